@@ -1,0 +1,175 @@
+/*
+ * nyles_hip.h -- C ABI of libnyles_hip.so, the MI355X (gfx950) time step of
+ * the Nyles incompressible Boussinesq LES.
+ *
+ * The library replaces the native layer the reference's Python host code
+ * calls into:
+ *   - the f2py stencil modules fortran_upwind, fortran_vortex_force,
+ *     fortran_vorticity, fortran_bernoulli, fortran_kinenergy
+ *     (core/Makefile:1-12; signatures core/fortran_upwind.f90:156,
+ *     core/fortran_vortex_force.f90:10,87, core/fortran_vorticity.f90:2,
+ *     core/fortran_bernoulli.f90:2,29,61, core/fortran_kinenergy.f90:101);
+ *   - the ctypes ABI of the mgfor multigrid library
+ *     (core/mgfordriver.py:5-105 over core/build.py:262-314:
+ *     get_ptrmg, solve, set_pyarray, get_pyarray, get_pyshape);
+ *   - the one-rank MPI halo exchange (core/mpi/halo.py:123-188).
+ *
+ * Conventions
+ *   - Every entry point returns 0 on success and a negative NYL_E_* code on
+ *     failure; nyl_last_error() then describes the failure (thread-local).
+ *     Nothing in the library calls exit() or abort().
+ *   - The library allocates no device memory.  The caller (the Python host
+ *     layer, through PyTorch's allocator) provides one workspace buffer of
+ *     nyl_workspace_bytes() bytes, which must outlive the context.
+ *   - All work is enqueued on the stream given by nyl_set_stream() (default:
+ *     the null stream).  Entry points that return host values
+ *     (nyl_reduce, nyl_mg_solve, nyl_diagnose, nyl_step_rk3) synchronise
+ *     that stream.  A context must not be used from two host threads at once.
+ *   - Fields are stored interior-only, C order [z][y][x] (x contiguous).
+ *     The halo of the reference layout is not stored: periodic neighbours
+ *     are read at their wrapped index and wall closures are applied in the
+ *     kernels, which is value-identical to the reference's filled halos.
+ */
+#ifndef NYLES_HIP_H
+#define NYLES_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NYL_ABI_VERSION 1
+
+/* status codes */
+#define NYL_OK 0
+#define NYL_E_ARG (-1)      /* invalid argument */
+#define NYL_E_STATE (-2)    /* call out of order (e.g. workspace not bound) */
+#define NYL_E_HIP (-3)      /* HIP runtime error */
+#define NYL_E_SIZE (-4)     /* unsupported grid size */
+
+/* precision modes */
+#define NYL_PREC_FP64 0     /* loop-faithful FP64, bit-compatible with the oracle */
+#define NYL_PREC_MIXED 1    /* FP32 stencils and V-cycles, FP64 outer MG residual */
+
+typedef struct nyl_ctx nyl_ctx;
+
+/* Geometry and numerics of one model instance.
+ * Mirrors the parameters core/model_les.py:33-95 and core/grid.py:15-53
+ * read, plus the mgfor MG_Param defaults (core/mgfor/mg_types.f90:15-26). */
+typedef struct nyl_desc {
+    int32_t nx, ny, nz;          /* interior cells (global_n* on one rank) */
+    int32_t periodic_x, periodic_y, periodic_z;   /* geometry perio_* flags */
+    double dx, dy, dz;           /* core/grid.py:30-32 */
+    double idx2, idy2, idz2;     /* 1/dx**2 ... as computed by core/grid.py:42-44 */
+    double dx2;                  /* dx**2 as computed by core/mgfordriver.py:78 */
+    double fparameter;           /* coriolis*dx*dy, 0 if not rotating (core/model_les.py:61-67) */
+    int32_t precision;           /* NYL_PREC_* */
+    int32_t mg_maxite;           /* 20 (core/mgfor/mg_types.f90:20) */
+    double mg_tol;               /* 1e-6 on the squared ratio (core/mgfor/mg_types.f90:21) */
+    double mg_omega;             /* 0.9 (core/mgfor/mg_types.f90:22) */
+} nyl_desc;
+
+/* Per-solve statistics (core/mgfor/mg_types.f90:46-49 MG_Stats, plus normb). */
+typedef struct nyl_mg_stats {
+    int32_t nite;                /* V-cycles of the last solve */
+    int32_t total_cycles;        /* V-cycles since context creation */
+    double res;                  /* last squared relative residual */
+    double normb;                /* squared norm of the right-hand side */
+} nyl_mg_stats;
+
+/* field identifiers for nyl_field_offset */
+enum {
+    NYL_F_B = 0, NYL_F_U, NYL_F_V, NYL_F_W,          /* prognostic state b, u_i, u_j, u_k */
+    NYL_F_KE, NYL_F_VORX, NYL_F_VORY, NYL_F_VORZ,    /* diagnostics ke, vor */
+    NYL_F_UCX, NYL_F_UCY, NYL_F_UCZ,                 /* contravariant U (core/cov_to_contra.py) */
+    NYL_F_P, NYL_F_DIV,                              /* pressure, divergence */
+    NYL_F_DS0_B, NYL_F_DS0_U, NYL_F_DS0_V, NYL_F_DS0_W,   /* RK tendencies, slot 0 */
+    NYL_F_DS1_B, NYL_F_DS1_U, NYL_F_DS1_V, NYL_F_DS1_W,   /* slot 1 */
+    NYL_F_DS2_B, NYL_F_DS2_U, NYL_F_DS2_V, NYL_F_DS2_W,   /* slot 2 */
+    NYL_F_MGX,                                       /* MG level-1 solution x (warm start) */
+    NYL_F_MGB,                                       /* MG level-1 right-hand side b */
+    NYL_NFIELDS
+};
+
+/* dtype codes returned by nyl_field_offset */
+#define NYL_DT_F64 0
+#define NYL_DT_F32 1
+
+/* what nyl_reduce computes */
+#define NYL_RED_MAXU2 0   /* max over cells of U^2+V^2+W^2 (core/nyles.py:249) */
+#define NYL_RED_KE 1      /* sum of ke over the interior (FP64) */
+#define NYL_RED_BSUM 2    /* sum of b over the interior */
+#define NYL_RED_NONFINITE 3  /* number of non-finite values in b,u,v,w */
+
+int nyl_abi_version(void);
+const char *nyl_last_error(void);
+
+/* core/mgfordriver.py:6-32 MG.__init__ + core/model_les.py:33-95 LES.__init__ */
+int nyl_ctx_create(const nyl_desc *desc, nyl_ctx **out);
+int nyl_ctx_destroy(nyl_ctx *ctx);
+
+/* Workspace sizing and binding.  nyl_bind_workspace zero-fills the buffer
+ * (the reference starts every array at zero, core/variables.py:146-150). */
+int nyl_workspace_bytes(const nyl_ctx *ctx, uint64_t *bytes);
+int nyl_bind_workspace(nyl_ctx *ctx, void *dptr, uint64_t bytes);
+int nyl_set_stream(nyl_ctx *ctx, void *hip_stream);
+
+/* Byte offset, element count and dtype of a field in the workspace.  The
+ * state fields are double-buffered: query again after every step. */
+int nyl_field_offset(const nyl_ctx *ctx, int field, uint64_t *offset_bytes,
+                     int64_t *nelem, int32_t *dtype);
+
+/* MG hierarchy: number of levels and interior size of a level
+ * (core/mgfor/mg_setup.f90:225-307, get_pyshape core/mgfor/pytools.f90:8-15). */
+int nyl_mg_levels(const nyl_ctx *ctx, int32_t *nlevels);
+int nyl_mg_level_shape(const nyl_ctx *ctx, int level, int32_t *nx, int32_t *ny, int32_t *nz);
+
+/* LES.rhs (core/model_les.py:129-144): tracer advection, vortex force and
+ * Bernoulli tendencies of the current state into tendency slot 0..2. */
+int nyl_rhs(nyl_ctx *ctx, int slot, int last);
+
+/* One RK3_SSP stage (core/timescheme.py:403-444) without the projection:
+ * rhs into the stage's slot and the stage update of b,u,v,w.
+ * stage 1: s+=c*ds0; 2: s+=c*(ds1-3*ds0); 3: s+=c*(8*ds2-ds0-ds1), with
+ * c = dt, dt/4., dt/12. computed by the caller. */
+int nyl_rk3_stage(nyl_ctx *ctx, int stage, double c);
+
+/* LES.diagnose_var (core/model_les.py:97-127): U, div, MG projection
+ * (core/projection.py:38-87, core/mgfordriver.py:66-78), u -= grad p,
+ * then U, vorticity and ke. */
+int nyl_diagnose(nyl_ctx *ctx, nyl_mg_stats *stats);
+
+/* Timescheme.RK3_SSP forward (core/timescheme.py:403-444) with three
+ * diagnose_var calls; stats accumulates the three solves (nite = sum). */
+int nyl_step_rk3(nyl_ctx *ctx, double dt, nyl_mg_stats *stats);
+
+/* mgfor solve (core/mgfor/solvers.f90:8-33) on the current MG right-hand
+ * side NYL_F_MGB, warm-started from NYL_F_MGX. */
+int nyl_mg_solve(nyl_ctx *ctx, nyl_mg_stats *stats);
+
+/* Diagnostic pieces, each one reference call:
+ *   NYL_OP_U_FROM_U   core/cov_to_contra.py:4-20
+ *   NYL_OP_DIV        core/projection.py:15-35 (into NYL_F_DIV and NYL_F_MGB)
+ *   NYL_OP_CORRECT    core/projection.py:82-87 with p = x*dx**2
+ *   NYL_OP_VORTICITY  core/vorticity.py:7-34
+ *   NYL_OP_KINENERGY  core/kinenergy.py:7-24
+ *   NYL_OP_PRESSURE   p = x*dx**2 into NYL_F_P (core/mgfordriver.py:78) */
+#define NYL_OP_U_FROM_U 0
+#define NYL_OP_DIV 1
+#define NYL_OP_CORRECT 2
+#define NYL_OP_VORTICITY 3
+#define NYL_OP_KINENERGY 4
+#define NYL_OP_PRESSURE 5
+int nyl_apply(nyl_ctx *ctx, int op);
+
+/* Reductions (synchronous). */
+int nyl_reduce(nyl_ctx *ctx, int what, double *out);
+
+/* Wait for all work of the context's stream. */
+int nyl_sync(nyl_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* NYLES_HIP_H */

@@ -1,0 +1,505 @@
+// nyl_api.hip -- context, workspace layout and the extern "C" entry points of
+// libnyles_hip.so (declared in include/nyles_hip.h).
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <vector>
+
+#include "nyl_internal.h"
+#include "../../include/nyles_hip.h"
+
+using namespace nyl;
+
+namespace {
+
+thread_local char g_err[512] = "";
+
+int fail(int code, const char *fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+#define HIPCHK(...)                                                                    \
+    do {                                                                               \
+        hipError_t e_ = (__VA_ARGS__);                                                 \
+        if (e_ != hipSuccess)                                                          \
+            return fail(NYL_E_HIP, "%s: %s (%s:%d)", #__VA_ARGS__,                    \
+                        hipGetErrorString(e_), __FILE__, __LINE__);                    \
+    } while (0)
+
+constexpr int kMaxLevels = 32;
+
+struct Level {
+    Geo g;
+    uint64_t x, b, y;   // byte offsets in the workspace
+};
+
+}  // namespace
+
+struct nyl_ctx {
+    nyl_desc d;
+    Geo g;
+    int64_t ncell;
+    int nlev;
+    Level lev[kMaxLevels];
+    MGTabs tabs;
+    // byte offsets of the fields
+    uint64_t state[2][4];
+    uint64_t ds[3][4];
+    uint64_t ke, vor[3], uc[3], p;
+    uint64_t partials, result;
+    uint64_t ws_bytes;
+    int cur;                 // which state buffer is current
+    char *ws;
+    hipStream_t stream;
+    double *host_out;        // pinned, 8 doubles
+    int total_cycles;
+};
+
+namespace {
+
+template <typename T>
+T *F(nyl_ctx *c, uint64_t off) { return reinterpret_cast<T *>(c->ws + off); }
+
+uint64_t align256(uint64_t v) { return (v + 255) & ~uint64_t(255); }
+
+Geo make_geo(int nx, int ny, int nz, int px, int py, int pz) {
+    Geo g;
+    g.nx = nx; g.ny = ny; g.nz = nz;
+    g.px = px; g.py = py; g.pz = pz;
+    g.sy = nx;
+    g.sz = (int64_t)nx * ny;
+    return g;
+}
+
+// core/mgfor/mg_setup.f90:225-307 on one rank: halve x and y until one of
+// them equals 2; z (interior) halves alongside.  Every level that is
+// restricted must have even sizes so that its cells pair up exactly.
+int build_hierarchy(nyl_ctx *c) {
+    int x = c->d.nx, y = c->d.ny, z = c->d.nz;
+    int n = 0;
+    c->lev[n++].g = make_geo(x, y, z, c->g.px, c->g.py, c->g.pz);
+    while (!(x == 2 || y == 2)) {
+        if ((x & 1) || (y & 1) || (z & 1) || x < 2 || y < 2 || z < 2)
+            return fail(NYL_E_SIZE,
+                        "grid %dx%dx%d: multigrid level %d has size %dx%dx%d that does not "
+                        "halve evenly (the mgfor hierarchy needs x or y to reach 2)",
+                        c->d.nx, c->d.ny, c->d.nz, n - 1, x, y, z);
+        x /= 2; y /= 2; z /= 2;
+        if (n >= kMaxLevels) return fail(NYL_E_SIZE, "too many multigrid levels");
+        c->lev[n++].g = make_geo(x, y, z, c->g.px, c->g.py, c->g.pz);
+    }
+    c->nlev = n;
+    return NYL_OK;
+}
+
+void build_tabs(MGTabs &t) {
+    for (int d = 0; d <= 6; d++) {
+        t.diag[d] = (double)d;
+        t.idiag[d] = d > 0 ? 1.0 / (double)d : 0.0;
+    }
+    for (int m = 0; m < 8; m++) {
+        const int mx = m & 1, my = (m >> 1) & 1, mz = (m >> 2) & 1;
+        t.pcoef[m] = 1.0 / (double)((3 + mx) * (3 + my) * (3 + mz));
+    }
+}
+
+void build_layout(nyl_ctx *c) {
+    const uint64_t fb = align256((uint64_t)c->ncell * sizeof(double));
+    uint64_t off = 0;
+    auto take = [&](uint64_t bytes) { uint64_t o = off; off += align256(bytes); return o; };
+    for (int s = 0; s < 2; s++)
+        for (int f = 0; f < 4; f++) c->state[s][f] = take(fb);
+    for (int s = 0; s < 3; s++)
+        for (int f = 0; f < 4; f++) c->ds[s][f] = take(fb);
+    c->ke = take(fb);
+    for (int f = 0; f < 3; f++) c->vor[f] = take(fb);
+    for (int f = 0; f < 3; f++) c->uc[f] = take(fb);
+    c->p = take(fb);
+    for (int l = 0; l < c->nlev; l++) {
+        const uint64_t lb = (uint64_t)c->lev[l].g.nx * c->lev[l].g.ny * c->lev[l].g.nz * sizeof(double);
+        c->lev[l].x = take(lb);
+        c->lev[l].b = take(lb);
+        c->lev[l].y = take(lb);
+    }
+    c->partials = take(NYL_RED_BLOCKS * sizeof(double));
+    c->result = take(8 * sizeof(double));
+    c->ws_bytes = off;
+}
+
+int check_ready(const nyl_ctx *c) {
+    if (!c) return fail(NYL_E_ARG, "null context");
+    if (!c->ws) return fail(NYL_E_STATE, "workspace not bound (call nyl_bind_workspace)");
+    return NYL_OK;
+}
+
+int read_result(nyl_ctx *c, double *out) {
+    HIPCHK(hipMemcpyAsync(c->host_out, c->ws + c->result, sizeof(double), hipMemcpyDeviceToHost,
+                          c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    *out = c->host_out[0];
+    return NYL_OK;
+}
+
+StageArgs<double> stage_args(nyl_ctx *c) {
+    StageArgs<double> a;
+    const int s = c->cur, t = 1 - c->cur;
+    a.b = F<double>(c, c->state[s][0]); a.u = F<double>(c, c->state[s][1]);
+    a.v = F<double>(c, c->state[s][2]); a.w = F<double>(c, c->state[s][3]);
+    a.bn = F<double>(c, c->state[t][0]); a.un = F<double>(c, c->state[t][1]);
+    a.vn = F<double>(c, c->state[t][2]); a.wn = F<double>(c, c->state[t][3]);
+    a.Ux = F<double>(c, c->uc[0]); a.Uy = F<double>(c, c->uc[1]); a.Uz = F<double>(c, c->uc[2]);
+    a.ox = F<double>(c, c->vor[0]); a.oy = F<double>(c, c->vor[1]); a.oz = F<double>(c, c->vor[2]);
+    a.ke = F<double>(c, c->ke);
+    a.d0b = F<double>(c, c->ds[0][0]); a.d0u = F<double>(c, c->ds[0][1]);
+    a.d0v = F<double>(c, c->ds[0][2]); a.d0w = F<double>(c, c->ds[0][3]);
+    a.d1b = F<double>(c, c->ds[1][0]); a.d1u = F<double>(c, c->ds[1][1]);
+    a.d1v = F<double>(c, c->ds[1][2]); a.d1w = F<double>(c, c->ds[1][3]);
+    a.tb = a.tu = a.tv = a.tw = nullptr;
+    a.dz = c->d.dz;
+    a.c = 0.0;
+    return a;
+}
+
+void set_tendency_out(nyl_ctx *c, StageArgs<double> &a, int slot) {
+    a.tb = F<double>(c, c->ds[slot][0]); a.tu = F<double>(c, c->ds[slot][1]);
+    a.tv = F<double>(c, c->ds[slot][2]); a.tw = F<double>(c, c->ds[slot][3]);
+}
+
+// core/mgfor/solvers.f90:35-55
+int vcycle(nyl_ctx *c) {
+    const int L = c->nlev;
+    const double om = c->d.mg_omega;
+    for (int l = 0; l < L - 1; l++) {
+        Level &f = c->lev[l], &q = c->lev[l + 1];
+        double *x = F<double>(c, f.x), *b = F<double>(c, f.b), *y = F<double>(c, f.y);
+        HIPCHK(launch_mg_sweep<double>(f.g, x, y, b, om, c->tabs, c->stream));
+        HIPCHK(launch_mg_sweep<double>(f.g, y, x, b, om, c->tabs, c->stream));
+        HIPCHK(launch_mg_resid_restrict<double>(f.g, q.g, x, b, F<double>(c, q.b),
+                                                F<double>(c, q.x), c->tabs, c->stream));
+    }
+    {
+        Level &f = c->lev[L - 1];
+        double *x = F<double>(c, f.x), *b = F<double>(c, f.b), *y = F<double>(c, f.y);
+        HIPCHK(launch_mg_sweep<double>(f.g, x, y, b, om, c->tabs, c->stream));
+        HIPCHK(launch_mg_sweep<double>(f.g, y, x, b, om, c->tabs, c->stream));
+    }
+    for (int l = L - 2; l >= 0; l--) {
+        Level &f = c->lev[l], &q = c->lev[l + 1];
+        double *x = F<double>(c, f.x), *b = F<double>(c, f.b), *y = F<double>(c, f.y);
+        HIPCHK(launch_mg_prolong<double>(f.g, q.g, x, F<double>(c, q.x), c->tabs, c->stream));
+        HIPCHK(launch_mg_sweep<double>(f.g, x, y, b, om, c->tabs, c->stream));
+        HIPCHK(launch_mg_sweep<double>(f.g, y, x, b, om, c->tabs, c->stream));
+    }
+    return NYL_OK;
+}
+
+// core/mgfor/solvers.f90:8-33 with normresidual (core/mgfor/operators.f90:81-96)
+int mg_solve(nyl_ctx *c, nyl_mg_stats *st) {
+    Level &f = c->lev[0];
+    const double *x = F<double>(c, f.x), *b = F<double>(c, f.b);
+    double *part = F<double>(c, c->partials), *res_dev = F<double>(c, c->result);
+    int rc;
+    double normb, rn, res;
+    HIPCHK(launch_norm2<double>(f.g, b, part, res_dev, c->stream));
+    if ((rc = read_result(c, &normb))) return rc;
+    auto normresidual = [&](double *out) -> int {
+        if (normb > 0.0) {
+            HIPCHK(launch_resnorm2<double>(f.g, x, b, c->tabs, part, res_dev, c->stream));
+            int r2 = read_result(c, &rn);
+            if (r2) return r2;
+            *out = rn / normb;
+        } else {
+            *out = 0.0;
+        }
+        return NYL_OK;
+    };
+    if ((rc = normresidual(&res))) return rc;
+    int nite = 0;
+    for (;;) {
+        if (res < c->d.mg_tol) break;
+        if ((rc = vcycle(c))) return rc;
+        nite = nite + 1;
+        if (nite >= c->d.mg_maxite) break;
+        if ((rc = normresidual(&res))) return rc;
+    }
+    c->total_cycles += nite;
+    if (st) {
+        st->nite = nite;
+        st->res = res;
+        st->normb = normb;
+        st->total_cycles = c->total_cycles;
+    }
+    return NYL_OK;
+}
+
+DiagArgs<double> diag_args(nyl_ctx *c) {
+    DiagArgs<double> a;
+    const int s = c->cur;
+    a.u = F<double>(c, c->state[s][1]); a.v = F<double>(c, c->state[s][2]);
+    a.w = F<double>(c, c->state[s][3]);
+    a.Ux = F<double>(c, c->uc[0]); a.Uy = F<double>(c, c->uc[1]); a.Uz = F<double>(c, c->uc[2]);
+    a.ox = F<double>(c, c->vor[0]); a.oy = F<double>(c, c->vor[1]); a.oz = F<double>(c, c->vor[2]);
+    a.ke = F<double>(c, c->ke);
+    a.idx2 = c->d.idx2; a.idy2 = c->d.idy2; a.idz2 = c->d.idz2;
+    a.fpar = c->d.fparameter > 0.0 ? c->d.fparameter : 0.0;
+    return a;
+}
+
+int apply_op(nyl_ctx *c, int op) {
+    const int s = c->cur;
+    double *u = F<double>(c, c->state[s][1]), *v = F<double>(c, c->state[s][2]),
+           *w = F<double>(c, c->state[s][3]);
+    switch (op) {
+    case NYL_OP_U_FROM_U:
+        HIPCHK(launch_diag<double>(c->g, diag_args(c), 1, c->stream));
+        break;
+    case NYL_OP_DIV:
+        HIPCHK(launch_div<double, double>(c->g, u, v, w, c->d.idx2, c->d.idy2, c->d.idz2,
+                                          F<double>(c, c->lev[0].b), c->stream));
+        break;
+    case NYL_OP_CORRECT:
+        HIPCHK(launch_correct<double, double>(c->g, u, v, w, F<double>(c, c->lev[0].x), c->d.dx2,
+                                              c->stream));
+        break;
+    case NYL_OP_VORTICITY:
+        HIPCHK(launch_diag<double>(c->g, diag_args(c), 2, c->stream));
+        break;
+    case NYL_OP_KINENERGY:
+        HIPCHK(launch_diag<double>(c->g, diag_args(c), 4, c->stream));
+        break;
+    case NYL_OP_PRESSURE:
+        HIPCHK(launch_pressure<double, double>(c->g, F<double>(c, c->lev[0].x), c->d.dx2,
+                                               F<double>(c, c->p), c->stream));
+        break;
+    default:
+        return fail(NYL_E_ARG, "unknown op %d", op);
+    }
+    return NYL_OK;
+}
+
+// core/model_les.py:97-127 (halo fills are implicit in the wrapped reads)
+int diagnose(nyl_ctx *c, nyl_mg_stats *st) {
+    int rc;
+    if ((rc = apply_op(c, NYL_OP_DIV))) return rc;
+    if ((rc = mg_solve(c, st))) return rc;
+    if ((rc = apply_op(c, NYL_OP_CORRECT))) return rc;
+    HIPCHK(launch_diag<double>(c->g, diag_args(c), 7, c->stream));
+    return NYL_OK;
+}
+
+int stage(nyl_ctx *c, int st, double coef) {
+    StageArgs<double> a = stage_args(c);
+    if (st == 1 || st == 2) set_tendency_out(c, a, st - 1);
+    a.c = coef;
+    HIPCHK(launch_stage<double>(c->g, a, st, c->stream));
+    c->cur = 1 - c->cur;
+    return NYL_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int nyl_abi_version(void) { return NYL_ABI_VERSION; }
+
+const char *nyl_last_error(void) { return g_err; }
+
+int nyl_ctx_create(const nyl_desc *desc, nyl_ctx **out) {
+    if (!desc || !out) return fail(NYL_E_ARG, "null argument");
+    *out = nullptr;
+    const nyl_desc &d = *desc;
+    if (d.precision != NYL_PREC_FP64)
+        return fail(NYL_E_ARG, "precision mode %d is not available in this build", d.precision);
+    const int n3[3] = {d.nx, d.ny, d.nz};
+    const int p3[3] = {d.periodic_x, d.periodic_y, d.periodic_z};
+    for (int a = 0; a < 3; a++) {
+        // wall axes need n >= 6 for the flux1d closures (core/weno.f90:116-150)
+        if (n3[a] < 8)
+            return fail(NYL_E_SIZE, "axis %d has %d cells; at least 8 are needed", a, n3[a]);
+        if (p3[a] != 0 && p3[a] != 1) return fail(NYL_E_ARG, "periodic flags must be 0 or 1");
+    }
+    if (!(d.dx > 0 && d.dy > 0 && d.dz > 0)) return fail(NYL_E_ARG, "grid spacings must be > 0");
+    if (d.mg_maxite < 0) return fail(NYL_E_ARG, "mg_maxite must be >= 0");
+    nyl_ctx *c = new nyl_ctx();
+    memset(c, 0, sizeof(*c));
+    c->d = d;
+    c->g = make_geo(d.nx, d.ny, d.nz, d.periodic_x, d.periodic_y, d.periodic_z);
+    c->ncell = (int64_t)d.nx * d.ny * d.nz;
+    int rc = build_hierarchy(c);
+    if (rc) { delete c; return rc; }
+    build_tabs(c->tabs);
+    build_layout(c);
+    c->stream = nullptr;
+    *out = c;
+    return NYL_OK;
+}
+
+int nyl_ctx_destroy(nyl_ctx *c) {
+    if (!c) return NYL_OK;
+    if (c->host_out) (void)hipHostFree(c->host_out);
+    delete c;
+    return NYL_OK;
+}
+
+int nyl_workspace_bytes(const nyl_ctx *c, uint64_t *bytes) {
+    if (!c || !bytes) return fail(NYL_E_ARG, "null argument");
+    *bytes = c->ws_bytes;
+    return NYL_OK;
+}
+
+int nyl_set_stream(nyl_ctx *c, void *stream) {
+    if (!c) return fail(NYL_E_ARG, "null context");
+    c->stream = (hipStream_t)stream;
+    return NYL_OK;
+}
+
+int nyl_bind_workspace(nyl_ctx *c, void *dptr, uint64_t bytes) {
+    if (!c || !dptr) return fail(NYL_E_ARG, "null argument");
+    if (bytes < c->ws_bytes)
+        return fail(NYL_E_ARG, "workspace of %llu bytes is smaller than the %llu needed",
+                    (unsigned long long)bytes, (unsigned long long)c->ws_bytes);
+    if (((uintptr_t)dptr) & 255) return fail(NYL_E_ARG, "workspace must be 256-byte aligned");
+    if (!c->host_out) HIPCHK(hipHostMalloc((void **)&c->host_out, 8 * sizeof(double), 0));
+    c->ws = (char *)dptr;
+    HIPCHK(launch_fill((double *)c->ws, (int64_t)(c->ws_bytes / sizeof(double)), 0.0, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->cur = 0;
+    c->total_cycles = 0;
+    return NYL_OK;
+}
+
+int nyl_field_offset(const nyl_ctx *c, int field, uint64_t *off, int64_t *nelem, int32_t *dtype) {
+    if (!c || !off) return fail(NYL_E_ARG, "null argument");
+    uint64_t o;
+    const int s = c->cur;
+    switch (field) {
+    case NYL_F_B: case NYL_F_U: case NYL_F_V: case NYL_F_W: o = c->state[s][field - NYL_F_B]; break;
+    case NYL_F_KE: o = c->ke; break;
+    case NYL_F_VORX: case NYL_F_VORY: case NYL_F_VORZ: o = c->vor[field - NYL_F_VORX]; break;
+    case NYL_F_UCX: case NYL_F_UCY: case NYL_F_UCZ: o = c->uc[field - NYL_F_UCX]; break;
+    case NYL_F_P: o = c->p; break;
+    case NYL_F_DIV: o = c->lev[0].b; break;
+    case NYL_F_MGX: o = c->lev[0].x; break;
+    case NYL_F_MGB: o = c->lev[0].b; break;
+    default:
+        if (field >= NYL_F_DS0_B && field <= NYL_F_DS2_W) {
+            const int q = field - NYL_F_DS0_B;
+            o = c->ds[q / 4][q % 4];
+            break;
+        }
+        return fail(NYL_E_ARG, "unknown field %d", field);
+    }
+    *off = o;
+    if (nelem) *nelem = c->ncell;
+    if (dtype) *dtype = NYL_DT_F64;
+    return NYL_OK;
+}
+
+int nyl_mg_levels(const nyl_ctx *c, int32_t *n) {
+    if (!c || !n) return fail(NYL_E_ARG, "null argument");
+    *n = c->nlev;
+    return NYL_OK;
+}
+
+int nyl_mg_level_shape(const nyl_ctx *c, int level, int32_t *nx, int32_t *ny, int32_t *nz) {
+    if (!c || !nx || !ny || !nz) return fail(NYL_E_ARG, "null argument");
+    if (level < 0 || level >= c->nlev) return fail(NYL_E_ARG, "level %d out of range", level);
+    *nx = c->lev[level].g.nx; *ny = c->lev[level].g.ny; *nz = c->lev[level].g.nz;
+    return NYL_OK;
+}
+
+int nyl_rhs(nyl_ctx *c, int slot, int last) {
+    int rc = check_ready(c);
+    if (rc) return rc;
+    (void)last;   // viscosity at the last stage is not part of this model path
+    if (slot < 0 || slot > 2) return fail(NYL_E_ARG, "slot must be 0..2");
+    StageArgs<double> a = stage_args(c);
+    set_tendency_out(c, a, slot);
+    HIPCHK(launch_stage<double>(c->g, a, 0, c->stream));
+    return NYL_OK;
+}
+
+int nyl_rk3_stage(nyl_ctx *c, int st, double coef) {
+    int rc = check_ready(c);
+    if (rc) return rc;
+    if (st < 1 || st > 3) return fail(NYL_E_ARG, "stage must be 1..3");
+    return stage(c, st, coef);
+}
+
+int nyl_diagnose(nyl_ctx *c, nyl_mg_stats *st) {
+    int rc = check_ready(c);
+    if (rc) return rc;
+    return diagnose(c, st);
+}
+
+int nyl_step_rk3(nyl_ctx *c, double dt, nyl_mg_stats *st) {
+    int rc = check_ready(c);
+    if (rc) return rc;
+    nyl_mg_stats s1, acc;
+    memset(&acc, 0, sizeof(acc));
+    const double coef[3] = {dt, dt / 4., dt / 12.};
+    for (int k = 1; k <= 3; k++) {
+        if ((rc = stage(c, k, coef[k - 1]))) return rc;
+        if ((rc = diagnose(c, &s1))) return rc;
+        acc.nite += s1.nite;
+        acc.res = s1.res;
+        acc.normb = s1.normb;
+    }
+    acc.total_cycles = c->total_cycles;
+    if (st) *st = acc;
+    return NYL_OK;
+}
+
+int nyl_mg_solve(nyl_ctx *c, nyl_mg_stats *st) {
+    int rc = check_ready(c);
+    if (rc) return rc;
+    return mg_solve(c, st);
+}
+
+int nyl_apply(nyl_ctx *c, int op) {
+    int rc = check_ready(c);
+    if (rc) return rc;
+    return apply_op(c, op);
+}
+
+int nyl_reduce(nyl_ctx *c, int what, double *out) {
+    int rc = check_ready(c);
+    if (rc) return rc;
+    if (!out) return fail(NYL_E_ARG, "null argument");
+    const int s = c->cur;
+    double *part = F<double>(c, c->partials), *res = F<double>(c, c->result);
+    switch (what) {
+    case NYL_RED_MAXU2:
+        HIPCHK(launch_maxu2<double>(c->g, F<double>(c, c->state[s][1]), F<double>(c, c->state[s][2]),
+                                    F<double>(c, c->state[s][3]), c->d.idx2, c->d.idy2, c->d.idz2,
+                                    part, res, c->stream));
+        break;
+    case NYL_RED_KE:
+        HIPCHK(launch_sum<double>(c->g, F<double>(c, c->ke), part, res, c->stream));
+        break;
+    case NYL_RED_BSUM:
+        HIPCHK(launch_sum<double>(c->g, F<double>(c, c->state[s][0]), part, res, c->stream));
+        break;
+    case NYL_RED_NONFINITE:
+        HIPCHK(launch_nonfinite<double>(c->g, F<double>(c, c->state[s][0]),
+                                        F<double>(c, c->state[s][1]), F<double>(c, c->state[s][2]),
+                                        F<double>(c, c->state[s][3]), part, res, c->stream));
+        break;
+    default:
+        return fail(NYL_E_ARG, "unknown reduction %d", what);
+    }
+    return read_result(c, out);
+}
+
+int nyl_sync(nyl_ctx *c) {
+    if (!c) return fail(NYL_E_ARG, "null context");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return NYL_OK;
+}
+
+}  // extern "C"

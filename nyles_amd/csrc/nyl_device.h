@@ -1,0 +1,113 @@
+// nyl_device.h -- device helpers shared by the Nyles HIP kernels (gfx950).
+//
+// Fields are interior-only, C order [z][y][x].  Along a periodic axis a
+// neighbour index is wrapped; along a wall axis the reference's array-edge
+// closures are applied (the reference stores no halo on wall axes,
+// core/mpi/topology.py:253-304).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace nyl {
+
+struct Geo {
+    int nx, ny, nz;      // interior cells
+    int px, py, pz;      // 1 = periodic axis
+    int64_t sy, sz;      // strides of y and z (x stride is 1)
+};
+
+__device__ __forceinline__ int wrapi(int m, int n) {
+    // |m - [0,n)| < n by construction (offsets are at most 3, n >= 4)
+    return m < 0 ? m + n : (m >= n ? m - n : m);
+}
+
+// ---------------------------------------------------------------------------
+// WENO reconstructions of core/weno.f90.  Real literals there are REAL(4)
+// constants widened to the working precision (the f2py build has no
+// -fdefault-real-8); tau5 is an implicitly typed REAL(4) (core/weno.f90:45).
+// Each operation is written in the reference's evaluation order; FP64
+// instantiations are compiled without contraction.
+
+// core/weno.f90:1-22
+template <typename T>
+__device__ __forceinline__ T weno3(T qm, T q0, T qp) {
+    const T eps = T(1e-14f);
+    T qi1 = (-qm + T(3) * q0) * T(0.5);
+    T qi2 = (q0 + qp) * T(0.5);
+    T d1 = q0 - qm, d2 = qp - q0;
+    T beta1 = d1 * d1;
+    T beta2 = d2 * d2;
+    T tau = fabs(beta2 - beta1);
+    T w1 = (T(1) + tau / (beta1 + eps));
+    T w2 = (T(1) + tau / (beta2 + eps)) * T(2);
+    return (w1 * qi1 + w2 * qi2) / (w1 + w2);
+}
+
+// core/weno.f90:25-54
+template <typename T>
+__device__ __forceinline__ T weno5(T qmm, T qm, T q0, T qp, T qpp) {
+    const T eps = T(1e-16f);
+    const T c13 = T(1.0f / 3.0f), c76 = T(7.0f / 6.0f), c116 = T(11.0f / 6.0f);
+    const T c16 = T(1.0f / 6.0f), c56 = T(5.0f / 6.0f);
+    const T k1 = T(13.0f / 12.0f), k2 = T(0.25);
+
+    T qi1 = c13 * qmm - c76 * qm + c116 * q0;
+    T qi2 = -(c16 * qm) + c56 * q0 + c13 * qp;
+    T qi3 = c13 * q0 + c56 * qp - c16 * qpp;
+
+    T t1 = qmm - T(2) * qm + q0, t2 = qmm - T(4) * qm + T(3) * q0;
+    T beta1 = k1 * (t1 * t1) + k2 * (t2 * t2);
+    T t3 = qm - T(2) * q0 + qp, t4 = qm - qp;
+    T beta2 = k1 * (t3 * t3) + k2 * (t4 * t4);
+    T t5 = q0 - T(2) * qp + qpp, t6 = T(3) * q0 - T(4) * qp + qpp;
+    T beta3 = k1 * (t5 * t5) + k2 * (t6 * t6);
+
+    T tau5 = T((float)fabs(beta1 - beta3));
+
+    T w1 = (T(1) + tau5 / (beta1 + eps));
+    T w2 = T(6) * (T(1) + tau5 / (beta2 + eps));
+    T w3 = T(3) * (T(1) + tau5 / (beta3 + eps));
+    return (w1 * qi1 + w2 * qi2 + w3 * qi3) / (w1 + w2 + w3);
+}
+
+// Flux at line position f (0-based) of core/weno.f90:106-153 flux1d, for a
+// line of n cells; u is the advecting velocity at f, q(m) the advected
+// quantity at line position m.  On a periodic axis every position uses the
+// interior WENO5 stencil (the reference's halo makes the closures fall in
+// the halo).  On a wall axis the closures are applied with the reference's
+// precedence (later assignments win): n-1 > n-2 > n-3 > {0, 1} > interior.
+// The stencil values are selected first and one reconstruction evaluated,
+// so a wavefront with mixed upwind directions runs WENO5 once.
+template <typename T, typename QF>
+__device__ __forceinline__ T face_flux(int f, int n, int per, T u, const QF &q) {
+    const bool pos = u > T(0);
+    int kind;
+    T a0 = T(0), a1 = T(0), a2 = T(0), a3 = T(0), a4 = T(0);
+    if (per || (f >= 2 && f <= n - 4)) {
+        kind = 5;
+        if (pos) { a0 = q(f - 2); a1 = q(f - 1); a2 = q(f); a3 = q(f + 1); a4 = q(f + 2); }
+        else     { a0 = q(f + 3); a1 = q(f + 2); a2 = q(f + 1); a3 = q(f); a4 = q(f - 1); }
+    } else if (f == n - 1) {
+        kind = 0;
+    } else if (f == n - 2) {
+        if (pos) { kind = 3; a0 = q(f - 1); a1 = q(f); a2 = q(f + 1); }
+        else     { kind = 1; a0 = q(f + 1); }
+    } else if (f == n - 3) {
+        if (pos) { kind = 5; a0 = q(f - 2); a1 = q(f - 1); a2 = q(f); a3 = q(f + 1); a4 = q(f + 2); }
+        else     { kind = 3; a0 = q(f + 2); a1 = q(f + 1); a2 = q(f); }
+    } else if (f == 0) {
+        if (pos) { kind = 1; a0 = q(0); }
+        else     { kind = 3; a0 = q(2); a1 = q(1); a2 = q(0); }
+    } else {  // f == 1
+        if (pos) { kind = 3; a0 = q(0); a1 = q(1); a2 = q(2); }
+        else     { kind = 5; a0 = q(4); a1 = q(3); a2 = q(2); a3 = q(1); a4 = q(0); }
+    }
+    T w;
+    if (kind == 5) w = weno5<T>(a0, a1, a2, a3, a4);
+    else if (kind == 3) w = weno3<T>(a0, a1, a2);
+    else if (kind == 1) w = a0;
+    else return T(0);
+    return u * w;
+}
+
+}  // namespace nyl

@@ -1,0 +1,94 @@
+// nyl_internal.h -- host-side launcher declarations shared by the
+// translation units of libnyles_hip.so (not part of the public ABI).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "nyl_device.h"
+
+namespace nyl {
+
+// Arguments of the fused RK stage kernel (rhs + update).
+template <typename T>
+struct StageArgs {
+    const T *b, *u, *v, *w;          // current state
+    T *bn, *un, *vn, *wn;            // next state (stage update target)
+    const T *Ux, *Uy, *Uz;           // contravariant velocity
+    const T *ox, *oy, *oz;           // vorticity
+    const T *ke;                     // kinetic energy
+    const T *d0b, *d0u, *d0v, *d0w;  // tendency slot 0 (read in stages 2, 3)
+    const T *d1b, *d1u, *d1v, *d1w;  // tendency slot 1 (read in stage 3)
+    T *tb, *tu, *tv, *tw;            // tendency output (modes 0, 1, 2)
+    double dz;                       // for cff = 0.5*dz (core/fortran_bernoulli.f90:47)
+    double c;                        // RK coefficient dt, dt/4., dt/12.
+};
+
+// mode 0: tendencies only; 1..3: RK3_SSP stage update
+template <typename T>
+hipError_t launch_stage(const Geo &g, const StageArgs<T> &a, int mode, hipStream_t s);
+
+template <typename T>
+struct DiagArgs {
+    const T *u, *v, *w;
+    T *Ux, *Uy, *Uz;
+    T *ox, *oy, *oz;
+    T *ke;
+    double idx2, idy2, idz2;
+    double fpar;                     // Coriolis parameter added to vor_k when > 0
+};
+// what: bit 0 = U, bit 1 = vorticity, bit 2 = ke
+template <typename T>
+hipError_t launch_diag(const Geo &g, const DiagArgs<T> &a, int what, hipStream_t s);
+
+// div of U (computed from u) into out (core/projection.py:15-35)
+template <typename T, typename TO>
+hipError_t launch_div(const Geo &g, const T *u, const T *v, const T *w, double idx2,
+                      double idy2, double idz2, TO *out, hipStream_t s);
+
+// u -= grad(p), p = x*dx2 (core/projection.py:82-87)
+template <typename T, typename TX>
+hipError_t launch_correct(const Geo &g, T *u, T *v, T *w, const TX *x, double dx2,
+                          hipStream_t s);
+
+// p = x*dx2
+template <typename TX, typename T>
+hipError_t launch_pressure(const Geo &g, const TX *x, double dx2, T *p, hipStream_t s);
+
+// ---- multigrid (core/mgfor) ----------------------------------------------
+struct MGTabs {
+    double idiag[7];     // 1/diag for diag = 0..6 (0 -> 0)
+    double diag[7];      // diag as a real
+    double pcoef[8];     // 1/((3+mx)(3+my)(3+mz)) indexed mx + 2 my + 4 mz
+};
+
+template <typename T>
+hipError_t launch_mg_sweep(const Geo &g, const T *src, T *dst, const T *b, double omega,
+                           const MGTabs &tab, hipStream_t s);
+template <typename T>
+hipError_t launch_mg_resid_restrict(const Geo &gf, const Geo &gc, const T *x, const T *b,
+                                    T *bc, T *xc, const MGTabs &tab, hipStream_t s);
+template <typename T>
+hipError_t launch_mg_prolong(const Geo &gf, const Geo &gc, T *xf, const T *xc,
+                             const MGTabs &tab, hipStream_t s);
+
+// reductions: partials then one final block; result written to dev_out[0]
+#define NYL_RED_BLOCKS 1024
+template <typename T>
+hipError_t launch_norm2(const Geo &g, const T *v, double *partials, double *dev_out,
+                        hipStream_t s);
+template <typename T>
+hipError_t launch_resnorm2(const Geo &g, const T *x, const T *b, const MGTabs &tab,
+                           double *partials, double *dev_out, hipStream_t s);
+template <typename T>
+hipError_t launch_maxu2(const Geo &g, const T *u, const T *v, const T *w, double idx2,
+                        double idy2, double idz2, double *partials, double *dev_out,
+                        hipStream_t s);
+template <typename T>
+hipError_t launch_sum(const Geo &g, const T *v, double *partials, double *dev_out,
+                      hipStream_t s);
+template <typename T>
+hipError_t launch_nonfinite(const Geo &g, const T *a, const T *b, const T *c, const T *d,
+                            double *partials, double *dev_out, hipStream_t s);
+
+hipError_t launch_fill(double *p, int64_t n, double v, hipStream_t s);
+
+}  // namespace nyl

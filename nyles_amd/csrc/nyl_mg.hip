@@ -1,0 +1,351 @@
+// nyl_mg.hip -- geometric multigrid of the pressure projection (the mgfor
+// library of the reference) on gfx950, and the deterministic reductions.
+//
+// Level arrays are interior-only [z][y][x].  mgfor's coefficient arrays are
+// closed forms of the position here (they are what core/mgfor/operators.f90
+// :299-457 computes for a box without land): msk = 1 on the interior,
+// diag = number of in-domain neighbours, idiag = 1/diag, Rcoef = 1/2 and
+// Pcoef = 1/((3+mx)(3+my)(3+mz)) with m* = 1 when the coarse neighbour used
+// by the trilinear stencil is in the domain (or periodic), 0 at a wall.
+// Out-of-domain neighbours read as zero, which is what mgfor's unfilled wall
+// halos hold; periodic neighbours are wrapped, which is what its filled
+// halos hold (core/mgfor/mod_halo.f90:235-262).
+#include "nyl_internal.h"
+
+namespace nyl {
+
+#define MG_BX 64
+#define MG_BY 4
+
+static inline dim3 mg_grid(const Geo &g) {
+    return dim3((g.nx + MG_BX - 1) / MG_BX, (g.ny + MG_BY - 1) / MG_BY, g.nz);
+}
+
+// Face-neighbour values of a cell, zero outside a wall axis, and the number
+// of in-domain neighbours (mgfor's diag).
+template <typename T>
+struct Nbrs {
+    T xm, xp, ym, yp, zm, zp;
+    int nwet;
+};
+
+template <typename T>
+__device__ __forceinline__ Nbrs<T> neighbours(const Geo &g, const T *__restrict__ a, int i, int j,
+                                              int k, int64_t c) {
+    Nbrs<T> r;
+    r.nwet = 0;
+    if (g.px || i > 0) { r.xm = a[c - (g.px ? (int64_t)(i - wrapi(i - 1, g.nx)) : 1)]; r.nwet++; } else r.xm = T(0);
+    if (g.px || i < g.nx - 1) { r.xp = a[c + (g.px ? (int64_t)(wrapi(i + 1, g.nx) - i) : 1)]; r.nwet++; } else r.xp = T(0);
+    if (g.py || j > 0) { r.ym = a[c - (g.py ? (int64_t)(j - wrapi(j - 1, g.ny)) * g.sy : g.sy)]; r.nwet++; } else r.ym = T(0);
+    if (g.py || j < g.ny - 1) { r.yp = a[c + (g.py ? (int64_t)(wrapi(j + 1, g.ny) - j) * g.sy : g.sy)]; r.nwet++; } else r.yp = T(0);
+    if (g.pz || k > 0) { r.zm = a[c - (g.pz ? (int64_t)(k - wrapi(k - 1, g.nz)) * g.sz : g.sz)]; r.nwet++; } else r.zm = T(0);
+    if (g.pz || k < g.nz - 1) { r.zp = a[c + (g.pz ? (int64_t)(wrapi(k + 1, g.nz) - k) * g.sz : g.sz)]; r.nwet++; } else r.zp = T(0);
+    return r;
+}
+
+// One damped-Jacobi sweep of fsmoother3d (core/mgfor/basicoperators.f90:363-400):
+// dst = cff1*src + omega*(sum6(src) - b)*idiag.
+template <typename T>
+__global__ __launch_bounds__(256) void k_mg_sweep(Geo g, const T *__restrict__ src,
+                                                  T *__restrict__ dst, const T *__restrict__ b,
+                                                  double omega, MGTabs tab) {
+    const int i = blockIdx.x * MG_BX + threadIdx.x;
+    const int j = blockIdx.y * MG_BY + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i >= g.nx || j >= g.ny) return;
+    const int64_t c = (int64_t)k * g.sz + (int64_t)j * g.sy + i;
+    const Nbrs<T> n = neighbours<T>(g, src, i, j, k, c);
+    const T S = n.xm + n.xp + n.ym + n.yp + n.zm + n.zp;
+    const T om = T(omega), cff1 = T(1.0 - omega);
+    dst[c] = cff1 * src[c] + om * (S - b[c]) * T(tab.idiag[n.nwet]);
+}
+
+template <typename T>
+hipError_t launch_mg_sweep(const Geo &g, const T *src, T *dst, const T *b, double omega,
+                           const MGTabs &tab, hipStream_t s) {
+    hipLaunchKernelGGL((k_mg_sweep<T>), mg_grid(g), dim3(MG_BX, MG_BY, 1), 0, s, g, src, dst, b,
+                       omega, tab);
+    return hipGetLastError();
+}
+
+// fresidual3d (core/mgfor/basicoperators.f90:300-323): r = b + diag*x - sum6(x)
+template <typename T>
+__device__ __forceinline__ T residual_at(const Geo &g, const T *__restrict__ x,
+                                         const T *__restrict__ b, int i, int j, int k,
+                                         const MGTabs &tab) {
+    const int64_t c = (int64_t)k * g.sz + (int64_t)j * g.sy + i;
+    const Nbrs<T> n = neighbours<T>(g, x, i, j, k, c);
+    return b[c] + T(tab.diag[n.nwet]) * x[c] - (n.xm + n.xp + n.ym + n.yp + n.zm + n.zp);
+}
+
+// residual then restriction (core/mgfor/operators.f90:121-122 of vcycle:
+// residual + restriction(rb)); frestrict_centers3d
+// (core/mgfor/basicoperators.f90:32-60) with Rcoef = 1/2; coarse x = 0
+// (core/mgfor/operators.f90:209).
+template <typename T>
+__global__ __launch_bounds__(256) void k_mg_resid_restrict(Geo gf, Geo gc,
+                                                           const T *__restrict__ x,
+                                                           const T *__restrict__ b,
+                                                           T *__restrict__ bc, T *__restrict__ xc,
+                                                           MGTabs tab) {
+    const int ic = blockIdx.x * MG_BX + threadIdx.x;
+    const int jc = blockIdx.y * MG_BY + threadIdx.y;
+    const int kc = blockIdx.z;
+    if (ic >= gc.nx || jc >= gc.ny) return;
+    const int i = 2 * ic, j = 2 * jc, k = 2 * kc;
+    const T r000 = residual_at<T>(gf, x, b, i, j, k, tab);
+    const T r100 = residual_at<T>(gf, x, b, i + 1, j, k, tab);
+    const T r010 = residual_at<T>(gf, x, b, i, j + 1, k, tab);
+    const T r110 = residual_at<T>(gf, x, b, i + 1, j + 1, k, tab);
+    const T r001 = residual_at<T>(gf, x, b, i, j, k + 1, tab);
+    const T r101 = residual_at<T>(gf, x, b, i + 1, j, k + 1, tab);
+    const T r011 = residual_at<T>(gf, x, b, i, j + 1, k + 1, tab);
+    const T r111 = residual_at<T>(gf, x, b, i + 1, j + 1, k + 1, tab);
+    const int64_t cc = (int64_t)kc * gc.sz + (int64_t)jc * gc.sy + ic;
+    bc[cc] = T(0.5) * (r000 + r100 + r010 + r110 + r001 + r101 + r011 + r111);
+    xc[cc] = T(0);
+}
+
+template <typename T>
+hipError_t launch_mg_resid_restrict(const Geo &gf, const Geo &gc, const T *x, const T *b, T *bc,
+                                    T *xc, const MGTabs &tab, hipStream_t s) {
+    hipLaunchKernelGGL((k_mg_resid_restrict<T>), mg_grid(gc), dim3(MG_BX, MG_BY, 1), 0, s, gf, gc,
+                       x, b, bc, xc, tab);
+    return hipGetLastError();
+}
+
+// fprolongation_centers3d (core/mgfor/basicoperators.f90:173-231), one thread
+// per fine cell: xf += Pcoef*(3*b + a) with the (9,3,3,1) planar weights.
+template <typename T>
+__global__ __launch_bounds__(256) void k_mg_prolong(Geo gf, Geo gc, T *__restrict__ xf,
+                                                    const T *__restrict__ xc, MGTabs tab) {
+    const int i = blockIdx.x * MG_BX + threadIdx.x;
+    const int j = blockIdx.y * MG_BY + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i >= gf.nx || j >= gf.ny) return;
+    const int ic = i >> 1, jc = j >> 1, kc = k >> 1;
+    // coarse neighbour on the side of the fine cell within its parent
+    int icn = (i & 1) ? ic + 1 : ic - 1;
+    int jcn = (j & 1) ? jc + 1 : jc - 1;
+    int kcn = (k & 1) ? kc + 1 : kc - 1;
+    int mx = 1, my = 1, mz = 1;
+    if (gc.px) icn = wrapi(icn, gc.nx); else if (icn < 0 || icn >= gc.nx) mx = 0;
+    if (gc.py) jcn = wrapi(jcn, gc.ny); else if (jcn < 0 || jcn >= gc.ny) my = 0;
+    if (gc.pz) kcn = wrapi(kcn, gc.nz); else if (kcn < 0 || kcn >= gc.nz) mz = 0;
+    auto X = [&](int a, int bb, int cc, bool ok) -> T {
+        return ok ? xc[(int64_t)cc * gc.sz + (int64_t)bb * gc.sy + a] : T(0);
+    };
+    auto A = [&](int kk, bool okz) -> T {
+        return T(9) * X(ic, jc, kk, okz) + T(3) * X(icn, jc, kk, okz && mx)
+             + T(3) * X(ic, jcn, kk, okz && my) + X(icn, jcn, kk, okz && mx && my);
+    };
+    const T bmid = A(kc, true);
+    const T aside = A(kcn, mz != 0);
+    const int64_t c = (int64_t)k * gf.sz + (int64_t)j * gf.sy + i;
+    xf[c] = xf[c] + T(tab.pcoef[mx + 2 * my + 4 * mz]) * (T(3) * bmid + aside);
+}
+
+template <typename T>
+hipError_t launch_mg_prolong(const Geo &gf, const Geo &gc, T *xf, const T *xc, const MGTabs &tab,
+                             hipStream_t s) {
+    hipLaunchKernelGGL((k_mg_prolong<T>), mg_grid(gf), dim3(MG_BX, MG_BY, 1), 0, s, gf, gc, xf,
+                       xc, tab);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Deterministic two-stage reductions.  The grid size depends only on the
+// cell count, each thread sums a fixed set of cells in a fixed order and the
+// block trees are fixed, so a result does not change from run to run.
+
+#define RED_T 256
+
+__device__ __forceinline__ void block_sum_store(double v, double *out) {
+    __shared__ double sh[RED_T];
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = RED_T / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) sh[threadIdx.x] = sh[threadIdx.x] + sh[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = sh[0];
+}
+
+__device__ __forceinline__ void block_max_store(double v, double *out) {
+    __shared__ double sh[RED_T];
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = RED_T / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = sh[0];
+}
+
+static inline int red_blocks(int64_t n) {
+    int64_t b = (n + RED_T - 1) / RED_T;
+    return (int)(b < NYL_RED_BLOCKS ? (b < 1 ? 1 : b) : NYL_RED_BLOCKS);
+}
+
+__global__ __launch_bounds__(RED_T) void k_final_sum(const double *__restrict__ partials, int np,
+                                                     double *__restrict__ out) {
+    double acc = 0.0;
+    for (int q = threadIdx.x; q < np; q += RED_T) acc = acc + partials[q];
+    block_sum_store(acc, out);
+}
+
+__global__ __launch_bounds__(RED_T) void k_final_max(const double *__restrict__ partials, int np,
+                                                     double *__restrict__ out) {
+    double acc = 0.0;
+    for (int q = threadIdx.x; q < np; q += RED_T) acc = fmax(acc, partials[q]);
+    block_max_store(acc, out);
+}
+
+template <typename T>
+__global__ __launch_bounds__(RED_T) void k_norm2(int64_t n, const T *__restrict__ v,
+                                                 double *__restrict__ partials) {
+    double acc = 0.0;
+    for (int64_t q = (int64_t)blockIdx.x * RED_T + threadIdx.x; q < n;
+         q += (int64_t)gridDim.x * RED_T) {
+        const double x = (double)v[q];
+        acc = acc + x * x;
+    }
+    block_sum_store(acc, partials + blockIdx.x);
+}
+
+template <typename T>
+hipError_t launch_norm2(const Geo &g, const T *v, double *partials, double *dev_out,
+                        hipStream_t s) {
+    const int64_t n = (int64_t)g.nx * g.ny * g.nz;
+    const int nb = red_blocks(n);
+    hipLaunchKernelGGL((k_norm2<T>), dim3(nb), dim3(RED_T), 0, s, n, v, partials);
+    hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(RED_T), 0, s, partials, nb, dev_out);
+    return hipGetLastError();
+}
+
+template <typename T>
+__global__ __launch_bounds__(RED_T) void k_resnorm2(Geo g, const T *__restrict__ x,
+                                                    const T *__restrict__ b, MGTabs tab,
+                                                    double *__restrict__ partials) {
+    const int64_t n = (int64_t)g.nx * g.ny * g.nz;
+    double acc = 0.0;
+    for (int64_t q = (int64_t)blockIdx.x * RED_T + threadIdx.x; q < n;
+         q += (int64_t)gridDim.x * RED_T) {
+        const int i = (int)(q % g.nx);
+        const int64_t t = q / g.nx;
+        const int j = (int)(t % g.ny);
+        const int k = (int)(t / g.ny);
+        const double r = (double)residual_at<T>(g, x, b, i, j, k, tab);
+        acc = acc + r * r;
+    }
+    block_sum_store(acc, partials + blockIdx.x);
+}
+
+template <typename T>
+hipError_t launch_resnorm2(const Geo &g, const T *x, const T *b, const MGTabs &tab,
+                           double *partials, double *dev_out, hipStream_t s) {
+    const int64_t n = (int64_t)g.nx * g.ny * g.nz;
+    const int nb = red_blocks(n);
+    hipLaunchKernelGGL((k_resnorm2<T>), dim3(nb), dim3(RED_T), 0, s, g, x, b, tab, partials);
+    hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(RED_T), 0, s, partials, nb, dev_out);
+    return hipGetLastError();
+}
+
+// max of U**2 + V**2 + W**2 at equal indices (core/nyles.py:249)
+template <typename T>
+__global__ __launch_bounds__(RED_T) void k_maxu2(int64_t n, const T *__restrict__ u,
+                                                 const T *__restrict__ v,
+                                                 const T *__restrict__ w, double idx2,
+                                                 double idy2, double idz2,
+                                                 double *__restrict__ partials) {
+    double acc = 0.0;
+    for (int64_t q = (int64_t)blockIdx.x * RED_T + threadIdx.x; q < n;
+         q += (int64_t)gridDim.x * RED_T) {
+        const double U = (double)(u[q] * T(idx2)), V = (double)(v[q] * T(idy2)),
+                     W = (double)(w[q] * T(idz2));
+        const double s2 = U * U + V * V + W * W;
+        acc = fmax(acc, s2);
+    }
+    block_max_store(acc, partials + blockIdx.x);
+}
+
+template <typename T>
+hipError_t launch_maxu2(const Geo &g, const T *u, const T *v, const T *w, double idx2,
+                        double idy2, double idz2, double *partials, double *dev_out,
+                        hipStream_t s) {
+    const int64_t n = (int64_t)g.nx * g.ny * g.nz;
+    const int nb = red_blocks(n);
+    hipLaunchKernelGGL((k_maxu2<T>), dim3(nb), dim3(RED_T), 0, s, n, u, v, w, idx2, idy2, idz2,
+                       partials);
+    hipLaunchKernelGGL(k_final_max, dim3(1), dim3(RED_T), 0, s, partials, nb, dev_out);
+    return hipGetLastError();
+}
+
+template <typename T>
+__global__ __launch_bounds__(RED_T) void k_sum(int64_t n, const T *__restrict__ v,
+                                               double *__restrict__ partials) {
+    double acc = 0.0;
+    for (int64_t q = (int64_t)blockIdx.x * RED_T + threadIdx.x; q < n;
+         q += (int64_t)gridDim.x * RED_T)
+        acc = acc + (double)v[q];
+    block_sum_store(acc, partials + blockIdx.x);
+}
+
+template <typename T>
+hipError_t launch_sum(const Geo &g, const T *v, double *partials, double *dev_out,
+                      hipStream_t s) {
+    const int64_t n = (int64_t)g.nx * g.ny * g.nz;
+    const int nb = red_blocks(n);
+    hipLaunchKernelGGL((k_sum<T>), dim3(nb), dim3(RED_T), 0, s, n, v, partials);
+    hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(RED_T), 0, s, partials, nb, dev_out);
+    return hipGetLastError();
+}
+
+template <typename T>
+__global__ __launch_bounds__(RED_T) void k_nonfinite(int64_t n, const T *__restrict__ a,
+                                                     const T *__restrict__ b,
+                                                     const T *__restrict__ c,
+                                                     const T *__restrict__ d,
+                                                     double *__restrict__ partials) {
+    double acc = 0.0;
+    for (int64_t q = (int64_t)blockIdx.x * RED_T + threadIdx.x; q < n;
+         q += (int64_t)gridDim.x * RED_T) {
+        acc += isfinite((double)a[q]) ? 0.0 : 1.0;
+        acc += isfinite((double)b[q]) ? 0.0 : 1.0;
+        acc += isfinite((double)c[q]) ? 0.0 : 1.0;
+        acc += isfinite((double)d[q]) ? 0.0 : 1.0;
+    }
+    block_sum_store(acc, partials + blockIdx.x);
+}
+
+template <typename T>
+hipError_t launch_nonfinite(const Geo &g, const T *a, const T *b, const T *c, const T *d,
+                            double *partials, double *dev_out, hipStream_t s) {
+    const int64_t n = (int64_t)g.nx * g.ny * g.nz;
+    const int nb = red_blocks(n);
+    hipLaunchKernelGGL((k_nonfinite<T>), dim3(nb), dim3(RED_T), 0, s, n, a, b, c, d, partials);
+    hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(RED_T), 0, s, partials, nb, dev_out);
+    return hipGetLastError();
+}
+
+template hipError_t launch_mg_sweep<double>(const Geo &, const double *, double *, const double *,
+                                            double, const MGTabs &, hipStream_t);
+template hipError_t launch_mg_resid_restrict<double>(const Geo &, const Geo &, const double *,
+                                                     const double *, double *, double *,
+                                                     const MGTabs &, hipStream_t);
+template hipError_t launch_mg_prolong<double>(const Geo &, const Geo &, double *, const double *,
+                                              const MGTabs &, hipStream_t);
+template hipError_t launch_norm2<double>(const Geo &, const double *, double *, double *,
+                                         hipStream_t);
+template hipError_t launch_resnorm2<double>(const Geo &, const double *, const double *,
+                                            const MGTabs &, double *, double *, hipStream_t);
+template hipError_t launch_maxu2<double>(const Geo &, const double *, const double *,
+                                         const double *, double, double, double, double *,
+                                         double *, hipStream_t);
+template hipError_t launch_sum<double>(const Geo &, const double *, double *, double *,
+                                       hipStream_t);
+template hipError_t launch_nonfinite<double>(const Geo &, const double *, const double *,
+                                             const double *, const double *, double *, double *,
+                                             hipStream_t);
+
+}  // namespace nyl

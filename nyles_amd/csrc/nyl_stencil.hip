@@ -1,0 +1,300 @@
+// nyl_stencil.hip -- explicit right-hand side, RK3 stage update and the
+// diagnostic stencils of the Nyles time step on gfx950.
+//
+// Every FP64 instantiation reproduces the reference's arithmetic operation
+// by operation (the library is compiled with -ffp-contract=off), so its
+// results are bit-identical to the oracle's.
+#include "nyl_internal.h"
+
+namespace nyl {
+
+#define NYL_BX 64
+#define NYL_BY 4
+
+static inline dim3 cell_grid(const Geo &g) {
+    return dim3((g.nx + NYL_BX - 1) / NYL_BX, (g.ny + NYL_BY - 1) / NYL_BY, g.nz);
+}
+
+// Vortex-force flux of one term at a cell (core/fortran_vortex_force.f90
+// WENO branch, :66-81 and :145-159): the vorticity component `om` is
+// upwinded along axis d with the velocity component `Ud` averaged along the
+// target axis e (UU) and then along d (u1d).  `based` is the index of the
+// cell's line start along d, `sd` the stride of d, `md` the cell's position
+// on that line, `oe` the offset to the e+1 neighbour.
+template <typename T>
+__device__ __forceinline__ T vf_flux(const T *__restrict__ Ud, const T *__restrict__ om,
+                                     int64_t based, int64_t sd, int md, int nd, int pd,
+                                     int64_t oe) {
+    auto UU = [&](int m) -> T {
+        const int64_t p = based + (int64_t)m * sd;
+        return T(0.5) * (Ud[p] + Ud[p + oe]);
+    };
+    T uu_prev;
+    if (!pd && md == 0) uu_prev = T(0);   // UU_0 = 0. at the line start
+    else uu_prev = UU(pd ? wrapi(md - 1, nd) : md - 1);
+    const T u1d = T(0.5) * (uu_prev + UU(md));
+    auto q = [&](int m) -> T {            // q(1) = 0., q(k) = vort(k-1)
+        if (pd) return om[based + (int64_t)wrapi(m - 1, nd) * sd];
+        if (m == 0) return T(0);
+        return om[based + (int64_t)(m - 1) * sd];
+    };
+    return face_flux<T>(md, nd, pd, u1d, q);
+}
+
+// Tracer flux divergence along one axis (core/fortran_upwind.f90:228-234):
+// d = d + flux(m-1) - flux(m), or d - flux(1) at a wall start.
+template <typename T>
+__device__ __forceinline__ T tracer_axis(T d, const T *__restrict__ q_arr,
+                                         const T *__restrict__ U, int64_t based, int64_t sd,
+                                         int md, int nd, int pd) {
+    auto q = [&](int m) -> T { return q_arr[based + (int64_t)(pd ? wrapi(m, nd) : m) * sd]; };
+    const T fr = face_flux<T>(md, nd, pd, U[based + (int64_t)md * sd], q);
+    if (!pd && md == 0) return d - fr;
+    const int ml = pd ? wrapi(md - 1, nd) : md - 1;
+    const T fl = face_flux<T>(ml, nd, pd, U[based + (int64_t)ml * sd], q);
+    return (d + fl) - fr;
+}
+
+// Fused LES.rhs (core/model_les.py:129-144) + RK3_SSP stage update
+// (core/timescheme.py:422-443) for one cell.
+template <typename T, int MODE>
+__global__ __launch_bounds__(256) void k_stage(Geo g, StageArgs<T> a) {
+    const int i = blockIdx.x * NYL_BX + threadIdx.x;
+    const int j = blockIdx.y * NYL_BY + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i >= g.nx || j >= g.ny) return;
+    const int64_t c = (int64_t)k * g.sz + (int64_t)j * g.sy + i;
+    const int64_t bx = c - i, by = c - (int64_t)j * g.sy, bz = c - (int64_t)k * g.sz;
+
+    // tracer b: rhstrac, directions i, j, k (core/tracer.py:63-72)
+    T db = T(0);
+    db = tracer_axis<T>(db, a.b, a.Ux, bx, 1, i, g.nx, g.px);
+    db = tracer_axis<T>(db, a.b, a.Uy, by, g.sy, j, g.ny, g.py);
+    db = tracer_axis<T>(db, a.b, a.Uz, bz, g.sz, k, g.nz, g.pz);
+
+    // vortex force, triplets 'ikj', 'jik', 'kji' (core/vortex_force.py:69-81)
+    const int64_t oex = g.px ? (int64_t)(wrapi(i + 1, g.nx) - i) : 1;
+    const int64_t oey = g.py ? (int64_t)(wrapi(j + 1, g.ny) - j) * g.sy : g.sy;
+    const int64_t oez = g.pz ? (int64_t)(wrapi(k + 1, g.nz) - k) * g.sz : g.sz;
+    const bool ex = g.px || i < g.nx - 1;   // loops stop at n-1 on a wall axis
+    const bool ey = g.py || j < g.ny - 1;
+    const bool ez = g.pz || k < g.nz - 1;
+    T du = T(0), dv = T(0), dw = T(0);
+    if (ey) dv = dv - vf_flux<T>(a.Ux, a.oz, bx, 1, i, g.nx, g.px, oey);      // direc(U_i, vor_k, du_j)
+    if (ex) du = du + vf_flux<T>(a.Uy, a.oz, by, g.sy, j, g.ny, g.py, oex);   // flip (U_j, vor_k, du_i)
+    if (ez) dw = dw - vf_flux<T>(a.Uy, a.ox, by, g.sy, j, g.ny, g.py, oez);   // direc(U_j, vor_i, du_k)
+    if (ey) dv = dv + vf_flux<T>(a.Uz, a.ox, bz, g.sz, k, g.nz, g.pz, oey);   // flip (U_k, vor_i, du_j)
+    if (ex) du = du - vf_flux<T>(a.Uz, a.oy, bz, g.sz, k, g.nz, g.pz, oex);   // direc(U_k, vor_j, du_i)
+    if (ez) dw = dw + vf_flux<T>(a.Ux, a.oy, bx, 1, i, g.nx, g.px, oez);      // flip (U_i, vor_j, du_k)
+
+    // bernoulli (core/bernoulli.py:14-23, core/fortran_bernoulli.f90:2-58)
+    const T kec = a.ke[c];
+    if (ex) du = du - (a.ke[c + oex] - kec);
+    if (ey) dv = dv - (a.ke[c + oey] - kec);
+    if (ez) {
+        const T cff = T(0.5 * a.dz);
+        dw = dw - (a.ke[c + oez] - kec) + cff * (a.b[c + oez] + a.b[c]);
+    }
+
+    if (MODE == 0 || MODE == 1 || MODE == 2) {
+        a.tb[c] = db; a.tu[c] = du; a.tv[c] = dv; a.tw[c] = dw;
+    }
+    const T cc = T(a.c);
+    if (MODE == 1) {            // s += dt*ds0
+        a.bn[c] = a.b[c] + cc * db;
+        a.un[c] = a.u[c] + cc * du;
+        a.vn[c] = a.v[c] + cc * dv;
+        a.wn[c] = a.w[c] + cc * dw;
+    } else if (MODE == 2) {     // s += (dt/4.)*(ds1-3*ds0)
+        a.bn[c] = a.b[c] + cc * (db - T(3) * a.d0b[c]);
+        a.un[c] = a.u[c] + cc * (du - T(3) * a.d0u[c]);
+        a.vn[c] = a.v[c] + cc * (dv - T(3) * a.d0v[c]);
+        a.wn[c] = a.w[c] + cc * (dw - T(3) * a.d0w[c]);
+    } else if (MODE == 3) {     // s += (dt/12.)*(8*ds2-ds0-ds1)
+        a.bn[c] = a.b[c] + cc * ((T(8) * db - a.d0b[c]) - a.d1b[c]);
+        a.un[c] = a.u[c] + cc * ((T(8) * du - a.d0u[c]) - a.d1u[c]);
+        a.vn[c] = a.v[c] + cc * ((T(8) * dv - a.d0v[c]) - a.d1v[c]);
+        a.wn[c] = a.w[c] + cc * ((T(8) * dw - a.d0w[c]) - a.d1w[c]);
+    }
+}
+
+template <typename T>
+hipError_t launch_stage(const Geo &g, const StageArgs<T> &a, int mode, hipStream_t s) {
+    dim3 blk(NYL_BX, NYL_BY, 1), grd = cell_grid(g);
+    switch (mode) {
+    case 0: hipLaunchKernelGGL((k_stage<T, 0>), grd, blk, 0, s, g, a); break;
+    case 1: hipLaunchKernelGGL((k_stage<T, 1>), grd, blk, 0, s, g, a); break;
+    case 2: hipLaunchKernelGGL((k_stage<T, 2>), grd, blk, 0, s, g, a); break;
+    case 3: hipLaunchKernelGGL((k_stage<T, 3>), grd, blk, 0, s, g, a); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+// U_from_u (core/cov_to_contra.py:15-20), vorticity (core/vorticity.py:25-34,
+// core/fortran_vorticity.f90:2-28) and kinenergy (core/kinenergy.py:13-24,
+// core/fortran_kinenergy.f90:145-153) from the covariant velocity.
+template <typename T>
+__global__ __launch_bounds__(256) void k_diag(Geo g, DiagArgs<T> a, int what) {
+    const int i = blockIdx.x * NYL_BX + threadIdx.x;
+    const int j = blockIdx.y * NYL_BY + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i >= g.nx || j >= g.ny) return;
+    const int64_t c = (int64_t)k * g.sz + (int64_t)j * g.sy + i;
+    const T uc = a.u[c], vc = a.v[c], wc = a.w[c];
+    if (what & 1) {
+        a.Ux[c] = uc * T(a.idx2);
+        a.Uy[c] = vc * T(a.idy2);
+        a.Uz[c] = wc * T(a.idz2);
+    }
+    const int64_t oxp = g.px ? (int64_t)(wrapi(i + 1, g.nx) - i) : 1;
+    const int64_t oyp = g.py ? (int64_t)(wrapi(j + 1, g.ny) - j) * g.sy : g.sy;
+    const int64_t ozp = g.pz ? (int64_t)(wrapi(k + 1, g.nz) - k) * g.sz : g.sz;
+    const bool ex = g.px || i < g.nx - 1, ey = g.py || j < g.ny - 1, ez = g.pz || k < g.nz - 1;
+    if (what & 2) {
+        // vor_k = delta_i u_j - delta_j u_i (flipview('k'): ui=u, uj=v)
+        T oz = T(0);
+        if (ex && ey) {
+            oz = a.v[c + oxp] - vc - a.u[c + oyp] + uc;
+            if (a.fpar > 0.0) oz = oz + T(a.fpar);
+        }
+        // vor_i (flipview('i') = [x][z][y]: ui=v, uj=w)
+        T ox = T(0);
+        if (ey && ez) ox = a.w[c + oyp] - wc - a.v[c + ozp] + vc;
+        // vor_j (flipview('j') = [y][x][z]: ui=w, uj=u)
+        T oy = T(0);
+        if (ez && ex) oy = a.u[c + ozp] - uc - a.w[c + oxp] + wc;
+        a.ox[c] = ox; a.oy[c] = oy; a.oz[c] = oz;
+    }
+    if (what & 4) {
+        // ke = 0; ke(i) += (cff2*0.5)*(u(i)**2+u(i-1)**2) for i >= 2, per direction
+        T ke = T(0);
+        if (g.px || i > 0) {
+            const T um = a.u[c - (g.px ? (int64_t)(i - wrapi(i - 1, g.nx)) : 1)];
+            const T cff2 = T(0.5 * a.idx2);
+            ke = ke + cff2 * T(0.5) * (uc * uc + um * um);
+        }
+        if (g.py || j > 0) {
+            const T vm = a.v[c - (g.py ? (int64_t)(j - wrapi(j - 1, g.ny)) * g.sy : g.sy)];
+            const T cff2 = T(0.5 * a.idy2);
+            ke = ke + cff2 * T(0.5) * (vc * vc + vm * vm);
+        }
+        if (g.pz || k > 0) {
+            const T wm = a.w[c - (g.pz ? (int64_t)(k - wrapi(k - 1, g.nz)) * g.sz : g.sz)];
+            const T cff2 = T(0.5 * a.idz2);
+            ke = ke + cff2 * T(0.5) * (wc * wc + wm * wm);
+        }
+        a.ke[c] = ke;
+    }
+}
+
+template <typename T>
+hipError_t launch_diag(const Geo &g, const DiagArgs<T> &a, int what, hipStream_t s) {
+    hipLaunchKernelGGL((k_diag<T>), cell_grid(g), dim3(NYL_BX, NYL_BY, 1), 0, s, g, a, what);
+    return hipGetLastError();
+}
+
+// compute_div (core/projection.py:15-35 over core/fortran_bernoulli.f90:61-97)
+// with U = u*idx2 evaluated in place (core/cov_to_contra.py:18-20).
+template <typename T, typename TO>
+__global__ __launch_bounds__(256) void k_div(Geo g, const T *__restrict__ u,
+                                             const T *__restrict__ v, const T *__restrict__ w,
+                                             double idx2, double idy2, double idz2,
+                                             TO *__restrict__ out) {
+    const int i = blockIdx.x * NYL_BX + threadIdx.x;
+    const int j = blockIdx.y * NYL_BY + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i >= g.nx || j >= g.ny) return;
+    const int64_t c = (int64_t)k * g.sz + (int64_t)j * g.sy + i;
+    const T cx = T(idx2), cy = T(idy2), cz = T(idz2);
+    T d;
+    const T Uc = u[c] * cx;
+    if (!g.px && i == 0) d = Uc;
+    else d = Uc - u[c - (g.px ? (int64_t)(i - wrapi(i - 1, g.nx)) : 1)] * cx;
+    const T Vc = v[c] * cy;
+    if (!g.py && j == 0) d = d + Vc;
+    else d = d + (Vc - v[c - (g.py ? (int64_t)(j - wrapi(j - 1, g.ny)) * g.sy : g.sy)] * cy);
+    const T Wc = w[c] * cz;
+    if (!g.pz && k == 0) d = d + Wc;
+    else d = d + (Wc - w[c - (g.pz ? (int64_t)(k - wrapi(k - 1, g.nz)) * g.sz : g.sz)] * cz);
+    out[c] = TO(d);
+}
+
+template <typename T, typename TO>
+hipError_t launch_div(const Geo &g, const T *u, const T *v, const T *w, double idx2,
+                      double idy2, double idz2, TO *out, hipStream_t s) {
+    hipLaunchKernelGGL((k_div<T, TO>), cell_grid(g), dim3(NYL_BX, NYL_BY, 1), 0, s, g, u, v, w,
+                       idx2, idy2, idz2, out);
+    return hipGetLastError();
+}
+
+// u -= delta[p] with p = x*dx**2 (core/projection.py:84-87 over
+// core/fortran_bernoulli.f90:2-26; p from core/mgfordriver.py:78)
+template <typename T, typename TX>
+__global__ __launch_bounds__(256) void k_correct(Geo g, T *__restrict__ u, T *__restrict__ v,
+                                                 T *__restrict__ w, const TX *__restrict__ x,
+                                                 double dx2) {
+    const int i = blockIdx.x * NYL_BX + threadIdx.x;
+    const int j = blockIdx.y * NYL_BY + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i >= g.nx || j >= g.ny) return;
+    const int64_t c = (int64_t)k * g.sz + (int64_t)j * g.sy + i;
+    const TX pc = x[c] * TX(dx2);
+    if (g.px || i < g.nx - 1) {
+        const int64_t o = g.px ? (int64_t)(wrapi(i + 1, g.nx) - i) : 1;
+        u[c] = T(TX(u[c]) - (x[c + o] * TX(dx2) - pc));
+    }
+    if (g.py || j < g.ny - 1) {
+        const int64_t o = g.py ? (int64_t)(wrapi(j + 1, g.ny) - j) * g.sy : g.sy;
+        v[c] = T(TX(v[c]) - (x[c + o] * TX(dx2) - pc));
+    }
+    if (g.pz || k < g.nz - 1) {
+        const int64_t o = g.pz ? (int64_t)(wrapi(k + 1, g.nz) - k) * g.sz : g.sz;
+        w[c] = T(TX(w[c]) - (x[c + o] * TX(dx2) - pc));
+    }
+}
+
+template <typename T, typename TX>
+hipError_t launch_correct(const Geo &g, T *u, T *v, T *w, const TX *x, double dx2,
+                          hipStream_t s) {
+    hipLaunchKernelGGL((k_correct<T, TX>), cell_grid(g), dim3(NYL_BX, NYL_BY, 1), 0, s, g, u, v,
+                       w, x, dx2);
+    return hipGetLastError();
+}
+
+template <typename TX, typename T>
+__global__ void k_pressure(int64_t n, const TX *__restrict__ x, double dx2, T *__restrict__ p) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) p[q] = T(x[q] * TX(dx2));
+}
+
+template <typename TX, typename T>
+hipError_t launch_pressure(const Geo &g, const TX *x, double dx2, T *p, hipStream_t s) {
+    const int64_t n = (int64_t)g.nx * g.ny * g.nz;
+    hipLaunchKernelGGL((k_pressure<TX, T>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
+                       n, x, dx2, p);
+    return hipGetLastError();
+}
+
+__global__ void k_fill(double *p, int64_t n, double v) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) p[q] = v;
+}
+
+hipError_t launch_fill(double *p, int64_t n, double v, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n, v);
+    return hipGetLastError();
+}
+
+template hipError_t launch_stage<double>(const Geo &, const StageArgs<double> &, int, hipStream_t);
+template hipError_t launch_diag<double>(const Geo &, const DiagArgs<double> &, int, hipStream_t);
+template hipError_t launch_div<double, double>(const Geo &, const double *, const double *,
+                                               const double *, double, double, double, double *,
+                                               hipStream_t);
+template hipError_t launch_correct<double, double>(const Geo &, double *, double *, double *,
+                                                   const double *, double, hipStream_t);
+template hipError_t launch_pressure<double, double>(const Geo &, const double *, double,
+                                                    double *, hipStream_t);
+
+}  // namespace nyl

@@ -1,0 +1,151 @@
+"""One Nyles model instance resident on the GPU.
+
+``DeviceModel`` owns a libnyles_hip context and the device workspace it
+computes in.  PyTorch is used only to allocate that workspace and to move
+arrays between host and device; every arithmetic operation on the step path
+runs in the library's HIP kernels.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib as L
+
+STATE_FIELDS = {"b": L.NYL_F_B, "u_i": L.NYL_F_U, "u_j": L.NYL_F_V, "u_k": L.NYL_F_W}
+DIAG_FIELDS = {"ke": L.NYL_F_KE, "vor_i": L.NYL_F_VORX, "vor_j": L.NYL_F_VORY,
+               "vor_k": L.NYL_F_VORZ, "U_i": L.NYL_F_UCX, "U_j": L.NYL_F_UCY,
+               "U_k": L.NYL_F_UCZ, "p": L.NYL_F_P, "div": L.NYL_F_DIV}
+_DT = {L.NYL_DT_F64: torch.float64, L.NYL_DT_F32: torch.float32}
+
+
+class DeviceModel:
+    """Device state and time step of one LES instance.
+
+    Parameters mirror what core/model_les.py:33-95 reads from ``param`` and
+    ``grid``; the spacing-derived constants are passed as the caller computed
+    them (core/grid.py:30-44, core/mgfordriver.py:78) so the device uses the
+    same doubles as the reference.
+    """
+
+    def __init__(self, nx, ny, nz, periodic, dx, dy, dz, idx2=None, idy2=None, idz2=None,
+                 dx2=None, fparameter=0.0, precision=L.NYL_PREC_FP64, mg_maxite=20,
+                 mg_tol=1e-6, mg_omega=0.9, device=None):
+        lib = L.load()
+        if not torch.cuda.is_available():
+            raise L.NylesHipError("no GPU visible: the Nyles time step runs only on the GPU")
+        self.device = torch.device(device if device is not None else "cuda:0")
+        self.shape = (nz, ny, nx)
+        d = L.NylDesc()
+        d.nx, d.ny, d.nz = nx, ny, nz
+        d.periodic_x, d.periodic_y, d.periodic_z = (int(bool(p)) for p in periodic)
+        d.dx, d.dy, d.dz = dx, dy, dz
+        d.idx2 = idx2 if idx2 is not None else 1 / dx ** 2
+        d.idy2 = idy2 if idy2 is not None else 1 / dy ** 2
+        d.idz2 = idz2 if idz2 is not None else 1 / dz ** 2
+        d.dx2 = dx2 if dx2 is not None else dx ** 2
+        d.fparameter = fparameter
+        d.precision = precision
+        d.mg_maxite = mg_maxite
+        d.mg_tol = mg_tol
+        d.mg_omega = mg_omega
+        self.desc = d
+        ctx = ctypes.c_void_p()
+        L.check(lib.nyl_ctx_create(ctypes.byref(d), ctypes.byref(ctx)))
+        self.ctx = ctx
+        nbytes = ctypes.c_uint64()
+        L.check(lib.nyl_workspace_bytes(ctx, ctypes.byref(nbytes)))
+        self.workspace_bytes = nbytes.value
+        with torch.cuda.device(self.device):
+            self.ws = torch.empty(self.workspace_bytes, dtype=torch.uint8, device=self.device)
+            self.stream = torch.cuda.current_stream(self.device)
+        L.check(lib.nyl_set_stream(ctx, ctypes.c_void_p(self.stream.cuda_stream)))
+        L.check(lib.nyl_bind_workspace(ctx, ctypes.c_void_p(self.ws.data_ptr()),
+                                       self.workspace_bytes))
+        self.lib = lib
+        self.last_stats = L.NylMGStats()
+
+    # -- fields -------------------------------------------------------------
+    def field(self, fid):
+        """torch view (nz, ny, nx) of a workspace field (state views move
+        between the two state buffers after every stage)."""
+        off, n, dt = ctypes.c_uint64(), ctypes.c_int64(), ctypes.c_int32()
+        L.check(self.lib.nyl_field_offset(self.ctx, fid, ctypes.byref(off), ctypes.byref(n),
+                                          ctypes.byref(dt)))
+        tdt = _DT[dt.value]
+        esz = torch.tensor([], dtype=tdt).element_size()
+        flat = self.ws[off.value: off.value + n.value * esz].view(tdt)
+        return flat.view(self.shape)
+
+    def upload(self, fid, array):
+        a = np.ascontiguousarray(array)
+        assert a.shape == self.shape, (a.shape, self.shape)
+        dst = self.field(fid)
+        with torch.cuda.stream(self.stream):
+            dst.copy_(torch.from_numpy(a).to(dst.dtype), non_blocking=False)
+
+    def download(self, fid):
+        src = self.field(fid)
+        with torch.cuda.stream(self.stream):
+            out = src.to("cpu")
+        self.stream.synchronize()
+        return out.numpy().astype(np.float64, copy=False)
+
+    # -- operations ---------------------------------------------------------
+    def rhs(self, slot, last=False):
+        L.check(self.lib.nyl_rhs(self.ctx, slot, int(bool(last))))
+
+    def rk3_stage(self, stage, coef):
+        L.check(self.lib.nyl_rk3_stage(self.ctx, stage, float(coef)))
+
+    def diagnose(self):
+        st = L.NylMGStats()
+        L.check(self.lib.nyl_diagnose(self.ctx, ctypes.byref(st)))
+        self.last_stats = st
+        return st.as_dict()
+
+    def step_rk3(self, dt):
+        st = L.NylMGStats()
+        L.check(self.lib.nyl_step_rk3(self.ctx, float(dt), ctypes.byref(st)))
+        self.last_stats = st
+        return st.as_dict()
+
+    def mg_solve(self):
+        st = L.NylMGStats()
+        L.check(self.lib.nyl_mg_solve(self.ctx, ctypes.byref(st)))
+        self.last_stats = st
+        return st.as_dict()
+
+    def apply(self, op):
+        L.check(self.lib.nyl_apply(self.ctx, op))
+
+    def reduce(self, what):
+        out = ctypes.c_double()
+        L.check(self.lib.nyl_reduce(self.ctx, what, ctypes.byref(out)))
+        return out.value
+
+    def sync(self):
+        L.check(self.lib.nyl_sync(self.ctx))
+
+    def mg_levels(self):
+        n = ctypes.c_int32()
+        L.check(self.lib.nyl_mg_levels(self.ctx, ctypes.byref(n)))
+        shapes = []
+        for l in range(n.value):
+            a, b, c = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+            L.check(self.lib.nyl_mg_level_shape(self.ctx, l, ctypes.byref(a), ctypes.byref(b),
+                                                ctypes.byref(c)))
+            shapes.append((c.value, b.value, a.value))
+        return shapes
+
+    def close(self):
+        if getattr(self, "ctx", None) is not None and self.ctx.value:
+            self.sync()
+            self.lib.nyl_ctx_destroy(self.ctx)
+            self.ctx = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
