@@ -166,6 +166,23 @@ int nyl_apply(nyl_ctx *ctx, int op);
 /* Reductions (synchronous). */
 int nyl_reduce(nyl_ctx *ctx, int what, double *out);
 
+/* Kernel-class timing with HIP events on the context stream: every launch
+ * of the class between start and stop is bracketed by two events; stop
+ * synchronises and returns the summed kernel time, the launch count and the
+ * summed algorithmic HBM bytes of those launches (each input field read
+ * once, each output written once).  No counterpart in the reference; the
+ * benchmark uses it to price the dominant kernel against the HBM roofline. */
+#define NYL_K_STAGE 1        /* fused rhs + RK update */
+#define NYL_K_MG_SWEEP 2     /* one damped-Jacobi sweep */
+#define NYL_K_MG_RESTRICT 3  /* residual + restriction */
+#define NYL_K_MG_PROLONG 4   /* prolongation */
+#define NYL_K_DIAG 5         /* U, vorticity, ke */
+#define NYL_K_DIV 6          /* divergence */
+#define NYL_K_CORRECT 7      /* u -= grad p */
+#define NYL_K_RESNORM 8      /* residual norm (two launches) */
+int nyl_prof_start(nyl_ctx *ctx, int kernel_class);
+int nyl_prof_stop(nyl_ctx *ctx, double *total_ms, int64_t *launches, double *alg_bytes);
+
 /* Wait for all work of the context's stream. */
 int nyl_sync(nyl_ctx *ctx);
 

@@ -42,8 +42,13 @@ EXPORTED = [
     "nyl_abi_version", "nyl_last_error", "nyl_ctx_create", "nyl_ctx_destroy",
     "nyl_workspace_bytes", "nyl_bind_workspace", "nyl_set_stream", "nyl_field_offset",
     "nyl_mg_levels", "nyl_mg_level_shape", "nyl_rhs", "nyl_rk3_stage", "nyl_diagnose",
-    "nyl_step_rk3", "nyl_mg_solve", "nyl_apply", "nyl_reduce", "nyl_sync",
+    "nyl_step_rk3", "nyl_mg_solve", "nyl_apply", "nyl_reduce", "nyl_prof_start",
+    "nyl_prof_stop", "nyl_sync",
 ]
+
+# kernel classes of nyl_prof_start
+KERNEL_CLASSES = {"stage": 1, "mg_sweep": 2, "mg_restrict": 3, "mg_prolong": 4, "diag": 5,
+                  "div": 6, "correct": 7, "resnorm": 8}
 
 
 class NylesHipError(RuntimeError):
@@ -107,6 +112,8 @@ def load():
         "nyl_mg_solve": (ci, [vp, P(NylMGStats)]),
         "nyl_apply": (ci, [vp, ci]),
         "nyl_reduce": (ci, [vp, ci, P(dbl)]),
+        "nyl_prof_start": (ci, [vp, ci]),
+        "nyl_prof_stop": (ci, [vp, P(dbl), P(i64), P(dbl)]),
         "nyl_sync": (ci, [vp]),
     }
     for name, (res, args) in sig.items():

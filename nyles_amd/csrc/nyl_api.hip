@@ -58,6 +58,13 @@ struct nyl_ctx {
     hipStream_t stream;
     double *host_out;        // pinned, 8 doubles
     int total_cycles;
+    // HIP-event timing of one kernel class (nyl_prof_start / nyl_prof_stop)
+    int prof_cls;
+    int prof_used;
+    hipEvent_t prof_ev[2 * 4096];
+    int prof_nev;            // events created
+    double prof_ms, prof_bytes;
+    int64_t prof_launches;
 };
 
 namespace {
@@ -137,6 +144,55 @@ int check_ready(const nyl_ctx *c) {
     return NYL_OK;
 }
 
+int prof_drain(nyl_ctx *c) {
+    if (c->prof_used == 0) return NYL_OK;
+    HIPCHK(hipEventSynchronize(c->prof_ev[2 * c->prof_used - 1]));
+    for (int q = 0; q < c->prof_used; q++) {
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, c->prof_ev[2 * q], c->prof_ev[2 * q + 1]));
+        c->prof_ms += ms;
+    }
+    c->prof_used = 0;
+    return NYL_OK;
+}
+
+// Launch through `fn`; when the kernel class is being profiled, bracket the
+// launch with events on the context stream and count its algorithmic bytes.
+template <typename Fn>
+int timed_launch(nyl_ctx *c, int cls, double bytes, Fn fn) {
+    const bool on = c->prof_cls == cls;
+    if (on) {
+        if (c->prof_used * 2 + 2 > c->prof_nev) {
+            const int cap = (int)(sizeof(c->prof_ev) / sizeof(c->prof_ev[0]));
+            if (c->prof_nev + 2 <= cap) {
+                HIPCHK(hipEventCreate(&c->prof_ev[c->prof_nev]));
+                HIPCHK(hipEventCreate(&c->prof_ev[c->prof_nev + 1]));
+                c->prof_nev += 2;
+            } else {
+                int r = prof_drain(c);
+                if (r) return r;
+            }
+        }
+        HIPCHK(hipEventRecord(c->prof_ev[2 * c->prof_used], c->stream));
+    }
+    HIPCHK(fn());
+    if (on) {
+        HIPCHK(hipEventRecord(c->prof_ev[2 * c->prof_used + 1], c->stream));
+        c->prof_used++;
+        c->prof_launches++;
+        c->prof_bytes += bytes;
+    }
+    return NYL_OK;
+}
+
+#define LAUNCH(cls, bytes, ...)                                                        \
+    do {                                                                               \
+        int r_ = timed_launch(c, (cls), (double)(bytes), [&]() { return (__VA_ARGS__); }); \
+        if (r_) return r_;                                                             \
+    } while (0)
+
+double cells(const Geo &g) { return (double)g.nx * g.ny * g.nz; }
+
 int read_result(nyl_ctx *c, double *out) {
     HIPCHK(hipMemcpyAsync(c->host_out, c->ws + c->result, sizeof(double), hipMemcpyDeviceToHost,
                           c->stream));
@@ -177,23 +233,31 @@ int vcycle(nyl_ctx *c) {
     for (int l = 0; l < L - 1; l++) {
         Level &f = c->lev[l], &q = c->lev[l + 1];
         double *x = F<double>(c, f.x), *b = F<double>(c, f.b), *y = F<double>(c, f.y);
-        HIPCHK(launch_mg_sweep<double>(f.g, x, y, b, om, c->tabs, c->stream));
-        HIPCHK(launch_mg_sweep<double>(f.g, y, x, b, om, c->tabs, c->stream));
-        HIPCHK(launch_mg_resid_restrict<double>(f.g, q.g, x, b, F<double>(c, q.b),
+        LAUNCH(NYL_K_MG_SWEEP, 3 * cells(f.g) * sizeof(double),
+               launch_mg_sweep<double>(f.g, x, y, b, om, c->tabs, c->stream));
+        LAUNCH(NYL_K_MG_SWEEP, 3 * cells(f.g) * sizeof(double),
+               launch_mg_sweep<double>(f.g, y, x, b, om, c->tabs, c->stream));
+        LAUNCH(NYL_K_MG_RESTRICT, 2.25 * cells(f.g) * sizeof(double),
+               launch_mg_resid_restrict<double>(f.g, q.g, x, b, F<double>(c, q.b),
                                                 F<double>(c, q.x), c->tabs, c->stream));
     }
     {
         Level &f = c->lev[L - 1];
         double *x = F<double>(c, f.x), *b = F<double>(c, f.b), *y = F<double>(c, f.y);
-        HIPCHK(launch_mg_sweep<double>(f.g, x, y, b, om, c->tabs, c->stream));
-        HIPCHK(launch_mg_sweep<double>(f.g, y, x, b, om, c->tabs, c->stream));
+        LAUNCH(NYL_K_MG_SWEEP, 3 * cells(f.g) * sizeof(double),
+               launch_mg_sweep<double>(f.g, x, y, b, om, c->tabs, c->stream));
+        LAUNCH(NYL_K_MG_SWEEP, 3 * cells(f.g) * sizeof(double),
+               launch_mg_sweep<double>(f.g, y, x, b, om, c->tabs, c->stream));
     }
     for (int l = L - 2; l >= 0; l--) {
         Level &f = c->lev[l], &q = c->lev[l + 1];
         double *x = F<double>(c, f.x), *b = F<double>(c, f.b), *y = F<double>(c, f.y);
-        HIPCHK(launch_mg_prolong<double>(f.g, q.g, x, F<double>(c, q.x), c->tabs, c->stream));
-        HIPCHK(launch_mg_sweep<double>(f.g, x, y, b, om, c->tabs, c->stream));
-        HIPCHK(launch_mg_sweep<double>(f.g, y, x, b, om, c->tabs, c->stream));
+        LAUNCH(NYL_K_MG_PROLONG, 2.125 * cells(f.g) * sizeof(double),
+               launch_mg_prolong<double>(f.g, q.g, x, F<double>(c, q.x), c->tabs, c->stream));
+        LAUNCH(NYL_K_MG_SWEEP, 3 * cells(f.g) * sizeof(double),
+               launch_mg_sweep<double>(f.g, x, y, b, om, c->tabs, c->stream));
+        LAUNCH(NYL_K_MG_SWEEP, 3 * cells(f.g) * sizeof(double),
+               launch_mg_sweep<double>(f.g, y, x, b, om, c->tabs, c->stream));
     }
     return NYL_OK;
 }
@@ -209,7 +273,8 @@ int mg_solve(nyl_ctx *c, nyl_mg_stats *st) {
     if ((rc = read_result(c, &normb))) return rc;
     auto normresidual = [&](double *out) -> int {
         if (normb > 0.0) {
-            HIPCHK(launch_resnorm2<double>(f.g, x, b, c->tabs, part, res_dev, c->stream));
+            LAUNCH(NYL_K_RESNORM, 2 * cells(f.g) * sizeof(double),
+                   launch_resnorm2<double>(f.g, x, b, c->tabs, part, res_dev, c->stream));
             int r2 = read_result(c, &rn);
             if (r2) return r2;
             *out = rn / normb;
@@ -259,12 +324,14 @@ int apply_op(nyl_ctx *c, int op) {
         HIPCHK(launch_diag<double>(c->g, diag_args(c), 1, c->stream));
         break;
     case NYL_OP_DIV:
-        HIPCHK(launch_div<double, double>(c->g, u, v, w, c->d.idx2, c->d.idy2, c->d.idz2,
+        LAUNCH(NYL_K_DIV, 4 * cells(c->g) * sizeof(double),
+               launch_div<double, double>(c->g, u, v, w, c->d.idx2, c->d.idy2, c->d.idz2,
                                           F<double>(c, c->lev[0].b), c->stream));
         break;
     case NYL_OP_CORRECT:
-        HIPCHK(launch_correct<double, double>(c->g, u, v, w, F<double>(c, c->lev[0].x), c->d.dx2,
-                                              c->stream));
+        LAUNCH(NYL_K_CORRECT, 7 * cells(c->g) * sizeof(double),
+               launch_correct<double, double>(c->g, u, v, w, F<double>(c, c->lev[0].x),
+                                              c->d.dx2, c->stream));
         break;
     case NYL_OP_VORTICITY:
         HIPCHK(launch_diag<double>(c->g, diag_args(c), 2, c->stream));
@@ -288,7 +355,7 @@ int diagnose(nyl_ctx *c, nyl_mg_stats *st) {
     if ((rc = apply_op(c, NYL_OP_DIV))) return rc;
     if ((rc = mg_solve(c, st))) return rc;
     if ((rc = apply_op(c, NYL_OP_CORRECT))) return rc;
-    HIPCHK(launch_diag<double>(c->g, diag_args(c), 7, c->stream));
+    LAUNCH(NYL_K_DIAG, 10 * cells(c->g) * sizeof(double), launch_diag<double>(c->g, diag_args(c), 7, c->stream));
     return NYL_OK;
 }
 
@@ -296,7 +363,8 @@ int stage(nyl_ctx *c, int st, double coef) {
     StageArgs<double> a = stage_args(c);
     if (st == 1 || st == 2) set_tendency_out(c, a, st - 1);
     a.c = coef;
-    HIPCHK(launch_stage<double>(c->g, a, st, c->stream));
+    const double words = st == 1 ? 19 : 23;
+    LAUNCH(NYL_K_STAGE, words * cells(c->g) * sizeof(double), launch_stage<double>(c->g, a, st, c->stream));
     c->cur = 1 - c->cur;
     return NYL_OK;
 }
@@ -341,6 +409,7 @@ int nyl_ctx_create(const nyl_desc *desc, nyl_ctx **out) {
 
 int nyl_ctx_destroy(nyl_ctx *c) {
     if (!c) return NYL_OK;
+    for (int q = 0; q < c->prof_nev; q++) (void)hipEventDestroy(c->prof_ev[q]);
     if (c->host_out) (void)hipHostFree(c->host_out);
     delete c;
     return NYL_OK;
@@ -420,7 +489,7 @@ int nyl_rhs(nyl_ctx *c, int slot, int last) {
     if (slot < 0 || slot > 2) return fail(NYL_E_ARG, "slot must be 0..2");
     StageArgs<double> a = stage_args(c);
     set_tendency_out(c, a, slot);
-    HIPCHK(launch_stage<double>(c->g, a, 0, c->stream));
+    LAUNCH(NYL_K_STAGE, 12 * cells(c->g) * sizeof(double), launch_stage<double>(c->g, a, 0, c->stream));
     return NYL_OK;
 }
 
@@ -494,6 +563,28 @@ int nyl_reduce(nyl_ctx *c, int what, double *out) {
         return fail(NYL_E_ARG, "unknown reduction %d", what);
     }
     return read_result(c, out);
+}
+
+int nyl_prof_start(nyl_ctx *c, int kernel_class) {
+    int rc = check_ready(c);
+    if (rc) return rc;
+    if ((rc = prof_drain(c))) return rc;
+    c->prof_cls = kernel_class;
+    c->prof_ms = 0.0;
+    c->prof_bytes = 0.0;
+    c->prof_launches = 0;
+    return NYL_OK;
+}
+
+int nyl_prof_stop(nyl_ctx *c, double *total_ms, int64_t *launches, double *alg_bytes) {
+    int rc = check_ready(c);
+    if (rc) return rc;
+    if ((rc = prof_drain(c))) return rc;
+    if (total_ms) *total_ms = c->prof_ms;
+    if (launches) *launches = c->prof_launches;
+    if (alg_bytes) *alg_bytes = c->prof_bytes;
+    c->prof_cls = 0;
+    return NYL_OK;
 }
 
 int nyl_sync(nyl_ctx *c) {

@@ -124,6 +124,16 @@ class DeviceModel:
         L.check(self.lib.nyl_reduce(self.ctx, what, ctypes.byref(out)))
         return out.value
 
+    def prof_start(self, kernel_class):
+        L.check(self.lib.nyl_prof_start(self.ctx, L.KERNEL_CLASSES[kernel_class]))
+
+    def prof_stop(self):
+        """(total kernel ms, launches, algorithmic bytes) since prof_start."""
+        ms, n, by = ctypes.c_double(), ctypes.c_int64(), ctypes.c_double()
+        L.check(self.lib.nyl_prof_stop(self.ctx, ctypes.byref(ms), ctypes.byref(n),
+                                       ctypes.byref(by)))
+        return ms.value, n.value, by.value
+
     def sync(self):
         L.check(self.lib.nyl_sync(self.ctx))
 
