@@ -144,6 +144,18 @@ int nyl_diagnose(nyl_ctx *ctx, nyl_mg_stats *stats);
  * diagnose_var calls; stats accumulates the three solves (nite = sum). */
 int nyl_step_rk3(nyl_ctx *ctx, double dt, nyl_mg_stats *stats);
 
+/* Timescheme.EulerForward (core/timescheme.py:336-345): rhs, s += dt*ds,
+ * diagnose_var. */
+int nyl_step_ef(nyl_ctx *ctx, double dt, nyl_mg_stats *stats);
+
+/* Timescheme.LFAM3 (core/timescheme.py:348-400): the first call is an Euler
+ * step that also stores the 'before' state; later calls are the
+ * leap-frog / Adams-Moulton predictor, diagnose_var, the corrector at n+1/2
+ * and diagnose_var (stats.nite sums both solves).  nyl_lfam3_reset makes the
+ * next call a first step again. */
+int nyl_step_lfam3(nyl_ctx *ctx, double dt, nyl_mg_stats *stats);
+int nyl_lfam3_reset(nyl_ctx *ctx);
+
 /* mgfor solve (core/mgfor/solvers.f90:8-33) on the current MG right-hand
  * side NYL_F_MGB, warm-started from NYL_F_MGX. */
 int nyl_mg_solve(nyl_ctx *ctx, nyl_mg_stats *stats);

@@ -42,7 +42,7 @@ EXPORTED = [
     "nyl_abi_version", "nyl_last_error", "nyl_ctx_create", "nyl_ctx_destroy",
     "nyl_workspace_bytes", "nyl_bind_workspace", "nyl_set_stream", "nyl_field_offset",
     "nyl_mg_levels", "nyl_mg_level_shape", "nyl_rhs", "nyl_rk3_stage", "nyl_diagnose",
-    "nyl_step_rk3", "nyl_mg_solve", "nyl_apply", "nyl_reduce", "nyl_prof_start",
+    "nyl_step_rk3", "nyl_step_ef", "nyl_step_lfam3", "nyl_lfam3_reset", "nyl_mg_solve", "nyl_apply", "nyl_reduce", "nyl_prof_start",
     "nyl_prof_stop", "nyl_sync",
 ]
 
@@ -109,6 +109,9 @@ def load():
         "nyl_rk3_stage": (ci, [vp, ci, dbl]),
         "nyl_diagnose": (ci, [vp, P(NylMGStats)]),
         "nyl_step_rk3": (ci, [vp, dbl, P(NylMGStats)]),
+        "nyl_step_ef": (ci, [vp, dbl, P(NylMGStats)]),
+        "nyl_step_lfam3": (ci, [vp, dbl, P(NylMGStats)]),
+        "nyl_lfam3_reset": (ci, [vp]),
         "nyl_mg_solve": (ci, [vp, P(NylMGStats)]),
         "nyl_apply": (ci, [vp, ci]),
         "nyl_reduce": (ci, [vp, ci, P(dbl)]),

@@ -1,0 +1,168 @@
+"""LES model (interface of core/model_les.py) running on one GPU.
+
+``LES(param, grid)`` builds the host State (views for scripts), a device
+context holding the same fields, and the time scheme bound to it.
+``rhs``, ``diagnose_var`` and ``forward`` run in the HIP library; the
+host mirrors are synchronised lazily (see variables.py).
+
+Differences from the reference that a script can see:
+* geometries perio_x and perio_y are accepted (the reference asserts,
+  core/model_les.py:80-88, although its multigrid supports them);
+* viscosity/diffusion (diff_coef), forcing, passive tracers and the linear
+  model are not available on the device path and raise NotImplementedError;
+* ``forward`` returns True when b or u holds a non-finite value (the
+  reference's blow-up flag is never set, core/mgfordriver.py:32).
+"""
+import os as _os
+import sys as _sys
+_ROOT = _os.path.dirname(_os.path.dirname(_os.path.dirname(_os.path.abspath(__file__))))
+if _ROOT not in _sys.path:
+    _sys.path.insert(0, _ROOT)
+import numpy as np
+
+from nyles_amd import _lib as L
+from nyles_amd.device import DeviceModel
+
+from nyles_amd.core import halo
+from nyles_amd.core import timescheme as ts
+from nyles_amd.core import variables as var
+from nyles_amd.core.timing import timing
+
+GEOMETRY_PERIODIC_XYZ = {
+    "closed": (0, 0, 0), "perio_x": (1, 0, 0), "perio_y": (0, 1, 0),
+    "perio_xy": (1, 1, 0), "perio_xyz": (1, 1, 1),
+}
+
+
+class MGInfo(object):
+    """What the reference's MG driver object exposes (core/mgfordriver.py:32)."""
+
+    def __init__(self):
+        self.stats = {"normb": 0, "res": [0], "blowup": False, "nite": 0}
+
+
+class LES(object):
+    def __init__(self, param, grid, linear=False):
+        if linear:
+            raise NotImplementedError("the linear model is not available on the GPU path")
+        if param["n_tracers"] > 0:
+            raise NotImplementedError("passive tracers are not available on the GPU path")
+        if param["forced"]:
+            raise NotImplementedError("user forcing is not available on the GPU path")
+        if len(param["diff_coef"]) > 0:
+            raise NotImplementedError("viscosity / diffusion is not available on the GPU path")
+        self.nonlinear = True
+        self.grid = grid
+        self.param = param
+        self.traclist = ["b"]
+        self.state = var.get_state(param)
+        self.halo = halo.set_halo(param, self.state)
+        self.neighbours = param["neighbours"]
+        self.timescheme = ts.Timescheme(param, self.state)
+        self.timescheme.set(self.rhs, self.diagnose_var)
+        self.orderA = param["orderA"]
+        self.orderVF = param["orderVF"]
+        self.orderKE = param["orderKE"]
+        self.rotating = param["rotating"]
+        self.forced = param["forced"]
+        self.diff_coef = param["diff_coef"]
+        self.add_viscosity = False
+        if self.rotating:
+            self.fparameter = param["coriolis"] * (grid.dx * grid.dy)
+        else:
+            self.fparameter = 0.0
+        precision = {"fp64": L.NYL_PREC_FP64, "mixed": L.NYL_PREC_MIXED}[param["precision"]]
+        self.blowup_check = param["blowup_check"]
+        self.device = DeviceModel(
+            param["nx"], param["ny"], param["nz"], GEOMETRY_PERIODIC_XYZ[param["geometry"]],
+            grid.dx, grid.dy, grid.dz, idx2=grid.idx2, idy2=grid.idy2, idz2=grid.idz2,
+            dx2=grid.dx ** 2, fparameter=self.fparameter, precision=precision,
+            mg_maxite=param["mg_maxite"], mg_tol=param["mg_tol"], mg_omega=param["mg_omega"])
+        self.timescheme.bind(self.device)
+        self.mg = MGInfo()
+        st = self.state
+        pairs = [(st.b, L.NYL_F_B), (st.u["i"], L.NYL_F_U), (st.u["j"], L.NYL_F_V),
+                 (st.u["k"], L.NYL_F_W), (st.ke, L.NYL_F_KE), (st.vor["i"], L.NYL_F_VORX),
+                 (st.vor["j"], L.NYL_F_VORY), (st.vor["k"], L.NYL_F_VORZ),
+                 (st.U["i"], L.NYL_F_UCX), (st.U["j"], L.NYL_F_UCY), (st.U["k"], L.NYL_F_UCZ),
+                 (st.div, L.NYL_F_DIV)]
+        for scalar, fid in pairs:
+            scalar.attach(self.device, fid)
+        st.p.attach(self.device, L.NYL_F_P,
+                    before_pull=lambda: self.device.apply(L.NYL_OP_PRESSURE))
+        if self.timescheme.timestepping == "RK3_SSP":
+            for slot, ds in enumerate((self.timescheme.ds0, self.timescheme.ds1,
+                                       self.timescheme.ds2)):
+                base = L.NYL_F_DS0_B + 4 * slot
+                for k, name in enumerate(("b", "u_i", "u_j", "u_k")):
+                    ds.get(name).attach(self.device, base + k)
+        self.stats = []
+
+    # -- host <-> device ------------------------------------------------------
+    def _attached(self):
+        out = self.state.scalars()
+        if self.timescheme.timestepping == "RK3_SSP":
+            for ds in (self.timescheme.ds0, self.timescheme.ds1, self.timescheme.ds2):
+                out += ds.scalars()
+        return out
+
+    def push(self):
+        for s in self._attached():
+            s.push()
+
+    def mark_device_newer(self):
+        for s in self._attached():
+            s.mark_device_newer()
+
+    def _update_stats(self, st):
+        self.mg.stats["nite"] = st["nite"]
+        self.mg.stats["res"] = [st["res"]]
+        self.mg.stats["normb"] = st["normb"]
+
+    # -- reference interface --------------------------------------------------
+    @timing
+    def diagnose_var(self, state):
+        self.push()
+        st = self.device.diagnose()
+        self.mark_device_newer()
+        self._update_stats(st)
+
+    @timing
+    def rhs(self, state, t, dstate, last=False):
+        """Tendencies of the current state; into the RK3 slot that ``dstate``
+        is, slot 0 otherwise."""
+        slot = 0
+        for k, name in enumerate(("ds0", "ds1", "ds2")):
+            if getattr(self.timescheme, name, None) is dstate:
+                slot = k
+        self.push()
+        self.device.rhs(slot, last)
+        self.mark_device_newer()
+
+    @timing
+    def forward(self, t, dt):
+        self.push()
+        self.timescheme.forward(self.state, t, dt)
+        self.mark_device_newer()
+        self._update_stats(self.timescheme.stats)
+        if self.blowup_check and self.device.reduce(L.NYL_RED_NONFINITE) > 0:
+            self.mg.stats["blowup"] = True
+        return self.mg.stats["blowup"]
+
+    def max_U2(self):
+        """max over cells of U**2+V**2+W**2 (core/nyles.py:249), on the device."""
+        self.push()
+        return self.device.reduce(L.NYL_RED_MAXU2)
+
+    def update_stats(self):
+        self.stats.append(dict(self.mg.stats))
+
+    def write_stats(self, path):
+        import pickle
+        with open("%s/stats.pkl" % path, "wb") as fid:
+            pickle.dump(self.stats, fid)
+
+
+def reset_state(state):
+    for s in state.scalars():
+        s.view()[...] = 0.0

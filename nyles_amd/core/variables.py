@@ -1,0 +1,275 @@
+"""Model variables: Scalar, Vector, State -- host views of device fields.
+
+Interface of core/variables.py of the reference (Scalar.view / flipview /
+viewlike / duplicate, Vector, State.get / get_prognostic_scalars ...), so
+experiment scripts that write initial conditions into
+``state.b.view('i')`` keep working.
+
+A Scalar attached to a device field (``attach``) is a lazily synchronised
+mirror of it:
+
+* ``view()`` first downloads the field when the device holds newer data,
+  then fills the host halo with the periodic images (the values the
+  reference's halo exchange leaves there), and marks the mirror as possibly
+  modified by the caller;
+* before the next device operation the model uploads every mirror that
+  may have been modified (``push``).
+
+The three orientation arrays and the transposing ``view`` of the reference
+(core/variables.py:146-182) are kept for the host side; the device stores
+one interior-only [z][y][x] array per field.
+"""
+import os as _os
+import sys as _sys
+_ROOT = _os.path.dirname(_os.path.dirname(_os.path.dirname(_os.path.abspath(__file__))))
+if _ROOT not in _sys.path:
+    _sys.path.insert(0, _ROOT)
+from collections import namedtuple
+
+import numpy as np
+
+from nyles_amd.core import topology as topo
+
+ModelVariable = namedtuple("ModelVariable", ["type", "name", "dimension", "prognostic"])
+
+modelvar = {
+    "b": ModelVariable("scalar", "buoyancy", "L.T^-2", prognostic=True),
+    "p": ModelVariable("scalar", "pressure", "L^2.T^-2", prognostic=False),
+    "ke": ModelVariable("scalar", "kinetic energy", "L^2.T^-2", prognostic=False),
+    "div": ModelVariable("scalar", "divergence", "T^-1", prognostic=False),
+    "u": ModelVariable("velocity", "covariant velocity", "L^2.T^-1", prognostic=True),
+    "U": ModelVariable("velocity", "contravariant velocity", "T^-1", prognostic=False),
+    "vor": ModelVariable("vorticity", "vorticity", "L^2.T^-1", prognostic=False),
+    "work": ModelVariable("scalar", "work", "L^2.T^-2", prognostic=False),
+}
+
+
+def rightswap(src, dest):
+    dest[:, :, :] = np.transpose(src, (2, 0, 1))
+
+
+def leftswap(src, dest):
+    dest[:, :, :] = np.transpose(src, (1, 2, 0))
+
+
+def to_i(arr, orientation):
+    """View of an array of the given orientation in the 'i' convention."""
+    if orientation == "i":
+        return arr
+    if orientation == "j":        # [i, k, j] -> [k, j, i]
+        return np.transpose(arr, (1, 2, 0))
+    return np.transpose(arr, (2, 0, 1))   # [j, i, k] -> [k, j, i]
+
+
+def fill_halo(x, neighbours, nh, domainindices):
+    """Periodic images into the halo of an 'i' array (one-rank semantics of
+    core/mpi/halo.py:140-178)."""
+    k0, k1, j0, j1, i0, i1 = domainindices
+    inner = ((k0, k1), (j0, j1), (i0, i1))
+    size = x.shape
+    for d in neighbours:
+        dst, src = [], []
+        for a in range(3):
+            if d[a] == -1:
+                dst.append(slice(0, nh))
+                src.append(slice(size[a] - 2 * nh, size[a] - nh))
+            elif d[a] == 1:
+                dst.append(slice(size[a] - nh, size[a]))
+                src.append(slice(nh, 2 * nh))
+            else:
+                dst.append(slice(*inner[a]))
+                src.append(slice(*inner[a]))
+        x[tuple(dst)] = x[tuple(src)]
+
+
+class Scalar(object):
+    """A scalar field, stored three times (one array per orientation) on the
+    host, optionally mirrored from a device field."""
+
+    def __init__(self, param, name, nickname, dimension, prognostic: bool = False):
+        required = ["nx", "ny", "nz", "nh", "neighbours"]
+        self.param = {k: param[k] for k in required}
+        self.name = name
+        self.nickname = nickname
+        self.dimension = dimension
+        self.prognostic = prognostic
+        nx, ny, nz, nh = param["nx"], param["ny"], param["nz"], param["nh"]
+        shape = [nz, ny, nx]
+        self.shape = shape
+        size, domainindices = topo.get_variable_shape(shape, param["neighbours"], nh)
+        nzl, nyl, nxl = size
+        self.size = {"i": nxl, "j": nyl, "k": nzl}
+        self.domainindices = domainindices
+        k0, k1, j0, j1, i0, i1 = domainindices
+        sk, ek = max(0, k0 - 1), min(nzl, k1 + 1)
+        sj, ej = max(0, j0 - 1), min(nyl, j1 + 1)
+        si, ei = max(0, i0 - 1), min(nxl, i1 + 1)
+        self.mg_idx = (slice(sk, ek), slice(sj, ej), slice(si, ei))
+        self.mg_idx2 = np.array([sk, ek, sj, ej, si, ei], dtype=int)
+        self.data = {
+            "i": np.zeros((nzl, nyl, nxl)),
+            "j": np.zeros((nxl, nzl, nyl)),
+            "k": np.zeros((nyl, nxl, nzl)),
+        }
+        self.activeview = "i"
+        self._dev = None
+        self._fid = None
+        self._stale = False     # device holds newer data than the host
+        self._dirty = False     # host views were handed out since the last upload
+
+    # -- device mirror -------------------------------------------------------
+    def attach(self, device_model, field_id, before_pull=None):
+        self._dev = device_model
+        self._fid = field_id
+        self._before_pull = before_pull
+
+    def interior(self, arr=None):
+        k0, k1, j0, j1, i0, i1 = self.domainindices
+        a = self.data["i"] if arr is None else arr
+        return a[k0:k1, j0:j1, i0:i1]
+
+    def mark_device_newer(self):
+        if self._dev is not None:
+            self._stale = True
+
+    def pull(self):
+        if self._dev is None or not self._stale:
+            return
+        if self._before_pull is not None:
+            self._before_pull()
+        vals = self._dev.download(self._fid)
+        a = self.data["i"]
+        self.interior(a)[...] = vals
+        fill_halo(a, self.param["neighbours"], self.param["nh"], self.domainindices)
+        self.activeview = "i"
+        self._stale = False
+
+    def push(self):
+        if self._dev is None or not self._dirty:
+            return
+        if self._stale:          # a view was never taken after the last step
+            self._dirty = False
+            return
+        a = to_i(self.data[self.activeview], self.activeview)
+        self._dev.upload(self._fid, np.ascontiguousarray(self.interior(a)))
+        self._dirty = False
+
+    # -- reference interface -------------------------------------------------
+    def duplicate(self):
+        return Scalar(self.param, self.name, self.nickname, self.dimension, self.prognostic)
+
+    def view(self, idx=None):
+        """Array of the field with direction idx innermost (i: [k,j,i],
+        j: [i,k,j], k: [j,i,k]); None keeps the current orientation."""
+        self.pull()
+        if idx == self.activeview or idx is None:
+            field = self.data[self.activeview]
+        else:
+            current = self.data[self.activeview]
+            field = self.data[idx]
+            if self.activeview + idx in ["ij", "jk", "ki"]:
+                rightswap(current, field)
+            else:
+                leftswap(current, field)
+            self.activeview = idx
+        if self._dev is not None:
+            self._dirty = True
+        return field
+
+    def flipview(self, idx):
+        if idx == "i":
+            return self.view("j")
+        if idx == "j":
+            return self.view("k")
+        if idx == "k":
+            return self.view("i")
+        raise ValueError('argument idx of Scalar.flipview must be in ["i","j","k"], not '
+                         + repr(idx))
+
+    def viewlike(self, scalar):
+        return self.view(scalar.activeview)
+
+    @staticmethod
+    def get_nature():
+        return "scalar"
+
+
+class Vector(dict):
+    """Three Scalar components 'i', 'j', 'k' of a velocity or vorticity."""
+
+    def __init__(self, param, name, nickname, dimension, prognostic: bool = False,
+                 is_velocity: bool = True):
+        dirname = {"i": "x", "j": "y", "k": "z"}
+        for d in "ijk":
+            self[d] = Scalar(param, name + " %s-component" % dirname[d], nickname + "_" + d,
+                             dimension, prognostic)
+        self.param = self["i"].param
+        self.name = name
+        self.nickname = nickname
+        self.dimension = dimension
+        self.prognostic = prognostic
+        self.is_velocity = is_velocity
+
+    def duplicate(self):
+        return Vector(self.param, self.name, self.nickname, self.dimension, self.prognostic,
+                      self.is_velocity)
+
+    def get_nature(self):
+        return "velocity" if self.is_velocity else "vorticity"
+
+
+class State(object):
+    """Table of contents plus one attribute per model variable."""
+
+    def __init__(self, listvar):
+        self.toc = {}
+        for var in listvar:
+            self.toc[var.nickname] = var.get_nature()
+            setattr(self, var.nickname, var)
+
+    def __str__(self):
+        return "\n".join(["{:10}: {!r}".format(v, getattr(self, v)) for v in self.toc])
+
+    def duplicate_prognostic_variables(self):
+        return State([getattr(self, v).duplicate() for v in self.toc
+                      if getattr(self, v).prognostic])
+
+    def get(self, variable):
+        if len(variable) > 2 and variable[-2] == "_" and variable[-1] in "ijk":
+            return getattr(self, variable[:-2])[variable[-1]]
+        return getattr(self, variable)
+
+    def get_prognostic_variables(self):
+        return [v for v in self.toc if getattr(self, v).prognostic]
+
+    def get_prognostic_scalars(self):
+        out = []
+        for v in self.get_prognostic_variables():
+            if self.toc[v] == "scalar":
+                out.append(v)
+            else:
+                out += ["{}_{}".format(v, d) for d in "ijk"]
+        return out
+
+    def scalars(self):
+        """Every Scalar of the state (vector components included)."""
+        out = []
+        for v in self.toc:
+            obj = getattr(self, v)
+            out += [obj] if self.toc[v] == "scalar" else [obj[d] for d in "ijk"]
+        return out
+
+
+def get_state(param):
+    listvar = []
+    for nickname, var in modelvar.items():
+        if var.type == "scalar":
+            listvar.append(Scalar(param, var.name, nickname, var.dimension, var.prognostic))
+        else:
+            listvar.append(Vector(param, var.name, nickname, var.dimension, var.prognostic,
+                                  is_velocity=(var.type == "velocity")))
+    return State(listvar)
+
+
+def get_work(param):
+    return Scalar(param, "work", "w", "any", True)

@@ -50,6 +50,8 @@ struct nyl_ctx {
     // byte offsets of the fields
     uint64_t state[2][4];
     uint64_t ds[3][4];
+    uint64_t sb[4];          // LFAM3 'before' state
+    int lf_first;            // next LFAM3 step is the first one
     uint64_t ke, vor[3], uc[3], p;
     uint64_t partials, result;
     uint64_t ws_bytes;
@@ -123,6 +125,7 @@ void build_layout(nyl_ctx *c) {
         for (int f = 0; f < 4; f++) c->state[s][f] = take(fb);
     for (int s = 0; s < 3; s++)
         for (int f = 0; f < 4; f++) c->ds[s][f] = take(fb);
+    for (int f = 0; f < 4; f++) c->sb[f] = take(fb);
     c->ke = take(fb);
     for (int f = 0; f < 3; f++) c->vor[f] = take(fb);
     for (int f = 0; f < 3; f++) c->uc[f] = take(fb);
@@ -216,8 +219,11 @@ StageArgs<double> stage_args(nyl_ctx *c) {
     a.d1b = F<double>(c, c->ds[1][0]); a.d1u = F<double>(c, c->ds[1][1]);
     a.d1v = F<double>(c, c->ds[1][2]); a.d1w = F<double>(c, c->ds[1][3]);
     a.tb = a.tu = a.tv = a.tw = nullptr;
+    a.sbb = F<double>(c, c->sb[0]); a.sbu = F<double>(c, c->sb[1]);
+    a.sbv = F<double>(c, c->sb[2]); a.sbw = F<double>(c, c->sb[3]);
     a.dz = c->d.dz;
     a.c = 0.0;
+    a.c2 = 0.0;
     return a;
 }
 
@@ -359,13 +365,29 @@ int diagnose(nyl_ctx *c, nyl_mg_stats *st) {
     return NYL_OK;
 }
 
-int stage(nyl_ctx *c, int st, double coef) {
+// words of HBM traffic per cell of each stage-kernel mode: reads b,u,v,w,
+// U(3), vor(3), ke plus the mode's extra inputs; writes the mode's outputs
+double stage_words(int mode) {
+    switch (mode) {
+    case 0: return 8 + 4;            // tendencies only
+    case 1: return 11 + 8;           // + next state + ds0
+    case 2: return 15 + 8;           // + ds0 read
+    case 3: return 19 + 4;           // + ds0, ds1 read
+    case 4: return 15 + 8;           // + sb read; next state + sb written
+    case 5: return 11 + 4 + 4;       // next state read and written
+    case 6: return 11 + 8;           // next state + sb written
+    default: return 0;
+    }
+}
+
+int stage(nyl_ctx *c, int st, double coef, double coef2 = 0.0) {
     StageArgs<double> a = stage_args(c);
     if (st == 1 || st == 2) set_tendency_out(c, a, st - 1);
     a.c = coef;
-    const double words = st == 1 ? 19 : 23;
-    LAUNCH(NYL_K_STAGE, words * cells(c->g) * sizeof(double), launch_stage<double>(c->g, a, st, c->stream));
-    c->cur = 1 - c->cur;
+    a.c2 = coef2;
+    LAUNCH(NYL_K_STAGE, stage_words(st) * cells(c->g) * sizeof(double),
+           launch_stage<double>(c->g, a, st, c->stream));
+    if (st != 5) c->cur = 1 - c->cur;
     return NYL_OK;
 }
 
@@ -439,6 +461,7 @@ int nyl_bind_workspace(nyl_ctx *c, void *dptr, uint64_t bytes) {
     HIPCHK(hipStreamSynchronize(c->stream));
     c->cur = 0;
     c->total_cycles = 0;
+    c->lf_first = 1;
     return NYL_OK;
 }
 
@@ -521,6 +544,42 @@ int nyl_step_rk3(nyl_ctx *c, double dt, nyl_mg_stats *st) {
     }
     acc.total_cycles = c->total_cycles;
     if (st) *st = acc;
+    return NYL_OK;
+}
+
+int nyl_step_ef(nyl_ctx *c, double dt, nyl_mg_stats *st) {
+    int rc = check_ready(c);
+    if (rc) return rc;
+    if ((rc = stage(c, 1, dt))) return rc;
+    return diagnose(c, st);
+}
+
+int nyl_step_lfam3(nyl_ctx *c, double dt, nyl_mg_stats *st) {
+    int rc = check_ready(c);
+    if (rc) return rc;
+    if (c->lf_first) {
+        if ((rc = stage(c, 6, dt))) return rc;
+        c->lf_first = 0;
+        return diagnose(c, st);
+    }
+    nyl_mg_stats s1, s2;
+    // predictor: the current buffer keeps sn, the next one receives s
+    if ((rc = stage(c, 4, 0.0, 2. * dt))) return rc;
+    if ((rc = diagnose(c, &s1))) return rc;
+    // corrector: rhs at n+1/2, then s = sn + dt*ds written over sn
+    if ((rc = stage(c, 5, dt))) return rc;
+    c->cur = 1 - c->cur;
+    if ((rc = diagnose(c, &s2))) return rc;
+    if (st) {
+        *st = s2;
+        st->nite = s1.nite + s2.nite;
+    }
+    return NYL_OK;
+}
+
+int nyl_lfam3_reset(nyl_ctx *c) {
+    if (!c) return fail(NYL_E_ARG, "null context");
+    c->lf_first = 1;
     return NYL_OK;
 }
 

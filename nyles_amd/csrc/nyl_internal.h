@@ -18,11 +18,15 @@ struct StageArgs {
     const T *d0b, *d0u, *d0v, *d0w;  // tendency slot 0 (read in stages 2, 3)
     const T *d1b, *d1u, *d1v, *d1w;  // tendency slot 1 (read in stage 3)
     T *tb, *tu, *tv, *tw;            // tendency output (modes 0, 1, 2)
+    T *sbb, *sbu, *sbv, *sbw;        // LFAM3 'before' state (modes 4, 6)
     double dz;                       // for cff = 0.5*dz (core/fortran_bernoulli.f90:47)
     double c;                        // RK coefficient dt, dt/4., dt/12.
+    double c2;                       // LFAM3: 2.*dt (mode 4)
 };
 
-// mode 0: tendencies only; 1..3: RK3_SSP stage update
+// mode 0: tendencies only; 1..3: RK3_SSP stage update;
+// 4: LFAM3 predictor; 5: LFAM3 corrector (next = next + c*ds);
+// 6: Euler step that also copies the state into the LFAM3 'before' state
 template <typename T>
 hipError_t launch_stage(const Geo &g, const StageArgs<T> &a, int mode, hipStream_t s);
 

@@ -115,6 +115,25 @@ __global__ __launch_bounds__(256) void k_stage(Geo g, StageArgs<T> a) {
         a.un[c] = a.u[c] + cc * ((T(8) * du - a.d0u[c]) - a.d1u[c]);
         a.vn[c] = a.v[c] + cc * ((T(8) * dv - a.d0v[c]) - a.d1v[c]);
         a.wn[c] = a.w[c] + cc * ((T(8) * dw - a.d0w[c]) - a.d1w[c]);
+    } else if (MODE == 4) {     // LFAM3 predictor (core/timescheme.py:367-385)
+        // s = sb + (2.*dt)*ds; s = (1./12.)*(5.*s+8.*sn-sb); sb = sn
+        const T c2 = T(a.c2), c12 = T(1. / 12.);
+        T sb, t1;
+        sb = a.sbb[c]; t1 = sb + c2 * db; a.bn[c] = c12 * (T(5) * t1 + T(8) * a.b[c] - sb); a.sbb[c] = a.b[c];
+        sb = a.sbu[c]; t1 = sb + c2 * du; a.un[c] = c12 * (T(5) * t1 + T(8) * a.u[c] - sb); a.sbu[c] = a.u[c];
+        sb = a.sbv[c]; t1 = sb + c2 * dv; a.vn[c] = c12 * (T(5) * t1 + T(8) * a.v[c] - sb); a.sbv[c] = a.v[c];
+        sb = a.sbw[c]; t1 = sb + c2 * dw; a.wn[c] = c12 * (T(5) * t1 + T(8) * a.w[c] - sb); a.sbw[c] = a.w[c];
+    } else if (MODE == 5) {     // LFAM3 corrector: s = sn + dt*ds (core/timescheme.py:393-398)
+        a.bn[c] = a.bn[c] + cc * db;
+        a.un[c] = a.un[c] + cc * du;
+        a.vn[c] = a.vn[c] + cc * dv;
+        a.wn[c] = a.wn[c] + cc * dw;
+    } else if (MODE == 6) {     // first LFAM3 step: sn = s; sb = s; s += dt*ds (:352-364)
+        a.sbb[c] = a.b[c]; a.sbu[c] = a.u[c]; a.sbv[c] = a.v[c]; a.sbw[c] = a.w[c];
+        a.bn[c] = a.b[c] + cc * db;
+        a.un[c] = a.u[c] + cc * du;
+        a.vn[c] = a.v[c] + cc * dv;
+        a.wn[c] = a.w[c] + cc * dw;
     }
 }
 
@@ -126,6 +145,9 @@ hipError_t launch_stage(const Geo &g, const StageArgs<T> &a, int mode, hipStream
     case 1: hipLaunchKernelGGL((k_stage<T, 1>), grd, blk, 0, s, g, a); break;
     case 2: hipLaunchKernelGGL((k_stage<T, 2>), grd, blk, 0, s, g, a); break;
     case 3: hipLaunchKernelGGL((k_stage<T, 3>), grd, blk, 0, s, g, a); break;
+    case 4: hipLaunchKernelGGL((k_stage<T, 4>), grd, blk, 0, s, g, a); break;
+    case 5: hipLaunchKernelGGL((k_stage<T, 5>), grd, blk, 0, s, g, a); break;
+    case 6: hipLaunchKernelGGL((k_stage<T, 6>), grd, blk, 0, s, g, a); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

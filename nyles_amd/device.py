@@ -110,6 +110,21 @@ class DeviceModel:
         self.last_stats = st
         return st.as_dict()
 
+    def step_ef(self, dt):
+        st = L.NylMGStats()
+        L.check(self.lib.nyl_step_ef(self.ctx, float(dt), ctypes.byref(st)))
+        self.last_stats = st
+        return st.as_dict()
+
+    def step_lfam3(self, dt):
+        st = L.NylMGStats()
+        L.check(self.lib.nyl_step_lfam3(self.ctx, float(dt), ctypes.byref(st)))
+        self.last_stats = st
+        return st.as_dict()
+
+    def lfam3_reset(self):
+        L.check(self.lib.nyl_lfam3_reset(self.ctx))
+
     def mg_solve(self):
         st = L.NylMGStats()
         L.check(self.lib.nyl_mg_solve(self.ctx, ctypes.byref(st)))

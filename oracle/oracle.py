@@ -479,6 +479,41 @@ class OracleLES:
             s += (dt / 12.) * (8 * d2 - d0 - d1)
         self.diagnose_var()
 
+    # -- core/timescheme.py:336-345 ----------------------------------------
+    def euler_forward(self, dt):
+        ds = self.ds[0]
+        self.rhs(ds, last=True)
+        for s, d in self._pairs(ds):
+            s += dt * d
+        self.diagnose_var()
+
+    # -- core/timescheme.py:348-400 ----------------------------------------
+    def lfam3(self, dt):
+        if not hasattr(self, "lf_first"):
+            self.lf_first = True
+            self.stateb = self._new_tendency()
+            self.staten = self._new_tendency()
+        ds = self.ds[0]
+        self.rhs(ds)
+        if self.lf_first:
+            for s, d, sb, sn in self._pairs(ds, self.stateb, self.staten):
+                sn[:] = s
+                sb[:] = s
+                s += dt * d
+            self.lf_first = False
+            self.diagnose_var()
+        else:
+            for s, d, sb, sn in self._pairs(ds, self.stateb, self.staten):
+                sn[:] = s
+                s[:] = sb + (2. * dt) * d
+                s[:] = (1. / 12.) * (5. * s + 8. * sn - sb)
+                sb[:] = sn
+            self.diagnose_var()
+            self.rhs(ds, last=True)
+            for s, d, sn in self._pairs(ds, self.staten):
+                s[:] = sn + dt * d
+            self.diagnose_var()
+
     # -- core/nyles.py:227-260 ---------------------------------------------
     def compute_dt(self, cfl, dt_max):
         U, V, W = self.U["i"], self.U["j"], self.U["k"]

@@ -1,0 +1,121 @@
+"""An experiment script written against the reference API runs unchanged on
+the GPU path and reproduces the oracle bit for bit (GPU)."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CORE = os.path.join(ROOT, "nyles_amd", "core")
+
+
+def _script(tmp_path, scheme, steps=4):
+    """lock-exchange style setup, as experiments/lockechange/lockexchange.py"""
+    if CORE not in sys.path:
+        sys.path.insert(0, CORE)
+    import parameters
+    import nyles as nyles_module
+    parameters.InextensibleDict.frozen = False
+    param = parameters.UserParameters()
+    param.model["modelname"] = "LES"
+    param.model["geometry"] = "perio_x"
+    param.model["Lx"] = 16.0
+    param.model["Ly"] = 8.0
+    param.model["Lz"] = 8.0
+    param.IO["datadir"] = str(tmp_path)
+    param.IO["expname"] = "lock"
+    param.IO["variables_in_history"] = ["b", "u", "vor"]
+    param.IO["timestep_history"] = 0.1
+    param.time["timestepping"] = scheme
+    param.time["auto_dt"] = False
+    param.time["dt"] = 0.05
+    param.time["tend"] = steps * 0.05 - 1e-9
+    param.discretization["global_nx"] = 64
+    param.discretization["global_ny"] = 32
+    param.discretization["global_nz"] = 32
+    param.MPI["nh"] = 3
+    nyl = nyles_module.Nyles(param)
+    b = nyl.model.state.b.view("i")
+    x = nyl.grid.x_b.view("i")
+    rng = np.random.default_rng(1)
+    b[:] = np.where(x < 8.0, 0.5, -0.5) + 1e-3 * rng.uniform(-1, 1, x.shape)
+    b0 = b.copy()
+    nyl.run()
+    parameters.InextensibleDict.frozen = False
+    return nyl, b0
+
+
+@pytest.mark.parametrize("scheme", ["RK3_SSP", "LFAM3"])
+def test_lock_exchange_script_matches_oracle(tmp_path, scheme):
+    import oracle as O
+    nyl, b0 = _script(tmp_path, scheme)
+    st = nyl.model.state
+    orc = O.OracleLES(64, 32, 32, 16.0, 8.0, 8.0, "perio_x")
+    orc.set_interior(orc.b, st.b.interior(b0))
+    orc.diagnose_var()
+    for n in range(nyl.n):
+        if scheme == "RK3_SSP":
+            orc.rk3_ssp(0.05)
+        else:
+            orc.lfam3(0.05)
+    for name, mine, ref in (("b", st.b, orc.b), ("u", st.u["i"], orc.u["i"]),
+                            ("w", st.u["k"], orc.u["k"]), ("vor_j", st.vor["j"], orc.vor["j"]),
+                            ("ke", st.ke, orc.ke)):
+        got = mine.view("i")
+        np.testing.assert_array_equal(got, ref, err_msg=name)
+    files = sorted(os.listdir(nyl.IO.output_directory))
+    assert any(f.endswith(".npz") for f in files)
+    assert "param.pkl" in files
+
+
+def test_auto_dt_closed_forms(tmp_path):
+    """compute_dt: dt_max at rest, cfl/|U| otherwise (core/nyles.py:306-321)."""
+    if CORE not in sys.path:
+        sys.path.insert(0, CORE)
+    import parameters
+    import nyles as nyles_module
+    parameters.InextensibleDict.frozen = False
+    param = parameters.UserParameters()
+    param.IO["datadir"] = str(tmp_path)
+    param.time["dt_max"] = 1.5
+    param.discretization["global_nx"] = 32
+    param.discretization["global_ny"] = 32
+    param.discretization["global_nz"] = 32
+    nyl = nyles_module.Nyles(param)
+    parameters.InextensibleDict.frozen = False
+    assert nyl.compute_dt() == 1.5
+    cfl = param.time["cfl"]
+    dx = nyl.grid.dx
+    for k, comp in enumerate("ijk"):
+        # U = u/dx**2 = 1 in every cell for the component
+        nyl.model.state.u[comp].view("i")[...] = dx ** 2
+        assert abs(nyl.compute_dt() - min(cfl / np.sqrt(k + 1), 1.5)) < 1e-12
+
+
+def test_blowup_stops_the_run(tmp_path, capsys):
+    """A non-finite value in the state makes forward() report a blow-up;
+    run() writes a last frame and aborts (core/nyles.py:183-216)."""
+    if CORE not in sys.path:
+        sys.path.insert(0, CORE)
+    import parameters
+    import nyles as nyles_module
+    parameters.InextensibleDict.frozen = False
+    param = parameters.UserParameters()
+    param.IO["datadir"] = str(tmp_path)
+    param.time["timestepping"] = "RK3_SSP"
+    param.time["auto_dt"] = False
+    param.time["dt"] = 0.01
+    param.time["tend"] = 1.0
+    param.discretization["global_nx"] = 16
+    param.discretization["global_ny"] = 16
+    param.discretization["global_nz"] = 16
+    nyl = nyles_module.Nyles(param)
+    parameters.InextensibleDict.frozen = False
+    nyl.model.state.b.view("i")[3, 4, 5] = np.nan
+    nyl.run()
+    out = capsys.readouterr().out
+    assert "BLOW UP!" in out and "Job is aborted" in out
+    assert nyl.n == 1

@@ -140,3 +140,28 @@ def test_rotating_vorticity(cmp):
     got = cmp.device_fields(dev)
     _assert_same("vor_k", got["vor_k"], want["vor_k"])
     dev.close()
+
+
+@pytest.mark.parametrize("scheme,geometry", [("EF", "perio_xy"), ("LFAM3", "closed"),
+                                             ("LFAM3", "perio_xyz")])
+def test_ef_lfam3_parity(cmp, scheme, geometry):
+    """Euler forward and the LFAM3 predictor-corrector, three steps, bitwise."""
+    print("case", scheme, geometry, flush=True)
+    orc, dev = cmp.make_pair(32, 16, 16, geometry, Lx=2.0, Ly=1.0, Lz=1.0)
+    st = cmp.random_state(orc, seed=9, amp_u=3e-3)
+    cmp.load_state(orc, dev, st)
+    orc.diagnose_var()
+    dev.diagnose()
+    for n in range(3):
+        dt = orc.compute_dt(cfl=0.5, dt_max=0.1)
+        if scheme == "EF":
+            orc.euler_forward(dt)
+            dev.step_ef(dt)
+        else:
+            orc.lfam3(dt)
+            dev.step_lfam3(dt)
+    want = cmp.oracle_fields(orc)
+    got = cmp.device_fields(dev)
+    for k in want:
+        _assert_same(k, got[k], want[k])
+    dev.close()
