@@ -89,6 +89,7 @@ enum {
     NYL_F_DS2_B, NYL_F_DS2_U, NYL_F_DS2_V, NYL_F_DS2_W,   /* slot 2 */
     NYL_F_MGX,                                       /* MG level-1 solution x (warm start) */
     NYL_F_MGB,                                       /* MG level-1 right-hand side b */
+    NYL_F_MGY,                                       /* MG level-1 smoother scratch y */
     NYL_NFIELDS
 };
 
@@ -194,6 +195,15 @@ int nyl_reduce(nyl_ctx *ctx, int what, double *out);
 #define NYL_K_RESNORM 8      /* residual norm (two launches) */
 int nyl_prof_start(nyl_ctx *ctx, int kernel_class);
 int nyl_prof_stop(nyl_ctx *ctx, double *total_ms, int64_t *launches, double *alg_bytes);
+
+/* Kernel-level checks of the level-1 multigrid kernels (synchronous):
+ * 0: one fsmoother3d sweep NYL_F_MGX -> NYL_F_MGY with the per-cell kernel;
+ * 1: the same sweep with the tiled kernel; 2: tiled sweep that also reduces
+ * the squared residual of NYL_F_MGX (read it with nyl_reduce_last); 3: that
+ * residual norm alone.  Used by the parity tests; no reference counterpart
+ * beyond the sweep itself (core/mgfor/basicoperators.f90:363-400). */
+int nyl_debug_mg(nyl_ctx *ctx, int what);
+int nyl_reduce_last(nyl_ctx *ctx, double *out);
 
 /* Wait for all work of the context's stream. */
 int nyl_sync(nyl_ctx *ctx);

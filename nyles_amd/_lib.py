@@ -20,7 +20,7 @@ NYL_PREC_MIXED = 1
  NYL_F_DS0_B, NYL_F_DS0_U, NYL_F_DS0_V, NYL_F_DS0_W,
  NYL_F_DS1_B, NYL_F_DS1_U, NYL_F_DS1_V, NYL_F_DS1_W,
  NYL_F_DS2_B, NYL_F_DS2_U, NYL_F_DS2_V, NYL_F_DS2_W,
- NYL_F_MGX, NYL_F_MGB, NYL_NFIELDS) = range(28)
+ NYL_F_MGX, NYL_F_MGB, NYL_F_MGY, NYL_NFIELDS) = range(29)
 
 NYL_DT_F64 = 0
 NYL_DT_F32 = 1
@@ -43,7 +43,7 @@ EXPORTED = [
     "nyl_workspace_bytes", "nyl_bind_workspace", "nyl_set_stream", "nyl_field_offset",
     "nyl_mg_levels", "nyl_mg_level_shape", "nyl_rhs", "nyl_rk3_stage", "nyl_diagnose",
     "nyl_step_rk3", "nyl_step_ef", "nyl_step_lfam3", "nyl_lfam3_reset", "nyl_mg_solve", "nyl_apply", "nyl_reduce", "nyl_prof_start",
-    "nyl_prof_stop", "nyl_sync",
+    "nyl_prof_stop", "nyl_debug_mg", "nyl_reduce_last", "nyl_sync",
 ]
 
 # kernel classes of nyl_prof_start
@@ -117,6 +117,8 @@ def load():
         "nyl_reduce": (ci, [vp, ci, P(dbl)]),
         "nyl_prof_start": (ci, [vp, ci]),
         "nyl_prof_stop": (ci, [vp, P(dbl), P(i64), P(dbl)]),
+        "nyl_debug_mg": (ci, [vp, ci]),
+        "nyl_reduce_last": (ci, [vp, P(dbl)]),
         "nyl_sync": (ci, [vp]),
     }
     for name, (res, args) in sig.items():

@@ -4,6 +4,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <utility>
 #include <vector>
 
 #include "nyl_internal.h"
@@ -232,55 +233,80 @@ void set_tendency_out(nyl_ctx *c, StageArgs<double> &a, int slot) {
     a.tv = F<double>(c, c->ds[slot][2]); a.tw = F<double>(c, c->ds[slot][3]);
 }
 
-// core/mgfor/solvers.f90:35-55
-int vcycle(nyl_ctx *c) {
-    const int L = c->nlev;
+// One smooth() of level l: the two fsmoother3d sweeps x -> y -> x.
+// xc_level >= 0 first adds the prolongation from that coarse level.
+// pre_done: the first sweep already ran (speculatively, with the
+// convergence test of the previous V-cycle) into y.
+int smooth_level(nyl_ctx *c, int l, int xc_level, bool pre_done = false) {
+    Level &f = c->lev[l];
     const double om = c->d.mg_omega;
-    for (int l = 0; l < L - 1; l++) {
-        Level &f = c->lev[l], &q = c->lev[l + 1];
-        double *x = F<double>(c, f.x), *b = F<double>(c, f.b), *y = F<double>(c, f.y);
-        LAUNCH(NYL_K_MG_SWEEP, 3 * cells(f.g) * sizeof(double),
-               launch_mg_sweep<double>(f.g, x, y, b, om, c->tabs, c->stream));
-        LAUNCH(NYL_K_MG_SWEEP, 3 * cells(f.g) * sizeof(double),
-               launch_mg_sweep<double>(f.g, y, x, b, om, c->tabs, c->stream));
-        LAUNCH(NYL_K_MG_RESTRICT, 2.25 * cells(f.g) * sizeof(double),
-               launch_mg_resid_restrict<double>(f.g, q.g, x, b, F<double>(c, q.b),
-                                                F<double>(c, q.x), c->tabs, c->stream));
-    }
-    {
-        Level &f = c->lev[L - 1];
-        double *x = F<double>(c, f.x), *b = F<double>(c, f.b), *y = F<double>(c, f.y);
-        LAUNCH(NYL_K_MG_SWEEP, 3 * cells(f.g) * sizeof(double),
-               launch_mg_sweep<double>(f.g, x, y, b, om, c->tabs, c->stream));
-        LAUNCH(NYL_K_MG_SWEEP, 3 * cells(f.g) * sizeof(double),
-               launch_mg_sweep<double>(f.g, y, x, b, om, c->tabs, c->stream));
-    }
-    for (int l = L - 2; l >= 0; l--) {
-        Level &f = c->lev[l], &q = c->lev[l + 1];
-        double *x = F<double>(c, f.x), *b = F<double>(c, f.b), *y = F<double>(c, f.y);
+    double *x = F<double>(c, f.x), *b = F<double>(c, f.b), *y = F<double>(c, f.y);
+    if (xc_level >= 0) {
+        Level &q = c->lev[xc_level];
         LAUNCH(NYL_K_MG_PROLONG, 2.125 * cells(f.g) * sizeof(double),
                launch_mg_prolong<double>(f.g, q.g, x, F<double>(c, q.x), c->tabs, c->stream));
-        LAUNCH(NYL_K_MG_SWEEP, 3 * cells(f.g) * sizeof(double),
-               launch_mg_sweep<double>(f.g, x, y, b, om, c->tabs, c->stream));
-        LAUNCH(NYL_K_MG_SWEEP, 3 * cells(f.g) * sizeof(double),
-               launch_mg_sweep<double>(f.g, y, x, b, om, c->tabs, c->stream));
     }
+    const double by = 3 * cells(f.g) * sizeof(double);
+    if (sweep25_fits(f.g)) {
+        if (!pre_done)
+            LAUNCH(NYL_K_MG_SWEEP, by, launch_mg_sweep25<double>(f.g, x, y, b, om, c->tabs,
+                                                                  nullptr, nullptr, c->stream));
+        LAUNCH(NYL_K_MG_SWEEP, by, launch_mg_sweep25<double>(f.g, y, x, b, om, c->tabs, nullptr,
+                                                              nullptr, c->stream));
+        return NYL_OK;
+    }
+    if (!pre_done)
+        LAUNCH(NYL_K_MG_SWEEP, by, launch_mg_sweep<double>(f.g, x, y, b, om, c->tabs, c->stream));
+    LAUNCH(NYL_K_MG_SWEEP, by, launch_mg_sweep<double>(f.g, y, x, b, om, c->tabs, c->stream));
     return NYL_OK;
 }
 
-// core/mgfor/solvers.f90:8-33 with normresidual (core/mgfor/operators.f90:81-96)
+// core/mgfor/solvers.f90:35-55.  pre_done: the level-1 pre-smooth already
+// ran (speculatively, with the convergence test) into the spare buffer.
+int vcycle(nyl_ctx *c, bool pre_done) {
+    const int L = c->nlev;
+    int rc;
+    for (int l = 0; l < L - 1; l++) {
+        Level &f = c->lev[l], &q = c->lev[l + 1];
+        if ((rc = smooth_level(c, l, -1, l == 0 && pre_done))) return rc;
+        LAUNCH(NYL_K_MG_RESTRICT, 2.25 * cells(f.g) * sizeof(double),
+               launch_mg_resid_restrict<double>(f.g, q.g, F<double>(c, f.x), F<double>(c, f.b),
+                                                F<double>(c, q.b), F<double>(c, q.x), c->tabs,
+                                                c->stream));
+    }
+    if ((rc = smooth_level(c, L - 1, -1))) return rc;
+    for (int l = L - 2; l >= 0; l--)
+        if ((rc = smooth_level(c, l, l + 1))) return rc;
+    return NYL_OK;
+}
+
+// core/mgfor/solvers.f90:8-33 with normresidual (core/mgfor/operators.f90:81-96).
+// When level 1 uses the tiled sweep, the residual norm of x is computed by
+// the first pre-smoothing sweep of the next V-cycle, launched before the
+// test: if the test stops the solve, that sweep's output (in y) is unused.
 int mg_solve(nyl_ctx *c, nyl_mg_stats *st) {
     Level &f = c->lev[0];
-    const double *x = F<double>(c, f.x), *b = F<double>(c, f.b);
     double *part = F<double>(c, c->partials), *res_dev = F<double>(c, c->result);
+    const bool spec = sweep25_fits(f.g) && sweep25_blocks(f.g) <= NYL_RED_BLOCKS;
     int rc;
     double normb, rn, res;
-    HIPCHK(launch_norm2<double>(f.g, b, part, res_dev, c->stream));
+    HIPCHK(launch_norm2<double>(f.g, F<double>(c, f.b), part, res_dev, c->stream));
     if ((rc = read_result(c, &normb))) return rc;
+    bool pending = false;
     auto normresidual = [&](double *out) -> int {
+        pending = false;
         if (normb > 0.0) {
-            LAUNCH(NYL_K_RESNORM, 2 * cells(f.g) * sizeof(double),
-                   launch_resnorm2<double>(f.g, x, b, c->tabs, part, res_dev, c->stream));
+            if (spec) {
+                LAUNCH(NYL_K_MG_SWEEP, 3 * cells(f.g) * sizeof(double),
+                       launch_mg_sweep25<double>(f.g, F<double>(c, f.x), F<double>(c, f.y),
+                                                 F<double>(c, f.b), c->d.mg_omega, c->tabs,
+                                                 part, res_dev, c->stream));
+                pending = true;
+            } else {
+                LAUNCH(NYL_K_RESNORM, 2 * cells(f.g) * sizeof(double),
+                       launch_resnorm2<double>(f.g, F<double>(c, f.x), F<double>(c, f.b), c->tabs,
+                                               part, res_dev, c->stream));
+            }
             int r2 = read_result(c, &rn);
             if (r2) return r2;
             *out = rn / normb;
@@ -293,7 +319,7 @@ int mg_solve(nyl_ctx *c, nyl_mg_stats *st) {
     int nite = 0;
     for (;;) {
         if (res < c->d.mg_tol) break;
-        if ((rc = vcycle(c))) return rc;
+        if ((rc = vcycle(c, pending))) return rc;
         nite = nite + 1;
         if (nite >= c->d.mg_maxite) break;
         if ((rc = normresidual(&res))) return rc;
@@ -478,6 +504,7 @@ int nyl_field_offset(const nyl_ctx *c, int field, uint64_t *off, int64_t *nelem,
     case NYL_F_DIV: o = c->lev[0].b; break;
     case NYL_F_MGX: o = c->lev[0].x; break;
     case NYL_F_MGB: o = c->lev[0].b; break;
+    case NYL_F_MGY: o = c->lev[0].y; break;
     default:
         if (field >= NYL_F_DS0_B && field <= NYL_F_DS2_W) {
             const int q = field - NYL_F_DS0_B;
@@ -644,6 +671,43 @@ int nyl_prof_stop(nyl_ctx *c, double *total_ms, int64_t *launches, double *alg_b
     if (alg_bytes) *alg_bytes = c->prof_bytes;
     c->prof_cls = 0;
     return NYL_OK;
+}
+
+int nyl_debug_mg(nyl_ctx *c, int what) {
+    int rc = check_ready(c);
+    if (rc) return rc;
+    Level &f = c->lev[0];
+    double *x = F<double>(c, f.x), *b = F<double>(c, f.b), *y = F<double>(c, f.y);
+    double *part = F<double>(c, c->partials), *res_dev = F<double>(c, c->result);
+    switch (what) {
+    case 0:
+        HIPCHK(launch_mg_sweep<double>(f.g, x, y, b, c->d.mg_omega, c->tabs, c->stream));
+        break;
+    case 1:
+        if (!sweep25_fits(f.g)) return fail(NYL_E_SIZE, "level 1 does not fit the tiled sweep");
+        HIPCHK(launch_mg_sweep25<double>(f.g, x, y, b, c->d.mg_omega, c->tabs, nullptr, nullptr,
+                                         c->stream));
+        break;
+    case 2:
+        if (!sweep25_fits(f.g)) return fail(NYL_E_SIZE, "level 1 does not fit the tiled sweep");
+        HIPCHK(launch_mg_sweep25<double>(f.g, x, y, b, c->d.mg_omega, c->tabs, part, res_dev,
+                                         c->stream));
+        break;
+    case 3:
+        HIPCHK(launch_resnorm2<double>(f.g, x, b, c->tabs, part, res_dev, c->stream));
+        break;
+    default:
+        return fail(NYL_E_ARG, "unknown debug operation %d", what);
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return NYL_OK;
+}
+
+int nyl_reduce_last(nyl_ctx *c, double *out) {
+    int rc = check_ready(c);
+    if (rc) return rc;
+    if (!out) return fail(NYL_E_ARG, "null argument");
+    return read_result(c, out);
 }
 
 int nyl_sync(nyl_ctx *c) {

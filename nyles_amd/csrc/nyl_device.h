@@ -21,6 +21,28 @@ __device__ __forceinline__ int wrapi(int m, int n) {
     return m < 0 ? m + n : (m >= n ? m - n : m);
 }
 
+// Block coordinates remapped so that each of the 8 XCDs (which receive
+// workgroups round-robin in dispatch order) owns a contiguous range of the
+// grid: with x fastest, then y, then z, an XCD sweeps a slab of z planes and
+// the z-neighbours of a plane stay in its own L2.
+struct Blk3 {
+    int x, y, z;
+};
+
+__device__ __forceinline__ Blk3 xcd_block() {
+    const uint32_t gx = gridDim.x, gy = gridDim.y, gz = gridDim.z;
+    const uint32_t nb = gx * gy * gz;
+    uint32_t b = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+    const uint32_t per = nb >> 3;
+    if (b < (per << 3)) b = (b & 7u) * per + (b >> 3);
+    Blk3 r;
+    r.x = (int)(b % gx);
+    const uint32_t t = b / gx;
+    r.y = (int)(t % gy);
+    r.z = (int)(t / gy);
+    return r;
+}
+
 // ---------------------------------------------------------------------------
 // WENO reconstructions of core/weno.f90.  Real literals there are REAL(4)
 // constants widened to the working precision (the f2py build has no

@@ -74,8 +74,18 @@ template <typename T>
 hipError_t launch_mg_prolong(const Geo &gf, const Geo &gc, T *xf, const T *xc,
                              const MGTabs &tab, hipStream_t s);
 
+// One fsmoother3d sweep with 2.5-D LDS tiling (levels with nx % 64 == 0 and
+// ny % 8 == 0).  partials: also reduce the squared residual of src into
+// dev_out (needs sweep25_blocks(g) <= NYL_RED_BLOCKS partials).
+bool sweep25_fits(const Geo &g);
+int sweep25_blocks(const Geo &g);
+template <typename T>
+hipError_t launch_mg_sweep25(const Geo &g, const T *src, T *dst, const T *b, double omega,
+                             const MGTabs &tab, double *partials, double *dev_out,
+                             hipStream_t s);
+
 // reductions: partials then one final block; result written to dev_out[0]
-#define NYL_RED_BLOCKS 1024
+#define NYL_RED_BLOCKS 16384
 template <typename T>
 hipError_t launch_norm2(const Geo &g, const T *v, double *partials, double *dev_out,
                         hipStream_t s);

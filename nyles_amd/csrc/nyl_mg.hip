@@ -49,9 +49,10 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_mg_sweep(Geo g, const T *__restrict__ src,
                                                   T *__restrict__ dst, const T *__restrict__ b,
                                                   double omega, MGTabs tab) {
-    const int i = blockIdx.x * MG_BX + threadIdx.x;
-    const int j = blockIdx.y * MG_BY + threadIdx.y;
-    const int k = blockIdx.z;
+    const Blk3 bk = xcd_block();
+    const int i = bk.x * MG_BX + threadIdx.x;
+    const int j = bk.y * MG_BY + threadIdx.y;
+    const int k = bk.z;
     if (i >= g.nx || j >= g.ny) return;
     const int64_t c = (int64_t)k * g.sz + (int64_t)j * g.sy + i;
     const Nbrs<T> n = neighbours<T>(g, src, i, j, k, c);
@@ -88,9 +89,10 @@ __global__ __launch_bounds__(256) void k_mg_resid_restrict(Geo gf, Geo gc,
                                                            const T *__restrict__ b,
                                                            T *__restrict__ bc, T *__restrict__ xc,
                                                            MGTabs tab) {
-    const int ic = blockIdx.x * MG_BX + threadIdx.x;
-    const int jc = blockIdx.y * MG_BY + threadIdx.y;
-    const int kc = blockIdx.z;
+    const Blk3 bk = xcd_block();
+    const int ic = bk.x * MG_BX + threadIdx.x;
+    const int jc = bk.y * MG_BY + threadIdx.y;
+    const int kc = bk.z;
     if (ic >= gc.nx || jc >= gc.ny) return;
     const int i = 2 * ic, j = 2 * jc, k = 2 * kc;
     const T r000 = residual_at<T>(gf, x, b, i, j, k, tab);
@@ -114,15 +116,12 @@ hipError_t launch_mg_resid_restrict(const Geo &gf, const Geo &gc, const T *x, co
     return hipGetLastError();
 }
 
-// fprolongation_centers3d (core/mgfor/basicoperators.f90:173-231), one thread
-// per fine cell: xf += Pcoef*(3*b + a) with the (9,3,3,1) planar weights.
+// fprolongation_centers3d (core/mgfor/basicoperators.f90:173-231) at one
+// fine cell: Pcoef*(3*b + a) with the (9,3,3,1) planar weights, b the
+// parent's coarse plane and a the coarse plane on the child's side.
 template <typename T>
-__global__ __launch_bounds__(256) void k_mg_prolong(Geo gf, Geo gc, T *__restrict__ xf,
-                                                    const T *__restrict__ xc, MGTabs tab) {
-    const int i = blockIdx.x * MG_BX + threadIdx.x;
-    const int j = blockIdx.y * MG_BY + threadIdx.y;
-    const int k = blockIdx.z;
-    if (i >= gf.nx || j >= gf.ny) return;
+__device__ __forceinline__ T prolong_term(const Geo &gc, const T *__restrict__ xc,
+                                          const MGTabs &tab, int i, int j, int k) {
     const int ic = i >> 1, jc = j >> 1, kc = k >> 1;
     // coarse neighbour on the side of the fine cell within its parent
     int icn = (i & 1) ? ic + 1 : ic - 1;
@@ -141,8 +140,20 @@ __global__ __launch_bounds__(256) void k_mg_prolong(Geo gf, Geo gc, T *__restric
     };
     const T bmid = A(kc, true);
     const T aside = A(kcn, mz != 0);
+    return T(tab.pcoef[mx + 2 * my + 4 * mz]) * (T(3) * bmid + aside);
+}
+
+// one thread per fine cell: xf = xf + Pcoef*(3*b + a)
+template <typename T>
+__global__ __launch_bounds__(256) void k_mg_prolong(Geo gf, Geo gc, T *__restrict__ xf,
+                                                    const T *__restrict__ xc, MGTabs tab) {
+    const Blk3 bk = xcd_block();
+    const int i = bk.x * MG_BX + threadIdx.x;
+    const int j = bk.y * MG_BY + threadIdx.y;
+    const int k = bk.z;
+    if (i >= gf.nx || j >= gf.ny) return;
     const int64_t c = (int64_t)k * gf.sz + (int64_t)j * gf.sy + i;
-    xf[c] = xf[c] + T(tab.pcoef[mx + 2 * my + 4 * mz]) * (T(3) * bmid + aside);
+    xf[c] = xf[c] + prolong_term<T>(gc, xc, tab, i, j, k);
 }
 
 template <typename T>
@@ -160,26 +171,129 @@ hipError_t launch_mg_prolong(const Geo &gf, const Geo &gc, T *xf, const T *xc, c
 
 #define RED_T 256
 
-__device__ __forceinline__ void block_sum_store(double v, double *out) {
+// Fixed-order tree over the RED_T threads of a block; tid is the linear
+// thread index (blocks may be 2-D).
+__device__ __forceinline__ void block_sum_store(double v, double *out, int tid) {
     __shared__ double sh[RED_T];
-    sh[threadIdx.x] = v;
+    sh[tid] = v;
     __syncthreads();
     for (int s = RED_T / 2; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) sh[threadIdx.x] = sh[threadIdx.x] + sh[threadIdx.x + s];
+        if (tid < s) sh[tid] = sh[tid] + sh[tid + s];
         __syncthreads();
     }
-    if (threadIdx.x == 0) *out = sh[0];
+    if (tid == 0) *out = sh[0];
 }
 
-__device__ __forceinline__ void block_max_store(double v, double *out) {
+__device__ __forceinline__ void block_max_store(double v, double *out, int tid) {
     __shared__ double sh[RED_T];
-    sh[threadIdx.x] = v;
+    sh[tid] = v;
     __syncthreads();
     for (int s = RED_T / 2; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + s]);
+        if (tid < s) sh[tid] = fmax(sh[tid], sh[tid + s]);
         __syncthreads();
     }
-    if (threadIdx.x == 0) *out = sh[0];
+    if (tid == 0) *out = sh[0];
+}
+
+// One fsmoother3d sweep with 2.5-D tiling: a block owns a 64 x 8 column of
+// cells (two rows per thread) and marches up KZ planes.  The current plane,
+// with a one-cell ring, sits in LDS (three buffers, one barrier per plane);
+// the z-neighbours of a thread's own cells stay in registers.  Each global
+// value is loaded about once.  The arithmetic is that of k_mg_sweep (same
+// operand order), so the result is bit-identical; out-of-domain neighbours
+// along a wall axis are zero.
+//
+// RESNORM: the block also sums the squared residual b + diag*src - sum6(src)
+//          of its INPUT over its cells into partials[block] (the convergence
+//          test of the previous V-cycle, core/mgfor/operators.f90:81-96).
+#define SW_TX 64
+#define SW_TY 8
+#define SW_R (SW_TY / 4)
+
+template <typename T, bool RESNORM>
+__global__ __launch_bounds__(256) void k_mg_sweep25(Geo g, const T *__restrict__ src,
+                                                    T *__restrict__ dst,
+                                                    const T *__restrict__ b, double omega,
+                                                    MGTabs tab, int kz,
+                                                    double *__restrict__ partials) {
+    constexpr int LX = SW_TX + 2, LY = SW_TY + 2, NRING = 2 * LX + 2 * SW_TY;
+    __shared__ T sp[3][LY][LX];
+    const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * SW_TX + tx;
+    const Blk3 bk = xcd_block();
+    const int i0 = bk.x * SW_TX, j0 = bk.y * SW_TY, k0 = bk.z * kz;
+    const T om = T(omega), cff1 = T(1.0 - omega);
+    const int i = i0 + tx;
+
+    // ring position of this thread (one value per plane), -1 when none/outside
+    int64_t roff = -1;
+    int rlx = 0, rly = 0;
+    if (tid < NRING) {
+        const int q = tid;
+        if (q < LX) { rlx = q; rly = 0; }
+        else if (q < 2 * LX) { rlx = q - LX; rly = LY - 1; }
+        else { const int t = q - 2 * LX; rly = 1 + (t >> 1); rlx = (t & 1) ? LX - 1 : 0; }
+        int gi = i0 + rlx - 1, gj = j0 + rly - 1;
+        bool in = true;
+        if (g.px) gi = wrapi(gi, g.nx); else in = in && gi >= 0 && gi < g.nx;
+        if (g.py) gj = wrapi(gj, g.ny); else in = in && gj >= 0 && gj < g.ny;
+        if (in) roff = (int64_t)gj * g.sy + gi;
+    }
+    auto zok = [&](int m) { return g.pz || (m >= 0 && m < g.nz); };
+    auto zw = [&](int m) { return g.pz ? wrapi(m, g.nz) : m; };
+    // load plane m: own cells into registers `own` and LDS, ring into LDS
+    auto load_plane = [&](int m, int buf, T *own) {
+        const bool zin = zok(m);
+        const int64_t base = (int64_t)zw(m) * g.sz;
+#pragma unroll
+        for (int r = 0; r < SW_R; r++) {
+            const int j = j0 + ty + 4 * r;
+            own[r] = zin ? src[base + (int64_t)j * g.sy + i] : T(0);
+            sp[buf][1 + ty + 4 * r][1 + tx] = own[r];
+        }
+        if (tid < NRING) sp[buf][rly][rlx] = (zin && roff >= 0) ? src[base + roff] : T(0);
+    };
+    // in-domain neighbour count per own row (x and y parts fixed, z varies)
+    int nxy[SW_R];
+#pragma unroll
+    for (int r = 0; r < SW_R; r++) {
+        const int j = j0 + ty + 4 * r;
+        nxy[r] = (g.px || i > 0) + (g.px || i < g.nx - 1) + (g.py || j > 0) + (g.py || j < g.ny - 1);
+    }
+
+    T zm[SW_R], z0[SW_R], zp[SW_R];
+    double acc = 0.0;
+    // plane k lives in buffer (k - k0 + 1) % 3; loading plane k+1 reuses the
+    // buffer of plane k-2, whose readers all passed the previous barrier
+    load_plane(k0 - 1, 0, zm);
+    load_plane(k0, 1, z0);
+    for (int k = k0; k < k0 + kz; k++) {
+        const int cur = (k - k0 + 1) % 3;
+        load_plane(k + 1, (k - k0 + 2) % 3, zp);
+        __syncthreads();
+        const int nz_ = (g.pz || k > 0) + (g.pz || k < g.nz - 1);
+#pragma unroll
+        for (int r = 0; r < SW_R; r++) {
+            const int ly = 1 + ty + 4 * r, lx = 1 + tx;
+            const int j = j0 + ty + 4 * r;
+            const T S = sp[cur][ly][lx - 1] + sp[cur][ly][lx + 1] + sp[cur][ly - 1][lx]
+                      + sp[cur][ly + 1][lx] + zm[r] + zp[r];
+            const int64_t c = (int64_t)k * g.sz + (int64_t)j * g.sy + i;
+            const T bv = b[c];
+            const int nw = nxy[r] + nz_;
+            if (RESNORM) {
+                const double rr = (double)(bv + T(tab.diag[nw]) * z0[r] - S);
+                acc = acc + rr * rr;
+            }
+            dst[c] = cff1 * z0[r] + om * (S - bv) * T(tab.idiag[nw]);
+            zm[r] = z0[r];
+            z0[r] = zp[r];
+        }
+    }
+    if (RESNORM) {
+        __syncthreads();
+        const int nb = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+        block_sum_store(acc, partials + nb, tid);
+    }
 }
 
 static inline int red_blocks(int64_t n) {
@@ -191,14 +305,14 @@ __global__ __launch_bounds__(RED_T) void k_final_sum(const double *__restrict__ 
                                                      double *__restrict__ out) {
     double acc = 0.0;
     for (int q = threadIdx.x; q < np; q += RED_T) acc = acc + partials[q];
-    block_sum_store(acc, out);
+    block_sum_store(acc, out, threadIdx.x);
 }
 
 __global__ __launch_bounds__(RED_T) void k_final_max(const double *__restrict__ partials, int np,
                                                      double *__restrict__ out) {
     double acc = 0.0;
     for (int q = threadIdx.x; q < np; q += RED_T) acc = fmax(acc, partials[q]);
-    block_max_store(acc, out);
+    block_max_store(acc, out, threadIdx.x);
 }
 
 template <typename T>
@@ -210,7 +324,7 @@ __global__ __launch_bounds__(RED_T) void k_norm2(int64_t n, const T *__restrict_
         const double x = (double)v[q];
         acc = acc + x * x;
     }
-    block_sum_store(acc, partials + blockIdx.x);
+    block_sum_store(acc, partials + blockIdx.x, threadIdx.x);
 }
 
 template <typename T>
@@ -238,7 +352,7 @@ __global__ __launch_bounds__(RED_T) void k_resnorm2(Geo g, const T *__restrict__
         const double r = (double)residual_at<T>(g, x, b, i, j, k, tab);
         acc = acc + r * r;
     }
-    block_sum_store(acc, partials + blockIdx.x);
+    block_sum_store(acc, partials + blockIdx.x, threadIdx.x);
 }
 
 template <typename T>
@@ -266,7 +380,7 @@ __global__ __launch_bounds__(RED_T) void k_maxu2(int64_t n, const T *__restrict_
         const double s2 = U * U + V * V + W * W;
         acc = fmax(acc, s2);
     }
-    block_max_store(acc, partials + blockIdx.x);
+    block_max_store(acc, partials + blockIdx.x, threadIdx.x);
 }
 
 template <typename T>
@@ -288,7 +402,7 @@ __global__ __launch_bounds__(RED_T) void k_sum(int64_t n, const T *__restrict__ 
     for (int64_t q = (int64_t)blockIdx.x * RED_T + threadIdx.x; q < n;
          q += (int64_t)gridDim.x * RED_T)
         acc = acc + (double)v[q];
-    block_sum_store(acc, partials + blockIdx.x);
+    block_sum_store(acc, partials + blockIdx.x, threadIdx.x);
 }
 
 template <typename T>
@@ -315,7 +429,7 @@ __global__ __launch_bounds__(RED_T) void k_nonfinite(int64_t n, const T *__restr
         acc += isfinite((double)c[q]) ? 0.0 : 1.0;
         acc += isfinite((double)d[q]) ? 0.0 : 1.0;
     }
-    block_sum_store(acc, partials + blockIdx.x);
+    block_sum_store(acc, partials + blockIdx.x, threadIdx.x);
 }
 
 template <typename T>
@@ -328,6 +442,43 @@ hipError_t launch_nonfinite(const Geo &g, const T *a, const T *b, const T *c, co
     return hipGetLastError();
 }
 
+bool sweep25_fits(const Geo &g) {
+    return g.nx % SW_TX == 0 && g.ny % SW_TY == 0 && g.nx >= SW_TX;
+}
+
+// planes per block: up to 8, fewer when that leaves the chip short of blocks
+static int sweep25_kz(const Geo &g) {
+    const int64_t cols = (int64_t)(g.nx / SW_TX) * (g.ny / SW_TY);
+    int kz = 8;
+    while (kz > 1 && (g.nz % kz != 0 || cols * (g.nz / kz) < 1024)) kz >>= 1;
+    return kz;
+}
+
+int sweep25_blocks(const Geo &g) {
+    return (g.nx / SW_TX) * (g.ny / SW_TY) * (g.nz / sweep25_kz(g));
+}
+
+template <typename T>
+hipError_t launch_mg_sweep25(const Geo &g, const T *src, T *dst, const T *b, double omega,
+                             const MGTabs &tab, double *partials, double *dev_out,
+                             hipStream_t s) {
+    const int kz = sweep25_kz(g);
+    dim3 grd(g.nx / SW_TX, g.ny / SW_TY, g.nz / kz), blk(SW_TX, 4);
+    if (partials) {
+        hipLaunchKernelGGL((k_mg_sweep25<T, true>), grd, blk, 0, s, g, src, dst, b, omega, tab,
+                           kz, partials);
+        const int nb = grd.x * grd.y * grd.z;
+        hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(RED_T), 0, s, partials, nb, dev_out);
+    } else {
+        hipLaunchKernelGGL((k_mg_sweep25<T, false>), grd, blk, 0, s, g, src, dst, b, omega, tab,
+                           kz, partials);
+    }
+    return hipGetLastError();
+}
+
+template hipError_t launch_mg_sweep25<double>(const Geo &, const double *, double *,
+                                              const double *, double, const MGTabs &, double *,
+                                              double *, hipStream_t);
 template hipError_t launch_mg_sweep<double>(const Geo &, const double *, double *, const double *,
                                             double, const MGTabs &, hipStream_t);
 template hipError_t launch_mg_resid_restrict<double>(const Geo &, const Geo &, const double *,

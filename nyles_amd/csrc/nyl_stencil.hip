@@ -59,9 +59,10 @@ __device__ __forceinline__ T tracer_axis(T d, const T *__restrict__ q_arr,
 // (core/timescheme.py:422-443) for one cell.
 template <typename T, int MODE>
 __global__ __launch_bounds__(256) void k_stage(Geo g, StageArgs<T> a) {
-    const int i = blockIdx.x * NYL_BX + threadIdx.x;
-    const int j = blockIdx.y * NYL_BY + threadIdx.y;
-    const int k = blockIdx.z;
+    const Blk3 bk = xcd_block();
+    const int i = bk.x * NYL_BX + threadIdx.x;
+    const int j = bk.y * NYL_BY + threadIdx.y;
+    const int k = bk.z;
     if (i >= g.nx || j >= g.ny) return;
     const int64_t c = (int64_t)k * g.sz + (int64_t)j * g.sy + i;
     const int64_t bx = c - i, by = c - (int64_t)j * g.sy, bz = c - (int64_t)k * g.sz;
@@ -158,9 +159,10 @@ hipError_t launch_stage(const Geo &g, const StageArgs<T> &a, int mode, hipStream
 // core/fortran_kinenergy.f90:145-153) from the covariant velocity.
 template <typename T>
 __global__ __launch_bounds__(256) void k_diag(Geo g, DiagArgs<T> a, int what) {
-    const int i = blockIdx.x * NYL_BX + threadIdx.x;
-    const int j = blockIdx.y * NYL_BY + threadIdx.y;
-    const int k = blockIdx.z;
+    const Blk3 bk = xcd_block();
+    const int i = bk.x * NYL_BX + threadIdx.x;
+    const int j = bk.y * NYL_BY + threadIdx.y;
+    const int k = bk.z;
     if (i >= g.nx || j >= g.ny) return;
     const int64_t c = (int64_t)k * g.sz + (int64_t)j * g.sy + i;
     const T uc = a.u[c], vc = a.v[c], wc = a.w[c];
@@ -223,9 +225,10 @@ __global__ __launch_bounds__(256) void k_div(Geo g, const T *__restrict__ u,
                                              const T *__restrict__ v, const T *__restrict__ w,
                                              double idx2, double idy2, double idz2,
                                              TO *__restrict__ out) {
-    const int i = blockIdx.x * NYL_BX + threadIdx.x;
-    const int j = blockIdx.y * NYL_BY + threadIdx.y;
-    const int k = blockIdx.z;
+    const Blk3 bk = xcd_block();
+    const int i = bk.x * NYL_BX + threadIdx.x;
+    const int j = bk.y * NYL_BY + threadIdx.y;
+    const int k = bk.z;
     if (i >= g.nx || j >= g.ny) return;
     const int64_t c = (int64_t)k * g.sz + (int64_t)j * g.sy + i;
     const T cx = T(idx2), cy = T(idy2), cz = T(idz2);
@@ -256,9 +259,10 @@ template <typename T, typename TX>
 __global__ __launch_bounds__(256) void k_correct(Geo g, T *__restrict__ u, T *__restrict__ v,
                                                  T *__restrict__ w, const TX *__restrict__ x,
                                                  double dx2) {
-    const int i = blockIdx.x * NYL_BX + threadIdx.x;
-    const int j = blockIdx.y * NYL_BY + threadIdx.y;
-    const int k = blockIdx.z;
+    const Blk3 bk = xcd_block();
+    const int i = bk.x * NYL_BX + threadIdx.x;
+    const int j = bk.y * NYL_BY + threadIdx.y;
+    const int k = bk.z;
     if (i >= g.nx || j >= g.ny) return;
     const int64_t c = (int64_t)k * g.sz + (int64_t)j * g.sy + i;
     const TX pc = x[c] * TX(dx2);

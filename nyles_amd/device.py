@@ -149,6 +149,16 @@ class DeviceModel:
                                        ctypes.byref(by)))
         return ms.value, n.value, by.value
 
+    def debug_mg(self, what):
+        """Level-1 multigrid kernel check (see nyl_debug_mg); returns the
+        reduced residual norm for what = 2, 3."""
+        L.check(self.lib.nyl_debug_mg(self.ctx, what))
+        if what in (2, 3):
+            out = ctypes.c_double()
+            L.check(self.lib.nyl_reduce_last(self.ctx, ctypes.byref(out)))
+            return out.value
+        return None
+
     def sync(self):
         L.check(self.lib.nyl_sync(self.ctx))
 

@@ -78,17 +78,25 @@ def test_rhs_parity_random_inputs(cmp, geometry):
     dev.close()
 
 
+MG_SIZES = [(32, 32, 16), (128, 64, 32)]
+
+
+@pytest.mark.parametrize("size", MG_SIZES, ids=["%dx%dx%d" % s for s in MG_SIZES])
 @pytest.mark.parametrize("geometry", GEOMETRIES)
-def test_mg_solve_parity(cmp, geometry):
-    """mgfor solve on a compatible random right-hand side, cold then warm start."""
+def test_mg_solve_parity(cmp, geometry, size):
+    """mgfor solve on a compatible random right-hand side: cold start, warm
+    start on a new b, then the same b again (no V-cycle needed)."""
     from nyles_amd import _lib as L
-    print("case mg", geometry, flush=True)
-    orc, dev = cmp.make_pair(32, 32, 16, geometry)
+    print("case mg", geometry, size, flush=True)
+    nx, ny, nz = size
+    orc, dev = cmp.make_pair(nx, ny, nz, geometry)
     mg = orc.mg
     rng = np.random.default_rng(3)
-    for rep in range(2):
-        r = rng.standard_normal((16, 32, 32))
-        r -= r.mean()
+    r = None
+    for rep in range(3):
+        if rep < 2:
+            r = rng.standard_normal((nz, ny, nx))
+            r -= r.mean()
         sub = np.zeros(orc.shape)
         orc.set_interior(sub, r)
         orc.fill(sub)
@@ -98,16 +106,24 @@ def test_mg_solve_parity(cmp, geometry):
         dev.upload(L.NYL_F_MGB, r)
         s = dev.mg_solve()
         assert s["nite"] == nite, (rep, s, nite)
+        if rep == 2:
+            assert nite == 0
         x_or = mg.x[tuple(slice(3, -3) for _ in range(3))]
         _assert_same("x rep%d" % rep, dev.download(L.NYL_F_MGX), x_or)
     dev.close()
 
 
-@pytest.mark.parametrize("geometry", ["closed", "perio_x", "perio_xyz"])
-def test_rk3_steps_parity(cmp, geometry):
+RK_CASES = [("closed", (32, 16, 16)), ("perio_x", (32, 16, 16)), ("perio_xyz", (32, 16, 16)),
+            ("closed", (64, 16, 16)), ("perio_xyz", (128, 32, 16)), ("perio_xy", (64, 64, 32))]
+
+
+@pytest.mark.parametrize("geometry,size", RK_CASES,
+                         ids=["%s-%dx%dx%d" % ((g,) + s) for g, s in RK_CASES])
+def test_rk3_steps_parity(cmp, geometry, size):
     """Three RK3_SSP steps of the whole model, bitwise."""
-    print("case rk3", geometry, flush=True)
-    orc, dev = cmp.make_pair(32, 16, 16, geometry, Lx=2.0, Ly=1.0, Lz=1.0)
+    print("case rk3", geometry, size, flush=True)
+    nx, ny, nz = size
+    orc, dev = cmp.make_pair(nx, ny, nz, geometry, Lx=nx / 16, Ly=ny / 16, Lz=nz / 16)
     st = cmp.random_state(orc, seed=7, amp_u=3e-3)
     cmp.load_state(orc, dev, st)
     orc.diagnose_var()
@@ -164,4 +180,32 @@ def test_ef_lfam3_parity(cmp, scheme, geometry):
     got = cmp.device_fields(dev)
     for k in want:
         _assert_same(k, got[k], want[k])
+    dev.close()
+
+
+SWEEP_SIZES = [(64, 8, 8), (128, 64, 32), (64, 16, 8), (256, 32, 16)]
+
+
+@pytest.mark.parametrize("size", SWEEP_SIZES, ids=["%dx%dx%d" % s for s in SWEEP_SIZES])
+@pytest.mark.parametrize("geometry", GEOMETRIES)
+def test_tiled_sweep_matches_per_cell_sweep(cmp, geometry, size):
+    """The 2.5-D tiled fsmoother3d sweep equals the per-cell sweep bitwise, and
+    its fused residual norm equals the standalone one to rounding."""
+    from nyles_amd import _lib as L
+    print("case sweep", geometry, size, flush=True)
+    nx, ny, nz = size
+    orc, dev = cmp.make_pair(nx, ny, nz, geometry)
+    rng = np.random.default_rng(12)
+    dev.upload(L.NYL_F_MGX, rng.standard_normal((nz, ny, nx)))
+    dev.upload(L.NYL_F_MGB, rng.standard_normal((nz, ny, nx)))
+    dev.debug_mg(0)
+    want = dev.download(L.NYL_F_MGY)
+    dev.upload(L.NYL_F_MGY, np.zeros((nz, ny, nx)))
+    dev.debug_mg(1)
+    _assert_same("tiled sweep", dev.download(L.NYL_F_MGY), want)
+    dev.upload(L.NYL_F_MGY, np.zeros((nz, ny, nx)))
+    r2 = dev.debug_mg(2)
+    _assert_same("tiled sweep + norm", dev.download(L.NYL_F_MGY), want)
+    r3 = dev.debug_mg(3)
+    assert abs(r2 - r3) <= 1e-12 * r3
     dev.close()
