@@ -9,11 +9,18 @@ fixed dt = 0.5 / max|U| at t = 0.  One step = one full RK3 step (3
 right-hand sides + 3 projections); one cell-step = one interior cell advanced
 by one step.  The state is resident in HBM before the timed region starts.
 
-Prints ONE JSON line.  ``roofline`` prices the dominant kernel: achieved =
-its algorithmic HBM bytes per launch (each input field read once, each
-output written once) divided by its mean launch time, both measured with HIP
-events on the library's stream over the timed steps.  ``cpu_baseline`` times
-the FP64 CPU oracle (C restatement of the reference kernels + numpy
+The headline value is the mixed-precision path (FP32 stencils and inner
+V-cycles, FP64 outer residual); the FP64 path (bit-identical to the oracle)
+is timed in the same run, and both are advanced --err-steps steps from the
+same initial state to state the error of the mixed path against FP64 (KE,
+velocity, buoyancy).
+
+Prints ONE JSON line.  ``roofline`` prices the dominant kernel class of the
+headline path (the class with the largest kernel time per step): achieved =
+its algorithmic HBM bytes per launch (each input field read once, each output
+written once) divided by its mean launch time, both measured with HIP events
+on the library's stream over the timed steps.  ``cpu_baseline`` times the
+FP64 CPU oracle (C restatement of the reference kernels + numpy
 orchestration, OpenMP) on a bounded sample of the same workload.
 """
 import argparse
@@ -33,12 +40,16 @@ def parse():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=256, help="grid cells per axis")
-    ap.add_argument("--precision", choices=["fp64"], default="fp64")
-    ap.add_argument("--roofline-kernel", default="stage")
+    ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed",
+                    help="path of the headline value (the other one is timed beside it)")
+    ap.add_argument("--roofline-kernel", default=None,
+                    help="kernel class priced against the roofline (default: dominant)")
+    ap.add_argument("--err-steps", type=int, default=200,
+                    help="steps of the mixed-vs-FP64 error check (0: skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--breakdown", action="store_true",
-                    help="also time every kernel class (one extra pass each)")
+                    help="report the per-class kernel times of both paths")
     return ap.parse_args()
 
 
@@ -76,6 +87,85 @@ def cpu_baseline(n, steps):
                       "%.1f s wall on %d OpenMP threads" % (n, steps, el, O.num_threads())}
 
 
+def make_model(n, precision):
+    import numpy as np
+    from nyles_amd import _lib as L
+    from nyles_amd import workloads
+    from nyles_amd.device import DeviceModel
+    st, meta = workloads.taylor_green(n, seed=2)
+    Lx, Ly, Lz = meta["L"]
+    dx, dy, dz = Lx / n, Ly / n, Lz / n
+    prec = {"fp64": L.NYL_PREC_FP64, "mixed": L.NYL_PREC_MIXED}[precision]
+    dev = DeviceModel(n, n, n, (1, 1, 1), dx, dy, dz, idx2=1 / dx ** 2, idy2=1 / dy ** 2,
+                      idz2=1 / dz ** 2, dx2=dx ** 2, precision=prec)
+    for k, fid in (("b", L.NYL_F_B), ("u_i", L.NYL_F_U), ("u_j", L.NYL_F_V),
+                   ("u_k", L.NYL_F_W)):
+        dev.upload(fid, st[k])
+    dev.diagnose()
+    return dev
+
+
+def class_times(dev, dt):
+    """kernel ms, launches and algorithmic GB/s per class over one step"""
+    from nyles_amd import _lib as L
+    br = {}
+    for kc in L.KERNEL_CLASSES:
+        dev.prof_start(kc)
+        dev.step_rk3(dt)
+        ms, nl, by = dev.prof_stop()
+        br[kc] = {"ms_per_step": round(ms, 4), "launches": nl,
+                  "GB/s": round(by / (ms / 1e3) / 1e9, 1) if ms > 0 else None}
+    return br
+
+
+def timed(dev, dt, steps, warmup, kclass):
+    import torch
+    for _ in range(warmup):
+        dev.step_rk3(dt)
+    torch.cuda.synchronize()
+    dev.prof_start(kclass)
+    nite = 0
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        nite += dev.step_rk3(dt)["nite"]
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    kms, klaunch, kbytes = dev.prof_stop()
+    return el, nite, (kms, klaunch, kbytes)
+
+
+def error_check(n, dt, steps):
+    """Advance both paths `steps` steps from the same state; errors of the
+    mixed path relative to FP64 (bit-identical to the oracle)."""
+    import numpy as np
+    from nyles_amd import _lib as L
+    res = {}
+    fields = {}
+    for prec in ("fp64", "mixed"):
+        dev = make_model(n, prec)
+        for _ in range(steps):
+            dev.step_rk3(dt)
+        res[prec] = {"ke": dev.reduce(L.NYL_RED_KE), "bsum": dev.reduce(L.NYL_RED_BSUM),
+                     "nonfinite": dev.reduce(L.NYL_RED_NONFINITE)}
+        fields[prec] = {k: dev.download(f) for k, f in
+                        (("b", L.NYL_F_B), ("u", L.NYL_F_U), ("v", L.NYL_F_V), ("w", L.NYL_F_W))}
+        dev.close()
+    a, m = fields["fp64"], fields["mixed"]
+    du2 = sum(float(np.sum((m[k] - a[k]) ** 2)) for k in ("u", "v", "w"))
+    u2 = sum(float(np.sum(a[k] ** 2)) for k in ("u", "v", "w"))
+    bn = float(np.sqrt(np.sum(a["b"] ** 2)))
+    return {
+        "steps": steps, "ke_fp64": res["fp64"]["ke"], "ke_mixed": res["mixed"]["ke"],
+        "ke_rel_err": abs(res["mixed"]["ke"] - res["fp64"]["ke"]) / abs(res["fp64"]["ke"]),
+        "u_l2_rel_err": (du2 / u2) ** 0.5 if u2 > 0 else None,
+        "b_max_abs_err": float(np.max(np.abs(m["b"] - a["b"]))),
+        "b_l2_rel_err": (float(np.sqrt(np.sum((m["b"] - a["b"]) ** 2))) / bn) if bn > 0 else None,
+        "nonfinite_mixed": res["mixed"]["nonfinite"],
+        "tolerance": "KE and b within 1e-5 relative of FP64",
+    }
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -85,68 +175,65 @@ def main():
     import numpy as np
     import torch
     from nyles_amd import _lib as L
-    from nyles_amd import workloads
-    from nyles_amd.device import DeviceModel
 
     n = args.n
-    st, meta = workloads.taylor_green(n, seed=2)
-    Lx, Ly, Lz = meta["L"]
-    dx, dy, dz = Lx / n, Ly / n, Lz / n
-    dev = DeviceModel(n, n, n, (1, 1, 1), dx, dy, dz, idx2=1 / dx ** 2, idy2=1 / dy ** 2,
-                      idz2=1 / dz ** 2, dx2=dx ** 2, precision=L.NYL_PREC_FP64)
-    for k, fid in (("b", L.NYL_F_B), ("u_i", L.NYL_F_U), ("u_j", L.NYL_F_V),
-                   ("u_k", L.NYL_F_W)):
-        dev.upload(fid, st[k])
-    dev.diagnose()
-    dt = 0.5 / float(np.sqrt(dev.reduce(L.NYL_RED_MAXU2)))
-    for _ in range(args.warmup):
-        dev.step_rk3(dt)
-    torch.cuda.synchronize()
+    other = "fp64" if args.precision == "mixed" else "mixed"
+    runs = {}
+    dt = None
+    for prec in (args.precision, other):
+        print("timing %s %d^3" % (prec, n), file=sys.stderr, flush=True)
+        dev = make_model(n, prec)
+        if dt is None:
+            dt = 0.5 / float(np.sqrt(dev.reduce(L.NYL_RED_MAXU2)))
+        for _ in range(args.warmup):
+            dev.step_rk3(dt)
+        br = class_times(dev, dt)
+        kclass = args.roofline_kernel or max(br, key=lambda k: br[k]["ms_per_step"])
+        el, nite, prof = timed(dev, dt, args.steps, 0, kclass)
+        runs[prec] = {"el": el, "nite": nite, "prof": prof, "kclass": kclass, "breakdown": br}
+        dev.close()
+        torch.cuda.synchronize()
 
-    dev.prof_start(args.roofline_kernel)
-    nite = 0
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        nite += dev.step_rk3(dt)["nite"]
-    torch.cuda.synchronize()
-    el = time.perf_counter() - t0
-    kms, klaunch, kbytes = dev.prof_stop()
-
-    value = n ** 3 * args.steps / el / 1e6
+    head = runs[args.precision]
+    value = n ** 3 * args.steps / head["el"] / 1e6
+    kms, klaunch, kbytes = head["prof"]
     achieved = (kbytes / klaunch) / (kms / klaunch / 1e3) / 1e9 if klaunch else None
+    oth = runs[other]
     out = {
         "metric": METRIC, "value": round(value, 3), "unit": "Mcell-steps/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": round(el / args.steps * 1e3, 4), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "ms_per_step": round(head["el"] / args.steps * 1e3, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32+f64" if args.precision == "mixed" else "f64", "data": "synthetic",
         "config": {
             "workload": "taylor-green vortex %d^3 perio_xyz: LES, RK3_SSP, WENO5 vortex force "
                         "and tracer, multigrid projection (mgfor V-cycles, tol 1e-6 squared)" % n,
             "grid": [n, n, n], "precision": args.precision, "dt": dt, "seed": 2,
-            "mg_vcycles_per_step": round(nite / args.steps, 3),
-            "mg_vcycles_per_solve": round(nite / (3 * args.steps), 3),
+            "mg_vcycles_per_step": round(head["nite"] / args.steps, 3),
+            "mg_vcycles_per_solve": round(head["nite"] / (3 * args.steps), 3),
         },
         "roofline": {
-            "bound": "hbm", "kernel": args.roofline_kernel,
+            "bound": "hbm", "kernel": head["kclass"],
             "achieved": round(achieved, 2) if achieved else None, "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
             "traffic": None, "launches": klaunch,
             "mean_launch_us": round(kms / klaunch * 1e3, 2) if klaunch else None,
             "alg_bytes_per_launch": kbytes / klaunch if klaunch else None,
         },
+        other: {
+            "value": round(n ** 3 * args.steps / oth["el"] / 1e6, 3),
+            "ms_per_step": round(oth["el"] / args.steps * 1e3, 4),
+            "mg_vcycles_per_step": round(oth["nite"] / args.steps, 3),
+        },
     }
+    out["speedup_mixed_vs_fp64"] = round(runs["fp64"]["el"] / runs["mixed"]["el"], 3)
     if args.breakdown:
-        br = {}
-        for kc in L.KERNEL_CLASSES:
-            dev.prof_start(kc)
-            dev.step_rk3(dt)
-            ms, nl, by = dev.prof_stop()
-            br[kc] = {"ms_per_step": round(ms, 4), "launches": nl,
-                      "GB/s": round(by / (ms / 1e3) / 1e9, 1) if ms > 0 else None}
-        out["breakdown"] = br
-    dev.close()
+        out["breakdown"] = {p: runs[p]["breakdown"] for p in runs}
+    if args.err_steps > 0:
+        print("error check, %d steps" % args.err_steps, file=sys.stderr, flush=True)
+        out["error_vs_fp64"] = error_check(n, dt, args.err_steps)
     if not args.no_cpu_baseline:
+        print("cpu baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(n, args.cpu_steps)
     print(json.dumps(out), flush=True)
 

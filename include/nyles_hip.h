@@ -50,7 +50,12 @@ extern "C" {
 
 /* precision modes */
 #define NYL_PREC_FP64 0     /* loop-faithful FP64, bit-compatible with the oracle */
-#define NYL_PREC_MIXED 1    /* FP32 stencils and V-cycles, FP64 outer MG residual */
+/* NYL_PREC_MIXED: model fields (state, tendencies, U, vor, ke) and their
+ * stencils in FP32; the projection is defect correction with FP32 V-cycles
+ * inside an FP64 outer residual, the right-hand side's mean removed in FP64,
+ * an FP64 coarsest level; NYL_RED_* reductions accumulate in FP64.  No
+ * counterpart in the reference (which is FP64 throughout). */
+#define NYL_PREC_MIXED 1
 
 typedef struct nyl_ctx nyl_ctx;
 
@@ -87,19 +92,21 @@ enum {
     NYL_F_DS0_B, NYL_F_DS0_U, NYL_F_DS0_V, NYL_F_DS0_W,   /* RK tendencies, slot 0 */
     NYL_F_DS1_B, NYL_F_DS1_U, NYL_F_DS1_V, NYL_F_DS1_W,   /* slot 1 */
     NYL_F_DS2_B, NYL_F_DS2_U, NYL_F_DS2_V, NYL_F_DS2_W,   /* slot 2 */
-    NYL_F_MGX,                                       /* MG level-1 solution x (warm start) */
-    NYL_F_MGB,                                       /* MG level-1 right-hand side b */
+    NYL_F_MGX,                                       /* MG level-1 solution x (warm start, FP64) */
+    NYL_F_MGB,                                       /* MG level-1 right-hand side b (FP64) */
     NYL_F_MGY,                                       /* MG level-1 smoother scratch y */
     NYL_NFIELDS
 };
 
-/* dtype codes returned by nyl_field_offset */
+/* dtype codes returned by nyl_field_offset: model fields are F32 in mixed
+ * mode; NYL_F_P, NYL_F_DIV, NYL_F_MGX and NYL_F_MGB are always F64 */
 #define NYL_DT_F64 0
 #define NYL_DT_F32 1
 
 /* what nyl_reduce computes */
 #define NYL_RED_MAXU2 0   /* max over cells of U^2+V^2+W^2 (core/nyles.py:249) */
-#define NYL_RED_KE 1      /* sum of ke over the interior (FP64) */
+#define NYL_RED_KE 1      /* sum over the interior of ke evaluated in FP64 from the current
+                             u, v, w (core/fortran_kinenergy.f90:145-153) */
 #define NYL_RED_BSUM 2    /* sum of b over the interior */
 #define NYL_RED_NONFINITE 3  /* number of non-finite values in b,u,v,w */
 
@@ -192,7 +199,8 @@ int nyl_reduce(nyl_ctx *ctx, int what, double *out);
 #define NYL_K_DIAG 5         /* U, vorticity, ke */
 #define NYL_K_DIV 6          /* divergence */
 #define NYL_K_CORRECT 7      /* u -= grad p */
-#define NYL_K_RESNORM 8      /* residual norm (two launches) */
+#define NYL_K_RESNORM 8      /* residual norm (two launches); mixed mode: outer FP64 residual */
+#define NYL_K_MG_UPDATE 9    /* mixed mode: x += e of the defect correction */
 int nyl_prof_start(nyl_ctx *ctx, int kernel_class);
 int nyl_prof_stop(nyl_ctx *ctx, double *total_ms, int64_t *launches, double *alg_bytes);
 

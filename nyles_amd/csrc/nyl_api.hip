@@ -4,6 +4,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <type_traits>
 #include <utility>
 #include <vector>
 
@@ -44,6 +45,7 @@ struct Level {
 struct nyl_ctx {
     nyl_desc d;
     Geo g;
+    int mixed;               // NYL_PREC_MIXED: FP32 model fields and inner V-cycles
     int64_t ncell;
     int nlev;
     Level lev[kMaxLevels];
@@ -54,6 +56,8 @@ struct nyl_ctx {
     uint64_t sb[4];          // LFAM3 'before' state
     int lf_first;            // next LFAM3 step is the first one
     uint64_t ke, vor[3], uc[3], p;
+    uint64_t mgx, mgb;       // FP64 solution and right-hand side of the projection
+    double mg_shift;         // mean removed from the right-hand side (mixed mode)
     uint64_t partials, result;
     uint64_t ws_bytes;
     int cur;                 // which state buffer is current
@@ -118,8 +122,14 @@ void build_tabs(MGTabs &t) {
     }
 }
 
+// Level l holds FP64 arrays in FP64 mode; in mixed mode the inner V-cycle
+// levels are FP32 and the coarsest level stays FP64.
+bool lev64(const nyl_ctx *c, int l) { return !c->mixed || l == c->nlev - 1; }
+
 void build_layout(nyl_ctx *c) {
-    const uint64_t fb = align256((uint64_t)c->ncell * sizeof(double));
+    const uint64_t es = c->mixed ? sizeof(float) : sizeof(double);
+    const uint64_t fb = align256((uint64_t)c->ncell * es);
+    const uint64_t fb64 = align256((uint64_t)c->ncell * sizeof(double));
     uint64_t off = 0;
     auto take = [&](uint64_t bytes) { uint64_t o = off; off += align256(bytes); return o; };
     for (int s = 0; s < 2; s++)
@@ -130,12 +140,20 @@ void build_layout(nyl_ctx *c) {
     c->ke = take(fb);
     for (int f = 0; f < 3; f++) c->vor[f] = take(fb);
     for (int f = 0; f < 3; f++) c->uc[f] = take(fb);
-    c->p = take(fb);
+    c->p = take(fb64);
     for (int l = 0; l < c->nlev; l++) {
-        const uint64_t lb = (uint64_t)c->lev[l].g.nx * c->lev[l].g.ny * c->lev[l].g.nz * sizeof(double);
+        const uint64_t lb = (uint64_t)c->lev[l].g.nx * c->lev[l].g.ny * c->lev[l].g.nz *
+                            (lev64(c, l) ? sizeof(double) : sizeof(float));
         c->lev[l].x = take(lb);
         c->lev[l].b = take(lb);
         c->lev[l].y = take(lb);
+    }
+    if (c->mixed) {
+        c->mgx = take(fb64);
+        c->mgb = take(fb64);
+    } else {
+        c->mgx = c->lev[0].x;
+        c->mgb = c->lev[0].b;
     }
     c->partials = take(NYL_RED_BLOCKS * sizeof(double));
     c->result = take(8 * sizeof(double));
@@ -197,82 +215,125 @@ int timed_launch(nyl_ctx *c, int cls, double bytes, Fn fn) {
 
 double cells(const Geo &g) { return (double)g.nx * g.ny * g.nz; }
 
-int read_result(nyl_ctx *c, double *out) {
-    HIPCHK(hipMemcpyAsync(c->host_out, c->ws + c->result, sizeof(double), hipMemcpyDeviceToHost,
-                          c->stream));
+int read_results(nyl_ctx *c, double *out, int n) {
+    HIPCHK(hipMemcpyAsync(c->host_out, c->ws + c->result, n * sizeof(double),
+                          hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
-    *out = c->host_out[0];
+    for (int q = 0; q < n; q++) out[q] = c->host_out[q];
     return NYL_OK;
 }
 
-StageArgs<double> stage_args(nyl_ctx *c) {
-    StageArgs<double> a;
+int read_result(nyl_ctx *c, double *out) { return read_results(c, out, 1); }
+
+template <typename T>
+StageArgs<T> stage_args(nyl_ctx *c) {
+    StageArgs<T> a;
     const int s = c->cur, t = 1 - c->cur;
-    a.b = F<double>(c, c->state[s][0]); a.u = F<double>(c, c->state[s][1]);
-    a.v = F<double>(c, c->state[s][2]); a.w = F<double>(c, c->state[s][3]);
-    a.bn = F<double>(c, c->state[t][0]); a.un = F<double>(c, c->state[t][1]);
-    a.vn = F<double>(c, c->state[t][2]); a.wn = F<double>(c, c->state[t][3]);
-    a.Ux = F<double>(c, c->uc[0]); a.Uy = F<double>(c, c->uc[1]); a.Uz = F<double>(c, c->uc[2]);
-    a.ox = F<double>(c, c->vor[0]); a.oy = F<double>(c, c->vor[1]); a.oz = F<double>(c, c->vor[2]);
-    a.ke = F<double>(c, c->ke);
-    a.d0b = F<double>(c, c->ds[0][0]); a.d0u = F<double>(c, c->ds[0][1]);
-    a.d0v = F<double>(c, c->ds[0][2]); a.d0w = F<double>(c, c->ds[0][3]);
-    a.d1b = F<double>(c, c->ds[1][0]); a.d1u = F<double>(c, c->ds[1][1]);
-    a.d1v = F<double>(c, c->ds[1][2]); a.d1w = F<double>(c, c->ds[1][3]);
+    a.b = F<T>(c, c->state[s][0]); a.u = F<T>(c, c->state[s][1]);
+    a.v = F<T>(c, c->state[s][2]); a.w = F<T>(c, c->state[s][3]);
+    a.bn = F<T>(c, c->state[t][0]); a.un = F<T>(c, c->state[t][1]);
+    a.vn = F<T>(c, c->state[t][2]); a.wn = F<T>(c, c->state[t][3]);
+    a.Ux = F<T>(c, c->uc[0]); a.Uy = F<T>(c, c->uc[1]); a.Uz = F<T>(c, c->uc[2]);
+    a.ox = F<T>(c, c->vor[0]); a.oy = F<T>(c, c->vor[1]); a.oz = F<T>(c, c->vor[2]);
+    a.ke = F<T>(c, c->ke);
+    a.d0b = F<T>(c, c->ds[0][0]); a.d0u = F<T>(c, c->ds[0][1]);
+    a.d0v = F<T>(c, c->ds[0][2]); a.d0w = F<T>(c, c->ds[0][3]);
+    a.d1b = F<T>(c, c->ds[1][0]); a.d1u = F<T>(c, c->ds[1][1]);
+    a.d1v = F<T>(c, c->ds[1][2]); a.d1w = F<T>(c, c->ds[1][3]);
     a.tb = a.tu = a.tv = a.tw = nullptr;
-    a.sbb = F<double>(c, c->sb[0]); a.sbu = F<double>(c, c->sb[1]);
-    a.sbv = F<double>(c, c->sb[2]); a.sbw = F<double>(c, c->sb[3]);
+    a.sbb = F<T>(c, c->sb[0]); a.sbu = F<T>(c, c->sb[1]);
+    a.sbv = F<T>(c, c->sb[2]); a.sbw = F<T>(c, c->sb[3]);
     a.dz = c->d.dz;
     a.c = 0.0;
     a.c2 = 0.0;
     return a;
 }
 
-void set_tendency_out(nyl_ctx *c, StageArgs<double> &a, int slot) {
-    a.tb = F<double>(c, c->ds[slot][0]); a.tu = F<double>(c, c->ds[slot][1]);
-    a.tv = F<double>(c, c->ds[slot][2]); a.tw = F<double>(c, c->ds[slot][3]);
+template <typename T>
+void set_tendency_out(nyl_ctx *c, StageArgs<T> &a, int slot) {
+    a.tb = F<T>(c, c->ds[slot][0]); a.tu = F<T>(c, c->ds[slot][1]);
+    a.tv = F<T>(c, c->ds[slot][2]); a.tw = F<T>(c, c->ds[slot][3]);
 }
 
 // One smooth() of level l: the two fsmoother3d sweeps x -> y -> x.
 // xc_level >= 0 first adds the prolongation from that coarse level.
-// pre_done: the first sweep already ran (speculatively, with the
-// convergence test of the previous V-cycle) into y.
-int smooth_level(nyl_ctx *c, int l, int xc_level, bool pre_done = false) {
+// pre_done: the first sweep already ran into y (speculatively, with the
+// convergence test of the previous V-cycle, or from e = 0 in the outer
+// residual of the mixed solve).
+template <typename Tf>
+int smooth_level_t(nyl_ctx *c, int l, int xc_level, bool pre_done) {
     Level &f = c->lev[l];
     const double om = c->d.mg_omega;
-    double *x = F<double>(c, f.x), *b = F<double>(c, f.b), *y = F<double>(c, f.y);
+    Tf *x = F<Tf>(c, f.x), *b = F<Tf>(c, f.b), *y = F<Tf>(c, f.y);
     if (xc_level >= 0) {
         Level &q = c->lev[xc_level];
-        LAUNCH(NYL_K_MG_PROLONG, 2.125 * cells(f.g) * sizeof(double),
-               launch_mg_prolong<double>(f.g, q.g, x, F<double>(c, q.x), c->tabs, c->stream));
+        const double bp = 2.125 * cells(f.g) * sizeof(Tf);
+        if (lev64(c, xc_level)) {
+            LAUNCH(NYL_K_MG_PROLONG, bp,
+                   launch_mg_prolong<Tf, double>(f.g, q.g, x, F<double>(c, q.x), c->tabs,
+                                                 c->stream));
+        } else if constexpr (std::is_same<Tf, float>::value) {
+            LAUNCH(NYL_K_MG_PROLONG, bp,
+                   launch_mg_prolong<float, float>(f.g, q.g, x, F<float>(c, q.x), c->tabs,
+                                                   c->stream));
+        } else {
+            return fail(NYL_E_STATE, "FP64 level above an FP32 level");
+        }
     }
-    const double by = 3 * cells(f.g) * sizeof(double);
+    const double by = 3 * cells(f.g) * sizeof(Tf);
     if (sweep25_fits(f.g)) {
         if (!pre_done)
-            LAUNCH(NYL_K_MG_SWEEP, by, launch_mg_sweep25<double>(f.g, x, y, b, om, c->tabs,
-                                                                  nullptr, nullptr, c->stream));
-        LAUNCH(NYL_K_MG_SWEEP, by, launch_mg_sweep25<double>(f.g, y, x, b, om, c->tabs, nullptr,
+            LAUNCH(NYL_K_MG_SWEEP, by, launch_mg_sweep25<Tf>(f.g, x, y, b, om, c->tabs, nullptr,
                                                               nullptr, c->stream));
+        LAUNCH(NYL_K_MG_SWEEP, by, launch_mg_sweep25<Tf>(f.g, y, x, b, om, c->tabs, nullptr,
+                                                          nullptr, c->stream));
         return NYL_OK;
     }
     if (!pre_done)
-        LAUNCH(NYL_K_MG_SWEEP, by, launch_mg_sweep<double>(f.g, x, y, b, om, c->tabs, c->stream));
-    LAUNCH(NYL_K_MG_SWEEP, by, launch_mg_sweep<double>(f.g, y, x, b, om, c->tabs, c->stream));
+        LAUNCH(NYL_K_MG_SWEEP, by, launch_mg_sweep<Tf>(f.g, x, y, b, om, c->tabs, c->stream));
+    LAUNCH(NYL_K_MG_SWEEP, by, launch_mg_sweep<Tf>(f.g, y, x, b, om, c->tabs, c->stream));
+    return NYL_OK;
+}
+
+int smooth_level(nyl_ctx *c, int l, int xc_level, bool pre_done = false) {
+    return lev64(c, l) ? smooth_level_t<double>(c, l, xc_level, pre_done)
+                       : smooth_level_t<float>(c, l, xc_level, pre_done);
+}
+
+// residual of level l restricted into level l + 1 (coarse x = 0)
+int restrict_level(nyl_ctx *c, int l) {
+    Level &f = c->lev[l], &q = c->lev[l + 1];
+    const bool f64 = lev64(c, l), c64 = lev64(c, l + 1);
+    const double bytes = 2.25 * cells(f.g) * (f64 ? sizeof(double) : sizeof(float));
+    if (f64 && c64) {
+        LAUNCH(NYL_K_MG_RESTRICT, bytes,
+               launch_mg_resid_restrict<double, double>(f.g, q.g, F<double>(c, f.x),
+                                                        F<double>(c, f.b), F<double>(c, q.b),
+                                                        F<double>(c, q.x), c->tabs, c->stream));
+    } else if (!f64 && c64) {
+        LAUNCH(NYL_K_MG_RESTRICT, bytes,
+               launch_mg_resid_restrict<float, double>(f.g, q.g, F<float>(c, f.x),
+                                                       F<float>(c, f.b), F<double>(c, q.b),
+                                                       F<double>(c, q.x), c->tabs, c->stream));
+    } else if (!f64 && !c64) {
+        LAUNCH(NYL_K_MG_RESTRICT, bytes,
+               launch_mg_resid_restrict<float, float>(f.g, q.g, F<float>(c, f.x),
+                                                      F<float>(c, f.b), F<float>(c, q.b),
+                                                      F<float>(c, q.x), c->tabs, c->stream));
+    } else {
+        return fail(NYL_E_STATE, "FP64 level above an FP32 level");
+    }
     return NYL_OK;
 }
 
 // core/mgfor/solvers.f90:35-55.  pre_done: the level-1 pre-smooth already
-// ran (speculatively, with the convergence test) into the spare buffer.
+// ran into the spare buffer.
 int vcycle(nyl_ctx *c, bool pre_done) {
     const int L = c->nlev;
     int rc;
     for (int l = 0; l < L - 1; l++) {
-        Level &f = c->lev[l], &q = c->lev[l + 1];
         if ((rc = smooth_level(c, l, -1, l == 0 && pre_done))) return rc;
-        LAUNCH(NYL_K_MG_RESTRICT, 2.25 * cells(f.g) * sizeof(double),
-               launch_mg_resid_restrict<double>(f.g, q.g, F<double>(c, f.x), F<double>(c, f.b),
-                                                F<double>(c, q.b), F<double>(c, q.x), c->tabs,
-                                                c->stream));
+        if ((rc = restrict_level(c, l))) return rc;
     }
     if ((rc = smooth_level(c, L - 1, -1))) return rc;
     for (int l = L - 2; l >= 0; l--)
@@ -280,11 +341,21 @@ int vcycle(nyl_ctx *c, bool pre_done) {
     return NYL_OK;
 }
 
+void fill_stats(nyl_ctx *c, nyl_mg_stats *st, int nite, double res, double normb) {
+    c->total_cycles += nite;
+    if (st) {
+        st->nite = nite;
+        st->res = res;
+        st->normb = normb;
+        st->total_cycles = c->total_cycles;
+    }
+}
+
 // core/mgfor/solvers.f90:8-33 with normresidual (core/mgfor/operators.f90:81-96).
 // When level 1 uses the tiled sweep, the residual norm of x is computed by
 // the first pre-smoothing sweep of the next V-cycle, launched before the
 // test: if the test stops the solve, that sweep's output (in y) is unused.
-int mg_solve(nyl_ctx *c, nyl_mg_stats *st) {
+int mg_solve_fp64(nyl_ctx *c, nyl_mg_stats *st) {
     Level &f = c->lev[0];
     double *part = F<double>(c, c->partials), *res_dev = F<double>(c, c->result);
     const bool spec = sweep25_fits(f.g) && sweep25_blocks(f.g) <= NYL_RED_BLOCKS;
@@ -324,55 +395,101 @@ int mg_solve(nyl_ctx *c, nyl_mg_stats *st) {
         if (nite >= c->d.mg_maxite) break;
         if ((rc = normresidual(&res))) return rc;
     }
-    c->total_cycles += nite;
-    if (st) {
-        st->nite = nite;
-        st->res = res;
-        st->normb = normb;
-        st->total_cycles = c->total_cycles;
-    }
+    fill_stats(c, st, nite, res, normb);
     return NYL_OK;
 }
 
-DiagArgs<double> diag_args(nyl_ctx *c) {
-    DiagArgs<double> a;
+// Mixed precision: the same stop test and iteration count rule as mgfor's
+// solve, as defect correction.  The right-hand side's mean is removed in FP64
+// (compatibility of the singular Neumann/periodic problem); each outer step
+// computes r = b - A x in FP64, runs one FP32 V-cycle on A e = r from e = 0
+// and adds e to x in FP64.  The norms are those of mgfor (squared ratio).
+int mg_solve_mixed(nyl_ctx *c, nyl_mg_stats *st) {
+    Level &f = c->lev[0];
+    double *part = F<double>(c, c->partials), *res_dev = F<double>(c, c->result);
+    double *xo = F<double>(c, c->mgx), *bo = F<double>(c, c->mgb);
+    float *e = F<float>(c, f.x), *r = F<float>(c, f.b), *y = F<float>(c, f.y);
+    int rc;
+    double s2[2], rn, res;
+    HIPCHK(launch_sum_sumsq(f.g, bo, part, res_dev, c->stream));
+    if ((rc = read_results(c, s2, 2))) return rc;
+    const double n = cells(f.g);
+    const double shift = s2[0] / n;
+    double normb = s2[1] - n * shift * shift;
+    if (normb < 0.0) normb = 0.0;
+    c->mg_shift = shift;
+    auto outer = [&](double *out) -> int {
+        if (normb > 0.0) {
+            LAUNCH(NYL_K_RESNORM, (2 * sizeof(double) + 2 * sizeof(float)) * cells(f.g),
+                   launch_mg_outer_resid<float>(f.g, xo, bo, shift, r, y, c->d.mg_omega, c->tabs,
+                                                part, res_dev, c->stream));
+            int r2 = read_result(c, &rn);
+            if (r2) return r2;
+            *out = rn / normb;
+        } else {
+            *out = 0.0;
+        }
+        return NYL_OK;
+    };
+    if ((rc = outer(&res))) return rc;
+    int nite = 0;
+    for (;;) {
+        if (res < c->d.mg_tol) break;
+        if ((rc = vcycle(c, true))) return rc;
+        LAUNCH(NYL_K_MG_UPDATE, (2 * sizeof(double) + sizeof(float)) * cells(f.g),
+               launch_add_correction<float>(f.g, xo, e, c->stream));
+        nite = nite + 1;
+        if (nite >= c->d.mg_maxite) break;
+        if ((rc = outer(&res))) return rc;
+    }
+    fill_stats(c, st, nite, res, normb);
+    return NYL_OK;
+}
+
+int mg_solve(nyl_ctx *c, nyl_mg_stats *st) {
+    return c->mixed ? mg_solve_mixed(c, st) : mg_solve_fp64(c, st);
+}
+
+template <typename T>
+DiagArgs<T> diag_args(nyl_ctx *c) {
+    DiagArgs<T> a;
     const int s = c->cur;
-    a.u = F<double>(c, c->state[s][1]); a.v = F<double>(c, c->state[s][2]);
-    a.w = F<double>(c, c->state[s][3]);
-    a.Ux = F<double>(c, c->uc[0]); a.Uy = F<double>(c, c->uc[1]); a.Uz = F<double>(c, c->uc[2]);
-    a.ox = F<double>(c, c->vor[0]); a.oy = F<double>(c, c->vor[1]); a.oz = F<double>(c, c->vor[2]);
-    a.ke = F<double>(c, c->ke);
+    a.u = F<T>(c, c->state[s][1]); a.v = F<T>(c, c->state[s][2]);
+    a.w = F<T>(c, c->state[s][3]);
+    a.Ux = F<T>(c, c->uc[0]); a.Uy = F<T>(c, c->uc[1]); a.Uz = F<T>(c, c->uc[2]);
+    a.ox = F<T>(c, c->vor[0]); a.oy = F<T>(c, c->vor[1]); a.oz = F<T>(c, c->vor[2]);
+    a.ke = F<T>(c, c->ke);
     a.idx2 = c->d.idx2; a.idy2 = c->d.idy2; a.idz2 = c->d.idz2;
     a.fpar = c->d.fparameter > 0.0 ? c->d.fparameter : 0.0;
     return a;
 }
 
-int apply_op(nyl_ctx *c, int op) {
+template <typename T>
+int apply_op_t(nyl_ctx *c, int op) {
     const int s = c->cur;
-    double *u = F<double>(c, c->state[s][1]), *v = F<double>(c, c->state[s][2]),
-           *w = F<double>(c, c->state[s][3]);
+    T *u = F<T>(c, c->state[s][1]), *v = F<T>(c, c->state[s][2]), *w = F<T>(c, c->state[s][3]);
     switch (op) {
     case NYL_OP_U_FROM_U:
-        HIPCHK(launch_diag<double>(c->g, diag_args(c), 1, c->stream));
+        HIPCHK(launch_diag<T>(c->g, diag_args<T>(c), 1, c->stream));
         break;
     case NYL_OP_DIV:
-        LAUNCH(NYL_K_DIV, 4 * cells(c->g) * sizeof(double),
-               launch_div<double, double>(c->g, u, v, w, c->d.idx2, c->d.idy2, c->d.idz2,
-                                          F<double>(c, c->lev[0].b), c->stream));
+        LAUNCH(NYL_K_DIV, (3 * sizeof(T) + sizeof(double)) * cells(c->g),
+               launch_div<T, double>(c->g, u, v, w, c->d.idx2, c->d.idy2, c->d.idz2,
+                                     F<double>(c, c->mgb), c->stream));
         break;
     case NYL_OP_CORRECT:
-        LAUNCH(NYL_K_CORRECT, 7 * cells(c->g) * sizeof(double),
-               launch_correct<double, double>(c->g, u, v, w, F<double>(c, c->lev[0].x),
-                                              c->d.dx2, c->stream));
+        LAUNCH(NYL_K_CORRECT, (6 * sizeof(T) + sizeof(double)) * cells(c->g),
+               launch_correct<T, double>(c->g, u, v, w, F<double>(c, c->mgx), c->d.dx2,
+                                         c->stream));
         break;
     case NYL_OP_VORTICITY:
-        HIPCHK(launch_diag<double>(c->g, diag_args(c), 2, c->stream));
+        HIPCHK(launch_diag<T>(c->g, diag_args<T>(c), 2, c->stream));
         break;
     case NYL_OP_KINENERGY:
-        HIPCHK(launch_diag<double>(c->g, diag_args(c), 4, c->stream));
+        HIPCHK(launch_diag<T>(c->g, diag_args<T>(c), 4, c->stream));
         break;
     case NYL_OP_PRESSURE:
-        HIPCHK(launch_pressure<double, double>(c->g, F<double>(c, c->lev[0].x), c->d.dx2,
+        HIPCHK(launch_pressure<double, double>(c->g, F<double>(c, c->mgx), c->d.dx2,
                                                F<double>(c, c->p), c->stream));
         break;
     default:
@@ -381,14 +498,24 @@ int apply_op(nyl_ctx *c, int op) {
     return NYL_OK;
 }
 
+int apply_op(nyl_ctx *c, int op) {
+    return c->mixed ? apply_op_t<float>(c, op) : apply_op_t<double>(c, op);
+}
+
 // core/model_les.py:97-127 (halo fills are implicit in the wrapped reads)
-int diagnose(nyl_ctx *c, nyl_mg_stats *st) {
+template <typename T>
+int diagnose_t(nyl_ctx *c, nyl_mg_stats *st) {
     int rc;
-    if ((rc = apply_op(c, NYL_OP_DIV))) return rc;
+    if ((rc = apply_op_t<T>(c, NYL_OP_DIV))) return rc;
     if ((rc = mg_solve(c, st))) return rc;
-    if ((rc = apply_op(c, NYL_OP_CORRECT))) return rc;
-    LAUNCH(NYL_K_DIAG, 10 * cells(c->g) * sizeof(double), launch_diag<double>(c->g, diag_args(c), 7, c->stream));
+    if ((rc = apply_op_t<T>(c, NYL_OP_CORRECT))) return rc;
+    LAUNCH(NYL_K_DIAG, 10 * cells(c->g) * sizeof(T),
+           launch_diag<T>(c->g, diag_args<T>(c), 7, c->stream));
     return NYL_OK;
+}
+
+int diagnose(nyl_ctx *c, nyl_mg_stats *st) {
+    return c->mixed ? diagnose_t<float>(c, st) : diagnose_t<double>(c, st);
 }
 
 // words of HBM traffic per cell of each stage-kernel mode: reads b,u,v,w,
@@ -406,14 +533,51 @@ double stage_words(int mode) {
     }
 }
 
-int stage(nyl_ctx *c, int st, double coef, double coef2 = 0.0) {
-    StageArgs<double> a = stage_args(c);
-    if (st == 1 || st == 2) set_tendency_out(c, a, st - 1);
+// mode st of the stage kernel; slot >= 0 also writes the tendencies there
+template <typename T>
+int stage_t(nyl_ctx *c, int st, int slot, double coef, double coef2) {
+    StageArgs<T> a = stage_args<T>(c);
+    if (slot >= 0) set_tendency_out<T>(c, a, slot);
     a.c = coef;
     a.c2 = coef2;
-    LAUNCH(NYL_K_STAGE, stage_words(st) * cells(c->g) * sizeof(double),
-           launch_stage<double>(c->g, a, st, c->stream));
+    const double words = st == 0 ? 12 : stage_words(st);
+    LAUNCH(NYL_K_STAGE, words * cells(c->g) * sizeof(T), launch_stage<T>(c->g, a, st, c->stream));
+    return NYL_OK;
+}
+
+int stage(nyl_ctx *c, int st, double coef, double coef2 = 0.0) {
+    const int slot = (st == 1 || st == 2) ? st - 1 : -1;
+    int rc = c->mixed ? stage_t<float>(c, st, slot, coef, coef2)
+                      : stage_t<double>(c, st, slot, coef, coef2);
+    if (rc) return rc;
     if (st != 5) c->cur = 1 - c->cur;
+    return NYL_OK;
+}
+
+template <typename T>
+int reduce_t(nyl_ctx *c, int what) {
+    const int s = c->cur;
+    double *part = F<double>(c, c->partials), *res = F<double>(c, c->result);
+    T *b = F<T>(c, c->state[s][0]), *u = F<T>(c, c->state[s][1]), *v = F<T>(c, c->state[s][2]),
+      *w = F<T>(c, c->state[s][3]);
+    switch (what) {
+    case NYL_RED_MAXU2:
+        HIPCHK(launch_maxu2<T>(c->g, u, v, w, c->d.idx2, c->d.idy2, c->d.idz2, part, res,
+                               c->stream));
+        break;
+    case NYL_RED_KE:
+        HIPCHK(launch_kesum<T>(c->g, u, v, w, c->d.idx2, c->d.idy2, c->d.idz2, part, res,
+                               c->stream));
+        break;
+    case NYL_RED_BSUM:
+        HIPCHK(launch_sum<T>(c->g, b, part, res, c->stream));
+        break;
+    case NYL_RED_NONFINITE:
+        HIPCHK(launch_nonfinite<T>(c->g, b, u, v, w, part, res, c->stream));
+        break;
+    default:
+        return fail(NYL_E_ARG, "unknown reduction %d", what);
+    }
     return NYL_OK;
 }
 
@@ -429,8 +593,8 @@ int nyl_ctx_create(const nyl_desc *desc, nyl_ctx **out) {
     if (!desc || !out) return fail(NYL_E_ARG, "null argument");
     *out = nullptr;
     const nyl_desc &d = *desc;
-    if (d.precision != NYL_PREC_FP64)
-        return fail(NYL_E_ARG, "precision mode %d is not available in this build", d.precision);
+    if (d.precision != NYL_PREC_FP64 && d.precision != NYL_PREC_MIXED)
+        return fail(NYL_E_ARG, "unknown precision mode %d", d.precision);
     const int n3[3] = {d.nx, d.ny, d.nz};
     const int p3[3] = {d.periodic_x, d.periodic_y, d.periodic_z};
     for (int a = 0; a < 3; a++) {
@@ -444,7 +608,13 @@ int nyl_ctx_create(const nyl_desc *desc, nyl_ctx **out) {
     nyl_ctx *c = new nyl_ctx();
     memset(c, 0, sizeof(*c));
     c->d = d;
+    c->mixed = d.precision == NYL_PREC_MIXED;
     c->g = make_geo(d.nx, d.ny, d.nz, d.periodic_x, d.periodic_y, d.periodic_z);
+    if (c->mixed && !outer_resid_fits(c->g)) {
+        delete c;
+        return fail(NYL_E_SIZE, "grid %dx%dx%d is too wide for the mixed-precision solve",
+                    d.nx, d.ny, d.nz);
+    }
     c->ncell = (int64_t)d.nx * d.ny * d.nz;
     int rc = build_hierarchy(c);
     if (rc) { delete c; return rc; }
@@ -495,16 +665,17 @@ int nyl_field_offset(const nyl_ctx *c, int field, uint64_t *off, int64_t *nelem,
     if (!c || !off) return fail(NYL_E_ARG, "null argument");
     uint64_t o;
     const int s = c->cur;
+    bool f64 = !c->mixed;   // model fields follow the precision mode
     switch (field) {
     case NYL_F_B: case NYL_F_U: case NYL_F_V: case NYL_F_W: o = c->state[s][field - NYL_F_B]; break;
     case NYL_F_KE: o = c->ke; break;
     case NYL_F_VORX: case NYL_F_VORY: case NYL_F_VORZ: o = c->vor[field - NYL_F_VORX]; break;
     case NYL_F_UCX: case NYL_F_UCY: case NYL_F_UCZ: o = c->uc[field - NYL_F_UCX]; break;
-    case NYL_F_P: o = c->p; break;
-    case NYL_F_DIV: o = c->lev[0].b; break;
-    case NYL_F_MGX: o = c->lev[0].x; break;
-    case NYL_F_MGB: o = c->lev[0].b; break;
-    case NYL_F_MGY: o = c->lev[0].y; break;
+    case NYL_F_P: o = c->p; f64 = true; break;
+    case NYL_F_DIV: o = c->mgb; f64 = true; break;
+    case NYL_F_MGX: o = c->mgx; f64 = true; break;
+    case NYL_F_MGB: o = c->mgb; f64 = true; break;
+    case NYL_F_MGY: o = c->lev[0].y; f64 = lev64(c, 0); break;
     default:
         if (field >= NYL_F_DS0_B && field <= NYL_F_DS2_W) {
             const int q = field - NYL_F_DS0_B;
@@ -515,7 +686,7 @@ int nyl_field_offset(const nyl_ctx *c, int field, uint64_t *off, int64_t *nelem,
     }
     *off = o;
     if (nelem) *nelem = c->ncell;
-    if (dtype) *dtype = NYL_DT_F64;
+    if (dtype) *dtype = f64 ? NYL_DT_F64 : NYL_DT_F32;
     return NYL_OK;
 }
 
@@ -537,10 +708,7 @@ int nyl_rhs(nyl_ctx *c, int slot, int last) {
     if (rc) return rc;
     (void)last;   // viscosity at the last stage is not part of this model path
     if (slot < 0 || slot > 2) return fail(NYL_E_ARG, "slot must be 0..2");
-    StageArgs<double> a = stage_args(c);
-    set_tendency_out(c, a, slot);
-    LAUNCH(NYL_K_STAGE, 12 * cells(c->g) * sizeof(double), launch_stage<double>(c->g, a, 0, c->stream));
-    return NYL_OK;
+    return c->mixed ? stage_t<float>(c, 0, slot, 0.0, 0.0) : stage_t<double>(c, 0, slot, 0.0, 0.0);
 }
 
 int nyl_rk3_stage(nyl_ctx *c, int st, double coef) {
@@ -626,28 +794,7 @@ int nyl_reduce(nyl_ctx *c, int what, double *out) {
     int rc = check_ready(c);
     if (rc) return rc;
     if (!out) return fail(NYL_E_ARG, "null argument");
-    const int s = c->cur;
-    double *part = F<double>(c, c->partials), *res = F<double>(c, c->result);
-    switch (what) {
-    case NYL_RED_MAXU2:
-        HIPCHK(launch_maxu2<double>(c->g, F<double>(c, c->state[s][1]), F<double>(c, c->state[s][2]),
-                                    F<double>(c, c->state[s][3]), c->d.idx2, c->d.idy2, c->d.idz2,
-                                    part, res, c->stream));
-        break;
-    case NYL_RED_KE:
-        HIPCHK(launch_sum<double>(c->g, F<double>(c, c->ke), part, res, c->stream));
-        break;
-    case NYL_RED_BSUM:
-        HIPCHK(launch_sum<double>(c->g, F<double>(c, c->state[s][0]), part, res, c->stream));
-        break;
-    case NYL_RED_NONFINITE:
-        HIPCHK(launch_nonfinite<double>(c->g, F<double>(c, c->state[s][0]),
-                                        F<double>(c, c->state[s][1]), F<double>(c, c->state[s][2]),
-                                        F<double>(c, c->state[s][3]), part, res, c->stream));
-        break;
-    default:
-        return fail(NYL_E_ARG, "unknown reduction %d", what);
-    }
+    if ((rc = c->mixed ? reduce_t<float>(c, what) : reduce_t<double>(c, what))) return rc;
     return read_result(c, out);
 }
 
@@ -676,6 +823,7 @@ int nyl_prof_stop(nyl_ctx *c, double *total_ms, int64_t *launches, double *alg_b
 int nyl_debug_mg(nyl_ctx *c, int what) {
     int rc = check_ready(c);
     if (rc) return rc;
+    if (c->mixed) return fail(NYL_E_STATE, "nyl_debug_mg works on FP64 contexts");
     Level &f = c->lev[0];
     double *x = F<double>(c, f.x), *b = F<double>(c, f.b), *y = F<double>(c, f.y);
     double *part = F<double>(c, c->partials), *res_dev = F<double>(c, c->result);

@@ -99,17 +99,20 @@ __device__ __forceinline__ T weno5(T qmm, T qm, T q0, T qp, T qpp) {
 // the halo).  On a wall axis the closures are applied with the reference's
 // precedence (later assignments win): n-1 > n-2 > n-3 > {0, 1} > interior.
 // The stencil values are selected first and one reconstruction evaluated,
-// so a wavefront with mixed upwind directions runs WENO5 once.
-template <typename T, typename QF>
-__device__ __forceinline__ T face_flux(int f, int n, int per, T u, const QF &q) {
+// so a wavefront with mixed upwind directions runs WENO5 once.  PER is a
+// compile-time flag so that periodic axes compile to the interior path only.
+template <typename T, int PER, typename QF>
+__device__ __forceinline__ T face_flux(int f, int n, T u, const QF &q) {
     const bool pos = u > T(0);
-    int kind;
-    T a0 = T(0), a1 = T(0), a2 = T(0), a3 = T(0), a4 = T(0);
-    if (per || (f >= 2 && f <= n - 4)) {
-        kind = 5;
+    if (PER || (f >= 2 && f <= n - 4)) {
+        T a0, a1, a2, a3, a4;
         if (pos) { a0 = q(f - 2); a1 = q(f - 1); a2 = q(f); a3 = q(f + 1); a4 = q(f + 2); }
         else     { a0 = q(f + 3); a1 = q(f + 2); a2 = q(f + 1); a3 = q(f); a4 = q(f - 1); }
-    } else if (f == n - 1) {
+        return u * weno5<T>(a0, a1, a2, a3, a4);
+    }
+    int kind;
+    T a0 = T(0), a1 = T(0), a2 = T(0), a3 = T(0), a4 = T(0);
+    if (f == n - 1) {
         kind = 0;
     } else if (f == n - 2) {
         if (pos) { kind = 3; a0 = q(f - 1); a1 = q(f); a2 = q(f + 1); }

@@ -67,12 +67,32 @@ struct MGTabs {
 template <typename T>
 hipError_t launch_mg_sweep(const Geo &g, const T *src, T *dst, const T *b, double omega,
                            const MGTabs &tab, hipStream_t s);
-template <typename T>
-hipError_t launch_mg_resid_restrict(const Geo &gf, const Geo &gc, const T *x, const T *b,
-                                    T *bc, T *xc, const MGTabs &tab, hipStream_t s);
-template <typename T>
-hipError_t launch_mg_prolong(const Geo &gf, const Geo &gc, T *xf, const T *xc,
+// residual + restriction from a level of type Tf to the next of type Tc
+template <typename Tf, typename Tc>
+hipError_t launch_mg_resid_restrict(const Geo &gf, const Geo &gc, const Tf *x, const Tf *b,
+                                    Tc *bc, Tc *xc, const MGTabs &tab, hipStream_t s);
+template <typename Tf, typename Tc>
+hipError_t launch_mg_prolong(const Geo &gf, const Geo &gc, Tf *xf, const Tc *xc,
                              const MGTabs &tab, hipStream_t s);
+
+// mixed precision defect correction: outer FP64 residual of x (b - shift)
+// into the inner right-hand side r and the inner first sweep y, with the sum
+// of r**2 in dev_out[0]; x += e
+bool outer_resid_fits(const Geo &g);
+template <typename TI>
+hipError_t launch_mg_outer_resid(const Geo &g, const double *x, const double *b, double shift,
+                                 TI *r, TI *y, double omega, const MGTabs &tab, double *partials,
+                                 double *dev_out, hipStream_t s);
+template <typename TI>
+hipError_t launch_add_correction(const Geo &g, double *x, const TI *e, hipStream_t s);
+// dev_out[0] = sum v, dev_out[1] = sum v**2
+hipError_t launch_sum_sumsq(const Geo &g, const double *v, double *partials, double *dev_out,
+                            hipStream_t s);
+// dev_out[0] = sum of the FP64 kinetic-energy field of u, v, w
+template <typename T>
+hipError_t launch_kesum(const Geo &g, const T *u, const T *v, const T *w, double idx2,
+                        double idy2, double idz2, double *partials, double *dev_out,
+                        hipStream_t s);
 
 // One fsmoother3d sweep with 2.5-D LDS tiling (levels with nx % 64 == 0 and
 // ny % 8 == 0).  partials: also reduce the squared residual of src into

@@ -83,11 +83,11 @@ __device__ __forceinline__ T residual_at(const Geo &g, const T *__restrict__ x,
 // residual + restriction(rb)); frestrict_centers3d
 // (core/mgfor/basicoperators.f90:32-60) with Rcoef = 1/2; coarse x = 0
 // (core/mgfor/operators.f90:209).
-template <typename T>
+template <typename Tf, typename Tc>
 __global__ __launch_bounds__(256) void k_mg_resid_restrict(Geo gf, Geo gc,
-                                                           const T *__restrict__ x,
-                                                           const T *__restrict__ b,
-                                                           T *__restrict__ bc, T *__restrict__ xc,
+                                                           const Tf *__restrict__ x,
+                                                           const Tf *__restrict__ b,
+                                                           Tc *__restrict__ bc, Tc *__restrict__ xc,
                                                            MGTabs tab) {
     const Blk3 bk = xcd_block();
     const int ic = bk.x * MG_BX + threadIdx.x;
@@ -95,33 +95,33 @@ __global__ __launch_bounds__(256) void k_mg_resid_restrict(Geo gf, Geo gc,
     const int kc = bk.z;
     if (ic >= gc.nx || jc >= gc.ny) return;
     const int i = 2 * ic, j = 2 * jc, k = 2 * kc;
-    const T r000 = residual_at<T>(gf, x, b, i, j, k, tab);
-    const T r100 = residual_at<T>(gf, x, b, i + 1, j, k, tab);
-    const T r010 = residual_at<T>(gf, x, b, i, j + 1, k, tab);
-    const T r110 = residual_at<T>(gf, x, b, i + 1, j + 1, k, tab);
-    const T r001 = residual_at<T>(gf, x, b, i, j, k + 1, tab);
-    const T r101 = residual_at<T>(gf, x, b, i + 1, j, k + 1, tab);
-    const T r011 = residual_at<T>(gf, x, b, i, j + 1, k + 1, tab);
-    const T r111 = residual_at<T>(gf, x, b, i + 1, j + 1, k + 1, tab);
+    const Tf r000 = residual_at<Tf>(gf, x, b, i, j, k, tab);
+    const Tf r100 = residual_at<Tf>(gf, x, b, i + 1, j, k, tab);
+    const Tf r010 = residual_at<Tf>(gf, x, b, i, j + 1, k, tab);
+    const Tf r110 = residual_at<Tf>(gf, x, b, i + 1, j + 1, k, tab);
+    const Tf r001 = residual_at<Tf>(gf, x, b, i, j, k + 1, tab);
+    const Tf r101 = residual_at<Tf>(gf, x, b, i + 1, j, k + 1, tab);
+    const Tf r011 = residual_at<Tf>(gf, x, b, i, j + 1, k + 1, tab);
+    const Tf r111 = residual_at<Tf>(gf, x, b, i + 1, j + 1, k + 1, tab);
     const int64_t cc = (int64_t)kc * gc.sz + (int64_t)jc * gc.sy + ic;
-    bc[cc] = T(0.5) * (r000 + r100 + r010 + r110 + r001 + r101 + r011 + r111);
-    xc[cc] = T(0);
+    bc[cc] = Tc(Tf(0.5) * (r000 + r100 + r010 + r110 + r001 + r101 + r011 + r111));
+    xc[cc] = Tc(0);
 }
 
-template <typename T>
-hipError_t launch_mg_resid_restrict(const Geo &gf, const Geo &gc, const T *x, const T *b, T *bc,
-                                    T *xc, const MGTabs &tab, hipStream_t s) {
-    hipLaunchKernelGGL((k_mg_resid_restrict<T>), mg_grid(gc), dim3(MG_BX, MG_BY, 1), 0, s, gf, gc,
-                       x, b, bc, xc, tab);
+template <typename Tf, typename Tc>
+hipError_t launch_mg_resid_restrict(const Geo &gf, const Geo &gc, const Tf *x, const Tf *b,
+                                    Tc *bc, Tc *xc, const MGTabs &tab, hipStream_t s) {
+    hipLaunchKernelGGL((k_mg_resid_restrict<Tf, Tc>), mg_grid(gc), dim3(MG_BX, MG_BY, 1), 0, s,
+                       gf, gc, x, b, bc, xc, tab);
     return hipGetLastError();
 }
 
 // fprolongation_centers3d (core/mgfor/basicoperators.f90:173-231) at one
 // fine cell: Pcoef*(3*b + a) with the (9,3,3,1) planar weights, b the
 // parent's coarse plane and a the coarse plane on the child's side.
-template <typename T>
-__device__ __forceinline__ T prolong_term(const Geo &gc, const T *__restrict__ xc,
-                                          const MGTabs &tab, int i, int j, int k) {
+template <typename Tf, typename Tc>
+__device__ __forceinline__ Tf prolong_term(const Geo &gc, const Tc *__restrict__ xc,
+                                           const MGTabs &tab, int i, int j, int k) {
     const int ic = i >> 1, jc = j >> 1, kc = k >> 1;
     // coarse neighbour on the side of the fine cell within its parent
     int icn = (i & 1) ? ic + 1 : ic - 1;
@@ -131,36 +131,36 @@ __device__ __forceinline__ T prolong_term(const Geo &gc, const T *__restrict__ x
     if (gc.px) icn = wrapi(icn, gc.nx); else if (icn < 0 || icn >= gc.nx) mx = 0;
     if (gc.py) jcn = wrapi(jcn, gc.ny); else if (jcn < 0 || jcn >= gc.ny) my = 0;
     if (gc.pz) kcn = wrapi(kcn, gc.nz); else if (kcn < 0 || kcn >= gc.nz) mz = 0;
-    auto X = [&](int a, int bb, int cc, bool ok) -> T {
-        return ok ? xc[(int64_t)cc * gc.sz + (int64_t)bb * gc.sy + a] : T(0);
+    auto X = [&](int a, int bb, int cc, bool ok) -> Tf {
+        return ok ? Tf(xc[(int64_t)cc * gc.sz + (int64_t)bb * gc.sy + a]) : Tf(0);
     };
-    auto A = [&](int kk, bool okz) -> T {
-        return T(9) * X(ic, jc, kk, okz) + T(3) * X(icn, jc, kk, okz && mx)
-             + T(3) * X(ic, jcn, kk, okz && my) + X(icn, jcn, kk, okz && mx && my);
+    auto A = [&](int kk, bool okz) -> Tf {
+        return Tf(9) * X(ic, jc, kk, okz) + Tf(3) * X(icn, jc, kk, okz && mx)
+             + Tf(3) * X(ic, jcn, kk, okz && my) + X(icn, jcn, kk, okz && mx && my);
     };
-    const T bmid = A(kc, true);
-    const T aside = A(kcn, mz != 0);
-    return T(tab.pcoef[mx + 2 * my + 4 * mz]) * (T(3) * bmid + aside);
+    const Tf bmid = A(kc, true);
+    const Tf aside = A(kcn, mz != 0);
+    return Tf(tab.pcoef[mx + 2 * my + 4 * mz]) * (Tf(3) * bmid + aside);
 }
 
 // one thread per fine cell: xf = xf + Pcoef*(3*b + a)
-template <typename T>
-__global__ __launch_bounds__(256) void k_mg_prolong(Geo gf, Geo gc, T *__restrict__ xf,
-                                                    const T *__restrict__ xc, MGTabs tab) {
+template <typename Tf, typename Tc>
+__global__ __launch_bounds__(256) void k_mg_prolong(Geo gf, Geo gc, Tf *__restrict__ xf,
+                                                    const Tc *__restrict__ xc, MGTabs tab) {
     const Blk3 bk = xcd_block();
     const int i = bk.x * MG_BX + threadIdx.x;
     const int j = bk.y * MG_BY + threadIdx.y;
     const int k = bk.z;
     if (i >= gf.nx || j >= gf.ny) return;
     const int64_t c = (int64_t)k * gf.sz + (int64_t)j * gf.sy + i;
-    xf[c] = xf[c] + prolong_term<T>(gc, xc, tab, i, j, k);
+    xf[c] = xf[c] + prolong_term<Tf, Tc>(gc, xc, tab, i, j, k);
 }
 
-template <typename T>
-hipError_t launch_mg_prolong(const Geo &gf, const Geo &gc, T *xf, const T *xc, const MGTabs &tab,
-                             hipStream_t s) {
-    hipLaunchKernelGGL((k_mg_prolong<T>), mg_grid(gf), dim3(MG_BX, MG_BY, 1), 0, s, gf, gc, xf,
-                       xc, tab);
+template <typename Tf, typename Tc>
+hipError_t launch_mg_prolong(const Geo &gf, const Geo &gc, Tf *xf, const Tc *xc,
+                             const MGTabs &tab, hipStream_t s) {
+    hipLaunchKernelGGL((k_mg_prolong<Tf, Tc>), mg_grid(gf), dim3(MG_BX, MG_BY, 1), 0, s, gf, gc,
+                       xf, xc, tab);
     return hipGetLastError();
 }
 
@@ -442,6 +442,165 @@ hipError_t launch_nonfinite(const Geo &g, const T *a, const T *b, const T *c, co
     return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// Mixed precision: multigrid as defect correction.  The outer iteration keeps
+// x and b in FP64; each correction e solves A e = r with one FP32 V-cycle
+// from e = 0, and x += e in FP64.  A V-cycle with linear smoothers is the
+// affine map x -> x + B(b - A x), so this is the same iteration as the FP64
+// solve up to the precision of B.
+
+// Outer residual r = (b - shift) + diag*x - sum6(x) in FP64 (shift: the mean
+// of b removed for compatibility), stored in the inner precision as the
+// right-hand side of the inner V-cycle; y receives that V-cycle's first
+// pre-smoothing sweep from e = 0 (k_mg_sweep with src = 0:
+// cff1*0 + omega*(0 - r)*idiag), and the blocks sum r**2 in FP64.  Each
+// block takes the planes blockIdx.z, blockIdx.z + gridDim.z, ... so that the
+// grid stays within NYL_RED_BLOCKS partial sums.
+template <typename TI>
+__global__ __launch_bounds__(256) void k_mg_outer_resid(Geo g, const double *__restrict__ x,
+                                                        const double *__restrict__ b,
+                                                        double shift, TI *__restrict__ r_out,
+                                                        TI *__restrict__ y_out, double omega,
+                                                        MGTabs tab,
+                                                        double *__restrict__ partials) {
+    const int i = blockIdx.x * MG_BX + threadIdx.x;
+    const int j = blockIdx.y * MG_BY + threadIdx.y;
+    const int tid = threadIdx.y * MG_BX + threadIdx.x;
+    const TI om = TI(omega), cff1 = TI(1.0 - omega);
+    double acc = 0.0;
+    if (i < g.nx && j < g.ny) {
+        for (int k = blockIdx.z; k < g.nz; k += gridDim.z) {
+            const int64_t c = (int64_t)k * g.sz + (int64_t)j * g.sy + i;
+            const Nbrs<double> n = neighbours<double>(g, x, i, j, k, c);
+            const double r = (b[c] - shift) + tab.diag[n.nwet] * x[c]
+                           - (n.xm + n.xp + n.ym + n.yp + n.zm + n.zp);
+            acc = acc + r * r;
+            const TI rf = TI(r);
+            r_out[c] = rf;
+            y_out[c] = cff1 * TI(0) + om * (TI(0) - rf) * TI(tab.idiag[n.nwet]);
+        }
+    }
+    const int nb = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+    block_sum_store(acc, partials + nb, tid);
+}
+
+static dim3 outer_grid(const Geo &g) {
+    dim3 grd = mg_grid(g);
+    const int64_t cols = (int64_t)grd.x * grd.y;
+    int64_t gz = NYL_RED_BLOCKS / cols;
+    if (gz > g.nz) gz = g.nz;
+    if (gz < 1) gz = 1;
+    grd.z = (unsigned)gz;
+    return grd;
+}
+
+bool outer_resid_fits(const Geo &g) {
+    const dim3 grd = mg_grid(g);
+    return (int64_t)grd.x * grd.y <= NYL_RED_BLOCKS;
+}
+
+template <typename TI>
+hipError_t launch_mg_outer_resid(const Geo &g, const double *x, const double *b, double shift,
+                                 TI *r, TI *y, double omega, const MGTabs &tab, double *partials,
+                                 double *dev_out, hipStream_t s) {
+    const dim3 grd = outer_grid(g);
+    hipLaunchKernelGGL((k_mg_outer_resid<TI>), grd, dim3(MG_BX, MG_BY, 1), 0, s, g, x, b, shift,
+                       r, y, omega, tab, partials);
+    const int nb = grd.x * grd.y * grd.z;
+    hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(RED_T), 0, s, partials, nb, dev_out);
+    return hipGetLastError();
+}
+
+// x = x + e (the outer correction, in FP64)
+template <typename TI>
+__global__ __launch_bounds__(256) void k_add_correction(int64_t n, double *__restrict__ x,
+                                                        const TI *__restrict__ e) {
+    const int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (q < n) x[q] = x[q] + (double)e[q];
+}
+
+template <typename TI>
+hipError_t launch_add_correction(const Geo &g, double *x, const TI *e, hipStream_t s) {
+    const int64_t n = (int64_t)g.nx * g.ny * g.nz;
+    hipLaunchKernelGGL((k_add_correction<TI>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
+                       n, x, e);
+    return hipGetLastError();
+}
+
+// sum and sum of squares in one pass: dev_out[0] = sum v, dev_out[1] = sum v**2
+__global__ __launch_bounds__(RED_T) void k_sum_sumsq(int64_t n, const double *__restrict__ v,
+                                                     double *__restrict__ partials, int nb) {
+    double a1 = 0.0, a2 = 0.0;
+    for (int64_t q = (int64_t)blockIdx.x * RED_T + threadIdx.x; q < n;
+         q += (int64_t)gridDim.x * RED_T) {
+        const double x = v[q];
+        a1 = a1 + x;
+        a2 = a2 + x * x;
+    }
+    block_sum_store(a1, partials + blockIdx.x, threadIdx.x);
+    __syncthreads();
+    block_sum_store(a2, partials + nb + blockIdx.x, threadIdx.x);
+}
+
+hipError_t launch_sum_sumsq(const Geo &g, const double *v, double *partials, double *dev_out,
+                            hipStream_t s) {
+    const int64_t n = (int64_t)g.nx * g.ny * g.nz;
+    int nb = red_blocks(n);
+    if (nb > NYL_RED_BLOCKS / 2) nb = NYL_RED_BLOCKS / 2;
+    hipLaunchKernelGGL(k_sum_sumsq, dim3(nb), dim3(RED_T), 0, s, n, v, partials, nb);
+    hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(RED_T), 0, s, partials, nb, dev_out);
+    hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(RED_T), 0, s, partials + nb, nb, dev_out + 1);
+    return hipGetLastError();
+}
+
+// Sum of the kinetic energy field evaluated in FP64 from u, v, w with the
+// formula of core/fortran_kinenergy.f90:145-153 (as the diagnostic kernel):
+// the energy diagnostic of both precision modes.
+template <typename T>
+__global__ __launch_bounds__(RED_T) void k_kesum(Geo g, const T *__restrict__ u,
+                                                 const T *__restrict__ v,
+                                                 const T *__restrict__ w, double idx2,
+                                                 double idy2, double idz2,
+                                                 double *__restrict__ partials) {
+    const int64_t n = (int64_t)g.nx * g.ny * g.nz;
+    const double cx = 0.5 * idx2, cy = 0.5 * idy2, cz = 0.5 * idz2;
+    double acc = 0.0;
+    for (int64_t q = (int64_t)blockIdx.x * RED_T + threadIdx.x; q < n;
+         q += (int64_t)gridDim.x * RED_T) {
+        const int i = (int)(q % g.nx);
+        const int64_t t = q / g.nx;
+        const int j = (int)(t % g.ny);
+        const int k = (int)(t / g.ny);
+        double ke = 0.0;
+        if (g.px || i > 0) {
+            const double a = u[q], m = u[q - (g.px ? (int64_t)(i - wrapi(i - 1, g.nx)) : 1)];
+            ke = ke + cx * 0.5 * (a * a + m * m);
+        }
+        if (g.py || j > 0) {
+            const double a = v[q], m = v[q - (g.py ? (int64_t)(j - wrapi(j - 1, g.ny)) * g.sy : g.sy)];
+            ke = ke + cy * 0.5 * (a * a + m * m);
+        }
+        if (g.pz || k > 0) {
+            const double a = w[q], m = w[q - (g.pz ? (int64_t)(k - wrapi(k - 1, g.nz)) * g.sz : g.sz)];
+            ke = ke + cz * 0.5 * (a * a + m * m);
+        }
+        acc = acc + ke;
+    }
+    block_sum_store(acc, partials + blockIdx.x, threadIdx.x);
+}
+
+template <typename T>
+hipError_t launch_kesum(const Geo &g, const T *u, const T *v, const T *w, double idx2,
+                        double idy2, double idz2, double *partials, double *dev_out,
+                        hipStream_t s) {
+    const int64_t n = (int64_t)g.nx * g.ny * g.nz;
+    const int nb = red_blocks(n);
+    hipLaunchKernelGGL((k_kesum<T>), dim3(nb), dim3(RED_T), 0, s, g, u, v, w, idx2, idy2, idz2,
+                       partials);
+    hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(RED_T), 0, s, partials, nb, dev_out);
+    return hipGetLastError();
+}
+
 bool sweep25_fits(const Geo &g) {
     return g.nx % SW_TX == 0 && g.ny % SW_TY == 0 && g.nx >= SW_TX;
 }
@@ -476,27 +635,41 @@ hipError_t launch_mg_sweep25(const Geo &g, const T *src, T *dst, const T *b, dou
     return hipGetLastError();
 }
 
-template hipError_t launch_mg_sweep25<double>(const Geo &, const double *, double *,
-                                              const double *, double, const MGTabs &, double *,
-                                              double *, hipStream_t);
-template hipError_t launch_mg_sweep<double>(const Geo &, const double *, double *, const double *,
-                                            double, const MGTabs &, hipStream_t);
-template hipError_t launch_mg_resid_restrict<double>(const Geo &, const Geo &, const double *,
-                                                     const double *, double *, double *,
-                                                     const MGTabs &, hipStream_t);
-template hipError_t launch_mg_prolong<double>(const Geo &, const Geo &, double *, const double *,
-                                              const MGTabs &, hipStream_t);
-template hipError_t launch_norm2<double>(const Geo &, const double *, double *, double *,
-                                         hipStream_t);
+#define NYL_MG_INST(T)                                                                          \
+    template hipError_t launch_mg_sweep25<T>(const Geo &, const T *, T *, const T *, double,    \
+                                             const MGTabs &, double *, double *, hipStream_t);  \
+    template hipError_t launch_mg_sweep<T>(const Geo &, const T *, T *, const T *, double,      \
+                                           const MGTabs &, hipStream_t);                        \
+    template hipError_t launch_mg_resid_restrict<T, T>(const Geo &, const Geo &, const T *,     \
+                                                       const T *, T *, T *, const MGTabs &,     \
+                                                       hipStream_t);                            \
+    template hipError_t launch_mg_prolong<T, T>(const Geo &, const Geo &, T *, const T *,       \
+                                                const MGTabs &, hipStream_t);                   \
+    template hipError_t launch_norm2<T>(const Geo &, const T *, double *, double *, hipStream_t); \
+    template hipError_t launch_maxu2<T>(const Geo &, const T *, const T *, const T *, double,   \
+                                        double, double, double *, double *, hipStream_t);       \
+    template hipError_t launch_sum<T>(const Geo &, const T *, double *, double *, hipStream_t); \
+    template hipError_t launch_nonfinite<T>(const Geo &, const T *, const T *, const T *,       \
+                                            const T *, double *, double *, hipStream_t);        \
+    template hipError_t launch_kesum<T>(const Geo &, const T *, const T *, const T *, double,   \
+                                        double, double, double *, double *, hipStream_t);
+NYL_MG_INST(double)
+NYL_MG_INST(float)
+#undef NYL_MG_INST
+
 template hipError_t launch_resnorm2<double>(const Geo &, const double *, const double *,
                                             const MGTabs &, double *, double *, hipStream_t);
-template hipError_t launch_maxu2<double>(const Geo &, const double *, const double *,
-                                         const double *, double, double, double, double *,
-                                         double *, hipStream_t);
-template hipError_t launch_sum<double>(const Geo &, const double *, double *, double *,
-                                       hipStream_t);
-template hipError_t launch_nonfinite<double>(const Geo &, const double *, const double *,
-                                             const double *, const double *, double *, double *,
-                                             hipStream_t);
+// FP32 inner levels above an FP64 coarsest level
+template hipError_t launch_mg_resid_restrict<float, double>(const Geo &, const Geo &,
+                                                            const float *, const float *,
+                                                            double *, double *, const MGTabs &,
+                                                            hipStream_t);
+template hipError_t launch_mg_prolong<float, double>(const Geo &, const Geo &, float *,
+                                                     const double *, const MGTabs &, hipStream_t);
+template hipError_t launch_mg_outer_resid<float>(const Geo &, const double *, const double *,
+                                                 double, float *, float *, double,
+                                                 const MGTabs &, double *, double *, hipStream_t);
+template hipError_t launch_add_correction<float>(const Geo &, double *, const float *,
+                                                 hipStream_t);
 
 }  // namespace nyl

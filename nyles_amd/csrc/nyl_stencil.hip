@@ -11,6 +11,13 @@ namespace nyl {
 #define NYL_BX 64
 #define NYL_BY 4
 
+// Minimum waves per SIMD for the stage kernel: caps its registers so that
+// enough waves are resident to hide the latency of its ~90 loads per cell
+// (uncapped, the compiler hoists them all and the kernel runs 2 waves/SIMD).
+#ifndef NYL_STAGE_WAVES
+#define NYL_STAGE_WAVES 4
+#endif
+
 static inline dim3 cell_grid(const Geo &g) {
     return dim3((g.nx + NYL_BX - 1) / NYL_BX, (g.ny + NYL_BY - 1) / NYL_BY, g.nz);
 }
@@ -21,10 +28,9 @@ static inline dim3 cell_grid(const Geo &g) {
 // target axis e (UU) and then along d (u1d).  `based` is the index of the
 // cell's line start along d, `sd` the stride of d, `md` the cell's position
 // on that line, `oe` the offset to the e+1 neighbour.
-template <typename T>
+template <typename T, int pd>
 __device__ __forceinline__ T vf_flux(const T *__restrict__ Ud, const T *__restrict__ om,
-                                     int64_t based, int64_t sd, int md, int nd, int pd,
-                                     int64_t oe) {
+                                     int64_t based, int64_t sd, int md, int nd, int64_t oe) {
     auto UU = [&](int m) -> T {
         const int64_t p = based + (int64_t)m * sd;
         return T(0.5) * (Ud[p] + Ud[p + oe]);
@@ -38,27 +44,28 @@ __device__ __forceinline__ T vf_flux(const T *__restrict__ Ud, const T *__restri
         if (m == 0) return T(0);
         return om[based + (int64_t)(m - 1) * sd];
     };
-    return face_flux<T>(md, nd, pd, u1d, q);
+    return face_flux<T, pd>(md, nd, u1d, q);
 }
 
 // Tracer flux divergence along one axis (core/fortran_upwind.f90:228-234):
 // d = d + flux(m-1) - flux(m), or d - flux(1) at a wall start.
-template <typename T>
+template <typename T, int pd>
 __device__ __forceinline__ T tracer_axis(T d, const T *__restrict__ q_arr,
                                          const T *__restrict__ U, int64_t based, int64_t sd,
-                                         int md, int nd, int pd) {
+                                         int md, int nd) {
     auto q = [&](int m) -> T { return q_arr[based + (int64_t)(pd ? wrapi(m, nd) : m) * sd]; };
-    const T fr = face_flux<T>(md, nd, pd, U[based + (int64_t)md * sd], q);
+    const T fr = face_flux<T, pd>(md, nd, U[based + (int64_t)md * sd], q);
     if (!pd && md == 0) return d - fr;
     const int ml = pd ? wrapi(md - 1, nd) : md - 1;
-    const T fl = face_flux<T>(ml, nd, pd, U[based + (int64_t)ml * sd], q);
+    const T fl = face_flux<T, pd>(ml, nd, U[based + (int64_t)ml * sd], q);
     return (d + fl) - fr;
 }
 
 // Fused LES.rhs (core/model_les.py:129-144) + RK3_SSP stage update
 // (core/timescheme.py:422-443) for one cell.
-template <typename T, int MODE>
-__global__ __launch_bounds__(256) void k_stage(Geo g, StageArgs<T> a) {
+template <typename T, int MODE, int PX, int PY, int PZ>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NYL_STAGE_WAVES)))
+void k_stage(Geo g, StageArgs<T> a) {
     const Blk3 bk = xcd_block();
     const int i = bk.x * NYL_BX + threadIdx.x;
     const int j = bk.y * NYL_BY + threadIdx.y;
@@ -69,24 +76,24 @@ __global__ __launch_bounds__(256) void k_stage(Geo g, StageArgs<T> a) {
 
     // tracer b: rhstrac, directions i, j, k (core/tracer.py:63-72)
     T db = T(0);
-    db = tracer_axis<T>(db, a.b, a.Ux, bx, 1, i, g.nx, g.px);
-    db = tracer_axis<T>(db, a.b, a.Uy, by, g.sy, j, g.ny, g.py);
-    db = tracer_axis<T>(db, a.b, a.Uz, bz, g.sz, k, g.nz, g.pz);
+    db = tracer_axis<T, PX>(db, a.b, a.Ux, bx, 1, i, g.nx);
+    db = tracer_axis<T, PY>(db, a.b, a.Uy, by, g.sy, j, g.ny);
+    db = tracer_axis<T, PZ>(db, a.b, a.Uz, bz, g.sz, k, g.nz);
 
     // vortex force, triplets 'ikj', 'jik', 'kji' (core/vortex_force.py:69-81)
-    const int64_t oex = g.px ? (int64_t)(wrapi(i + 1, g.nx) - i) : 1;
-    const int64_t oey = g.py ? (int64_t)(wrapi(j + 1, g.ny) - j) * g.sy : g.sy;
-    const int64_t oez = g.pz ? (int64_t)(wrapi(k + 1, g.nz) - k) * g.sz : g.sz;
-    const bool ex = g.px || i < g.nx - 1;   // loops stop at n-1 on a wall axis
-    const bool ey = g.py || j < g.ny - 1;
-    const bool ez = g.pz || k < g.nz - 1;
+    const int64_t oex = PX ? (int64_t)(wrapi(i + 1, g.nx) - i) : 1;
+    const int64_t oey = PY ? (int64_t)(wrapi(j + 1, g.ny) - j) * g.sy : g.sy;
+    const int64_t oez = PZ ? (int64_t)(wrapi(k + 1, g.nz) - k) * g.sz : g.sz;
+    const bool ex = PX || i < g.nx - 1;   // loops stop at n-1 on a wall axis
+    const bool ey = PY || j < g.ny - 1;
+    const bool ez = PZ || k < g.nz - 1;
     T du = T(0), dv = T(0), dw = T(0);
-    if (ey) dv = dv - vf_flux<T>(a.Ux, a.oz, bx, 1, i, g.nx, g.px, oey);      // direc(U_i, vor_k, du_j)
-    if (ex) du = du + vf_flux<T>(a.Uy, a.oz, by, g.sy, j, g.ny, g.py, oex);   // flip (U_j, vor_k, du_i)
-    if (ez) dw = dw - vf_flux<T>(a.Uy, a.ox, by, g.sy, j, g.ny, g.py, oez);   // direc(U_j, vor_i, du_k)
-    if (ey) dv = dv + vf_flux<T>(a.Uz, a.ox, bz, g.sz, k, g.nz, g.pz, oey);   // flip (U_k, vor_i, du_j)
-    if (ex) du = du - vf_flux<T>(a.Uz, a.oy, bz, g.sz, k, g.nz, g.pz, oex);   // direc(U_k, vor_j, du_i)
-    if (ez) dw = dw + vf_flux<T>(a.Ux, a.oy, bx, 1, i, g.nx, g.px, oez);      // flip (U_i, vor_j, du_k)
+    if (ey) dv = dv - vf_flux<T, PX>(a.Ux, a.oz, bx, 1, i, g.nx, oey);      // direc(U_i, vor_k, du_j)
+    if (ex) du = du + vf_flux<T, PY>(a.Uy, a.oz, by, g.sy, j, g.ny, oex);   // flip (U_j, vor_k, du_i)
+    if (ez) dw = dw - vf_flux<T, PY>(a.Uy, a.ox, by, g.sy, j, g.ny, oez);   // direc(U_j, vor_i, du_k)
+    if (ey) dv = dv + vf_flux<T, PZ>(a.Uz, a.ox, bz, g.sz, k, g.nz, oey);   // flip (U_k, vor_i, du_j)
+    if (ex) du = du - vf_flux<T, PZ>(a.Uz, a.oy, bz, g.sz, k, g.nz, oex);   // direc(U_k, vor_j, du_i)
+    if (ez) dw = dw + vf_flux<T, PX>(a.Ux, a.oy, bx, 1, i, g.nx, oez);      // flip (U_i, vor_j, du_k)
 
     // bernoulli (core/bernoulli.py:14-23, core/fortran_bernoulli.f90:2-58)
     const T kec = a.ke[c];
@@ -138,18 +145,33 @@ __global__ __launch_bounds__(256) void k_stage(Geo g, StageArgs<T> a) {
     }
 }
 
-template <typename T>
-hipError_t launch_stage(const Geo &g, const StageArgs<T> &a, int mode, hipStream_t s) {
+template <typename T, int PX, int PY, int PZ>
+static void launch_stage_p(const Geo &g, const StageArgs<T> &a, int mode, hipStream_t s) {
     dim3 blk(NYL_BX, NYL_BY, 1), grd = cell_grid(g);
     switch (mode) {
-    case 0: hipLaunchKernelGGL((k_stage<T, 0>), grd, blk, 0, s, g, a); break;
-    case 1: hipLaunchKernelGGL((k_stage<T, 1>), grd, blk, 0, s, g, a); break;
-    case 2: hipLaunchKernelGGL((k_stage<T, 2>), grd, blk, 0, s, g, a); break;
-    case 3: hipLaunchKernelGGL((k_stage<T, 3>), grd, blk, 0, s, g, a); break;
-    case 4: hipLaunchKernelGGL((k_stage<T, 4>), grd, blk, 0, s, g, a); break;
-    case 5: hipLaunchKernelGGL((k_stage<T, 5>), grd, blk, 0, s, g, a); break;
-    case 6: hipLaunchKernelGGL((k_stage<T, 6>), grd, blk, 0, s, g, a); break;
-    default: return hipErrorInvalidValue;
+    case 0: hipLaunchKernelGGL((k_stage<T, 0, PX, PY, PZ>), grd, blk, 0, s, g, a); break;
+    case 1: hipLaunchKernelGGL((k_stage<T, 1, PX, PY, PZ>), grd, blk, 0, s, g, a); break;
+    case 2: hipLaunchKernelGGL((k_stage<T, 2, PX, PY, PZ>), grd, blk, 0, s, g, a); break;
+    case 3: hipLaunchKernelGGL((k_stage<T, 3, PX, PY, PZ>), grd, blk, 0, s, g, a); break;
+    case 4: hipLaunchKernelGGL((k_stage<T, 4, PX, PY, PZ>), grd, blk, 0, s, g, a); break;
+    case 5: hipLaunchKernelGGL((k_stage<T, 5, PX, PY, PZ>), grd, blk, 0, s, g, a); break;
+    case 6: hipLaunchKernelGGL((k_stage<T, 6, PX, PY, PZ>), grd, blk, 0, s, g, a); break;
+    default: break;
+    }
+}
+
+template <typename T>
+hipError_t launch_stage(const Geo &g, const StageArgs<T> &a, int mode, hipStream_t s) {
+    if (mode < 0 || mode > 6) return hipErrorInvalidValue;
+    switch (g.px + 2 * g.py + 4 * g.pz) {
+    case 0: launch_stage_p<T, 0, 0, 0>(g, a, mode, s); break;
+    case 1: launch_stage_p<T, 1, 0, 0>(g, a, mode, s); break;
+    case 2: launch_stage_p<T, 0, 1, 0>(g, a, mode, s); break;
+    case 3: launch_stage_p<T, 1, 1, 0>(g, a, mode, s); break;
+    case 4: launch_stage_p<T, 0, 0, 1>(g, a, mode, s); break;
+    case 5: launch_stage_p<T, 1, 0, 1>(g, a, mode, s); break;
+    case 6: launch_stage_p<T, 0, 1, 1>(g, a, mode, s); break;
+    default: launch_stage_p<T, 1, 1, 1>(g, a, mode, s); break;
     }
     return hipGetLastError();
 }
@@ -231,18 +253,19 @@ __global__ __launch_bounds__(256) void k_div(Geo g, const T *__restrict__ u,
     const int k = bk.z;
     if (i >= g.nx || j >= g.ny) return;
     const int64_t c = (int64_t)k * g.sz + (int64_t)j * g.sy + i;
-    const T cx = T(idx2), cy = T(idy2), cz = T(idz2);
-    T d;
-    const T Uc = u[c] * cx;
+    // evaluated in the output precision (FP64 right-hand side of the mixed path)
+    const TO cx = TO(idx2), cy = TO(idy2), cz = TO(idz2);
+    TO d;
+    const TO Uc = TO(u[c]) * cx;
     if (!g.px && i == 0) d = Uc;
-    else d = Uc - u[c - (g.px ? (int64_t)(i - wrapi(i - 1, g.nx)) : 1)] * cx;
-    const T Vc = v[c] * cy;
+    else d = Uc - TO(u[c - (g.px ? (int64_t)(i - wrapi(i - 1, g.nx)) : 1)]) * cx;
+    const TO Vc = TO(v[c]) * cy;
     if (!g.py && j == 0) d = d + Vc;
-    else d = d + (Vc - v[c - (g.py ? (int64_t)(j - wrapi(j - 1, g.ny)) * g.sy : g.sy)] * cy);
-    const T Wc = w[c] * cz;
+    else d = d + (Vc - TO(v[c - (g.py ? (int64_t)(j - wrapi(j - 1, g.ny)) * g.sy : g.sy)]) * cy);
+    const TO Wc = TO(w[c]) * cz;
     if (!g.pz && k == 0) d = d + Wc;
-    else d = d + (Wc - w[c - (g.pz ? (int64_t)(k - wrapi(k - 1, g.nz)) * g.sz : g.sz)] * cz);
-    out[c] = TO(d);
+    else d = d + (Wc - TO(w[c - (g.pz ? (int64_t)(k - wrapi(k - 1, g.nz)) * g.sz : g.sz)]) * cz);
+    out[c] = d;
 }
 
 template <typename T, typename TO>
@@ -314,12 +337,19 @@ hipError_t launch_fill(double *p, int64_t n, double v, hipStream_t s) {
 }
 
 template hipError_t launch_stage<double>(const Geo &, const StageArgs<double> &, int, hipStream_t);
+template hipError_t launch_stage<float>(const Geo &, const StageArgs<float> &, int, hipStream_t);
 template hipError_t launch_diag<double>(const Geo &, const DiagArgs<double> &, int, hipStream_t);
+template hipError_t launch_diag<float>(const Geo &, const DiagArgs<float> &, int, hipStream_t);
 template hipError_t launch_div<double, double>(const Geo &, const double *, const double *,
                                                const double *, double, double, double, double *,
                                                hipStream_t);
+template hipError_t launch_div<float, double>(const Geo &, const float *, const float *,
+                                              const float *, double, double, double, double *,
+                                              hipStream_t);
 template hipError_t launch_correct<double, double>(const Geo &, double *, double *, double *,
                                                    const double *, double, hipStream_t);
+template hipError_t launch_correct<float, double>(const Geo &, float *, float *, float *,
+                                                  const double *, double, hipStream_t);
 template hipError_t launch_pressure<double, double>(const Geo &, const double *, double,
                                                     double *, hipStream_t);
 

@@ -11,14 +11,14 @@ GEOMETRY_XYZ = {g: tuple(reversed(p)) for g, p in O.GEOMETRY_PERIODIC.items()}
 
 
 def make_pair(nx, ny, nz, geometry, Lx=1.0, Ly=1.0, Lz=1.0, rotating=False, coriolis=1.0,
-              mg_maxite=20, mg_tol=1e-6, mg_omega=0.9):
+              mg_maxite=20, mg_tol=1e-6, mg_omega=0.9, precision=0):
     from nyles_amd.device import DeviceModel
     orc = O.OracleLES(nx, ny, nz, Lx, Ly, Lz, geometry, rotating=rotating, coriolis=coriolis,
                       mg_maxite=mg_maxite, mg_tol=mg_tol, mg_omega=mg_omega)
     dev = DeviceModel(nx, ny, nz, GEOMETRY_XYZ[geometry], orc.dx, orc.dy, orc.dz,
                       idx2=orc.idx2, idy2=orc.idy2, idz2=orc.idz2, dx2=orc.dx ** 2,
                       fparameter=orc.fparameter, mg_maxite=mg_maxite, mg_tol=mg_tol,
-                      mg_omega=mg_omega)
+                      mg_omega=mg_omega, precision=precision)
     return orc, dev
 
 
@@ -67,6 +67,15 @@ def device_fields(dev):
            "ke": L.NYL_F_KE, "vor_i": L.NYL_F_VORX, "vor_j": L.NYL_F_VORY,
            "vor_k": L.NYL_F_VORZ, "U_i": L.NYL_F_UCX, "U_j": L.NYL_F_UCY, "U_k": L.NYL_F_UCZ}
     return {k: dev.download(v) for k, v in ids.items()}
+
+
+def rel_l2(got, want):
+    """||got - want|| / ||want|| (0 when both vanish)"""
+    got = np.asarray(got, dtype=np.float64)
+    want = np.asarray(want, dtype=np.float64)
+    den = float(np.sqrt(np.sum(want ** 2)))
+    num = float(np.sqrt(np.sum((got - want) ** 2)))
+    return num / den if den > 0 else num
 
 
 def mismatches(a, b):
