@@ -87,7 +87,8 @@ typedef struct nyl_mg_stats {
 enum {
     NYL_F_B = 0, NYL_F_U, NYL_F_V, NYL_F_W,          /* prognostic state b, u_i, u_j, u_k */
     NYL_F_KE, NYL_F_VORX, NYL_F_VORY, NYL_F_VORZ,    /* diagnostics ke, vor */
-    NYL_F_UCX, NYL_F_UCY, NYL_F_UCZ,                 /* contravariant U (core/cov_to_contra.py) */
+    NYL_F_UCX, NYL_F_UCY, NYL_F_UCZ,                 /* contravariant U (core/cov_to_contra.py),
+                                                        filled by nyl_apply(NYL_OP_U_FROM_U) */
     NYL_F_P, NYL_F_DIV,                              /* pressure, divergence */
     NYL_F_DS0_B, NYL_F_DS0_U, NYL_F_DS0_V, NYL_F_DS0_W,   /* RK tendencies, slot 0 */
     NYL_F_DS1_B, NYL_F_DS1_U, NYL_F_DS1_V, NYL_F_DS1_W,   /* slot 1 */
@@ -134,7 +135,9 @@ int nyl_mg_levels(const nyl_ctx *ctx, int32_t *nlevels);
 int nyl_mg_level_shape(const nyl_ctx *ctx, int level, int32_t *nx, int32_t *ny, int32_t *nz);
 
 /* LES.rhs (core/model_les.py:129-144): tracer advection, vortex force and
- * Bernoulli tendencies of the current state into tendency slot 0..2. */
+ * Bernoulli tendencies of the current state into tendency slot 0..2.  The
+ * contravariant U is evaluated as u*idx2 ... (the product U_from_u stores,
+ * core/cov_to_contra.py:15-17); vor and ke are those of the last diagnose. */
 int nyl_rhs(nyl_ctx *ctx, int slot, int last);
 
 /* One RK3_SSP stage (core/timescheme.py:403-444) without the projection:
@@ -143,9 +146,10 @@ int nyl_rhs(nyl_ctx *ctx, int slot, int last);
  * c = dt, dt/4., dt/12. computed by the caller. */
 int nyl_rk3_stage(nyl_ctx *ctx, int stage, double c);
 
-/* LES.diagnose_var (core/model_les.py:97-127): U, div, MG projection
+/* LES.diagnose_var (core/model_les.py:97-127): div, MG projection
  * (core/projection.py:38-87, core/mgfordriver.py:66-78), u -= grad p,
- * then U, vorticity and ke. */
+ * then vorticity and ke.  U is not stored on the step path (the stage
+ * kernel recomputes it from u); nyl_apply(NYL_OP_U_FROM_U) fills NYL_F_UC*. */
 int nyl_diagnose(nyl_ctx *ctx, nyl_mg_stats *stats);
 
 /* Timescheme.RK3_SSP forward (core/timescheme.py:403-444) with three

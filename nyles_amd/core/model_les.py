@@ -84,10 +84,13 @@ class LES(object):
         pairs = [(st.b, L.NYL_F_B), (st.u["i"], L.NYL_F_U), (st.u["j"], L.NYL_F_V),
                  (st.u["k"], L.NYL_F_W), (st.ke, L.NYL_F_KE), (st.vor["i"], L.NYL_F_VORX),
                  (st.vor["j"], L.NYL_F_VORY), (st.vor["k"], L.NYL_F_VORZ),
-                 (st.U["i"], L.NYL_F_UCX), (st.U["j"], L.NYL_F_UCY), (st.U["k"], L.NYL_F_UCZ),
                  (st.div, L.NYL_F_DIV)]
         for scalar, fid in pairs:
             scalar.attach(self.device, fid)
+        # U is recomputed inside the stage kernel, not stored: fill it on demand
+        for d, fid in (("i", L.NYL_F_UCX), ("j", L.NYL_F_UCY), ("k", L.NYL_F_UCZ)):
+            st.U[d].attach(self.device, fid,
+                           before_pull=lambda: self.device.apply(L.NYL_OP_U_FROM_U))
         st.p.attach(self.device, L.NYL_F_P,
                     before_pull=lambda: self.device.apply(L.NYL_OP_PRESSURE))
         if self.timescheme.timestepping == "RK3_SSP":

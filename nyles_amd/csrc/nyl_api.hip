@@ -76,6 +76,11 @@ struct nyl_ctx {
 
 namespace {
 
+// prolongation fused into the first post-smoothing sweep of tiled levels
+#ifndef NYL_FUSE_PROLONG
+#define NYL_FUSE_PROLONG 1
+#endif
+
 template <typename T>
 T *F(nyl_ctx *c, uint64_t off) { return reinterpret_cast<T *>(c->ws + off); }
 
@@ -233,9 +238,9 @@ StageArgs<T> stage_args(nyl_ctx *c) {
     a.v = F<T>(c, c->state[s][2]); a.w = F<T>(c, c->state[s][3]);
     a.bn = F<T>(c, c->state[t][0]); a.un = F<T>(c, c->state[t][1]);
     a.vn = F<T>(c, c->state[t][2]); a.wn = F<T>(c, c->state[t][3]);
-    a.Ux = F<T>(c, c->uc[0]); a.Uy = F<T>(c, c->uc[1]); a.Uz = F<T>(c, c->uc[2]);
     a.ox = F<T>(c, c->vor[0]); a.oy = F<T>(c, c->vor[1]); a.oz = F<T>(c, c->vor[2]);
     a.ke = F<T>(c, c->ke);
+    a.idx2 = c->d.idx2; a.idy2 = c->d.idy2; a.idz2 = c->d.idz2;
     a.d0b = F<T>(c, c->ds[0][0]); a.d0u = F<T>(c, c->ds[0][1]);
     a.d0v = F<T>(c, c->ds[0][2]); a.d0w = F<T>(c, c->ds[0][3]);
     a.d1b = F<T>(c, c->ds[1][0]); a.d1u = F<T>(c, c->ds[1][1]);
@@ -265,8 +270,21 @@ int smooth_level_t(nyl_ctx *c, int l, int xc_level, bool pre_done) {
     Level &f = c->lev[l];
     const double om = c->d.mg_omega;
     Tf *x = F<Tf>(c, f.x), *b = F<Tf>(c, f.b), *y = F<Tf>(c, f.y);
+    const double by = 3 * cells(f.g) * sizeof(Tf);
+    const bool tiled = sweep25_fits(f.g);
     if (xc_level >= 0) {
         Level &q = c->lev[xc_level];
+        const bool same = lev64(c, xc_level) == std::is_same<Tf, double>::value;
+        if (NYL_FUSE_PROLONG && tiled && same) {
+            // prolongation fused into the first sweep: x + P(xc) -> y, then y -> x
+            const double bp = (3 + 0.125) * cells(f.g) * sizeof(Tf);
+            LAUNCH(NYL_K_MG_SWEEP, bp,
+                   launch_mg_sweep25<Tf>(f.g, x, y, b, om, c->tabs, nullptr, nullptr, c->stream,
+                                         &q.g, F<Tf>(c, q.x)));
+            LAUNCH(NYL_K_MG_SWEEP, by, launch_mg_sweep25<Tf>(f.g, y, x, b, om, c->tabs, nullptr,
+                                                              nullptr, c->stream));
+            return NYL_OK;
+        }
         const double bp = 2.125 * cells(f.g) * sizeof(Tf);
         if (lev64(c, xc_level)) {
             LAUNCH(NYL_K_MG_PROLONG, bp,
@@ -280,8 +298,7 @@ int smooth_level_t(nyl_ctx *c, int l, int xc_level, bool pre_done) {
             return fail(NYL_E_STATE, "FP64 level above an FP32 level");
         }
     }
-    const double by = 3 * cells(f.g) * sizeof(Tf);
-    if (sweep25_fits(f.g)) {
+    if (tiled) {
         if (!pre_done)
             LAUNCH(NYL_K_MG_SWEEP, by, launch_mg_sweep25<Tf>(f.g, x, y, b, om, c->tabs, nullptr,
                                                               nullptr, c->stream));
@@ -305,7 +322,18 @@ int restrict_level(nyl_ctx *c, int l) {
     Level &f = c->lev[l], &q = c->lev[l + 1];
     const bool f64 = lev64(c, l), c64 = lev64(c, l + 1);
     const double bytes = 2.25 * cells(f.g) * (f64 ? sizeof(double) : sizeof(float));
-    if (f64 && c64) {
+    if (f64 == c64 && restrict25_fits(f.g)) {
+        if (f64)
+            LAUNCH(NYL_K_MG_RESTRICT, bytes,
+                   launch_mg_restrict25<double, double>(f.g, q.g, F<double>(c, f.x),
+                                                        F<double>(c, f.b), F<double>(c, q.b),
+                                                        F<double>(c, q.x), c->tabs, c->stream));
+        else
+            LAUNCH(NYL_K_MG_RESTRICT, bytes,
+                   launch_mg_restrict25<float, float>(f.g, q.g, F<float>(c, f.x),
+                                                      F<float>(c, f.b), F<float>(c, q.b),
+                                                      F<float>(c, q.x), c->tabs, c->stream));
+    } else if (f64 && c64) {
         LAUNCH(NYL_K_MG_RESTRICT, bytes,
                launch_mg_resid_restrict<double, double>(f.g, q.g, F<double>(c, f.x),
                                                         F<double>(c, f.b), F<double>(c, q.b),
@@ -509,8 +537,10 @@ int diagnose_t(nyl_ctx *c, nyl_mg_stats *st) {
     if ((rc = apply_op_t<T>(c, NYL_OP_DIV))) return rc;
     if ((rc = mg_solve(c, st))) return rc;
     if ((rc = apply_op_t<T>(c, NYL_OP_CORRECT))) return rc;
-    LAUNCH(NYL_K_DIAG, 10 * cells(c->g) * sizeof(T),
-           launch_diag<T>(c->g, diag_args<T>(c), 7, c->stream));
+    // vorticity and ke; U is not stored (the stage kernel recomputes it,
+    // NYL_OP_U_FROM_U fills it on demand)
+    LAUNCH(NYL_K_DIAG, 7 * cells(c->g) * sizeof(T),
+           launch_diag<T>(c->g, diag_args<T>(c), 6, c->stream));
     return NYL_OK;
 }
 
@@ -519,16 +549,16 @@ int diagnose(nyl_ctx *c, nyl_mg_stats *st) {
 }
 
 // words of HBM traffic per cell of each stage-kernel mode: reads b,u,v,w,
-// U(3), vor(3), ke plus the mode's extra inputs; writes the mode's outputs
+// vor(3), ke plus the mode's extra inputs; writes the mode's outputs
 double stage_words(int mode) {
     switch (mode) {
     case 0: return 8 + 4;            // tendencies only
-    case 1: return 11 + 8;           // + next state + ds0
-    case 2: return 15 + 8;           // + ds0 read
-    case 3: return 19 + 4;           // + ds0, ds1 read
-    case 4: return 15 + 8;           // + sb read; next state + sb written
-    case 5: return 11 + 4 + 4;       // next state read and written
-    case 6: return 11 + 8;           // next state + sb written
+    case 1: return 8 + 8;            // + next state + ds0
+    case 2: return 12 + 8;           // + ds0 read
+    case 3: return 16 + 4;           // + ds0, ds1 read
+    case 4: return 12 + 8;           // + sb read; next state + sb written
+    case 5: return 8 + 4 + 4;        // next state read and written
+    case 6: return 8 + 8;            // next state + sb written
     default: return 0;
     }
 }
@@ -540,7 +570,7 @@ int stage_t(nyl_ctx *c, int st, int slot, double coef, double coef2) {
     if (slot >= 0) set_tendency_out<T>(c, a, slot);
     a.c = coef;
     a.c2 = coef2;
-    const double words = st == 0 ? 12 : stage_words(st);
+    const double words = stage_words(st);
     LAUNCH(NYL_K_STAGE, words * cells(c->g) * sizeof(T), launch_stage<T>(c->g, a, st, c->stream));
     return NYL_OK;
 }

@@ -12,13 +12,13 @@ template <typename T>
 struct StageArgs {
     const T *b, *u, *v, *w;          // current state
     T *bn, *un, *vn, *wn;            // next state (stage update target)
-    const T *Ux, *Uy, *Uz;           // contravariant velocity
     const T *ox, *oy, *oz;           // vorticity
     const T *ke;                     // kinetic energy
     const T *d0b, *d0u, *d0v, *d0w;  // tendency slot 0 (read in stages 2, 3)
     const T *d1b, *d1u, *d1v, *d1w;  // tendency slot 1 (read in stage 3)
     T *tb, *tu, *tv, *tw;            // tendency output (modes 0, 1, 2)
     T *sbb, *sbu, *sbv, *sbw;        // LFAM3 'before' state (modes 4, 6)
+    double idx2, idy2, idz2;         // U = u*idx2 ... (core/cov_to_contra.py:15-17), recomputed
     double dz;                       // for cff = 0.5*dz (core/fortran_bernoulli.f90:47)
     double c;                        // RK coefficient dt, dt/4., dt/12.
     double c2;                       // LFAM3: 2.*dt (mode 4)
@@ -42,6 +42,19 @@ struct DiagArgs {
 // what: bit 0 = U, bit 1 = vorticity, bit 2 = ke
 template <typename T>
 hipError_t launch_diag(const Geo &g, const DiagArgs<T> &a, int what, hipStream_t s);
+
+template <typename T, typename TO>
+struct DivArgs {
+    const T *u, *v, *w;
+    double idx2, idy2, idz2;
+    TO *out;
+};
+template <typename T, typename TX>
+struct CorrectArgs {
+    T *u, *v, *w;
+    const TX *x;
+    double dx2;
+};
 
 // div of U (computed from u) into out (core/projection.py:15-35)
 template <typename T, typename TO>
@@ -99,10 +112,17 @@ hipError_t launch_kesum(const Geo &g, const T *u, const T *v, const T *w, double
 // dev_out (needs sweep25_blocks(g) <= NYL_RED_BLOCKS partials).
 bool sweep25_fits(const Geo &g);
 int sweep25_blocks(const Geo &g);
+// gc/xc non-null: the sweep reads src + prolongation of xc (level gc) in
+// place of src (fused k_mg_prolong; src itself is not modified).
 template <typename T>
 hipError_t launch_mg_sweep25(const Geo &g, const T *src, T *dst, const T *b, double omega,
                              const MGTabs &tab, double *partials, double *dev_out,
-                             hipStream_t s);
+                             hipStream_t s, const Geo *gc = nullptr, const T *xc = nullptr);
+// residual + restriction with 2.5-D tiling (sweep25_fits and even nz)
+bool restrict25_fits(const Geo &g);
+template <typename Tf, typename Tc>
+hipError_t launch_mg_restrict25(const Geo &gf, const Geo &gc, const Tf *x, const Tf *b, Tc *bc,
+                                Tc *xc, const MGTabs &tab, hipStream_t s);
 
 // reductions: partials then one final block; result written to dev_out[0]
 #define NYL_RED_BLOCKS 16384

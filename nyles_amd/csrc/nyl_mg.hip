@@ -210,12 +210,18 @@ __device__ __forceinline__ void block_max_store(double v, double *out, int tid) 
 #define SW_TY 8
 #define SW_R (SW_TY / 4)
 
-template <typename T, bool RESNORM>
+//
+// PROLONG: src is first corrected by the prolongation of the coarse xc (the
+//          value k_mg_prolong would store, same expression), on the own cells
+//          and the ring; the corrected x is only read by this sweep (the next
+//          sweep reads dst), so it is not stored.
+template <typename T, bool RESNORM, bool PROLONG>
 __global__ __launch_bounds__(256) void k_mg_sweep25(Geo g, const T *__restrict__ src,
                                                     T *__restrict__ dst,
                                                     const T *__restrict__ b, double omega,
                                                     MGTabs tab, int kz,
-                                                    double *__restrict__ partials) {
+                                                    double *__restrict__ partials, Geo gc,
+                                                    const T *__restrict__ xc) {
     constexpr int LX = SW_TX + 2, LY = SW_TY + 2, NRING = 2 * LX + 2 * SW_TY;
     __shared__ T sp[3][LY][LX];
     const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * SW_TX + tx;
@@ -226,7 +232,7 @@ __global__ __launch_bounds__(256) void k_mg_sweep25(Geo g, const T *__restrict__
 
     // ring position of this thread (one value per plane), -1 when none/outside
     int64_t roff = -1;
-    int rlx = 0, rly = 0;
+    int rlx = 0, rly = 0, rgi = 0, rgj = 0;
     if (tid < NRING) {
         const int q = tid;
         if (q < LX) { rlx = q; rly = 0; }
@@ -236,21 +242,34 @@ __global__ __launch_bounds__(256) void k_mg_sweep25(Geo g, const T *__restrict__
         bool in = true;
         if (g.px) gi = wrapi(gi, g.nx); else in = in && gi >= 0 && gi < g.nx;
         if (g.py) gj = wrapi(gj, g.ny); else in = in && gj >= 0 && gj < g.ny;
-        if (in) roff = (int64_t)gj * g.sy + gi;
+        if (in) { roff = (int64_t)gj * g.sy + gi; rgi = gi; rgj = gj; }
     }
     auto zok = [&](int m) { return g.pz || (m >= 0 && m < g.nz); };
     auto zw = [&](int m) { return g.pz ? wrapi(m, g.nz) : m; };
     // load plane m: own cells into registers `own` and LDS, ring into LDS
     auto load_plane = [&](int m, int buf, T *own) {
         const bool zin = zok(m);
-        const int64_t base = (int64_t)zw(m) * g.sz;
+        const int mw = zw(m);
+        const int64_t base = (int64_t)mw * g.sz;
 #pragma unroll
         for (int r = 0; r < SW_R; r++) {
             const int j = j0 + ty + 4 * r;
-            own[r] = zin ? src[base + (int64_t)j * g.sy + i] : T(0);
-            sp[buf][1 + ty + 4 * r][1 + tx] = own[r];
+            T v = T(0);
+            if (zin) {
+                v = src[base + (int64_t)j * g.sy + i];
+                if (PROLONG) v = v + prolong_term<T, T>(gc, xc, tab, i, j, mw);
+            }
+            own[r] = v;
+            sp[buf][1 + ty + 4 * r][1 + tx] = v;
         }
-        if (tid < NRING) sp[buf][rly][rlx] = (zin && roff >= 0) ? src[base + roff] : T(0);
+        if (tid < NRING) {
+            T v = T(0);
+            if (zin && roff >= 0) {
+                v = src[base + roff];
+                if (PROLONG) v = v + prolong_term<T, T>(gc, xc, tab, rgi, rgj, mw);
+            }
+            sp[buf][rly][rlx] = v;
+        }
     };
     // in-domain neighbour count per own row (x and y parts fixed, z varies)
     int nxy[SW_R];
@@ -293,6 +312,93 @@ __global__ __launch_bounds__(256) void k_mg_sweep25(Geo g, const T *__restrict__
         __syncthreads();
         const int nb = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
         block_sum_store(acc, partials + nb, tid);
+    }
+}
+
+// Residual + restriction with 2.5-D tiling: a block owns fine 64 x 8 cells
+// (coarse 32 x 4) and marches up kz fine planes (kz even); thread (tx, ty)
+// holds the fine rows j0 + 2 ty and j0 + 2 ty + 1 of column i0 + tx, so the
+// y pair of a coarse cell is in one thread, the x pair in two adjacent lanes
+// and the z pair in consecutive planes.  The residuals are those of
+// residual_at and the eight are summed in the order of k_mg_resid_restrict,
+// so the coarse b is bitwise the same.
+template <typename Tf, typename Tc>
+__global__ __launch_bounds__(256) void k_mg_restrict25(Geo g, Geo gc, const Tf *__restrict__ x,
+                                                       const Tf *__restrict__ b,
+                                                       Tc *__restrict__ bc, Tc *__restrict__ xc,
+                                                       MGTabs tab, int kz) {
+    constexpr int LX = SW_TX + 2, LY = SW_TY + 2, NRING = 2 * LX + 2 * SW_TY;
+    __shared__ Tf sp[3][LY][LX];
+    const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * SW_TX + tx;
+    const Blk3 bk = xcd_block();
+    const int i0 = bk.x * SW_TX, j0 = bk.y * SW_TY, k0 = bk.z * kz;
+    const int i = i0 + tx;
+
+    int64_t roff = -1;
+    int rlx = 0, rly = 0;
+    if (tid < NRING) {
+        const int q = tid;
+        if (q < LX) { rlx = q; rly = 0; }
+        else if (q < 2 * LX) { rlx = q - LX; rly = LY - 1; }
+        else { const int t = q - 2 * LX; rly = 1 + (t >> 1); rlx = (t & 1) ? LX - 1 : 0; }
+        int gi = i0 + rlx - 1, gj = j0 + rly - 1;
+        bool in = true;
+        if (g.px) gi = wrapi(gi, g.nx); else in = in && gi >= 0 && gi < g.nx;
+        if (g.py) gj = wrapi(gj, g.ny); else in = in && gj >= 0 && gj < g.ny;
+        if (in) roff = (int64_t)gj * g.sy + gi;
+    }
+    auto zok = [&](int m) { return g.pz || (m >= 0 && m < g.nz); };
+    auto zw = [&](int m) { return g.pz ? wrapi(m, g.nz) : m; };
+    auto load_plane = [&](int m, int buf, Tf *own) {
+        const bool zin = zok(m);
+        const int64_t base = (int64_t)zw(m) * g.sz;
+#pragma unroll
+        for (int r = 0; r < 2; r++) {
+            const int j = j0 + 2 * ty + r;
+            own[r] = zin ? x[base + (int64_t)j * g.sy + i] : Tf(0);
+            sp[buf][1 + 2 * ty + r][1 + tx] = own[r];
+        }
+        if (tid < NRING) sp[buf][rly][rlx] = (zin && roff >= 0) ? x[base + roff] : Tf(0);
+    };
+    int nxy[2];
+#pragma unroll
+    for (int r = 0; r < 2; r++) {
+        const int j = j0 + 2 * ty + r;
+        nxy[r] = (g.px || i > 0) + (g.px || i < g.nx - 1) + (g.py || j > 0) + (g.py || j < g.ny - 1);
+    }
+    Tf zm[2], z0[2], zp[2];
+    Tf acc = Tf(0);
+    load_plane(k0 - 1, 0, zm);
+    load_plane(k0, 1, z0);
+    for (int k = k0; k < k0 + kz; k++) {
+        const int cur = (k - k0 + 1) % 3;
+        load_plane(k + 1, (k - k0 + 2) % 3, zp);
+        __syncthreads();
+        const int nz_ = (g.pz || k > 0) + (g.pz || k < g.nz - 1);
+        Tf r[2];
+#pragma unroll
+        for (int rr = 0; rr < 2; rr++) {
+            const int ly = 1 + 2 * ty + rr, lx = 1 + tx;
+            const int j = j0 + 2 * ty + rr;
+            const Tf S = sp[cur][ly][lx - 1] + sp[cur][ly][lx + 1] + sp[cur][ly - 1][lx]
+                       + sp[cur][ly + 1][lx] + zm[rr] + zp[rr];
+            const int64_t c = (int64_t)k * g.sz + (int64_t)j * g.sy + i;
+            r[rr] = b[c] + Tf(tab.diag[nxy[rr] + nz_]) * z0[rr] - S;
+            zm[rr] = z0[rr];
+            z0[rr] = zp[rr];
+        }
+        const Tf rx0 = __shfl_down(r[0], 1), rx1 = __shfl_down(r[1], 1);
+        if (((k - k0) & 1) == 0) {
+            acc = ((r[0] + rx0) + r[1]) + rx1;
+        } else {
+            acc = (((acc + r[0]) + rx0) + r[1]) + rx1;
+            if ((tx & 1) == 0) {
+                const int64_t cc = (int64_t)(k >> 1) * gc.sz + (int64_t)((j0 >> 1) + ty) * gc.sy
+                                 + (i >> 1);
+                bc[cc] = Tc(Tf(0.5) * acc);
+                xc[cc] = Tc(0);
+            }
+        }
     }
 }
 
@@ -606,38 +712,59 @@ bool sweep25_fits(const Geo &g) {
 }
 
 // planes per block: up to 8, fewer when that leaves the chip short of blocks
-static int sweep25_kz(const Geo &g) {
+static int sweep25_kz(const Geo &g, int kmin) {
     const int64_t cols = (int64_t)(g.nx / SW_TX) * (g.ny / SW_TY);
     int kz = 8;
-    while (kz > 1 && (g.nz % kz != 0 || cols * (g.nz / kz) < 1024)) kz >>= 1;
+    while (kz > kmin && (g.nz % kz != 0 || cols * (g.nz / kz) < 1024)) kz >>= 1;
     return kz;
 }
 
 int sweep25_blocks(const Geo &g) {
-    return (g.nx / SW_TX) * (g.ny / SW_TY) * (g.nz / sweep25_kz(g));
+    return (g.nx / SW_TX) * (g.ny / SW_TY) * (g.nz / sweep25_kz(g, 1));
 }
 
 template <typename T>
 hipError_t launch_mg_sweep25(const Geo &g, const T *src, T *dst, const T *b, double omega,
                              const MGTabs &tab, double *partials, double *dev_out,
-                             hipStream_t s) {
-    const int kz = sweep25_kz(g);
+                             hipStream_t s, const Geo *gc, const T *xc) {
+    const int kz = sweep25_kz(g, 1);
     dim3 grd(g.nx / SW_TX, g.ny / SW_TY, g.nz / kz), blk(SW_TX, 4);
+    const Geo gcv = gc ? *gc : g;
     if (partials) {
-        hipLaunchKernelGGL((k_mg_sweep25<T, true>), grd, blk, 0, s, g, src, dst, b, omega, tab,
-                           kz, partials);
+        hipLaunchKernelGGL((k_mg_sweep25<T, true, false>), grd, blk, 0, s, g, src, dst, b, omega,
+                           tab, kz, partials, gcv, xc);
         const int nb = grd.x * grd.y * grd.z;
         hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(RED_T), 0, s, partials, nb, dev_out);
+    } else if (xc) {
+        hipLaunchKernelGGL((k_mg_sweep25<T, false, true>), grd, blk, 0, s, g, src, dst, b, omega,
+                           tab, kz, partials, gcv, xc);
     } else {
-        hipLaunchKernelGGL((k_mg_sweep25<T, false>), grd, blk, 0, s, g, src, dst, b, omega, tab,
-                           kz, partials);
+        hipLaunchKernelGGL((k_mg_sweep25<T, false, false>), grd, blk, 0, s, g, src, dst, b,
+                           omega, tab, kz, partials, gcv, xc);
     }
+    return hipGetLastError();
+}
+
+bool restrict25_fits(const Geo &g) {
+    return sweep25_fits(g) && g.nz % 2 == 0;
+}
+
+template <typename Tf, typename Tc>
+hipError_t launch_mg_restrict25(const Geo &gf, const Geo &gc, const Tf *x, const Tf *b, Tc *bc,
+                                Tc *xc, const MGTabs &tab, hipStream_t s) {
+    const int kz = sweep25_kz(gf, 2);
+    dim3 grd(gf.nx / SW_TX, gf.ny / SW_TY, gf.nz / kz), blk(SW_TX, 4);
+    hipLaunchKernelGGL((k_mg_restrict25<Tf, Tc>), grd, blk, 0, s, gf, gc, x, b, bc, xc, tab, kz);
     return hipGetLastError();
 }
 
 #define NYL_MG_INST(T)                                                                          \
     template hipError_t launch_mg_sweep25<T>(const Geo &, const T *, T *, const T *, double,    \
-                                             const MGTabs &, double *, double *, hipStream_t);  \
+                                             const MGTabs &, double *, double *, hipStream_t,   \
+                                             const Geo *, const T *);                           \
+    template hipError_t launch_mg_restrict25<T, T>(const Geo &, const Geo &, const T *,         \
+                                                   const T *, T *, T *, const MGTabs &,         \
+                                                   hipStream_t);                                \
     template hipError_t launch_mg_sweep<T>(const Geo &, const T *, T *, const T *, double,      \
                                            const MGTabs &, hipStream_t);                        \
     template hipError_t launch_mg_resid_restrict<T, T>(const Geo &, const Geo &, const T *,     \

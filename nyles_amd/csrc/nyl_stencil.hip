@@ -11,11 +11,25 @@ namespace nyl {
 #define NYL_BX 64
 #define NYL_BY 4
 
-// Minimum waves per SIMD for the stage kernel: caps its registers so that
-// enough waves are resident to hide the latency of its ~90 loads per cell
-// (uncapped, the compiler hoists them all and the kernel runs 2 waves/SIMD).
+// Minimum waves per SIMD for the stage kernel (caps its registers).
 #ifndef NYL_STAGE_WAVES
-#define NYL_STAGE_WAVES 4
+#define NYL_STAGE_WAVES 1
+#endif
+// Sequencing of the flux evaluations of the stage kernel.  Its stencil loads
+// are invariant (const __restrict__), so the compiler is free to issue the
+// loads of all twelve reconstructions before any arithmetic, which needs
+// ~180 registers and leaves 2 waves per SIMD.  STAGE_SEQ(idx, dep) passes
+// the line index of the next evaluation through an empty asm that also takes
+// the previous evaluation's result: the next loads cannot be issued before
+// that result exists, each reconstruction holds only its own operands and
+// occupancy hides the load latency.  No instruction is emitted.
+#ifndef NYL_STAGE_SEQ
+#define NYL_STAGE_SEQ 1
+#endif
+#if NYL_STAGE_SEQ
+#define STAGE_SEQ(idx, dep) asm volatile("" : "+v"(idx) : "v"(dep))
+#else
+#define STAGE_SEQ(idx, dep) ((void)0)
 #endif
 
 static inline dim3 cell_grid(const Geo &g) {
@@ -24,16 +38,18 @@ static inline dim3 cell_grid(const Geo &g) {
 
 // Vortex-force flux of one term at a cell (core/fortran_vortex_force.f90
 // WENO branch, :66-81 and :145-159): the vorticity component `om` is
-// upwinded along axis d with the velocity component `Ud` averaged along the
-// target axis e (UU) and then along d (u1d).  `based` is the index of the
-// cell's line start along d, `sd` the stride of d, `md` the cell's position
-// on that line, `oe` the offset to the e+1 neighbour.
+// upwinded along axis d with the contravariant velocity U_d = ud*cu
+// averaged along the target axis e (UU) and then along d (u1d).  `based` is
+// the index of the cell's line start along d, `sd` the stride of d, `md` the
+// cell's position on that line, `oe` the offset to the e+1 neighbour.  U is
+// recomputed from the covariant u with the product U_from_u stores
+// (core/cov_to_contra.py:15-17), so it is the same double.
 template <typename T, int pd>
-__device__ __forceinline__ T vf_flux(const T *__restrict__ Ud, const T *__restrict__ om,
+__device__ __forceinline__ T vf_flux(const T *__restrict__ ud, T cu, const T *__restrict__ om,
                                      int64_t based, int64_t sd, int md, int nd, int64_t oe) {
     auto UU = [&](int m) -> T {
         const int64_t p = based + (int64_t)m * sd;
-        return T(0.5) * (Ud[p] + Ud[p + oe]);
+        return T(0.5) * (ud[p] * cu + ud[p + oe] * cu);
     };
     T uu_prev;
     if (!pd && md == 0) uu_prev = T(0);   // UU_0 = 0. at the line start
@@ -49,36 +65,37 @@ __device__ __forceinline__ T vf_flux(const T *__restrict__ Ud, const T *__restri
 
 // Tracer flux divergence along one axis (core/fortran_upwind.f90:228-234):
 // d = d + flux(m-1) - flux(m), or d - flux(1) at a wall start.
+// U_d = ud*cu as in vf_flux.
 template <typename T, int pd>
 __device__ __forceinline__ T tracer_axis(T d, const T *__restrict__ q_arr,
-                                         const T *__restrict__ U, int64_t based, int64_t sd,
-                                         int md, int nd) {
+                                         const T *__restrict__ ud, T cu, int64_t based,
+                                         int64_t sd, int md, int nd) {
     auto q = [&](int m) -> T { return q_arr[based + (int64_t)(pd ? wrapi(m, nd) : m) * sd]; };
-    const T fr = face_flux<T, pd>(md, nd, U[based + (int64_t)md * sd], q);
+    const T fr = face_flux<T, pd>(md, nd, ud[based + (int64_t)md * sd] * cu, q);
+    STAGE_SEQ(based, fr);
     if (!pd && md == 0) return d - fr;
     const int ml = pd ? wrapi(md - 1, nd) : md - 1;
-    const T fl = face_flux<T, pd>(ml, nd, U[based + (int64_t)ml * sd], q);
+    const T fl = face_flux<T, pd>(ml, nd, ud[based + (int64_t)ml * sd] * cu, q);
     return (d + fl) - fr;
 }
 
 // Fused LES.rhs (core/model_les.py:129-144) + RK3_SSP stage update
 // (core/timescheme.py:422-443) for one cell.
 template <typename T, int MODE, int PX, int PY, int PZ>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NYL_STAGE_WAVES)))
-void k_stage(Geo g, StageArgs<T> a) {
-    const Blk3 bk = xcd_block();
-    const int i = bk.x * NYL_BX + threadIdx.x;
-    const int j = bk.y * NYL_BY + threadIdx.y;
-    const int k = bk.z;
-    if (i >= g.nx || j >= g.ny) return;
+__device__ __forceinline__ void stage_cell(const Geo &g, const StageArgs<T> &a, int i, int j,
+                                           int k) {
     const int64_t c = (int64_t)k * g.sz + (int64_t)j * g.sy + i;
-    const int64_t bx = c - i, by = c - (int64_t)j * g.sy, bz = c - (int64_t)k * g.sz;
 
     // tracer b: rhstrac, directions i, j, k (core/tracer.py:63-72)
+    const T cx = T(a.idx2), cy = T(a.idy2), cz = T(a.idz2);
     T db = T(0);
-    db = tracer_axis<T, PX>(db, a.b, a.Ux, bx, 1, i, g.nx);
-    db = tracer_axis<T, PY>(db, a.b, a.Uy, by, g.sy, j, g.ny);
-    db = tracer_axis<T, PZ>(db, a.b, a.Uz, bz, g.sz, k, g.nz);
+    int64_t bx = c - i, by = c - (int64_t)j * g.sy, bz = c - (int64_t)k * g.sz;
+    db = tracer_axis<T, PX>(db, a.b, a.u, cx, bx, 1, i, g.nx);
+    STAGE_SEQ(by, db);
+    db = tracer_axis<T, PY>(db, a.b, a.v, cy, by, g.sy, j, g.ny);
+    STAGE_SEQ(bz, db);
+    db = tracer_axis<T, PZ>(db, a.b, a.w, cz, bz, g.sz, k, g.nz);
+    STAGE_SEQ(bx, db);
 
     // vortex force, triplets 'ikj', 'jik', 'kji' (core/vortex_force.py:69-81)
     const int64_t oex = PX ? (int64_t)(wrapi(i + 1, g.nx) - i) : 1;
@@ -88,12 +105,17 @@ void k_stage(Geo g, StageArgs<T> a) {
     const bool ey = PY || j < g.ny - 1;
     const bool ez = PZ || k < g.nz - 1;
     T du = T(0), dv = T(0), dw = T(0);
-    if (ey) dv = dv - vf_flux<T, PX>(a.Ux, a.oz, bx, 1, i, g.nx, oey);      // direc(U_i, vor_k, du_j)
-    if (ex) du = du + vf_flux<T, PY>(a.Uy, a.oz, by, g.sy, j, g.ny, oex);   // flip (U_j, vor_k, du_i)
-    if (ez) dw = dw - vf_flux<T, PY>(a.Uy, a.ox, by, g.sy, j, g.ny, oez);   // direc(U_j, vor_i, du_k)
-    if (ey) dv = dv + vf_flux<T, PZ>(a.Uz, a.ox, bz, g.sz, k, g.nz, oey);   // flip (U_k, vor_i, du_j)
-    if (ex) du = du - vf_flux<T, PZ>(a.Uz, a.oy, bz, g.sz, k, g.nz, oex);   // direc(U_k, vor_j, du_i)
-    if (ez) dw = dw + vf_flux<T, PX>(a.Ux, a.oy, bx, 1, i, g.nx, oez);      // flip (U_i, vor_j, du_k)
+    if (ey) dv = dv - vf_flux<T, PX>(a.u, cx, a.oz, bx, 1, i, g.nx, oey);      // direc(U_i, vor_k, du_j)
+    STAGE_SEQ(by, dv);
+    if (ex) du = du + vf_flux<T, PY>(a.v, cy, a.oz, by, g.sy, j, g.ny, oex);   // flip (U_j, vor_k, du_i)
+    STAGE_SEQ(by, du);
+    if (ez) dw = dw - vf_flux<T, PY>(a.v, cy, a.ox, by, g.sy, j, g.ny, oez);   // direc(U_j, vor_i, du_k)
+    STAGE_SEQ(bz, dw);
+    if (ey) dv = dv + vf_flux<T, PZ>(a.w, cz, a.ox, bz, g.sz, k, g.nz, oey);   // flip (U_k, vor_i, du_j)
+    STAGE_SEQ(bz, dv);
+    if (ex) du = du - vf_flux<T, PZ>(a.w, cz, a.oy, bz, g.sz, k, g.nz, oex);   // direc(U_k, vor_j, du_i)
+    STAGE_SEQ(bx, du);
+    if (ez) dw = dw + vf_flux<T, PX>(a.u, cx, a.oy, bx, 1, i, g.nx, oez);      // flip (U_i, vor_j, du_k)
 
     // bernoulli (core/bernoulli.py:14-23, core/fortran_bernoulli.f90:2-58)
     const T kec = a.ke[c];
@@ -145,6 +167,17 @@ void k_stage(Geo g, StageArgs<T> a) {
     }
 }
 
+template <typename T, int MODE, int PX, int PY, int PZ>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NYL_STAGE_WAVES)))
+void k_stage(Geo g, StageArgs<T> a) {
+    const Blk3 bk = xcd_block();
+    const int i = bk.x * NYL_BX + threadIdx.x;
+    const int j = bk.y * NYL_BY + threadIdx.y;
+    const int k = bk.z;
+    if (i >= g.nx || j >= g.ny) return;
+    stage_cell<T, MODE, PX, PY, PZ>(g, a, i, j, k);
+}
+
 template <typename T, int PX, int PY, int PZ>
 static void launch_stage_p(const Geo &g, const StageArgs<T> &a, int mode, hipStream_t s) {
     dim3 blk(NYL_BX, NYL_BY, 1), grd = cell_grid(g);
@@ -180,12 +213,8 @@ hipError_t launch_stage(const Geo &g, const StageArgs<T> &a, int mode, hipStream
 // core/fortran_vorticity.f90:2-28) and kinenergy (core/kinenergy.py:13-24,
 // core/fortran_kinenergy.f90:145-153) from the covariant velocity.
 template <typename T>
-__global__ __launch_bounds__(256) void k_diag(Geo g, DiagArgs<T> a, int what) {
-    const Blk3 bk = xcd_block();
-    const int i = bk.x * NYL_BX + threadIdx.x;
-    const int j = bk.y * NYL_BY + threadIdx.y;
-    const int k = bk.z;
-    if (i >= g.nx || j >= g.ny) return;
+__device__ __forceinline__ void diag_cell(const Geo &g, const DiagArgs<T> &a, int what, int i,
+                                          int j, int k) {
     const int64_t c = (int64_t)k * g.sz + (int64_t)j * g.sy + i;
     const T uc = a.u[c], vc = a.v[c], wc = a.w[c];
     if (what & 1) {
@@ -235,6 +264,16 @@ __global__ __launch_bounds__(256) void k_diag(Geo g, DiagArgs<T> a, int what) {
 }
 
 template <typename T>
+__global__ __launch_bounds__(256) void k_diag(Geo g, DiagArgs<T> a, int what) {
+    const Blk3 bk = xcd_block();
+    const int i = bk.x * NYL_BX + threadIdx.x;
+    const int j = bk.y * NYL_BY + threadIdx.y;
+    const int k = bk.z;
+    if (i >= g.nx || j >= g.ny) return;
+    diag_cell<T>(g, a, what, i, j, k);
+}
+
+template <typename T>
 hipError_t launch_diag(const Geo &g, const DiagArgs<T> &a, int what, hipStream_t s) {
     hipLaunchKernelGGL((k_diag<T>), cell_grid(g), dim3(NYL_BX, NYL_BY, 1), 0, s, g, a, what);
     return hipGetLastError();
@@ -243,71 +282,81 @@ hipError_t launch_diag(const Geo &g, const DiagArgs<T> &a, int what, hipStream_t
 // compute_div (core/projection.py:15-35 over core/fortran_bernoulli.f90:61-97)
 // with U = u*idx2 evaluated in place (core/cov_to_contra.py:18-20).
 template <typename T, typename TO>
-__global__ __launch_bounds__(256) void k_div(Geo g, const T *__restrict__ u,
-                                             const T *__restrict__ v, const T *__restrict__ w,
-                                             double idx2, double idy2, double idz2,
-                                             TO *__restrict__ out) {
+__device__ __forceinline__ void div_cell(const Geo &g, const DivArgs<T, TO> &a, int i, int j,
+                                         int k) {
+    const int64_t c = (int64_t)k * g.sz + (int64_t)j * g.sy + i;
+    // evaluated in the output precision (FP64 right-hand side of the mixed path)
+    const TO cx = TO(a.idx2), cy = TO(a.idy2), cz = TO(a.idz2);
+    TO d;
+    const TO Uc = TO(a.u[c]) * cx;
+    if (!g.px && i == 0) d = Uc;
+    else d = Uc - TO(a.u[c - (g.px ? (int64_t)(i - wrapi(i - 1, g.nx)) : 1)]) * cx;
+    const TO Vc = TO(a.v[c]) * cy;
+    if (!g.py && j == 0) d = d + Vc;
+    else d = d + (Vc - TO(a.v[c - (g.py ? (int64_t)(j - wrapi(j - 1, g.ny)) * g.sy : g.sy)]) * cy);
+    const TO Wc = TO(a.w[c]) * cz;
+    if (!g.pz && k == 0) d = d + Wc;
+    else d = d + (Wc - TO(a.w[c - (g.pz ? (int64_t)(k - wrapi(k - 1, g.nz)) * g.sz : g.sz)]) * cz);
+    a.out[c] = d;
+}
+
+template <typename T, typename TO>
+__global__ __launch_bounds__(256) void k_div(Geo g, DivArgs<T, TO> a) {
     const Blk3 bk = xcd_block();
     const int i = bk.x * NYL_BX + threadIdx.x;
     const int j = bk.y * NYL_BY + threadIdx.y;
     const int k = bk.z;
     if (i >= g.nx || j >= g.ny) return;
-    const int64_t c = (int64_t)k * g.sz + (int64_t)j * g.sy + i;
-    // evaluated in the output precision (FP64 right-hand side of the mixed path)
-    const TO cx = TO(idx2), cy = TO(idy2), cz = TO(idz2);
-    TO d;
-    const TO Uc = TO(u[c]) * cx;
-    if (!g.px && i == 0) d = Uc;
-    else d = Uc - TO(u[c - (g.px ? (int64_t)(i - wrapi(i - 1, g.nx)) : 1)]) * cx;
-    const TO Vc = TO(v[c]) * cy;
-    if (!g.py && j == 0) d = d + Vc;
-    else d = d + (Vc - TO(v[c - (g.py ? (int64_t)(j - wrapi(j - 1, g.ny)) * g.sy : g.sy)]) * cy);
-    const TO Wc = TO(w[c]) * cz;
-    if (!g.pz && k == 0) d = d + Wc;
-    else d = d + (Wc - TO(w[c - (g.pz ? (int64_t)(k - wrapi(k - 1, g.nz)) * g.sz : g.sz)]) * cz);
-    out[c] = d;
+    div_cell<T, TO>(g, a, i, j, k);
 }
 
 template <typename T, typename TO>
 hipError_t launch_div(const Geo &g, const T *u, const T *v, const T *w, double idx2,
                       double idy2, double idz2, TO *out, hipStream_t s) {
-    hipLaunchKernelGGL((k_div<T, TO>), cell_grid(g), dim3(NYL_BX, NYL_BY, 1), 0, s, g, u, v, w,
-                       idx2, idy2, idz2, out);
+    DivArgs<T, TO> a;
+    a.u = u; a.v = v; a.w = w; a.idx2 = idx2; a.idy2 = idy2; a.idz2 = idz2; a.out = out;
+    hipLaunchKernelGGL((k_div<T, TO>), cell_grid(g), dim3(NYL_BX, NYL_BY, 1), 0, s, g, a);
     return hipGetLastError();
 }
 
 // u -= delta[p] with p = x*dx**2 (core/projection.py:84-87 over
 // core/fortran_bernoulli.f90:2-26; p from core/mgfordriver.py:78)
 template <typename T, typename TX>
-__global__ __launch_bounds__(256) void k_correct(Geo g, T *__restrict__ u, T *__restrict__ v,
-                                                 T *__restrict__ w, const TX *__restrict__ x,
-                                                 double dx2) {
+__device__ __forceinline__ void correct_cell(const Geo &g, const CorrectArgs<T, TX> &a, int i,
+                                             int j, int k) {
+    const int64_t c = (int64_t)k * g.sz + (int64_t)j * g.sy + i;
+    const TX *x = a.x;
+    const TX pc = x[c] * TX(a.dx2);
+    if (g.px || i < g.nx - 1) {
+        const int64_t o = g.px ? (int64_t)(wrapi(i + 1, g.nx) - i) : 1;
+        a.u[c] = T(TX(a.u[c]) - (x[c + o] * TX(a.dx2) - pc));
+    }
+    if (g.py || j < g.ny - 1) {
+        const int64_t o = g.py ? (int64_t)(wrapi(j + 1, g.ny) - j) * g.sy : g.sy;
+        a.v[c] = T(TX(a.v[c]) - (x[c + o] * TX(a.dx2) - pc));
+    }
+    if (g.pz || k < g.nz - 1) {
+        const int64_t o = g.pz ? (int64_t)(wrapi(k + 1, g.nz) - k) * g.sz : g.sz;
+        a.w[c] = T(TX(a.w[c]) - (x[c + o] * TX(a.dx2) - pc));
+    }
+}
+
+template <typename T, typename TX>
+__global__ __launch_bounds__(256) void k_correct(Geo g, CorrectArgs<T, TX> a) {
     const Blk3 bk = xcd_block();
     const int i = bk.x * NYL_BX + threadIdx.x;
     const int j = bk.y * NYL_BY + threadIdx.y;
     const int k = bk.z;
     if (i >= g.nx || j >= g.ny) return;
-    const int64_t c = (int64_t)k * g.sz + (int64_t)j * g.sy + i;
-    const TX pc = x[c] * TX(dx2);
-    if (g.px || i < g.nx - 1) {
-        const int64_t o = g.px ? (int64_t)(wrapi(i + 1, g.nx) - i) : 1;
-        u[c] = T(TX(u[c]) - (x[c + o] * TX(dx2) - pc));
-    }
-    if (g.py || j < g.ny - 1) {
-        const int64_t o = g.py ? (int64_t)(wrapi(j + 1, g.ny) - j) * g.sy : g.sy;
-        v[c] = T(TX(v[c]) - (x[c + o] * TX(dx2) - pc));
-    }
-    if (g.pz || k < g.nz - 1) {
-        const int64_t o = g.pz ? (int64_t)(wrapi(k + 1, g.nz) - k) * g.sz : g.sz;
-        w[c] = T(TX(w[c]) - (x[c + o] * TX(dx2) - pc));
-    }
+    correct_cell<T, TX>(g, a, i, j, k);
 }
 
 template <typename T, typename TX>
 hipError_t launch_correct(const Geo &g, T *u, T *v, T *w, const TX *x, double dx2,
                           hipStream_t s) {
-    hipLaunchKernelGGL((k_correct<T, TX>), cell_grid(g), dim3(NYL_BX, NYL_BY, 1), 0, s, g, u, v,
-                       w, x, dx2);
+    CorrectArgs<T, TX> a;
+    a.u = u; a.v = v; a.w = w; a.x = x; a.dx2 = dx2;
+    hipLaunchKernelGGL((k_correct<T, TX>), cell_grid(g), dim3(NYL_BX, NYL_BY, 1), 0, s, g, a);
     return hipGetLastError();
 }
 

@@ -63,6 +63,7 @@ def oracle_fields(orc):
 
 def device_fields(dev):
     from nyles_amd import _lib as L
+    dev.apply(L.NYL_OP_U_FROM_U)   # U is filled on demand
     ids = {"b": L.NYL_F_B, "u_i": L.NYL_F_U, "u_j": L.NYL_F_V, "u_k": L.NYL_F_W,
            "ke": L.NYL_F_KE, "vor_i": L.NYL_F_VORX, "vor_j": L.NYL_F_VORY,
            "vor_k": L.NYL_F_VORZ, "U_i": L.NYL_F_UCX, "U_j": L.NYL_F_UCY, "U_k": L.NYL_F_UCZ}

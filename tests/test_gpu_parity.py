@@ -48,7 +48,9 @@ def test_diagnose_parity(cmp, geometry):
 
 @pytest.mark.parametrize("geometry", GEOMETRIES)
 def test_rhs_parity_random_inputs(cmp, geometry):
-    """LES.rhs on arbitrary (not mutually consistent) U, vor, ke, b fields."""
+    """LES.rhs on arbitrary (not mutually consistent) u, vor, ke, b fields; the
+    oracle's U is u*idx2 (core/cov_to_contra.py:15-17), which the device
+    recomputes in the stage kernel."""
     from nyles_amd import _lib as L
     import oracle as O
     print("case rhs", geometry, flush=True)
@@ -56,18 +58,21 @@ def test_rhs_parity_random_inputs(cmp, geometry):
     rng = np.random.default_rng(5)
     shape = (orc.nz, orc.ny, orc.nx)
     fields = {
-        "b": (orc.b, L.NYL_F_B), "U_i": (orc.U["i"], L.NYL_F_UCX), "U_j": (orc.U["j"], L.NYL_F_UCY),
-        "U_k": (orc.U["k"], L.NYL_F_UCZ), "vor_i": (orc.vor["i"], L.NYL_F_VORX),
+        "b": (orc.b, L.NYL_F_B), "u_i": (orc.U["i"], L.NYL_F_U), "u_j": (orc.U["j"], L.NYL_F_V),
+        "u_k": (orc.U["k"], L.NYL_F_W), "vor_i": (orc.vor["i"], L.NYL_F_VORX),
         "vor_j": (orc.vor["j"], L.NYL_F_VORY), "vor_k": (orc.vor["k"], L.NYL_F_VORZ),
         "ke": (orc.ke, L.NYL_F_KE),
     }
+    metric = {"u_i": orc.idx2, "u_j": orc.idy2, "u_k": orc.idz2}
     for name, (arr, fid) in fields.items():
         vals = rng.standard_normal(shape)
         # exact zeros on a few cells exercise the u > 0 / u <= 0 split
         vals.flat[::7] = 0.0
+        dev.upload(fid, vals)
+        if name in metric:
+            vals = vals * metric[name]
         orc.set_interior(arr, vals)
         O.halo_fill(arr, geometry, orc.nh)
-        dev.upload(fid, vals)
     ds = orc.ds[1]
     orc.rhs(ds)
     dev.rhs(1)
