@@ -8,8 +8,7 @@
 #include <utility>
 #include <vector>
 
-#include "nyl_internal.h"
-#include "../../include/nyles_hip.h"
+#include "nyl_ctx.h"
 
 using namespace nyl;
 
@@ -33,56 +32,15 @@ int fail(int code, const char *fmt, ...) {
                         hipGetErrorString(e_), __FILE__, __LINE__);                    \
     } while (0)
 
-constexpr int kMaxLevels = 32;
-
-struct Level {
-    Geo g;
-    uint64_t x, b, y;   // byte offsets in the workspace
-};
-
 }  // namespace
-
-struct nyl_ctx {
-    nyl_desc d;
-    Geo g;
-    int mixed;               // NYL_PREC_MIXED: FP32 model fields and inner V-cycles
-    int64_t ncell;
-    int nlev;
-    Level lev[kMaxLevels];
-    MGTabs tabs;
-    // byte offsets of the fields
-    uint64_t state[2][4];
-    uint64_t ds[3][4];
-    uint64_t sb[4];          // LFAM3 'before' state
-    int lf_first;            // next LFAM3 step is the first one
-    uint64_t ke, vor[3], uc[3], p;
-    uint64_t mgx, mgb;       // FP64 solution and right-hand side of the projection
-    double mg_shift;         // mean removed from the right-hand side (mixed mode)
-    uint64_t partials, result;
-    uint64_t ws_bytes;
-    int cur;                 // which state buffer is current
-    char *ws;
-    hipStream_t stream;
-    double *host_out;        // pinned, 8 doubles
-    int total_cycles;
-    // HIP-event timing of one kernel class (nyl_prof_start / nyl_prof_stop)
-    int prof_cls;
-    int prof_used;
-    hipEvent_t prof_ev[2 * 4096];
-    int prof_nev;            // events created
-    double prof_ms, prof_bytes;
-    int64_t prof_launches;
-};
 
 namespace {
 
 // prolongation fused into the first post-smoothing sweep of tiled levels
 #ifndef NYL_FUSE_PROLONG
-#define NYL_FUSE_PROLONG 1
+#define NYL_FUSE_PROLONG 0
 #endif
 
-template <typename T>
-T *F(nyl_ctx *c, uint64_t off) { return reinterpret_cast<T *>(c->ws + off); }
 
 uint64_t align256(uint64_t v) { return (v + 255) & ~uint64_t(255); }
 
@@ -127,9 +85,6 @@ void build_tabs(MGTabs &t) {
     }
 }
 
-// Level l holds FP64 arrays in FP64 mode; in mixed mode the inner V-cycle
-// levels are FP32 and the coarsest level stays FP64.
-bool lev64(const nyl_ctx *c, int l) { return !c->mixed || l == c->nlev - 1; }
 
 void build_layout(nyl_ctx *c) {
     const uint64_t es = c->mixed ? sizeof(float) : sizeof(double);
@@ -265,8 +220,10 @@ void set_tendency_out(nyl_ctx *c, StageArgs<T> &a, int slot) {
 // pre_done: the first sweep already ran into y (speculatively, with the
 // convergence test of the previous V-cycle, or from e = 0 in the outer
 // residual of the mixed solve).
+// addx (level 0 of the mixed solve): the last sweep adds its result to the
+// FP64 solution instead of storing it (tiled levels), or an add follows.
 template <typename Tf>
-int smooth_level_t(nyl_ctx *c, int l, int xc_level, bool pre_done) {
+int smooth_level_t(nyl_ctx *c, int l, int xc_level, bool pre_done, double *addx) {
     Level &f = c->lev[l];
     const double om = c->d.mg_omega;
     Tf *x = F<Tf>(c, f.x), *b = F<Tf>(c, f.b), *y = F<Tf>(c, f.y);
@@ -302,19 +259,35 @@ int smooth_level_t(nyl_ctx *c, int l, int xc_level, bool pre_done) {
         if (!pre_done)
             LAUNCH(NYL_K_MG_SWEEP, by, launch_mg_sweep25<Tf>(f.g, x, y, b, om, c->tabs, nullptr,
                                                               nullptr, c->stream));
-        LAUNCH(NYL_K_MG_SWEEP, by, launch_mg_sweep25<Tf>(f.g, y, x, b, om, c->tabs, nullptr,
-                                                          nullptr, c->stream));
+        if (addx) {
+            const double bx = (2 * sizeof(Tf) + 2 * sizeof(double)) * cells(f.g);
+            LAUNCH(NYL_K_MG_SWEEP, bx, launch_mg_sweep25<Tf>(f.g, y, x, b, om, c->tabs, nullptr,
+                                                              nullptr, c->stream, nullptr,
+                                                              nullptr, addx));
+        } else {
+            LAUNCH(NYL_K_MG_SWEEP, by, launch_mg_sweep25<Tf>(f.g, y, x, b, om, c->tabs, nullptr,
+                                                              nullptr, c->stream));
+        }
         return NYL_OK;
     }
     if (!pre_done)
         LAUNCH(NYL_K_MG_SWEEP, by, launch_mg_sweep<Tf>(f.g, x, y, b, om, c->tabs, c->stream));
     LAUNCH(NYL_K_MG_SWEEP, by, launch_mg_sweep<Tf>(f.g, y, x, b, om, c->tabs, c->stream));
+    if (addx) {
+        if constexpr (std::is_same<Tf, float>::value) {
+            LAUNCH(NYL_K_MG_UPDATE, (2 * sizeof(double) + sizeof(float)) * cells(f.g),
+                   launch_add_correction<float>(f.g, addx, x, c->stream));
+        } else {
+            return fail(NYL_E_STATE, "FP64 inner level");
+        }
+    }
     return NYL_OK;
 }
 
-int smooth_level(nyl_ctx *c, int l, int xc_level, bool pre_done = false) {
-    return lev64(c, l) ? smooth_level_t<double>(c, l, xc_level, pre_done)
-                       : smooth_level_t<float>(c, l, xc_level, pre_done);
+int smooth_level(nyl_ctx *c, int l, int xc_level, bool pre_done = false,
+                 double *addx = nullptr) {
+    return lev64(c, l) ? smooth_level_t<double>(c, l, xc_level, pre_done, addx)
+                       : smooth_level_t<float>(c, l, xc_level, pre_done, addx);
 }
 
 // residual of level l restricted into level l + 1 (coarse x = 0)
@@ -356,7 +329,8 @@ int restrict_level(nyl_ctx *c, int l) {
 
 // core/mgfor/solvers.f90:35-55.  pre_done: the level-1 pre-smooth already
 // ran into the spare buffer.
-int vcycle(nyl_ctx *c, bool pre_done) {
+// addx: the mixed solve's x, which receives the level-1 result (x += e)
+int vcycle(nyl_ctx *c, bool pre_done, double *addx = nullptr) {
     const int L = c->nlev;
     int rc;
     for (int l = 0; l < L - 1; l++) {
@@ -365,7 +339,7 @@ int vcycle(nyl_ctx *c, bool pre_done) {
     }
     if ((rc = smooth_level(c, L - 1, -1))) return rc;
     for (int l = L - 2; l >= 0; l--)
-        if ((rc = smooth_level(c, l, l + 1))) return rc;
+        if ((rc = smooth_level(c, l, l + 1, false, l == 0 ? addx : nullptr))) return rc;
     return NYL_OK;
 }
 
@@ -436,7 +410,7 @@ int mg_solve_mixed(nyl_ctx *c, nyl_mg_stats *st) {
     Level &f = c->lev[0];
     double *part = F<double>(c, c->partials), *res_dev = F<double>(c, c->result);
     double *xo = F<double>(c, c->mgx), *bo = F<double>(c, c->mgb);
-    float *e = F<float>(c, f.x), *r = F<float>(c, f.b), *y = F<float>(c, f.y);
+    float *r = F<float>(c, f.b), *y = F<float>(c, f.y);
     int rc;
     double s2[2], rn, res;
     HIPCHK(launch_sum_sumsq(f.g, bo, part, res_dev, c->stream));
@@ -463,9 +437,7 @@ int mg_solve_mixed(nyl_ctx *c, nyl_mg_stats *st) {
     int nite = 0;
     for (;;) {
         if (res < c->d.mg_tol) break;
-        if ((rc = vcycle(c, true))) return rc;
-        LAUNCH(NYL_K_MG_UPDATE, (2 * sizeof(double) + sizeof(float)) * cells(f.g),
-               launch_add_correction<float>(f.g, xo, e, c->stream));
+        if ((rc = vcycle(c, true, xo))) return rc;   // ends with x += e
         nite = nite + 1;
         if (nite >= c->d.mg_maxite) break;
         if ((rc = outer(&res))) return rc;

@@ -135,4 +135,28 @@ __device__ __forceinline__ T face_flux(int f, int n, T u, const QF &q) {
     return u * w;
 }
 
+// ---------------------------------------------------------------------------
+// Ensemble batches: the item (one per member) owning flattened block `blk`
+// (items sorted by h.start, h.start of the first item is 0).
+template <typename I>
+__device__ __forceinline__ const I &find_item(const I *items, int n, int64_t blk) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (items[mid].h.start <= blk) lo = mid;
+        else hi = mid - 1;
+    }
+    return items[lo];
+}
+
+// cell (i, j, k) of flattened index q of a level, false past its last cell
+__device__ __forceinline__ bool cell_of(const Geo &g, int64_t q, int &i, int &j, int &k) {
+    if (q >= (int64_t)g.nx * g.ny * g.nz) return false;
+    i = (int)(q % g.nx);
+    const int64_t t = q / g.nx;
+    j = (int)(t % g.ny);
+    k = (int)(t / g.ny);
+    return true;
+}
+
 }  // namespace nyl

@@ -114,15 +114,89 @@ bool sweep25_fits(const Geo &g);
 int sweep25_blocks(const Geo &g);
 // gc/xc non-null: the sweep reads src + prolongation of xc (level gc) in
 // place of src (fused k_mg_prolong; src itself is not modified).
+// addx non-null: the result is added to the FP64 addx instead of stored.
 template <typename T>
 hipError_t launch_mg_sweep25(const Geo &g, const T *src, T *dst, const T *b, double omega,
                              const MGTabs &tab, double *partials, double *dev_out,
-                             hipStream_t s, const Geo *gc = nullptr, const T *xc = nullptr);
+                             hipStream_t s, const Geo *gc = nullptr, const T *xc = nullptr,
+                             double *addx = nullptr);
 // residual + restriction with 2.5-D tiling (sweep25_fits and even nz)
 bool restrict25_fits(const Geo &g);
 template <typename Tf, typename Tc>
 hipError_t launch_mg_restrict25(const Geo &gf, const Geo &gc, const Tf *x, const Tf *b, Tc *bc,
                                 Tc *xc, const MGTabs &tab, hipStream_t s);
+
+// ---- ensemble batches -------------------------------------------------------
+// One member's share of a batched launch: the geometry of the level the
+// launch works on, the member's index in the batch, its number of blocks and
+// its first block in the flattened grid (items are sorted by start).  Cells
+// of a member are flattened in x-fastest order, one thread per cell.
+struct BHead {
+    Geo g;
+    int member;
+    int nblk;
+    int64_t start;
+};
+template <typename T> struct StageItem { BHead h; StageArgs<T> a; };
+template <typename T> struct DivItem { BHead h; DivArgs<T, double> a; };
+template <typename T> struct CorrItem { BHead h; CorrectArgs<T, double> a; };
+template <typename T> struct DiagItem { BHead h; DiagArgs<T> a; };
+template <typename T> struct SweepItem { BHead h; const T *src; T *dst; const T *b; double omega; };
+template <typename Tf, typename Tc>
+struct RestrItem { BHead h; Geo gc; const Tf *x; const Tf *b; Tc *bc; Tc *xc; };
+template <typename Tf, typename Tc>
+struct ProlItem { BHead h; Geo gc; Tf *xf; const Tc *xc; };
+// per-member partial sums: v (norm2 / sum and sum of squares) or x, b (residual)
+template <typename T> struct RedItem { BHead h; const T *v; const T *b; double *partials; };
+struct OuterItem { BHead h; const double *x; const double *b; float *r; float *y; double omega;
+                   double *partials; };
+struct AddItem { BHead h; double *x; const float *e; };
+
+// Device-side multigrid control of a batch: the stop rule of mgfor's solve
+// (core/mgfor/solvers.f90:17-30) evaluated per member on the device.
+struct BMG {
+    double *normb, *res, *shift, *tol;
+    int *nite, *maxite, *active, *any;
+    const double *dt;
+};
+// one member's final reduction: partials[0..nparts) of the member
+struct FinalItem { const double *partials; int nparts; int member; double ncell; };
+#define NYL_FIN_NORMB 0   // normb = sum
+#define NYL_FIN_INIT 1    // res = sum/normb, active = !(res < tol) && maxite > 0, nite = 0
+#define NYL_FIN_STEP 2    // nite++; active = nite < maxite && !(res = sum/normb < tol)
+#define NYL_FIN_SHIFT 3   // shift = sum/n, normb = sumsq - n shift^2 (two partial sets)
+
+#define NYL_BATCH_TPB 256
+int batch_blocks(int64_t ncell);   // blocks of one member at NYL_BATCH_TPB threads
+
+template <typename T>
+hipError_t launch_b_stage(const StageItem<T> *items, int n, int64_t nblk, int mode, int perio,
+                          int tpb, const double *dt, hipStream_t s);
+template <typename T>
+hipError_t launch_b_div(const DivItem<T> *items, int n, int64_t nblk, hipStream_t s);
+template <typename T>
+hipError_t launch_b_correct(const CorrItem<T> *items, int n, int64_t nblk, hipStream_t s);
+template <typename T>
+hipError_t launch_b_diag(const DiagItem<T> *items, int n, int64_t nblk, int what, hipStream_t s);
+template <typename T>
+hipError_t launch_b_sweep(const SweepItem<T> *items, int n, int64_t nblk, const MGTabs &tab,
+                          const int *active, hipStream_t s);
+template <typename Tf, typename Tc>
+hipError_t launch_b_restrict(const RestrItem<Tf, Tc> *items, int n, int64_t nblk,
+                             const MGTabs &tab, const int *active, hipStream_t s);
+template <typename Tf, typename Tc>
+hipError_t launch_b_prolong(const ProlItem<Tf, Tc> *items, int n, int64_t nblk,
+                            const MGTabs &tab, const int *active, hipStream_t s);
+// kind 0: sum v^2; 1: sum of squared residuals of (v = x, b); 2: sum v and sum v^2
+// (second set of partials at partials + nblk of the member)
+template <typename T>
+hipError_t launch_b_reduce(const RedItem<T> *items, int n, int64_t nblk, int kind,
+                           const MGTabs &tab, const int *active, hipStream_t s);
+hipError_t launch_b_outer_resid(const OuterItem *items, int n, int64_t nblk, const BMG &mg,
+                                const MGTabs &tab, hipStream_t s);
+hipError_t launch_b_add(const AddItem *items, int n, int64_t nblk, const int *active,
+                        hipStream_t s);
+hipError_t launch_b_final(const FinalItem *items, int n, int kind, const BMG &mg, hipStream_t s);
 
 // reductions: partials then one final block; result written to dev_out[0]
 #define NYL_RED_BLOCKS 16384

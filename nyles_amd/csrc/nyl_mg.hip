@@ -46,6 +46,17 @@ __device__ __forceinline__ Nbrs<T> neighbours(const Geo &g, const T *__restrict_
 // One damped-Jacobi sweep of fsmoother3d (core/mgfor/basicoperators.f90:363-400):
 // dst = cff1*src + omega*(sum6(src) - b)*idiag.
 template <typename T>
+__device__ __forceinline__ void sweep_cell(const Geo &g, const T *__restrict__ src,
+                                           T *__restrict__ dst, const T *__restrict__ b,
+                                           double omega, const MGTabs &tab, int i, int j, int k) {
+    const int64_t c = (int64_t)k * g.sz + (int64_t)j * g.sy + i;
+    const Nbrs<T> n = neighbours<T>(g, src, i, j, k, c);
+    const T S = n.xm + n.xp + n.ym + n.yp + n.zm + n.zp;
+    const T om = T(omega), cff1 = T(1.0 - omega);
+    dst[c] = cff1 * src[c] + om * (S - b[c]) * T(tab.idiag[n.nwet]);
+}
+
+template <typename T>
 __global__ __launch_bounds__(256) void k_mg_sweep(Geo g, const T *__restrict__ src,
                                                   T *__restrict__ dst, const T *__restrict__ b,
                                                   double omega, MGTabs tab) {
@@ -54,11 +65,7 @@ __global__ __launch_bounds__(256) void k_mg_sweep(Geo g, const T *__restrict__ s
     const int j = bk.y * MG_BY + threadIdx.y;
     const int k = bk.z;
     if (i >= g.nx || j >= g.ny) return;
-    const int64_t c = (int64_t)k * g.sz + (int64_t)j * g.sy + i;
-    const Nbrs<T> n = neighbours<T>(g, src, i, j, k, c);
-    const T S = n.xm + n.xp + n.ym + n.yp + n.zm + n.zp;
-    const T om = T(omega), cff1 = T(1.0 - omega);
-    dst[c] = cff1 * src[c] + om * (S - b[c]) * T(tab.idiag[n.nwet]);
+    sweep_cell<T>(g, src, dst, b, omega, tab, i, j, k);
 }
 
 template <typename T>
@@ -84,6 +91,12 @@ __device__ __forceinline__ T residual_at(const Geo &g, const T *__restrict__ x,
 // (core/mgfor/basicoperators.f90:32-60) with Rcoef = 1/2; coarse x = 0
 // (core/mgfor/operators.f90:209).
 template <typename Tf, typename Tc>
+__device__ __forceinline__ void restrict_cell(const Geo &gf, const Geo &gc,
+                                              const Tf *__restrict__ x, const Tf *__restrict__ b,
+                                              Tc *__restrict__ bc, Tc *__restrict__ xc,
+                                              const MGTabs &tab, int ic, int jc, int kc);
+
+template <typename Tf, typename Tc>
 __global__ __launch_bounds__(256) void k_mg_resid_restrict(Geo gf, Geo gc,
                                                            const Tf *__restrict__ x,
                                                            const Tf *__restrict__ b,
@@ -94,6 +107,14 @@ __global__ __launch_bounds__(256) void k_mg_resid_restrict(Geo gf, Geo gc,
     const int jc = bk.y * MG_BY + threadIdx.y;
     const int kc = bk.z;
     if (ic >= gc.nx || jc >= gc.ny) return;
+    restrict_cell<Tf, Tc>(gf, gc, x, b, bc, xc, tab, ic, jc, kc);
+}
+
+template <typename Tf, typename Tc>
+__device__ __forceinline__ void restrict_cell(const Geo &gf, const Geo &gc,
+                                              const Tf *__restrict__ x, const Tf *__restrict__ b,
+                                              Tc *__restrict__ bc, Tc *__restrict__ xc,
+                                              const MGTabs &tab, int ic, int jc, int kc) {
     const int i = 2 * ic, j = 2 * jc, k = 2 * kc;
     const Tf r000 = residual_at<Tf>(gf, x, b, i, j, k, tab);
     const Tf r100 = residual_at<Tf>(gf, x, b, i + 1, j, k, tab);
@@ -215,13 +236,18 @@ __device__ __forceinline__ void block_max_store(double v, double *out, int tid) 
 //          value k_mg_prolong would store, same expression), on the own cells
 //          and the ring; the corrected x is only read by this sweep (the next
 //          sweep reads dst), so it is not stored.
-template <typename T, bool RESNORM, bool PROLONG>
+//
+// ADDX: the result is not stored; it is added to the FP64 xo instead (the
+//       last sweep of an inner V-cycle of the mixed solve, whose output e is
+//       only ever used for x += e).
+template <typename T, bool RESNORM, bool PROLONG, bool ADDX>
 __global__ __launch_bounds__(256) void k_mg_sweep25(Geo g, const T *__restrict__ src,
                                                     T *__restrict__ dst,
                                                     const T *__restrict__ b, double omega,
                                                     MGTabs tab, int kz,
                                                     double *__restrict__ partials, Geo gc,
-                                                    const T *__restrict__ xc) {
+                                                    const T *__restrict__ xc,
+                                                    double *__restrict__ xo) {
     constexpr int LX = SW_TX + 2, LY = SW_TY + 2, NRING = 2 * LX + 2 * SW_TY;
     __shared__ T sp[3][LY][LX];
     const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * SW_TX + tx;
@@ -303,7 +329,9 @@ __global__ __launch_bounds__(256) void k_mg_sweep25(Geo g, const T *__restrict__
                 const double rr = (double)(bv + T(tab.diag[nw]) * z0[r] - S);
                 acc = acc + rr * rr;
             }
-            dst[c] = cff1 * z0[r] + om * (S - bv) * T(tab.idiag[nw]);
+            const T v = cff1 * z0[r] + om * (S - bv) * T(tab.idiag[nw]);
+            if (ADDX) xo[c] = xo[c] + (double)v;
+            else dst[c] = v;
             zm[r] = z0[r];
             z0[r] = zp[r];
         }
@@ -605,18 +633,6 @@ bool outer_resid_fits(const Geo &g) {
     return (int64_t)grd.x * grd.y <= NYL_RED_BLOCKS;
 }
 
-template <typename TI>
-hipError_t launch_mg_outer_resid(const Geo &g, const double *x, const double *b, double shift,
-                                 TI *r, TI *y, double omega, const MGTabs &tab, double *partials,
-                                 double *dev_out, hipStream_t s) {
-    const dim3 grd = outer_grid(g);
-    hipLaunchKernelGGL((k_mg_outer_resid<TI>), grd, dim3(MG_BX, MG_BY, 1), 0, s, g, x, b, shift,
-                       r, y, omega, tab, partials);
-    const int nb = grd.x * grd.y * grd.z;
-    hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(RED_T), 0, s, partials, nb, dev_out);
-    return hipGetLastError();
-}
-
 // x = x + e (the outer correction, in FP64)
 template <typename TI>
 __global__ __launch_bounds__(256) void k_add_correction(int64_t n, double *__restrict__ x,
@@ -707,6 +723,233 @@ hipError_t launch_kesum(const Geo &g, const T *u, const T *v, const T *w, double
     return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// Ensemble batches (items as in the stencil kernels; `active` skips the
+// blocks of members whose solve has stopped).
+#define BT NYL_BATCH_TPB
+
+template <typename T>
+__global__ __launch_bounds__(BT) void k_b_sweep(const SweepItem<T> *__restrict__ items, int n,
+                                                MGTabs tab, const int *__restrict__ active) {
+    const SweepItem<T> &it = find_item(items, n, blockIdx.x);
+    if (active && !active[it.h.member]) return;
+    int i, j, k;
+    if (!cell_of(it.h.g, ((int64_t)blockIdx.x - it.h.start) * BT + threadIdx.x, i, j, k)) return;
+    sweep_cell<T>(it.h.g, it.src, it.dst, it.b, it.omega, tab, i, j, k);
+}
+
+template <typename T>
+hipError_t launch_b_sweep(const SweepItem<T> *items, int n, int64_t nblk, const MGTabs &tab,
+                          const int *active, hipStream_t s) {
+    hipLaunchKernelGGL((k_b_sweep<T>), dim3((unsigned)nblk), BT, 0, s, items, n, tab, active);
+    return hipGetLastError();
+}
+
+// flattened over the coarse cells
+template <typename Tf, typename Tc>
+__global__ __launch_bounds__(BT) void k_b_restrict(const RestrItem<Tf, Tc> *__restrict__ items,
+                                                   int n, MGTabs tab,
+                                                   const int *__restrict__ active) {
+    const RestrItem<Tf, Tc> &it = find_item(items, n, blockIdx.x);
+    if (active && !active[it.h.member]) return;
+    int i, j, k;
+    if (!cell_of(it.gc, ((int64_t)blockIdx.x - it.h.start) * BT + threadIdx.x, i, j, k)) return;
+    restrict_cell<Tf, Tc>(it.h.g, it.gc, it.x, it.b, it.bc, it.xc, tab, i, j, k);
+}
+
+template <typename Tf, typename Tc>
+hipError_t launch_b_restrict(const RestrItem<Tf, Tc> *items, int n, int64_t nblk,
+                             const MGTabs &tab, const int *active, hipStream_t s) {
+    hipLaunchKernelGGL((k_b_restrict<Tf, Tc>), dim3((unsigned)nblk), BT, 0, s, items, n, tab,
+                       active);
+    return hipGetLastError();
+}
+
+template <typename Tf, typename Tc>
+__global__ __launch_bounds__(BT) void k_b_prolong(const ProlItem<Tf, Tc> *__restrict__ items,
+                                                  int n, MGTabs tab,
+                                                  const int *__restrict__ active) {
+    const ProlItem<Tf, Tc> &it = find_item(items, n, blockIdx.x);
+    if (active && !active[it.h.member]) return;
+    int i, j, k;
+    if (!cell_of(it.h.g, ((int64_t)blockIdx.x - it.h.start) * BT + threadIdx.x, i, j, k)) return;
+    const int64_t c = (int64_t)k * it.h.g.sz + (int64_t)j * it.h.g.sy + i;
+    it.xf[c] = it.xf[c] + prolong_term<Tf, Tc>(it.gc, it.xc, tab, i, j, k);
+}
+
+template <typename Tf, typename Tc>
+hipError_t launch_b_prolong(const ProlItem<Tf, Tc> *items, int n, int64_t nblk,
+                            const MGTabs &tab, const int *active, hipStream_t s) {
+    hipLaunchKernelGGL((k_b_prolong<Tf, Tc>), dim3((unsigned)nblk), BT, 0, s, items, n, tab,
+                       active);
+    return hipGetLastError();
+}
+
+// Per-member partial sums: each member owns h.nblk blocks that stride over
+// its cells, so its partials (and their order) do not depend on the batch.
+template <typename T, int KIND>
+__global__ __launch_bounds__(BT) void k_b_reduce(const RedItem<T> *__restrict__ items, int n,
+                                                 MGTabs tab, const int *__restrict__ active) {
+    const RedItem<T> &it = find_item(items, n, blockIdx.x);
+    if (active && !active[it.h.member]) return;
+    const Geo &g = it.h.g;
+    const int64_t nc = (int64_t)g.nx * g.ny * g.nz;
+    const int64_t lb = (int64_t)blockIdx.x - it.h.start;
+    double a1 = 0.0, a2 = 0.0;
+    for (int64_t q = lb * BT + threadIdx.x; q < nc; q += (int64_t)it.h.nblk * BT) {
+        if (KIND == 1) {
+            const int i = (int)(q % g.nx);
+            const int64_t t = q / g.nx;
+            const double r = (double)residual_at<T>(g, it.v, it.b, i, (int)(t % g.ny),
+                                                    (int)(t / g.ny), tab);
+            a1 = a1 + r * r;
+        } else {
+            const double x = (double)it.v[q];
+            if (KIND == 2) {
+                a1 = a1 + x;
+                a2 = a2 + x * x;
+            } else {
+                a1 = a1 + x * x;
+            }
+        }
+    }
+    block_sum_store(a1, it.partials + lb, threadIdx.x);
+    if (KIND == 2) {
+        __syncthreads();
+        block_sum_store(a2, it.partials + it.h.nblk + lb, threadIdx.x);
+    }
+}
+
+template <typename T>
+hipError_t launch_b_reduce(const RedItem<T> *items, int n, int64_t nblk, int kind,
+                           const MGTabs &tab, const int *active, hipStream_t s) {
+    const dim3 grd((unsigned)nblk);
+    if (kind == 0) hipLaunchKernelGGL((k_b_reduce<T, 0>), grd, BT, 0, s, items, n, tab, active);
+    else if (kind == 1) hipLaunchKernelGGL((k_b_reduce<T, 1>), grd, BT, 0, s, items, n, tab, active);
+    else hipLaunchKernelGGL((k_b_reduce<T, 2>), grd, BT, 0, s, items, n, tab, active);
+    return hipGetLastError();
+}
+
+// the outer FP64 residual of the mixed solve (k_mg_outer_resid per member)
+__global__ __launch_bounds__(BT) void k_b_outer_resid(const OuterItem *__restrict__ items, int n,
+                                                      BMG mg, MGTabs tab) {
+    const OuterItem &it = find_item(items, n, blockIdx.x);
+    if (!mg.active[it.h.member]) return;
+    const Geo &g = it.h.g;
+    const int64_t nc = (int64_t)g.nx * g.ny * g.nz;
+    const int64_t lb = (int64_t)blockIdx.x - it.h.start;
+    const double shift = mg.shift[it.h.member];
+    const float om = float(it.omega), cff1 = float(1.0 - it.omega);
+    double acc = 0.0;
+    for (int64_t q = lb * BT + threadIdx.x; q < nc; q += (int64_t)it.h.nblk * BT) {
+        const int i = (int)(q % g.nx);
+        const int64_t t = q / g.nx;
+        const int j = (int)(t % g.ny), k = (int)(t / g.ny);
+        const Nbrs<double> nb = neighbours<double>(g, it.x, i, j, k, q);
+        const double r = (it.b[q] - shift) + tab.diag[nb.nwet] * it.x[q]
+                       - (nb.xm + nb.xp + nb.ym + nb.yp + nb.zm + nb.zp);
+        acc = acc + r * r;
+        const float rf = float(r);
+        it.r[q] = rf;
+        it.y[q] = cff1 * 0.0f + om * (0.0f - rf) * float(tab.idiag[nb.nwet]);
+    }
+    block_sum_store(acc, it.partials + lb, threadIdx.x);
+}
+
+hipError_t launch_b_outer_resid(const OuterItem *items, int n, int64_t nblk, const BMG &mg,
+                                const MGTabs &tab, hipStream_t s) {
+    hipLaunchKernelGGL(k_b_outer_resid, dim3((unsigned)nblk), BT, 0, s, items, n, mg, tab);
+    return hipGetLastError();
+}
+
+__global__ __launch_bounds__(BT) void k_b_add(const AddItem *__restrict__ items, int n,
+                                              const int *__restrict__ active) {
+    const AddItem &it = find_item(items, n, blockIdx.x);
+    if (!active[it.h.member]) return;
+    const int64_t q = ((int64_t)blockIdx.x - it.h.start) * BT + threadIdx.x;
+    if (q < (int64_t)it.h.g.nx * it.h.g.ny * it.h.g.nz) it.x[q] = it.x[q] + (double)it.e[q];
+}
+
+hipError_t launch_b_add(const AddItem *items, int n, int64_t nblk, const int *active,
+                        hipStream_t s) {
+    hipLaunchKernelGGL(k_b_add, dim3((unsigned)nblk), BT, 0, s, items, n, active);
+    return hipGetLastError();
+}
+
+// One block per member: its partials summed in a fixed order, then the stop
+// rule of mgfor's solve loop (core/mgfor/solvers.f90:17-30).
+template <int KIND>
+__global__ __launch_bounds__(RED_T) void k_b_final(const FinalItem *__restrict__ items, BMG mg) {
+    const FinalItem &it = items[blockIdx.x];
+    const int m = it.member;
+    if ((KIND == NYL_FIN_STEP) && !mg.active[m]) return;
+    __shared__ double tot;
+    double acc = 0.0;
+    for (int q = threadIdx.x; q < it.nparts; q += RED_T) acc = acc + it.partials[q];
+    block_sum_store(acc, &tot, threadIdx.x);
+    double acc2 = 0.0, tot2 = 0.0;
+    if (KIND == NYL_FIN_SHIFT) {
+        __syncthreads();
+        for (int q = threadIdx.x; q < it.nparts; q += RED_T) acc2 = acc2 + it.partials[it.nparts + q];
+        __shared__ double t2;
+        block_sum_store(acc2, &t2, threadIdx.x);
+        __syncthreads();
+        tot2 = t2;
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    const double s1 = tot;
+    if (KIND == NYL_FIN_NORMB) {
+        mg.normb[m] = s1;
+    } else if (KIND == NYL_FIN_SHIFT) {
+        const double shift = s1 / it.ncell;
+        double nb = tot2 - it.ncell * shift * shift;
+        mg.shift[m] = shift;
+        mg.normb[m] = nb < 0.0 ? 0.0 : nb;
+    } else if (KIND == NYL_FIN_INIT) {
+        const double nb = mg.normb[m];
+        const double res = nb > 0.0 ? s1 / nb : 0.0;
+        mg.res[m] = res;
+        mg.nite[m] = 0;
+        mg.active[m] = (!(res < mg.tol[m]) && mg.maxite[m] > 0) ? 1 : 0;
+    } else {   // NYL_FIN_STEP
+        const int nite = mg.nite[m] + 1;
+        mg.nite[m] = nite;
+        if (nite >= mg.maxite[m]) {
+            mg.active[m] = 0;
+        } else {
+            const double nb = mg.normb[m];
+            const double res = nb > 0.0 ? s1 / nb : 0.0;
+            mg.res[m] = res;
+            mg.active[m] = res < mg.tol[m] ? 0 : 1;
+        }
+    }
+}
+
+// mg.any[0] = number of members still iterating
+__global__ __launch_bounds__(RED_T) void k_b_count(int n, BMG mg) {
+    __shared__ double tot;
+    double acc = 0.0;
+    for (int q = threadIdx.x; q < n; q += RED_T) acc = acc + (double)mg.active[q];
+    block_sum_store(acc, &tot, threadIdx.x);
+    __syncthreads();
+    if (threadIdx.x == 0) mg.any[0] = (int)tot;
+}
+
+hipError_t launch_b_final(const FinalItem *items, int n, int kind, const BMG &mg, hipStream_t s) {
+    switch (kind) {
+    case NYL_FIN_NORMB: hipLaunchKernelGGL((k_b_final<NYL_FIN_NORMB>), dim3(n), RED_T, 0, s, items, mg); break;
+    case NYL_FIN_INIT: hipLaunchKernelGGL((k_b_final<NYL_FIN_INIT>), dim3(n), RED_T, 0, s, items, mg); break;
+    case NYL_FIN_STEP: hipLaunchKernelGGL((k_b_final<NYL_FIN_STEP>), dim3(n), RED_T, 0, s, items, mg); break;
+    default: hipLaunchKernelGGL((k_b_final<NYL_FIN_SHIFT>), dim3(n), RED_T, 0, s, items, mg); break;
+    }
+    if (kind == NYL_FIN_INIT || kind == NYL_FIN_STEP)
+        hipLaunchKernelGGL(k_b_count, dim3(1), RED_T, 0, s, n, mg);
+    return hipGetLastError();
+}
+
+#undef BT
+
 bool sweep25_fits(const Geo &g) {
     return g.nx % SW_TX == 0 && g.ny % SW_TY == 0 && g.nx >= SW_TX;
 }
@@ -726,22 +969,140 @@ int sweep25_blocks(const Geo &g) {
 template <typename T>
 hipError_t launch_mg_sweep25(const Geo &g, const T *src, T *dst, const T *b, double omega,
                              const MGTabs &tab, double *partials, double *dev_out,
-                             hipStream_t s, const Geo *gc, const T *xc) {
+                             hipStream_t s, const Geo *gc, const T *xc, double *addx) {
     const int kz = sweep25_kz(g, 1);
     dim3 grd(g.nx / SW_TX, g.ny / SW_TY, g.nz / kz), blk(SW_TX, 4);
     const Geo gcv = gc ? *gc : g;
     if (partials) {
-        hipLaunchKernelGGL((k_mg_sweep25<T, true, false>), grd, blk, 0, s, g, src, dst, b, omega,
-                           tab, kz, partials, gcv, xc);
+        hipLaunchKernelGGL((k_mg_sweep25<T, true, false, false>), grd, blk, 0, s, g, src, dst, b,
+                           omega, tab, kz, partials, gcv, xc, addx);
         const int nb = grd.x * grd.y * grd.z;
         hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(RED_T), 0, s, partials, nb, dev_out);
+    } else if (addx) {
+        hipLaunchKernelGGL((k_mg_sweep25<T, false, false, true>), grd, blk, 0, s, g, src, dst, b,
+                           omega, tab, kz, partials, gcv, xc, addx);
     } else if (xc) {
-        hipLaunchKernelGGL((k_mg_sweep25<T, false, true>), grd, blk, 0, s, g, src, dst, b, omega,
-                           tab, kz, partials, gcv, xc);
+        hipLaunchKernelGGL((k_mg_sweep25<T, false, true, false>), grd, blk, 0, s, g, src, dst, b,
+                           omega, tab, kz, partials, gcv, xc, addx);
     } else {
-        hipLaunchKernelGGL((k_mg_sweep25<T, false, false>), grd, blk, 0, s, g, src, dst, b,
-                           omega, tab, kz, partials, gcv, xc);
+        hipLaunchKernelGGL((k_mg_sweep25<T, false, false, false>), grd, blk, 0, s, g, src, dst,
+                           b, omega, tab, kz, partials, gcv, xc, addx);
     }
+    return hipGetLastError();
+}
+
+// Outer residual of the mixed solve with the 2.5-D tiling of k_mg_sweep25
+// (FP64 planes of x in LDS): per cell the values of k_mg_outer_resid (same
+// operand order), the block sums of r**2 into partials[block].
+template <typename TI>
+__global__ __launch_bounds__(256) void k_mg_outer25(Geo g, const double *__restrict__ x,
+                                                    const double *__restrict__ b, double shift,
+                                                    TI *__restrict__ r_out,
+                                                    TI *__restrict__ y_out, double omega,
+                                                    MGTabs tab, int kz,
+                                                    double *__restrict__ partials) {
+    constexpr int LX = SW_TX + 2, LY = SW_TY + 2, NRING = 2 * LX + 2 * SW_TY;
+    __shared__ double sp[3][LY][LX];
+    const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * SW_TX + tx;
+    const Blk3 bk = xcd_block();
+    const int i0 = bk.x * SW_TX, j0 = bk.y * SW_TY, k0 = bk.z * kz;
+    const int i = i0 + tx;
+    const TI om = TI(omega), cff1 = TI(1.0 - omega);
+
+    int64_t roff = -1;
+    int rlx = 0, rly = 0;
+    if (tid < NRING) {
+        const int q = tid;
+        if (q < LX) { rlx = q; rly = 0; }
+        else if (q < 2 * LX) { rlx = q - LX; rly = LY - 1; }
+        else { const int t = q - 2 * LX; rly = 1 + (t >> 1); rlx = (t & 1) ? LX - 1 : 0; }
+        int gi = i0 + rlx - 1, gj = j0 + rly - 1;
+        bool in = true;
+        if (g.px) gi = wrapi(gi, g.nx); else in = in && gi >= 0 && gi < g.nx;
+        if (g.py) gj = wrapi(gj, g.ny); else in = in && gj >= 0 && gj < g.ny;
+        if (in) roff = (int64_t)gj * g.sy + gi;
+    }
+    auto zok = [&](int m) { return g.pz || (m >= 0 && m < g.nz); };
+    auto zw = [&](int m) { return g.pz ? wrapi(m, g.nz) : m; };
+    auto load_plane = [&](int m, int buf, double *own) {
+        const bool zin = zok(m);
+        const int64_t base = (int64_t)zw(m) * g.sz;
+#pragma unroll
+        for (int r = 0; r < SW_R; r++) {
+            const int j = j0 + ty + 4 * r;
+            own[r] = zin ? x[base + (int64_t)j * g.sy + i] : 0.0;
+            sp[buf][1 + ty + 4 * r][1 + tx] = own[r];
+        }
+        if (tid < NRING) sp[buf][rly][rlx] = (zin && roff >= 0) ? x[base + roff] : 0.0;
+    };
+    int nxy[SW_R];
+#pragma unroll
+    for (int r = 0; r < SW_R; r++) {
+        const int j = j0 + ty + 4 * r;
+        nxy[r] = (g.px || i > 0) + (g.px || i < g.nx - 1) + (g.py || j > 0) + (g.py || j < g.ny - 1);
+    }
+    double zm[SW_R], z0[SW_R], zp[SW_R];
+    double acc = 0.0;
+    load_plane(k0 - 1, 0, zm);
+    load_plane(k0, 1, z0);
+    for (int k = k0; k < k0 + kz; k++) {
+        const int cur = (k - k0 + 1) % 3;
+        load_plane(k + 1, (k - k0 + 2) % 3, zp);
+        __syncthreads();
+        const int nz_ = (g.pz || k > 0) + (g.pz || k < g.nz - 1);
+#pragma unroll
+        for (int r = 0; r < SW_R; r++) {
+            const int ly = 1 + ty + 4 * r, lx = 1 + tx;
+            const int j = j0 + ty + 4 * r;
+            const double S = sp[cur][ly][lx - 1] + sp[cur][ly][lx + 1] + sp[cur][ly - 1][lx]
+                           + sp[cur][ly + 1][lx] + zm[r] + zp[r];
+            const int64_t c = (int64_t)k * g.sz + (int64_t)j * g.sy + i;
+            const int nw = nxy[r] + nz_;
+            const double rr = (b[c] - shift) + tab.diag[nw] * z0[r] - S;
+            acc = acc + rr * rr;
+            const TI rf = TI(rr);
+            r_out[c] = rf;
+            y_out[c] = cff1 * TI(0) + om * (TI(0) - rf) * TI(tab.idiag[nw]);
+            zm[r] = z0[r];
+            z0[r] = zp[r];
+        }
+    }
+    __syncthreads();
+    const int nb = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+    block_sum_store(acc, partials + nb, tid);
+}
+
+// planes per block of the tiled outer residual: as many as divide nz up to 8
+static int outer25_kz(const Geo &g) {
+    int kz = 8;
+    while (kz > 1 && g.nz % kz != 0) kz >>= 1;
+    return kz;
+}
+
+bool outer25_fits(const Geo &g) {
+    if (!sweep25_fits(g)) return false;
+    const int64_t nb = (int64_t)(g.nx / SW_TX) * (g.ny / SW_TY) * (g.nz / outer25_kz(g));
+    return nb <= NYL_RED_BLOCKS;
+}
+
+template <typename TI>
+hipError_t launch_mg_outer_resid(const Geo &g, const double *x, const double *b, double shift,
+                                 TI *r, TI *y, double omega, const MGTabs &tab, double *partials,
+                                 double *dev_out, hipStream_t s) {
+    if (outer25_fits(g)) {
+        const int kz = outer25_kz(g);
+        const dim3 grd(g.nx / SW_TX, g.ny / SW_TY, g.nz / kz), blk(SW_TX, 4);
+        hipLaunchKernelGGL((k_mg_outer25<TI>), grd, blk, 0, s, g, x, b, shift, r, y, omega, tab,
+                           kz, partials);
+        const int nb = grd.x * grd.y * grd.z;
+        hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(RED_T), 0, s, partials, nb, dev_out);
+        return hipGetLastError();
+    }
+    const dim3 grd = outer_grid(g);
+    hipLaunchKernelGGL((k_mg_outer_resid<TI>), grd, dim3(MG_BX, MG_BY, 1), 0, s, g, x, b, shift,
+                       r, y, omega, tab, partials);
+    const int nb = grd.x * grd.y * grd.z;
+    hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(RED_T), 0, s, partials, nb, dev_out);
     return hipGetLastError();
 }
 
@@ -761,7 +1122,7 @@ hipError_t launch_mg_restrict25(const Geo &gf, const Geo &gc, const Tf *x, const
 #define NYL_MG_INST(T)                                                                          \
     template hipError_t launch_mg_sweep25<T>(const Geo &, const T *, T *, const T *, double,    \
                                              const MGTabs &, double *, double *, hipStream_t,   \
-                                             const Geo *, const T *);                           \
+                                             const Geo *, const T *, double *);                 \
     template hipError_t launch_mg_restrict25<T, T>(const Geo &, const Geo &, const T *,         \
                                                    const T *, T *, T *, const MGTabs &,         \
                                                    hipStream_t);                                \
@@ -783,6 +1144,24 @@ hipError_t launch_mg_restrict25(const Geo &gf, const Geo &gc, const Tf *x, const
 NYL_MG_INST(double)
 NYL_MG_INST(float)
 #undef NYL_MG_INST
+
+#define NYL_B_MG_INST(T)                                                                         \
+    template hipError_t launch_b_sweep<T>(const SweepItem<T> *, int, int64_t, const MGTabs &,   \
+                                          const int *, hipStream_t);                            \
+    template hipError_t launch_b_restrict<T, T>(const RestrItem<T, T> *, int, int64_t,          \
+                                                const MGTabs &, const int *, hipStream_t);      \
+    template hipError_t launch_b_prolong<T, T>(const ProlItem<T, T> *, int, int64_t,            \
+                                               const MGTabs &, const int *, hipStream_t);       \
+    template hipError_t launch_b_reduce<T>(const RedItem<T> *, int, int64_t, int,               \
+                                           const MGTabs &, const int *, hipStream_t);
+NYL_B_MG_INST(double)
+NYL_B_MG_INST(float)
+#undef NYL_B_MG_INST
+template hipError_t launch_b_restrict<float, double>(const RestrItem<float, double> *, int,
+                                                     int64_t, const MGTabs &, const int *,
+                                                     hipStream_t);
+template hipError_t launch_b_prolong<float, double>(const ProlItem<float, double> *, int, int64_t,
+                                                    const MGTabs &, const int *, hipStream_t);
 
 template hipError_t launch_resnorm2<double>(const Geo &, const double *, const double *,
                                             const MGTabs &, double *, double *, hipStream_t);

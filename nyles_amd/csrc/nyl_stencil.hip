@@ -385,6 +385,127 @@ hipError_t launch_fill(double *p, int64_t n, double v, hipStream_t s) {
     return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// Ensemble batches: one launch covers every member (BHead), each block a
+// tile of TPB consecutive cells of one member, running the single-member
+// cell functions.  The stage kernel is persistent: a fixed grid strides over
+// all tiles; its RK coefficient is the host's dt, dt/4., dt/12. of the
+// member's dt (same IEEE operations).
+template <typename T, int MODE, int PX, int PY, int PZ, int TPB>
+__global__ __launch_bounds__(TPB) void k_b_stage(const StageItem<T> *__restrict__ items, int n,
+                                                 int64_t nblk, const double *__restrict__ dt) {
+    for (int64_t blk = blockIdx.x; blk < nblk; blk += gridDim.x) {
+        const StageItem<T> &it = find_item(items, n, blk);
+        int i, j, k;
+        if (!cell_of(it.h.g, (blk - it.h.start) * TPB + threadIdx.x, i, j, k)) continue;
+        StageArgs<T> a = it.a;
+        const double d = dt[it.h.member];
+        a.c = MODE == 1 ? d : (MODE == 2 ? d / 4. : d / 12.);
+        stage_cell<T, MODE, PX, PY, PZ>(it.h.g, a, i, j, k);
+    }
+}
+
+int batch_blocks(int64_t ncell) { return (int)((ncell + NYL_BATCH_TPB - 1) / NYL_BATCH_TPB); }
+
+template <typename T, int PX, int PY, int PZ, int TPB>
+static void b_stage_p(const StageItem<T> *items, int n, int64_t nblk, int mode, const double *dt,
+                      hipStream_t s) {
+    const unsigned grid = (unsigned)(nblk < 2048 ? nblk : 2048);
+    switch (mode) {
+    case 1: hipLaunchKernelGGL((k_b_stage<T, 1, PX, PY, PZ, TPB>), grid, TPB, 0, s, items, n, nblk, dt); break;
+    case 2: hipLaunchKernelGGL((k_b_stage<T, 2, PX, PY, PZ, TPB>), grid, TPB, 0, s, items, n, nblk, dt); break;
+    default: hipLaunchKernelGGL((k_b_stage<T, 3, PX, PY, PZ, TPB>), grid, TPB, 0, s, items, n, nblk, dt); break;
+    }
+}
+
+template <typename T, int TPB>
+static void b_stage_t(const StageItem<T> *items, int n, int64_t nblk, int mode, int perio,
+                      const double *dt, hipStream_t s) {
+    switch (perio) {
+    case 0: b_stage_p<T, 0, 0, 0, TPB>(items, n, nblk, mode, dt, s); break;
+    case 1: b_stage_p<T, 1, 0, 0, TPB>(items, n, nblk, mode, dt, s); break;
+    case 2: b_stage_p<T, 0, 1, 0, TPB>(items, n, nblk, mode, dt, s); break;
+    case 3: b_stage_p<T, 1, 1, 0, TPB>(items, n, nblk, mode, dt, s); break;
+    case 4: b_stage_p<T, 0, 0, 1, TPB>(items, n, nblk, mode, dt, s); break;
+    case 5: b_stage_p<T, 1, 0, 1, TPB>(items, n, nblk, mode, dt, s); break;
+    case 6: b_stage_p<T, 0, 1, 1, TPB>(items, n, nblk, mode, dt, s); break;
+    default: b_stage_p<T, 1, 1, 1, TPB>(items, n, nblk, mode, dt, s); break;
+    }
+}
+
+// tpb: threads per tile (the autotuned shape: a tile is tpb consecutive cells
+// of a member, i.e. tpb/nx rows of its x lines)
+template <typename T>
+hipError_t launch_b_stage(const StageItem<T> *items, int n, int64_t nblk, int mode, int perio,
+                          int tpb, const double *dt, hipStream_t s) {
+    if (mode < 1 || mode > 3 || nblk <= 0) return hipErrorInvalidValue;
+    switch (tpb) {
+    case 128: b_stage_t<T, 128>(items, n, nblk, mode, perio, dt, s); break;
+    case 256: b_stage_t<T, 256>(items, n, nblk, mode, perio, dt, s); break;
+    case 512: b_stage_t<T, 512>(items, n, nblk, mode, perio, dt, s); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+template <typename T>
+__global__ __launch_bounds__(NYL_BATCH_TPB) void k_b_div(const DivItem<T> *__restrict__ items,
+                                                         int n) {
+    const DivItem<T> &it = find_item(items, n, blockIdx.x);
+    int i, j, k;
+    if (!cell_of(it.h.g, ((int64_t)blockIdx.x - it.h.start) * NYL_BATCH_TPB + threadIdx.x, i, j, k))
+        return;
+    div_cell<T, double>(it.h.g, it.a, i, j, k);
+}
+
+template <typename T>
+hipError_t launch_b_div(const DivItem<T> *items, int n, int64_t nblk, hipStream_t s) {
+    hipLaunchKernelGGL((k_b_div<T>), dim3((unsigned)nblk), NYL_BATCH_TPB, 0, s, items, n);
+    return hipGetLastError();
+}
+
+template <typename T>
+__global__ __launch_bounds__(NYL_BATCH_TPB) void k_b_correct(const CorrItem<T> *__restrict__ items,
+                                                             int n) {
+    const CorrItem<T> &it = find_item(items, n, blockIdx.x);
+    int i, j, k;
+    if (!cell_of(it.h.g, ((int64_t)blockIdx.x - it.h.start) * NYL_BATCH_TPB + threadIdx.x, i, j, k))
+        return;
+    correct_cell<T, double>(it.h.g, it.a, i, j, k);
+}
+
+template <typename T>
+hipError_t launch_b_correct(const CorrItem<T> *items, int n, int64_t nblk, hipStream_t s) {
+    hipLaunchKernelGGL((k_b_correct<T>), dim3((unsigned)nblk), NYL_BATCH_TPB, 0, s, items, n);
+    return hipGetLastError();
+}
+
+template <typename T>
+__global__ __launch_bounds__(NYL_BATCH_TPB) void k_b_diag(const DiagItem<T> *__restrict__ items,
+                                                          int n, int what) {
+    const DiagItem<T> &it = find_item(items, n, blockIdx.x);
+    int i, j, k;
+    if (!cell_of(it.h.g, ((int64_t)blockIdx.x - it.h.start) * NYL_BATCH_TPB + threadIdx.x, i, j, k))
+        return;
+    diag_cell<T>(it.h.g, it.a, what, i, j, k);
+}
+
+template <typename T>
+hipError_t launch_b_diag(const DiagItem<T> *items, int n, int64_t nblk, int what, hipStream_t s) {
+    hipLaunchKernelGGL((k_b_diag<T>), dim3((unsigned)nblk), NYL_BATCH_TPB, 0, s, items, n, what);
+    return hipGetLastError();
+}
+
+#define NYL_B_STENCIL_INST(T)                                                                    \
+    template hipError_t launch_b_stage<T>(const StageItem<T> *, int, int64_t, int, int, int,    \
+                                          const double *, hipStream_t);                         \
+    template hipError_t launch_b_div<T>(const DivItem<T> *, int, int64_t, hipStream_t);         \
+    template hipError_t launch_b_correct<T>(const CorrItem<T> *, int, int64_t, hipStream_t);    \
+    template hipError_t launch_b_diag<T>(const DiagItem<T> *, int, int64_t, int, hipStream_t);
+NYL_B_STENCIL_INST(double)
+NYL_B_STENCIL_INST(float)
+#undef NYL_B_STENCIL_INST
+
 template hipError_t launch_stage<double>(const Geo &, const StageArgs<double> &, int, hipStream_t);
 template hipError_t launch_stage<float>(const Geo &, const StageArgs<float> &, int, hipStream_t);
 template hipError_t launch_diag<double>(const Geo &, const DiagArgs<double> &, int, hipStream_t);
