@@ -50,11 +50,12 @@ extern "C" {
 
 /* precision modes */
 #define NYL_PREC_FP64 0     /* loop-faithful FP64, bit-compatible with the oracle */
-/* NYL_PREC_MIXED: model fields (state, tendencies, U, vor, ke) and their
- * stencils in FP32; the projection is defect correction with FP32 V-cycles
- * inside an FP64 outer residual, the right-hand side's mean removed in FP64,
- * an FP64 coarsest level; NYL_RED_* reductions accumulate in FP64.  No
- * counterpart in the reference (which is FP64 throughout). */
+/* NYL_PREC_MIXED: model fields (state, tendencies, vor, ke) and their
+ * stencils in FP32; the projection is iterative refinement: FP32 V-cycles
+ * (FP64 coarsest level) on the correction inside an FP64 outer residual, the
+ * right-hand side's mean removed in FP64, mgfor's stop rule and V-cycle
+ * count; NYL_RED_* reductions accumulate in FP64.  No counterpart in the
+ * reference (which is FP64 throughout). */
 #define NYL_PREC_MIXED 1
 
 typedef struct nyl_ctx nyl_ctx;
@@ -219,6 +220,37 @@ int nyl_reduce_last(nyl_ctx *ctx, double *out);
 
 /* Wait for all work of the context's stream. */
 int nyl_sync(nyl_ctx *ctx);
+
+/* ---------------------------------------------------------------------------
+ * Ensembles (no counterpart in the reference, which runs one model per
+ * process).  A batch advances existing contexts (the members: bound, any
+ * grid size and geometry, one precision mode) together by RK3_SSP steps:
+ * each RK stage is one persistent stage-kernel launch per periodicity class
+ * plus one launch per kernel class for the projection of every member, the
+ * multigrid stop rule of each member (core/mgfor/solvers.f90:17-30) runs on
+ * the device and every plan is replayed from a hipGraph.  A member's result
+ * does not depend on the other members.  The batch workspace (size from
+ * nyl_batch_workspace_bytes, 256-byte aligned) holds the item tables; the
+ * members keep their own workspaces.  Members may still be used on their
+ * own between batch steps.  Mixed-precision members run one FP32 V-cycle per
+ * FP64 outer residual (defect correction) in a batch. */
+typedef struct nyl_batch nyl_batch;
+int nyl_batch_create(nyl_ctx *const *members, int32_t n, nyl_batch **out);
+int nyl_batch_destroy(nyl_batch *batch);
+int nyl_batch_workspace_bytes(const nyl_batch *batch, uint64_t *bytes);
+int nyl_batch_bind_workspace(nyl_batch *batch, void *dptr, uint64_t bytes);
+int nyl_batch_set_stream(nyl_batch *batch, void *hip_stream);
+/* on = 0: launch the plans directly instead of from hipGraphs */
+int nyl_batch_set_graphs(nyl_batch *batch, int on);
+/* Times RK stage 1 of the batch for stage tiles of 128, 256 and 512 cells
+ * (ms_out[3], may be NULL), keeps the fastest (tpb_out).  It overwrites the
+ * members' RK3 scratch (next state buffer, tendency slot 0) only. */
+int nyl_batch_autotune(nyl_batch *batch, int32_t *tpb_out, double *ms_out);
+/* One RK3_SSP step of every member, member q with time step dt[q];
+ * stats[q] (may be NULL): V-cycles of the step, last residual and normb.
+ * Work queued on the members' own streams must be complete before the call;
+ * the call returns when the step is complete. */
+int nyl_batch_step_rk3(nyl_batch *batch, const double *dt, nyl_mg_stats *stats);
 
 #ifdef __cplusplus
 }

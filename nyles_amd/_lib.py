@@ -44,6 +44,9 @@ EXPORTED = [
     "nyl_mg_levels", "nyl_mg_level_shape", "nyl_rhs", "nyl_rk3_stage", "nyl_diagnose",
     "nyl_step_rk3", "nyl_step_ef", "nyl_step_lfam3", "nyl_lfam3_reset", "nyl_mg_solve", "nyl_apply", "nyl_reduce", "nyl_prof_start",
     "nyl_prof_stop", "nyl_debug_mg", "nyl_reduce_last", "nyl_sync",
+    "nyl_batch_create", "nyl_batch_destroy", "nyl_batch_workspace_bytes",
+    "nyl_batch_bind_workspace", "nyl_batch_set_stream", "nyl_batch_set_graphs",
+    "nyl_batch_autotune", "nyl_batch_step_rk3",
 ]
 
 # kernel classes of nyl_prof_start
@@ -120,6 +123,14 @@ def load():
         "nyl_debug_mg": (ci, [vp, ci]),
         "nyl_reduce_last": (ci, [vp, P(dbl)]),
         "nyl_sync": (ci, [vp]),
+        "nyl_batch_create": (ci, [P(vp), ctypes.c_int32, P(vp)]),
+        "nyl_batch_destroy": (ci, [vp]),
+        "nyl_batch_workspace_bytes": (ci, [vp, P(u64)]),
+        "nyl_batch_bind_workspace": (ci, [vp, vp, u64]),
+        "nyl_batch_set_stream": (ci, [vp, vp]),
+        "nyl_batch_set_graphs": (ci, [vp, ci]),
+        "nyl_batch_autotune": (ci, [vp, P(ctypes.c_int32), P(dbl)]),
+        "nyl_batch_step_rk3": (ci, [vp, P(dbl), P(NylMGStats)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

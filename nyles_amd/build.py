@@ -13,7 +13,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libnyles_hip.so")
-SOURCES = ["nyl_api.hip", "nyl_stencil.hip", "nyl_mg.hip"]
+SOURCES = ["nyl_api.hip", "nyl_stencil.hip", "nyl_mg.hip", "nyl_batch.hip"]
 ARCH = os.environ.get("NYL_OFFLOAD_ARCH", "gfx950")
 
 FLAGS = [

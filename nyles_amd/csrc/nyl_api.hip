@@ -24,6 +24,20 @@ int fail(int code, const char *fmt, ...) {
     return code;
 }
 
+}  // namespace
+
+namespace nyl {
+int set_error(int code, const char *fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+    return code;
+}
+}  // namespace nyl
+
+namespace {
+
 #define HIPCHK(...)                                                                    \
     do {                                                                               \
         hipError_t e_ = (__VA_ARGS__);                                                 \
@@ -401,16 +415,21 @@ int mg_solve_fp64(nyl_ctx *c, nyl_mg_stats *st) {
     return NYL_OK;
 }
 
-// Mixed precision: the same stop test and iteration count rule as mgfor's
-// solve, as defect correction.  The right-hand side's mean is removed in FP64
-// (compatibility of the singular Neumann/periodic problem); each outer step
-// computes r = b - A x in FP64, runs one FP32 V-cycle on A e = r from e = 0
-// and adds e to x in FP64.  The norms are those of mgfor (squared ratio).
+// Mixed precision: iterative refinement with the stop rule of mgfor's solve.
+// The right-hand side's mean is removed in FP64 (compatibility of the
+// singular Neumann/periodic problem).  An outer step computes r = b - A x in
+// FP64 and solves A e = r from e = 0 with FP32 V-cycles under mgfor's loop:
+// before each further V-cycle the squared residual r - A e = b - A(x + e),
+// evaluated in FP32 and summed in FP64, is tested against tol*|b|^2 (fused
+// into that V-cycle's first sweep, as in the FP64 path).  Then x += e in FP64
+// and the FP64 residual confirms the stop (or starts another outer step).
+// The V-cycle count follows the same rule as mgfor's (nite, maxite).
 int mg_solve_mixed(nyl_ctx *c, nyl_mg_stats *st) {
     Level &f = c->lev[0];
     double *part = F<double>(c, c->partials), *res_dev = F<double>(c, c->result);
     double *xo = F<double>(c, c->mgx), *bo = F<double>(c, c->mgb);
-    float *r = F<float>(c, f.b), *y = F<float>(c, f.y);
+    float *e = F<float>(c, f.x), *r = F<float>(c, f.b), *y = F<float>(c, f.y);
+    const bool spec = sweep25_fits(f.g) && sweep25_blocks(f.g) <= NYL_RED_BLOCKS;
     int rc;
     double s2[2], rn, res;
     HIPCHK(launch_sum_sumsq(f.g, bo, part, res_dev, c->stream));
@@ -420,6 +439,7 @@ int mg_solve_mixed(nyl_ctx *c, nyl_mg_stats *st) {
     double normb = s2[1] - n * shift * shift;
     if (normb < 0.0) normb = 0.0;
     c->mg_shift = shift;
+    // FP64 residual of x into r (and the first sweep from e = 0 into y)
     auto outer = [&](double *out) -> int {
         if (normb > 0.0) {
             LAUNCH(NYL_K_RESNORM, (2 * sizeof(double) + 2 * sizeof(float)) * cells(f.g),
@@ -433,13 +453,36 @@ int mg_solve_mixed(nyl_ctx *c, nyl_mg_stats *st) {
         }
         return NYL_OK;
     };
+    // FP32 residual r - A e (speculative first sweep of the next V-cycle)
+    auto inner = [&](double *out) -> int {
+        if (spec) {
+            LAUNCH(NYL_K_MG_SWEEP, 3 * cells(f.g) * sizeof(float),
+                   launch_mg_sweep25<float>(f.g, e, y, r, c->d.mg_omega, c->tabs, part, res_dev,
+                                            c->stream));
+        } else {
+            LAUNCH(NYL_K_RESNORM, 2 * cells(f.g) * sizeof(float),
+                   launch_resnorm2<float>(f.g, e, r, c->tabs, part, res_dev, c->stream));
+        }
+        int r2 = read_result(c, &rn);
+        if (r2) return r2;
+        *out = rn / normb;
+        return NYL_OK;
+    };
     if ((rc = outer(&res))) return rc;
     int nite = 0;
-    for (;;) {
-        if (res < c->d.mg_tol) break;
-        if ((rc = vcycle(c, true, xo))) return rc;   // ends with x += e
-        nite = nite + 1;
-        if (nite >= c->d.mg_maxite) break;
+    while (!(res < c->d.mg_tol) && nite < c->d.mg_maxite) {
+        bool pending = true;       // y holds the first sweep from e = 0
+        for (;;) {
+            if ((rc = vcycle(c, pending))) return rc;
+            nite = nite + 1;
+            if (nite >= c->d.mg_maxite) break;
+            double rin;
+            if ((rc = inner(&rin))) return rc;
+            pending = spec;
+            if (rin < c->d.mg_tol) break;
+        }
+        LAUNCH(NYL_K_MG_UPDATE, (2 * sizeof(double) + sizeof(float)) * cells(f.g),
+               launch_add_correction<float>(f.g, xo, e, c->stream));
         if ((rc = outer(&res))) return rc;
     }
     fill_stats(c, st, nite, res, normb);

@@ -52,6 +52,9 @@ namespace nyl {
 template <typename T>
 inline T *F(const nyl_ctx *c, uint64_t off) { return reinterpret_cast<T *>(c->ws + off); }
 
+// record an error message for nyl_last_error() and return code
+int set_error(int code, const char *fmt, ...);
+
 // Level l holds FP64 arrays in FP64 mode; in mixed mode the inner V-cycle
 // levels are FP32 and the coarsest level stays FP64.
 inline bool lev64(const nyl_ctx *c, int l) { return !c->mixed || l == c->nlev - 1; }

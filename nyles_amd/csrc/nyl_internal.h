@@ -157,6 +157,7 @@ struct AddItem { BHead h; double *x; const float *e; };
 struct BMG {
     double *normb, *res, *shift, *tol;
     int *nite, *maxite, *active, *any;
+    int *acc;              // V-cycles summed over the stages of a step
     const double *dt;
 };
 // one member's final reduction: partials[0..nparts) of the member
@@ -192,11 +193,14 @@ hipError_t launch_b_prolong(const ProlItem<Tf, Tc> *items, int n, int64_t nblk,
 template <typename T>
 hipError_t launch_b_reduce(const RedItem<T> *items, int n, int64_t nblk, int kind,
                            const MGTabs &tab, const int *active, hipStream_t s);
+// active null: every member (the initial residual of a solve)
 hipError_t launch_b_outer_resid(const OuterItem *items, int n, int64_t nblk, const BMG &mg,
-                                const MGTabs &tab, hipStream_t s);
+                                const MGTabs &tab, const int *active, hipStream_t s);
 hipError_t launch_b_add(const AddItem *items, int n, int64_t nblk, const int *active,
                         hipStream_t s);
 hipError_t launch_b_final(const FinalItem *items, int n, int kind, const BMG &mg, hipStream_t s);
+// mode 0: acc = 0; mode 1: acc += nite
+hipError_t launch_b_acc(int n, const BMG &mg, int mode, hipStream_t s);
 
 // reductions: partials then one final block; result written to dev_out[0]
 #define NYL_RED_BLOCKS 16384

@@ -832,9 +832,10 @@ hipError_t launch_b_reduce(const RedItem<T> *items, int n, int64_t nblk, int kin
 
 // the outer FP64 residual of the mixed solve (k_mg_outer_resid per member)
 __global__ __launch_bounds__(BT) void k_b_outer_resid(const OuterItem *__restrict__ items, int n,
-                                                      BMG mg, MGTabs tab) {
+                                                      BMG mg, MGTabs tab,
+                                                      const int *__restrict__ active) {
     const OuterItem &it = find_item(items, n, blockIdx.x);
-    if (!mg.active[it.h.member]) return;
+    if (active && !active[it.h.member]) return;
     const Geo &g = it.h.g;
     const int64_t nc = (int64_t)g.nx * g.ny * g.nz;
     const int64_t lb = (int64_t)blockIdx.x - it.h.start;
@@ -857,8 +858,9 @@ __global__ __launch_bounds__(BT) void k_b_outer_resid(const OuterItem *__restric
 }
 
 hipError_t launch_b_outer_resid(const OuterItem *items, int n, int64_t nblk, const BMG &mg,
-                                const MGTabs &tab, hipStream_t s) {
-    hipLaunchKernelGGL(k_b_outer_resid, dim3((unsigned)nblk), BT, 0, s, items, n, mg, tab);
+                                const MGTabs &tab, const int *active, hipStream_t s) {
+    hipLaunchKernelGGL(k_b_outer_resid, dim3((unsigned)nblk), BT, 0, s, items, n, mg, tab,
+                       active);
     return hipGetLastError();
 }
 
@@ -945,6 +947,15 @@ hipError_t launch_b_final(const FinalItem *items, int n, int kind, const BMG &mg
     }
     if (kind == NYL_FIN_INIT || kind == NYL_FIN_STEP)
         hipLaunchKernelGGL(k_b_count, dim3(1), RED_T, 0, s, n, mg);
+    return hipGetLastError();
+}
+
+__global__ __launch_bounds__(RED_T) void k_b_acc(int n, BMG mg, int mode) {
+    for (int q = threadIdx.x; q < n; q += RED_T) mg.acc[q] = mode ? mg.acc[q] + mg.nite[q] : 0;
+}
+
+hipError_t launch_b_acc(int n, const BMG &mg, int mode, hipStream_t s) {
+    hipLaunchKernelGGL(k_b_acc, dim3(1), RED_T, 0, s, n, mg, mode);
     return hipGetLastError();
 }
 
@@ -1165,6 +1176,8 @@ template hipError_t launch_b_prolong<float, double>(const ProlItem<float, double
 
 template hipError_t launch_resnorm2<double>(const Geo &, const double *, const double *,
                                             const MGTabs &, double *, double *, hipStream_t);
+template hipError_t launch_resnorm2<float>(const Geo &, const float *, const float *,
+                                           const MGTabs &, double *, double *, hipStream_t);
 // FP32 inner levels above an FP64 coarsest level
 template hipError_t launch_mg_resid_restrict<float, double>(const Geo &, const Geo &,
                                                             const float *, const float *,

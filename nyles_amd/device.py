@@ -184,3 +184,74 @@ class DeviceModel:
             self.close()
         except Exception:
             pass
+
+
+class DeviceBatch:
+    """An ensemble of DeviceModel members advanced together (nyl_batch_*).
+
+    Each RK stage is one persistent stage-kernel launch per periodicity class
+    and one launch per kernel class of the projection for all members, with
+    every member's multigrid stop rule evaluated on the device; the plans are
+    replayed from hipGraphs.  The batch runs on its own stream (default
+    priority); members keep their workspaces and may be used alone between
+    batch steps.
+    """
+
+    def __init__(self, members, graphs=True, autotune=True):
+        if not members:
+            raise ValueError("a batch needs at least one member")
+        self.lib = L.load()
+        self.members = list(members)
+        dev = self.members[0].device
+        arr = (ctypes.c_void_p * len(self.members))(*[m.ctx.value for m in self.members])
+        b = ctypes.c_void_p()
+        L.check(self.lib.nyl_batch_create(arr, len(self.members), ctypes.byref(b)))
+        self.batch = b
+        nbytes = ctypes.c_uint64()
+        L.check(self.lib.nyl_batch_workspace_bytes(b, ctypes.byref(nbytes)))
+        with torch.cuda.device(dev):
+            self.stream = torch.cuda.Stream(device=dev)
+            self.ws = torch.empty(max(nbytes.value, 256), dtype=torch.uint8, device=dev)
+        # the members' data must be ready before the batch stream reads it
+        for m in self.members:
+            m.stream.synchronize()
+        L.check(self.lib.nyl_batch_set_stream(b, ctypes.c_void_p(self.stream.cuda_stream)))
+        L.check(self.lib.nyl_batch_bind_workspace(b, ctypes.c_void_p(self.ws.data_ptr()),
+                                                  nbytes.value))
+        L.check(self.lib.nyl_batch_set_graphs(b, int(bool(graphs))))
+        self.tile = 256
+        self.tune_ms = None
+        if autotune:
+            self.autotune()
+
+    def autotune(self):
+        tpb = ctypes.c_int32()
+        ms = (ctypes.c_double * 3)()
+        L.check(self.lib.nyl_batch_autotune(self.batch, ctypes.byref(tpb), ms))
+        self.tile = tpb.value
+        self.tune_ms = {128: ms[0], 256: ms[1], 512: ms[2]}
+        return self.tile
+
+    def step_rk3(self, dts):
+        """one RK3_SSP step of every member (member q with dts[q])"""
+        n = len(self.members)
+        if len(dts) != n:
+            raise ValueError("one dt per member")
+        arr = (ctypes.c_double * n)(*[float(d) for d in dts])
+        st = (L.NylMGStats * n)()
+        for m in self.members:          # the batch stream reads what they wrote
+            m.stream.synchronize()
+        L.check(self.lib.nyl_batch_step_rk3(self.batch, arr, st))
+        return [s.as_dict() for s in st]
+
+    def close(self):
+        if getattr(self, "batch", None) is not None and self.batch.value:
+            self.stream.synchronize()
+            self.lib.nyl_batch_destroy(self.batch)
+            self.batch = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
