@@ -15,6 +15,11 @@ is timed in the same run, and both are advanced --err-steps steps from the
 same initial state to state the error of the mixed path against FP64 (KE,
 velocity, buoyancy).
 
+Beside the headline, the same run measures an ensemble of 64 members of
+uneven sizes advanced by batched steps against the same members stepped one
+after another, and a serving session (Poisson job arrivals, continuous
+batching) reported as p50/p99 time-to-completion and throughput.
+
 Prints ONE JSON line.  ``roofline`` prices the dominant kernel class of the
 headline path (the class with the largest kernel time per step): achieved =
 its algorithmic HBM bytes per launch (each input field read once, each output
@@ -50,7 +55,75 @@ def parse():
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--breakdown", action="store_true",
                     help="report the per-class kernel times of both paths")
+    ap.add_argument("--ensemble-members", type=int, default=64,
+                    help="members of the ensemble run (0: skip)")
+    ap.add_argument("--serving-jobs", type=int, default=40,
+                    help="jobs of the serving run (0: skip)")
+    ap.add_argument("--serving-rate", type=float, default=10.0, help="job arrivals per second")
     return ap.parse_args()
+
+
+def ensemble_run(members, steps, precision="mixed", seed=1000):
+    """Ensemble of stratified shear layers (perio_xy) of uneven sizes: members
+    stepped one after another through the single-model API, then the same
+    ensemble advanced by batched steps (one set of launches per RK stage)."""
+    import numpy as np
+    import torch
+    from nyles_amd import serving
+    from nyles_amd.device import DeviceBatch
+    rng = np.random.default_rng(seed)
+    # N log-uniform in [32, 96] restricted to the sizes the multigrid
+    # hierarchy accepts (it halves x and y down to 2): 32 and 64
+    sizes = [int(rng.choice([32, 64])) for _ in range(members)]
+    jobs = [serving.Job("shear", n, steps, seed=seed + q) for q, n in enumerate(sizes)]
+    cells = sum(n ** 3 for n in sizes)
+    out = {"members": members, "sizes": {"32": sizes.count(32), "64": sizes.count(64)},
+           "precision": precision, "steps": steps, "workload": "stratified shear layer "
+           "perio_xy (tanh jet, N^2 = 0.1, noise 1e-3), fixed dt at CFL 0.5"}
+    for mode in ("sequential", "batched"):
+        ms = [serving.make_model(j, precision) for j in jobs]
+        models = [m for m, _ in ms]
+        dts = [dt for _, dt in ms]
+        if mode == "batched":
+            b = DeviceBatch(models)
+            out["tile"], out["tune_ms"] = b.tile, b.tune_ms
+            b.step_rk3(dts)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                b.step_rk3(dts)
+            torch.cuda.synchronize()
+            el = time.perf_counter() - t0
+            b.close()
+        else:
+            for m, dt in zip(models, dts):
+                m.step_rk3(dt)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                for m, dt in zip(models, dts):
+                    m.step_rk3(dt)
+            torch.cuda.synchronize()
+            el = time.perf_counter() - t0
+        for m in models:
+            m.close()
+        out[mode + "_Mcell_steps_per_s"] = round(cells * steps / el / 1e6, 3)
+    out["batching_speedup"] = round(out["batched_Mcell_steps_per_s"] /
+                                    out["sequential_Mcell_steps_per_s"], 3)
+    return out
+
+
+def serving_run(count, rate, precision="mixed"):
+    """Poisson arrivals of lock-exchange, Taylor-Green and shear-layer jobs,
+    n in {32, 64, 128}, 50-200 steps each, served with continuous batching."""
+    from nyles_amd import serving
+    jobs = serving.poisson_jobs(count, rate)
+    rep = serving.Server(precision=precision).run(jobs)
+    rep["arrival_rate_per_s"] = rate
+    rep["workload"] = ("%d jobs, Poisson arrivals at %.1f/s; kind uniform over lock exchange "
+                       "(n x n/2 x n/2), Taylor-Green (n^3), shear layer (n^3); n uniform over "
+                       "{32, 64, 128}; steps uniform in [50, 200]" % (count, rate))
+    return {k: (round(v, 4) if isinstance(v, float) else v) for k, v in rep.items()}
 
 
 def cpu_baseline(n, steps):
@@ -232,6 +305,12 @@ def main():
     if args.err_steps > 0:
         print("error check, %d steps" % args.err_steps, file=sys.stderr, flush=True)
         out["error_vs_fp64"] = error_check(n, dt, args.err_steps)
+    if args.ensemble_members > 0:
+        print("ensemble, %d members" % args.ensemble_members, file=sys.stderr, flush=True)
+        out["ensemble"] = ensemble_run(args.ensemble_members, 10)
+    if args.serving_jobs > 0:
+        print("serving, %d jobs" % args.serving_jobs, file=sys.stderr, flush=True)
+        out["serving"] = serving_run(args.serving_jobs, args.serving_rate)
     if not args.no_cpu_baseline:
         print("cpu baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(n, args.cpu_steps)

@@ -246,6 +246,8 @@ int nyl_batch_set_graphs(nyl_batch *batch, int on);
  * (ms_out[3], may be NULL), keeps the fastest (tpb_out).  It overwrites the
  * members' RK3 scratch (next state buffer, tendency slot 0) only. */
 int nyl_batch_autotune(nyl_batch *batch, int32_t *tpb_out, double *ms_out);
+/* Sets the stage tile (128, 256 or 512 cells) without timing. */
+int nyl_batch_set_tile(nyl_batch *batch, int32_t tpb);
 /* One RK3_SSP step of every member, member q with time step dt[q];
  * stats[q] (may be NULL): V-cycles of the step, last residual and normb.
  * Work queued on the members' own streams must be complete before the call;

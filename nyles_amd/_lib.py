@@ -46,7 +46,7 @@ EXPORTED = [
     "nyl_prof_stop", "nyl_debug_mg", "nyl_reduce_last", "nyl_sync",
     "nyl_batch_create", "nyl_batch_destroy", "nyl_batch_workspace_bytes",
     "nyl_batch_bind_workspace", "nyl_batch_set_stream", "nyl_batch_set_graphs",
-    "nyl_batch_autotune", "nyl_batch_step_rk3",
+    "nyl_batch_autotune", "nyl_batch_set_tile", "nyl_batch_step_rk3",
 ]
 
 # kernel classes of nyl_prof_start
@@ -130,6 +130,7 @@ def load():
         "nyl_batch_set_stream": (ci, [vp, vp]),
         "nyl_batch_set_graphs": (ci, [vp, ci]),
         "nyl_batch_autotune": (ci, [vp, P(ctypes.c_int32), P(dbl)]),
+        "nyl_batch_set_tile": (ci, [vp, ctypes.c_int32]),
         "nyl_batch_step_rk3": (ci, [vp, P(dbl), P(NylMGStats)]),
     }
     for name, (res, args) in sig.items():

@@ -730,6 +730,15 @@ int nyl_batch_autotune(nyl_batch *b, int32_t *tpb_out, double *ms_out) {
     return NYL_OK;
 }
 
+int nyl_batch_set_tile(nyl_batch *b, int32_t tpb) {
+    if (!b || !b->ws) return set_error(NYL_E_STATE, "batch workspace not bound");
+    if (tpb != 128 && tpb != 256 && tpb != 512)
+        return set_error(NYL_E_ARG, "stage tile must be 128, 256 or 512 cells");
+    if (tpb == b->tpb) return NYL_OK;
+    b->tpb = tpb;
+    return rebuild(b);
+}
+
 int nyl_batch_step_rk3(nyl_batch *b, const double *dt, nyl_mg_stats *stats) {
     if (!b || !b->ws) return set_error(NYL_E_STATE, "batch workspace not bound");
     if (!dt) return set_error(NYL_E_ARG, "null dt");

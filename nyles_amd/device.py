@@ -197,7 +197,7 @@ class DeviceBatch:
     batch steps.
     """
 
-    def __init__(self, members, graphs=True, autotune=True):
+    def __init__(self, members, graphs=True, autotune=True, tile=None):
         if not members:
             raise ValueError("a batch needs at least one member")
         self.lib = L.load()
@@ -221,7 +221,9 @@ class DeviceBatch:
         L.check(self.lib.nyl_batch_set_graphs(b, int(bool(graphs))))
         self.tile = 256
         self.tune_ms = None
-        if autotune:
+        if tile is not None:
+            self.set_tile(tile)
+        elif autotune:
             self.autotune()
 
     def autotune(self):
@@ -231,6 +233,10 @@ class DeviceBatch:
         self.tile = tpb.value
         self.tune_ms = {128: ms[0], 256: ms[1], 512: ms[2]}
         return self.tile
+
+    def set_tile(self, tpb):
+        L.check(self.lib.nyl_batch_set_tile(self.batch, int(tpb)))
+        self.tile = int(tpb)
 
     def step_rk3(self, dts):
         """one RK3_SSP step of every member (member q with dts[q])"""
