@@ -20,8 +20,9 @@ uneven sizes advanced by batched steps against the same members stepped one
 after another, and a serving session (Poisson job arrivals, continuous
 batching) reported as p50/p99 time-to-completion and throughput.
 
-Prints ONE JSON line.  ``roofline`` prices the dominant kernel class of the
-headline path (the class with the largest kernel time per step): achieved =
+Prints ONE JSON line.  ``roofline`` prices the dominant kernel of the
+headline path, the damped-Jacobi smoother sweep on the finest multigrid level
+(k_mg_sweep25, the kernel with the largest time per step): achieved =
 its algorithmic HBM bytes per launch (each input field read once, each output
 written once) divided by its mean launch time, both measured with HIP events
 on the library's stream over the timed steps.  ``cpu_baseline`` times the
@@ -47,8 +48,9 @@ def parse():
     ap.add_argument("--n", type=int, default=256, help="grid cells per axis")
     ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed",
                     help="path of the headline value (the other one is timed beside it)")
-    ap.add_argument("--roofline-kernel", default=None,
-                    help="kernel class priced against the roofline (default: dominant)")
+    ap.add_argument("--roofline-kernel", default="mg_sweep_fine",
+                    help="kernel class priced against the roofline (default: the smoother "
+                         "sweep on the finest level, the kernel with the most time per step)")
     ap.add_argument("--err-steps", type=int, default=200,
                     help="steps of the mixed-vs-FP64 error check (0: skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -60,7 +62,24 @@ def parse():
     ap.add_argument("--serving-jobs", type=int, default=40,
                     help="jobs of the serving run (0: skip)")
     ap.add_argument("--serving-rate", type=float, default=10.0, help="job arrivals per second")
+    ap.add_argument("--single", action="store_true",
+                    help="time only the headline precision (short runs under a profiler)")
+    ap.add_argument("--copy-calibration-mb", type=int, default=0,
+                    help="only copy this many MB device to device 5 times (counter calibration)")
     return ap.parse_args()
+
+
+def copy_calibration(mb):
+    """A plain device-to-device copy of known size, to scale HBM byte counters."""
+    import torch
+    n = mb * (1 << 20)
+    a = torch.ones(n, dtype=torch.uint8, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes_read": n, "copy_calibration_bytes_written": n,
+                      "launches": 5}), flush=True)
 
 
 def ensemble_run(members, steps, precision="mixed", seed=1000):
@@ -124,6 +143,24 @@ def serving_run(count, rate, precision="mixed"):
                        "(n x n/2 x n/2), Taylor-Green (n^3), shear layer (n^3); n uniform over "
                        "{32, 64, 128}; steps uniform in [50, 200]" % (count, rate))
     return {k: (round(v, 4) if isinstance(v, float) else v) for k, v in rep.items()}
+
+
+def measured_traffic(kclass, precision, n):
+    """HBM bytes per launch of the roofline kernel, from the committed
+    rocprofv3 --pmc summary of the same kernel and workload (FETCH_SIZE and
+    WRITE_SIZE passes scaled by a copy of known size); None when no summary
+    covers this kernel and workload."""
+    if (kclass, precision, n) != ("mg_sweep_fine", "mixed", 256):
+        return None
+    path = os.path.join(ROOT, "profiles", "traffic_mg_sweep_fine_taylor_green_256_mixed.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch"):
+                    return int(line.split("=")[1])
+    except OSError:
+        return None
+    return None
 
 
 def cpu_baseline(n, steps):
@@ -249,19 +286,22 @@ def main():
     import torch
     from nyles_amd import _lib as L
 
+    if args.copy_calibration_mb > 0:
+        copy_calibration(args.copy_calibration_mb)
+        return
     n = args.n
     other = "fp64" if args.precision == "mixed" else "mixed"
     runs = {}
     dt = None
-    for prec in (args.precision, other):
+    for prec in (args.precision,) if args.single else (args.precision, other):
         print("timing %s %d^3" % (prec, n), file=sys.stderr, flush=True)
         dev = make_model(n, prec)
         if dt is None:
             dt = 0.5 / float(np.sqrt(dev.reduce(L.NYL_RED_MAXU2)))
         for _ in range(args.warmup):
             dev.step_rk3(dt)
-        br = class_times(dev, dt)
-        kclass = args.roofline_kernel or max(br, key=lambda k: br[k]["ms_per_step"])
+        br = class_times(dev, dt) if args.breakdown else {}
+        kclass = args.roofline_kernel
         el, nite, prof = timed(dev, dt, args.steps, 0, kclass)
         runs[prec] = {"el": el, "nite": nite, "prof": prof, "kclass": kclass, "breakdown": br}
         dev.close()
@@ -271,7 +311,7 @@ def main():
     value = n ** 3 * args.steps / head["el"] / 1e6
     kms, klaunch, kbytes = head["prof"]
     achieved = (kbytes / klaunch) / (kms / klaunch / 1e3) / 1e9 if klaunch else None
-    oth = runs[other]
+    oth = runs.get(other)
     out = {
         "metric": METRIC, "value": round(value, 3), "unit": "Mcell-steps/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -289,17 +329,18 @@ def main():
             "bound": "hbm", "kernel": head["kclass"],
             "achieved": round(achieved, 2) if achieved else None, "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
-            "traffic": None, "launches": klaunch,
+            "traffic": measured_traffic(kclass, args.precision, n), "launches": klaunch,
             "mean_launch_us": round(kms / klaunch * 1e3, 2) if klaunch else None,
             "alg_bytes_per_launch": kbytes / klaunch if klaunch else None,
         },
-        other: {
+    }
+    if oth is not None:
+        out[other] = {
             "value": round(n ** 3 * args.steps / oth["el"] / 1e6, 3),
             "ms_per_step": round(oth["el"] / args.steps * 1e3, 4),
             "mg_vcycles_per_step": round(oth["nite"] / args.steps, 3),
-        },
-    }
-    out["speedup_mixed_vs_fp64"] = round(runs["fp64"]["el"] / runs["mixed"]["el"], 3)
+        }
+        out["speedup_mixed_vs_fp64"] = round(runs["fp64"]["el"] / runs["mixed"]["el"], 3)
     if args.breakdown:
         out["breakdown"] = {p: runs[p]["breakdown"] for p in runs}
     if args.err_steps > 0:

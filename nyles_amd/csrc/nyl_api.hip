@@ -154,9 +154,13 @@ int prof_drain(nyl_ctx *c) {
 
 // Launch through `fn`; when the kernel class is being profiled, bracket the
 // launch with events on the context stream and count its algorithmic bytes.
+constexpr int kFineLevel = 0x100;   // launch works on the finest multigrid level
+
 template <typename Fn>
 int timed_launch(nyl_ctx *c, int cls, double bytes, Fn fn) {
-    const bool on = c->prof_cls == cls;
+    // a level-1 sweep also belongs to the NYL_K_MG_SWEEP_FINE class
+    const bool on = c->prof_cls == (cls & 0xff) ||
+                    (c->prof_cls == NYL_K_MG_SWEEP_FINE && (cls & kFineLevel));
     if (on) {
         if (c->prof_used * 2 + 2 > c->prof_nev) {
             const int cap = (int)(sizeof(c->prof_ev) / sizeof(c->prof_ev[0]));
@@ -241,6 +245,7 @@ int smooth_level_t(nyl_ctx *c, int l, int xc_level, bool pre_done, double *addx)
     Level &f = c->lev[l];
     const double om = c->d.mg_omega;
     Tf *x = F<Tf>(c, f.x), *b = F<Tf>(c, f.b), *y = F<Tf>(c, f.y);
+    const int SWC = NYL_K_MG_SWEEP | (l == 0 ? kFineLevel : 0);
     const double by = 3 * cells(f.g) * sizeof(Tf);
     const bool tiled = sweep25_fits(f.g);
     if (xc_level >= 0) {
@@ -249,10 +254,10 @@ int smooth_level_t(nyl_ctx *c, int l, int xc_level, bool pre_done, double *addx)
         if (NYL_FUSE_PROLONG && tiled && same) {
             // prolongation fused into the first sweep: x + P(xc) -> y, then y -> x
             const double bp = (3 + 0.125) * cells(f.g) * sizeof(Tf);
-            LAUNCH(NYL_K_MG_SWEEP, bp,
+            LAUNCH(SWC, bp,
                    launch_mg_sweep25<Tf>(f.g, x, y, b, om, c->tabs, nullptr, nullptr, c->stream,
                                          &q.g, F<Tf>(c, q.x)));
-            LAUNCH(NYL_K_MG_SWEEP, by, launch_mg_sweep25<Tf>(f.g, y, x, b, om, c->tabs, nullptr,
+            LAUNCH(SWC, by, launch_mg_sweep25<Tf>(f.g, y, x, b, om, c->tabs, nullptr,
                                                               nullptr, c->stream));
             return NYL_OK;
         }
@@ -271,22 +276,22 @@ int smooth_level_t(nyl_ctx *c, int l, int xc_level, bool pre_done, double *addx)
     }
     if (tiled) {
         if (!pre_done)
-            LAUNCH(NYL_K_MG_SWEEP, by, launch_mg_sweep25<Tf>(f.g, x, y, b, om, c->tabs, nullptr,
+            LAUNCH(SWC, by, launch_mg_sweep25<Tf>(f.g, x, y, b, om, c->tabs, nullptr,
                                                               nullptr, c->stream));
         if (addx) {
             const double bx = (2 * sizeof(Tf) + 2 * sizeof(double)) * cells(f.g);
-            LAUNCH(NYL_K_MG_SWEEP, bx, launch_mg_sweep25<Tf>(f.g, y, x, b, om, c->tabs, nullptr,
+            LAUNCH(SWC, bx, launch_mg_sweep25<Tf>(f.g, y, x, b, om, c->tabs, nullptr,
                                                               nullptr, c->stream, nullptr,
                                                               nullptr, addx));
         } else {
-            LAUNCH(NYL_K_MG_SWEEP, by, launch_mg_sweep25<Tf>(f.g, y, x, b, om, c->tabs, nullptr,
+            LAUNCH(SWC, by, launch_mg_sweep25<Tf>(f.g, y, x, b, om, c->tabs, nullptr,
                                                               nullptr, c->stream));
         }
         return NYL_OK;
     }
     if (!pre_done)
-        LAUNCH(NYL_K_MG_SWEEP, by, launch_mg_sweep<Tf>(f.g, x, y, b, om, c->tabs, c->stream));
-    LAUNCH(NYL_K_MG_SWEEP, by, launch_mg_sweep<Tf>(f.g, y, x, b, om, c->tabs, c->stream));
+        LAUNCH(SWC, by, launch_mg_sweep<Tf>(f.g, x, y, b, om, c->tabs, c->stream));
+    LAUNCH(SWC, by, launch_mg_sweep<Tf>(f.g, y, x, b, om, c->tabs, c->stream));
     if (addx) {
         if constexpr (std::is_same<Tf, float>::value) {
             LAUNCH(NYL_K_MG_UPDATE, (2 * sizeof(double) + sizeof(float)) * cells(f.g),
@@ -384,7 +389,7 @@ int mg_solve_fp64(nyl_ctx *c, nyl_mg_stats *st) {
         pending = false;
         if (normb > 0.0) {
             if (spec) {
-                LAUNCH(NYL_K_MG_SWEEP, 3 * cells(f.g) * sizeof(double),
+                LAUNCH(NYL_K_MG_SWEEP | kFineLevel, 3 * cells(f.g) * sizeof(double),
                        launch_mg_sweep25<double>(f.g, F<double>(c, f.x), F<double>(c, f.y),
                                                  F<double>(c, f.b), c->d.mg_omega, c->tabs,
                                                  part, res_dev, c->stream));
@@ -456,7 +461,7 @@ int mg_solve_mixed(nyl_ctx *c, nyl_mg_stats *st) {
     // FP32 residual r - A e (speculative first sweep of the next V-cycle)
     auto inner = [&](double *out) -> int {
         if (spec) {
-            LAUNCH(NYL_K_MG_SWEEP, 3 * cells(f.g) * sizeof(float),
+            LAUNCH(NYL_K_MG_SWEEP | kFineLevel, 3 * cells(f.g) * sizeof(float),
                    launch_mg_sweep25<float>(f.g, e, y, r, c->d.mg_omega, c->tabs, part, res_dev,
                                             c->stream));
         } else {
