@@ -262,7 +262,10 @@ int smooth_level_t(nyl_ctx *c, int l, int xc_level, bool pre_done, double *addx)
             return NYL_OK;
         }
         const double bp = 2.125 * cells(f.g) * sizeof(Tf);
-        if (lev64(c, xc_level)) {
+        if (same && restrict25_fits(f.g)) {
+            LAUNCH(NYL_K_MG_PROLONG, bp,
+                   launch_mg_prolong25<Tf, Tf>(f.g, q.g, x, F<Tf>(c, q.x), c->tabs, c->stream));
+        } else if (lev64(c, xc_level)) {
             LAUNCH(NYL_K_MG_PROLONG, bp,
                    launch_mg_prolong<Tf, double>(f.g, q.g, x, F<double>(c, q.x), c->tabs,
                                                  c->stream));

@@ -120,8 +120,12 @@ hipError_t launch_mg_sweep25(const Geo &g, const T *src, T *dst, const T *b, dou
                              const MGTabs &tab, double *partials, double *dev_out,
                              hipStream_t s, const Geo *gc = nullptr, const T *xc = nullptr,
                              double *addx = nullptr);
-// residual + restriction with 2.5-D tiling (sweep25_fits and even nz)
+// residual + restriction and prolongation with 2.5-D tiling (fine level
+// with sweep25_fits and even nz)
 bool restrict25_fits(const Geo &g);
+template <typename Tf, typename Tc>
+hipError_t launch_mg_prolong25(const Geo &gf, const Geo &gc, Tf *xf, const Tc *xc,
+                               const MGTabs &tab, hipStream_t s);
 template <typename Tf, typename Tc>
 hipError_t launch_mg_restrict25(const Geo &gf, const Geo &gc, const Tf *x, const Tf *b, Tc *bc,
                                 Tc *xc, const MGTabs &tab, hipStream_t s);

@@ -1117,6 +1117,104 @@ hipError_t launch_mg_outer_resid(const Geo &g, const double *x, const double *b,
     return hipGetLastError();
 }
 
+// Prolongation with 2.5-D tiling: a block owns fine 64 x 8 cells (rows
+// j0 + ty + 4r of column i0 + tx) and marches up kz fine planes (kz even).
+// The coarse values its cells use (a 34 x 6 footprint per coarse plane,
+// zero outside a wall axis, wrapped along a periodic one: what the
+// guarded reads of prolong_term return) sit in LDS for the three coarse
+// planes kc-1, kc, kc+1 of the current fine pair (2kc, 2kc+1).  Each fine
+// cell evaluates prolong_term's expression in its order: bitwise the same.
+#define PR_CX (SW_TX / 2 + 2)
+#define PR_CY (SW_TY / 2 + 2)
+template <typename Tf, typename Tc>
+__global__ __launch_bounds__(256) void k_mg_prolong25(Geo gf, Geo gc, Tf *__restrict__ xf,
+                                                      const Tc *__restrict__ xc, MGTabs tab,
+                                                      int kz) {
+    __shared__ Tf cp[3][PR_CY][PR_CX];
+    const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * SW_TX + tx;
+    const Blk3 bk = xcd_block();
+    const int i0 = bk.x * SW_TX, j0 = bk.y * SW_TY, k0 = bk.z * kz;
+    const int i = i0 + tx;
+    const int ci0 = (i0 >> 1) - 1, cj0 = (j0 >> 1) - 1;
+    // coarse footprint cell loaded by this thread (one per coarse plane)
+    int64_t coff = -1;
+    int clx = 0, cly = 0;
+    if (tid < PR_CX * PR_CY) {
+        clx = tid % PR_CX;
+        cly = tid / PR_CX;
+        int gi = ci0 + clx, gj = cj0 + cly;
+        bool in = true;
+        if (gc.px) gi = wrapi(gi, gc.nx); else in = in && gi >= 0 && gi < gc.nx;
+        if (gc.py) gj = wrapi(gj, gc.ny); else in = in && gj >= 0 && gj < gc.ny;
+        if (in) coff = (int64_t)gj * gc.sy + gi;
+    }
+    auto load_coarse = [&](int m, int buf) {
+        if (tid < PR_CX * PR_CY) {
+            const bool zin = gc.pz || (m >= 0 && m < gc.nz);
+            const int mw = gc.pz ? wrapi(m, gc.nz) : m;
+            cp[buf][cly][clx] = (zin && coff >= 0) ? Tf(xc[(int64_t)mw * gc.sz + coff]) : Tf(0);
+        }
+    };
+    // x and y parts of the stencil, fixed per thread and row
+    const int ic = i >> 1;
+    const int icn = (i & 1) ? ic + 1 : ic - 1;
+    const int mx = (gc.px || (icn >= 0 && icn < gc.nx)) ? 1 : 0;
+    const int lx = ic - ci0, lxn = icn - ci0;
+    int ly[SW_R], lyn[SW_R], my[SW_R];
+#pragma unroll
+    for (int r = 0; r < SW_R; r++) {
+        const int j = j0 + ty + 4 * r;
+        const int jc = j >> 1;
+        const int jcn = (j & 1) ? jc + 1 : jc - 1;
+        my[r] = (gc.py || (jcn >= 0 && jcn < gc.ny)) ? 1 : 0;
+        ly[r] = jc - cj0;
+        lyn[r] = jcn - cj0;
+    }
+    const int kc0 = k0 >> 1;
+    // ring slot of coarse plane m: (m - kc0 + 1) mod 3
+    load_coarse(kc0 - 1, 0);
+    load_coarse(kc0, 1);
+    load_coarse(kc0 + 1, 2);
+    __syncthreads();
+    for (int kc = kc0; kc < kc0 + kz / 2; kc++) {
+        if (kc > kc0) {
+            __syncthreads();                       // readers of plane kc-2 are done
+            load_coarse(kc + 1, (kc + 1 - kc0 + 1) % 3);
+            __syncthreads();
+        }
+        const int sm = (kc - 1 - kc0 + 1) % 3, s0 = (kc - kc0 + 1) % 3, sp = (kc + 1 - kc0 + 1) % 3;
+#pragma unroll
+        for (int h = 0; h < 2; h++) {              // fine planes 2kc (side kc-1), 2kc+1 (side kc+1)
+            const int k = 2 * kc + h;
+            const int kcn = h ? kc + 1 : kc - 1;
+            const int mz = (gc.pz || (kcn >= 0 && kcn < gc.nz)) ? 1 : 0;
+            const int sn = h ? sp : sm;
+#pragma unroll
+            for (int r = 0; r < SW_R; r++) {
+                auto A = [&](int s) -> Tf {
+                    return Tf(9) * cp[s][ly[r]][lx] + Tf(3) * cp[s][ly[r]][lxn]
+                         + Tf(3) * cp[s][lyn[r]][lx] + cp[s][lyn[r]][lxn];
+                };
+                const Tf bmid = A(s0);
+                const Tf aside = A(sn);
+                const Tf term = Tf(tab.pcoef[mx + 2 * my[r] + 4 * mz]) * (Tf(3) * bmid + aside);
+                const int j = j0 + ty + 4 * r;
+                const int64_t c = (int64_t)k * gf.sz + (int64_t)j * gf.sy + i;
+                xf[c] = xf[c] + term;
+            }
+        }
+    }
+}
+
+template <typename Tf, typename Tc>
+hipError_t launch_mg_prolong25(const Geo &gf, const Geo &gc, Tf *xf, const Tc *xc,
+                               const MGTabs &tab, hipStream_t s) {
+    const int kz = sweep25_kz(gf, 2);
+    dim3 grd(gf.nx / SW_TX, gf.ny / SW_TY, gf.nz / kz), blk(SW_TX, 4);
+    hipLaunchKernelGGL((k_mg_prolong25<Tf, Tc>), grd, blk, 0, s, gf, gc, xf, xc, tab, kz);
+    return hipGetLastError();
+}
+
 bool restrict25_fits(const Geo &g) {
     return sweep25_fits(g) && g.nz % 2 == 0;
 }
@@ -1134,6 +1232,8 @@ hipError_t launch_mg_restrict25(const Geo &gf, const Geo &gc, const Tf *x, const
     template hipError_t launch_mg_sweep25<T>(const Geo &, const T *, T *, const T *, double,    \
                                              const MGTabs &, double *, double *, hipStream_t,   \
                                              const Geo *, const T *, double *);                 \
+    template hipError_t launch_mg_prolong25<T, T>(const Geo &, const Geo &, T *, const T *,     \
+                                                  const MGTabs &, hipStream_t);                 \
     template hipError_t launch_mg_restrict25<T, T>(const Geo &, const Geo &, const T *,         \
                                                    const T *, T *, T *, const MGTabs &,         \
                                                    hipStream_t);                                \
