@@ -312,37 +312,44 @@ int smooth_level(nyl_ctx *c, int l, int xc_level, bool pre_done = false,
                        : smooth_level_t<float>(c, l, xc_level, pre_done, addx);
 }
 
-// residual of level l restricted into level l + 1 (coarse x = 0)
+// residual of level l restricted into level l + 1, with level l+1's first
+// smoothing sweep from x = 0 written to its y
 int restrict_level(nyl_ctx *c, int l) {
     Level &f = c->lev[l], &q = c->lev[l + 1];
     const bool f64 = lev64(c, l), c64 = lev64(c, l + 1);
+    const double om = c->d.mg_omega;
     const double bytes = 2.25 * cells(f.g) * (f64 ? sizeof(double) : sizeof(float));
     if (f64 == c64 && restrict25_fits(f.g)) {
         if (f64)
             LAUNCH(NYL_K_MG_RESTRICT, bytes,
                    launch_mg_restrict25<double, double>(f.g, q.g, F<double>(c, f.x),
                                                         F<double>(c, f.b), F<double>(c, q.b),
-                                                        F<double>(c, q.x), c->tabs, c->stream));
+                                                        F<double>(c, q.x), c->tabs, c->stream,
+                                                        F<double>(c, q.y), om));
         else
             LAUNCH(NYL_K_MG_RESTRICT, bytes,
                    launch_mg_restrict25<float, float>(f.g, q.g, F<float>(c, f.x),
                                                       F<float>(c, f.b), F<float>(c, q.b),
-                                                      F<float>(c, q.x), c->tabs, c->stream));
+                                                      F<float>(c, q.x), c->tabs, c->stream,
+                                                      F<float>(c, q.y), om));
     } else if (f64 && c64) {
         LAUNCH(NYL_K_MG_RESTRICT, bytes,
                launch_mg_resid_restrict<double, double>(f.g, q.g, F<double>(c, f.x),
                                                         F<double>(c, f.b), F<double>(c, q.b),
-                                                        F<double>(c, q.x), c->tabs, c->stream));
+                                                        F<double>(c, q.x), c->tabs, c->stream,
+                                                        F<double>(c, q.y), om));
     } else if (!f64 && c64) {
         LAUNCH(NYL_K_MG_RESTRICT, bytes,
                launch_mg_resid_restrict<float, double>(f.g, q.g, F<float>(c, f.x),
                                                        F<float>(c, f.b), F<double>(c, q.b),
-                                                       F<double>(c, q.x), c->tabs, c->stream));
+                                                       F<double>(c, q.x), c->tabs, c->stream,
+                                                       F<double>(c, q.y), om));
     } else if (!f64 && !c64) {
         LAUNCH(NYL_K_MG_RESTRICT, bytes,
                launch_mg_resid_restrict<float, float>(f.g, q.g, F<float>(c, f.x),
                                                       F<float>(c, f.b), F<float>(c, q.b),
-                                                      F<float>(c, q.x), c->tabs, c->stream));
+                                                      F<float>(c, q.x), c->tabs, c->stream,
+                                                      F<float>(c, q.y), om));
     } else {
         return fail(NYL_E_STATE, "FP64 level above an FP32 level");
     }
@@ -356,10 +363,11 @@ int vcycle(nyl_ctx *c, bool pre_done, double *addx = nullptr) {
     const int L = c->nlev;
     int rc;
     for (int l = 0; l < L - 1; l++) {
-        if ((rc = smooth_level(c, l, -1, l == 0 && pre_done))) return rc;
+        // below level 1 the first sweep was written by the restriction
+        if ((rc = smooth_level(c, l, -1, l == 0 ? pre_done : true))) return rc;
         if ((rc = restrict_level(c, l))) return rc;
     }
-    if ((rc = smooth_level(c, L - 1, -1))) return rc;
+    if ((rc = smooth_level(c, L - 1, -1, true))) return rc;
     for (int l = L - 2; l >= 0; l--)
         if ((rc = smooth_level(c, l, l + 1, false, l == 0 ? addx : nullptr))) return rc;
     return NYL_OK;
@@ -657,6 +665,9 @@ int nyl_ctx_create(const nyl_desc *desc, nyl_ctx **out) {
         if (p3[a] != 0 && p3[a] != 1) return fail(NYL_E_ARG, "periodic flags must be 0 or 1");
     }
     if (!(d.dx > 0 && d.dy > 0 && d.dz > 0)) return fail(NYL_E_ARG, "grid spacings must be > 0");
+    if ((int64_t)d.nx * d.ny * d.nz >= ((int64_t)1 << 31))
+        return fail(NYL_E_SIZE, "grid of %lld cells: at most 2^31 - 1 cells per model",
+                    (long long)d.nx * d.ny * d.nz);
     if (d.mg_maxite < 0) return fail(NYL_E_ARG, "mg_maxite must be >= 0");
     nyl_ctx *c = new nyl_ctx();
     memset(c, 0, sizeof(*c));

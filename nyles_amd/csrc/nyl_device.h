@@ -29,6 +29,24 @@ struct Blk3 {
     int x, y, z;
 };
 
+// Same, with the blocks of an XCD ordered x fastest, then z, then y: each
+// XCD owns a slab of y rows and sweeps it plane after plane, so a stencil
+// reaching several planes up and down (the stage kernel's +-3) finds them in
+// that XCD's L2 (the working set is a y slab, not whole planes).
+__device__ __forceinline__ Blk3 xcd_block_yslab() {
+    const uint32_t gx = gridDim.x, gy = gridDim.y, gz = gridDim.z;
+    const uint32_t nb = gx * gy * gz;
+    uint32_t b = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+    const uint32_t per = nb >> 3;
+    if (b < (per << 3)) b = (b & 7u) * per + (b >> 3);
+    Blk3 r;
+    r.x = (int)(b % gx);
+    const uint32_t t = b / gx;
+    r.z = (int)(t % gz);
+    r.y = (int)(t / gz);
+    return r;
+}
+
 __device__ __forceinline__ Blk3 xcd_block() {
     const uint32_t gx = gridDim.x, gy = gridDim.y, gz = gridDim.z;
     const uint32_t nb = gx * gy * gz;
@@ -50,6 +68,16 @@ __device__ __forceinline__ Blk3 xcd_block() {
 // Each operation is written in the reference's evaluation order; FP64
 // instantiations are compiled without contraction.
 
+// Division of the reconstruction weights: correctly rounded in FP64 (the
+// reference's arithmetic); the FP32 path (mixed precision, no bitwise
+// target) multiplies by the hardware reciprocal (v_rcp_f32, about 1 ulp).
+// Every divisor is >= eps (1e-16 or 1e-14, normal floats).
+template <typename T>
+__device__ __forceinline__ T wdiv(T a, T b) {
+    if constexpr (sizeof(T) == 4) return a * __builtin_amdgcn_rcpf(b);
+    else return a / b;
+}
+
 // core/weno.f90:1-22
 template <typename T>
 __device__ __forceinline__ T weno3(T qm, T q0, T qp) {
@@ -60,9 +88,9 @@ __device__ __forceinline__ T weno3(T qm, T q0, T qp) {
     T beta1 = d1 * d1;
     T beta2 = d2 * d2;
     T tau = fabs(beta2 - beta1);
-    T w1 = (T(1) + tau / (beta1 + eps));
-    T w2 = (T(1) + tau / (beta2 + eps)) * T(2);
-    return (w1 * qi1 + w2 * qi2) / (w1 + w2);
+    T w1 = (T(1) + wdiv(tau, beta1 + eps));
+    T w2 = (T(1) + wdiv(tau, beta2 + eps)) * T(2);
+    return wdiv(w1 * qi1 + w2 * qi2, w1 + w2);
 }
 
 // core/weno.f90:25-54
@@ -86,10 +114,10 @@ __device__ __forceinline__ T weno5(T qmm, T qm, T q0, T qp, T qpp) {
 
     T tau5 = T((float)fabs(beta1 - beta3));
 
-    T w1 = (T(1) + tau5 / (beta1 + eps));
-    T w2 = T(6) * (T(1) + tau5 / (beta2 + eps));
-    T w3 = T(3) * (T(1) + tau5 / (beta3 + eps));
-    return (w1 * qi1 + w2 * qi2 + w3 * qi3) / (w1 + w2 + w3);
+    T w1 = (T(1) + wdiv(tau5, beta1 + eps));
+    T w2 = T(6) * (T(1) + wdiv(tau5, beta2 + eps));
+    T w3 = T(3) * (T(1) + wdiv(tau5, beta3 + eps));
+    return wdiv(w1 * qi1 + w2 * qi2 + w3 * qi3, w1 + w2 + w3);
 }
 
 // Flux at line position f (0-based) of core/weno.f90:106-153 flux1d, for a

@@ -80,10 +80,13 @@ struct MGTabs {
 template <typename T>
 hipError_t launch_mg_sweep(const Geo &g, const T *src, T *dst, const T *b, double omega,
                            const MGTabs &tab, hipStream_t s);
-// residual + restriction from a level of type Tf to the next of type Tc
+// residual + restriction from a level of type Tf to the next of type Tc;
+// yc non-null: also the coarse level's first sweep from x = 0 into yc (coarse
+// x is then left as is, the second sweep overwrites it), else xc = 0
 template <typename Tf, typename Tc>
 hipError_t launch_mg_resid_restrict(const Geo &gf, const Geo &gc, const Tf *x, const Tf *b,
-                                    Tc *bc, Tc *xc, const MGTabs &tab, hipStream_t s);
+                                    Tc *bc, Tc *xc, const MGTabs &tab, hipStream_t s,
+                                    Tc *yc = nullptr, double omega = 0.0);
 template <typename Tf, typename Tc>
 hipError_t launch_mg_prolong(const Geo &gf, const Geo &gc, Tf *xf, const Tc *xc,
                              const MGTabs &tab, hipStream_t s);
@@ -128,7 +131,8 @@ hipError_t launch_mg_prolong25(const Geo &gf, const Geo &gc, Tf *xf, const Tc *x
                                const MGTabs &tab, hipStream_t s);
 template <typename Tf, typename Tc>
 hipError_t launch_mg_restrict25(const Geo &gf, const Geo &gc, const Tf *x, const Tf *b, Tc *bc,
-                                Tc *xc, const MGTabs &tab, hipStream_t s);
+                                Tc *xc, const MGTabs &tab, hipStream_t s, Tc *yc = nullptr,
+                                double omega = 0.0);
 
 // ---- ensemble batches -------------------------------------------------------
 // One member's share of a batched launch: the geometry of the level the

@@ -90,31 +90,48 @@ __device__ __forceinline__ T residual_at(const Geo &g, const T *__restrict__ x,
 // residual + restriction(rb)); frestrict_centers3d
 // (core/mgfor/basicoperators.f90:32-60) with Rcoef = 1/2; coarse x = 0
 // (core/mgfor/operators.f90:209).
+// The first smoothing sweep of a level whose x is zero (after restriction):
+// k_mg_sweep with src = 0, i.e. cff1*0 + omega*(0 - b)*idiag with the six
+// zero neighbours summing to +0, evaluated literally (same bits).
+template <typename T>
+__device__ __forceinline__ T sweep_from_zero(const Geo &g, T bv, double omega, const MGTabs &tab,
+                                             int i, int j, int k) {
+    const int nw = (g.px || i > 0) + (g.px || i < g.nx - 1) + (g.py || j > 0) +
+                   (g.py || j < g.ny - 1) + (g.pz || k > 0) + (g.pz || k < g.nz - 1);
+    const T om = T(omega), cff1 = T(1.0 - omega);
+    return cff1 * T(0) + om * (T(0) - bv) * T(tab.idiag[nw]);
+}
+
+// yc non-null: the coarse level's first sweep from x = 0 is written to yc
+// (then coarse x is not needed: the second sweep overwrites it); else xc = 0.
 template <typename Tf, typename Tc>
 __device__ __forceinline__ void restrict_cell(const Geo &gf, const Geo &gc,
                                               const Tf *__restrict__ x, const Tf *__restrict__ b,
                                               Tc *__restrict__ bc, Tc *__restrict__ xc,
-                                              const MGTabs &tab, int ic, int jc, int kc);
+                                              const MGTabs &tab, int ic, int jc, int kc,
+                                              Tc *__restrict__ yc = nullptr, double omega = 0.0);
 
 template <typename Tf, typename Tc>
 __global__ __launch_bounds__(256) void k_mg_resid_restrict(Geo gf, Geo gc,
                                                            const Tf *__restrict__ x,
                                                            const Tf *__restrict__ b,
                                                            Tc *__restrict__ bc, Tc *__restrict__ xc,
-                                                           MGTabs tab) {
+                                                           MGTabs tab, Tc *__restrict__ yc,
+                                                           double omega) {
     const Blk3 bk = xcd_block();
     const int ic = bk.x * MG_BX + threadIdx.x;
     const int jc = bk.y * MG_BY + threadIdx.y;
     const int kc = bk.z;
     if (ic >= gc.nx || jc >= gc.ny) return;
-    restrict_cell<Tf, Tc>(gf, gc, x, b, bc, xc, tab, ic, jc, kc);
+    restrict_cell<Tf, Tc>(gf, gc, x, b, bc, xc, tab, ic, jc, kc, yc, omega);
 }
 
 template <typename Tf, typename Tc>
 __device__ __forceinline__ void restrict_cell(const Geo &gf, const Geo &gc,
                                               const Tf *__restrict__ x, const Tf *__restrict__ b,
                                               Tc *__restrict__ bc, Tc *__restrict__ xc,
-                                              const MGTabs &tab, int ic, int jc, int kc) {
+                                              const MGTabs &tab, int ic, int jc, int kc,
+                                              Tc *__restrict__ yc, double omega) {
     const int i = 2 * ic, j = 2 * jc, k = 2 * kc;
     const Tf r000 = residual_at<Tf>(gf, x, b, i, j, k, tab);
     const Tf r100 = residual_at<Tf>(gf, x, b, i + 1, j, k, tab);
@@ -125,15 +142,18 @@ __device__ __forceinline__ void restrict_cell(const Geo &gf, const Geo &gc,
     const Tf r011 = residual_at<Tf>(gf, x, b, i, j + 1, k + 1, tab);
     const Tf r111 = residual_at<Tf>(gf, x, b, i + 1, j + 1, k + 1, tab);
     const int64_t cc = (int64_t)kc * gc.sz + (int64_t)jc * gc.sy + ic;
-    bc[cc] = Tc(Tf(0.5) * (r000 + r100 + r010 + r110 + r001 + r101 + r011 + r111));
-    xc[cc] = Tc(0);
+    const Tc bv = Tc(Tf(0.5) * (r000 + r100 + r010 + r110 + r001 + r101 + r011 + r111));
+    bc[cc] = bv;
+    if (yc) yc[cc] = sweep_from_zero<Tc>(gc, bv, omega, tab, ic, jc, kc);
+    else xc[cc] = Tc(0);
 }
 
 template <typename Tf, typename Tc>
 hipError_t launch_mg_resid_restrict(const Geo &gf, const Geo &gc, const Tf *x, const Tf *b,
-                                    Tc *bc, Tc *xc, const MGTabs &tab, hipStream_t s) {
+                                    Tc *bc, Tc *xc, const MGTabs &tab, hipStream_t s, Tc *yc,
+                                    double omega) {
     hipLaunchKernelGGL((k_mg_resid_restrict<Tf, Tc>), mg_grid(gc), dim3(MG_BX, MG_BY, 1), 0, s,
-                       gf, gc, x, b, bc, xc, tab);
+                       gf, gc, x, b, bc, xc, tab, yc, omega);
     return hipGetLastError();
 }
 
@@ -354,7 +374,8 @@ template <typename Tf, typename Tc>
 __global__ __launch_bounds__(256) void k_mg_restrict25(Geo g, Geo gc, const Tf *__restrict__ x,
                                                        const Tf *__restrict__ b,
                                                        Tc *__restrict__ bc, Tc *__restrict__ xc,
-                                                       MGTabs tab, int kz) {
+                                                       MGTabs tab, int kz, Tc *__restrict__ yc,
+                                                       double omega) {
     constexpr int LX = SW_TX + 2, LY = SW_TY + 2, NRING = 2 * LX + 2 * SW_TY;
     __shared__ Tf sp[3][LY][LX];
     const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * SW_TX + tx;
@@ -421,10 +442,12 @@ __global__ __launch_bounds__(256) void k_mg_restrict25(Geo g, Geo gc, const Tf *
         } else {
             acc = (((acc + r[0]) + rx0) + r[1]) + rx1;
             if ((tx & 1) == 0) {
-                const int64_t cc = (int64_t)(k >> 1) * gc.sz + (int64_t)((j0 >> 1) + ty) * gc.sy
-                                 + (i >> 1);
-                bc[cc] = Tc(Tf(0.5) * acc);
-                xc[cc] = Tc(0);
+                const int kc = k >> 1, jc = (j0 >> 1) + ty, ic = i >> 1;
+                const int64_t cc = (int64_t)kc * gc.sz + (int64_t)jc * gc.sy + ic;
+                const Tc bv = Tc(Tf(0.5) * acc);
+                bc[cc] = bv;
+                if (yc) yc[cc] = sweep_from_zero<Tc>(gc, bv, omega, tab, ic, jc, kc);
+                else xc[cc] = Tc(0);
             }
         }
     }
@@ -1221,10 +1244,11 @@ bool restrict25_fits(const Geo &g) {
 
 template <typename Tf, typename Tc>
 hipError_t launch_mg_restrict25(const Geo &gf, const Geo &gc, const Tf *x, const Tf *b, Tc *bc,
-                                Tc *xc, const MGTabs &tab, hipStream_t s) {
+                                Tc *xc, const MGTabs &tab, hipStream_t s, Tc *yc, double omega) {
     const int kz = sweep25_kz(gf, 2);
     dim3 grd(gf.nx / SW_TX, gf.ny / SW_TY, gf.nz / kz), blk(SW_TX, 4);
-    hipLaunchKernelGGL((k_mg_restrict25<Tf, Tc>), grd, blk, 0, s, gf, gc, x, b, bc, xc, tab, kz);
+    hipLaunchKernelGGL((k_mg_restrict25<Tf, Tc>), grd, blk, 0, s, gf, gc, x, b, bc, xc, tab, kz,
+                       yc, omega);
     return hipGetLastError();
 }
 
@@ -1236,12 +1260,12 @@ hipError_t launch_mg_restrict25(const Geo &gf, const Geo &gc, const Tf *x, const
                                                   const MGTabs &, hipStream_t);                 \
     template hipError_t launch_mg_restrict25<T, T>(const Geo &, const Geo &, const T *,         \
                                                    const T *, T *, T *, const MGTabs &,         \
-                                                   hipStream_t);                                \
+                                                   hipStream_t, T *, double);                   \
     template hipError_t launch_mg_sweep<T>(const Geo &, const T *, T *, const T *, double,      \
                                            const MGTabs &, hipStream_t);                        \
     template hipError_t launch_mg_resid_restrict<T, T>(const Geo &, const Geo &, const T *,     \
                                                        const T *, T *, T *, const MGTabs &,     \
-                                                       hipStream_t);                            \
+                                                       hipStream_t, T *, double);               \
     template hipError_t launch_mg_prolong<T, T>(const Geo &, const Geo &, T *, const T *,       \
                                                 const MGTabs &, hipStream_t);                   \
     template hipError_t launch_norm2<T>(const Geo &, const T *, double *, double *, hipStream_t); \
@@ -1282,7 +1306,7 @@ template hipError_t launch_resnorm2<float>(const Geo &, const float *, const flo
 template hipError_t launch_mg_resid_restrict<float, double>(const Geo &, const Geo &,
                                                             const float *, const float *,
                                                             double *, double *, const MGTabs &,
-                                                            hipStream_t);
+                                                            hipStream_t, double *, double);
 template hipError_t launch_mg_prolong<float, double>(const Geo &, const Geo &, float *,
                                                      const double *, const MGTabs &, hipStream_t);
 template hipError_t launch_mg_outer_resid<float>(const Geo &, const double *, const double *,

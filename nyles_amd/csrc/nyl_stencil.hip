@@ -26,6 +26,10 @@ namespace nyl {
 #ifndef NYL_STAGE_SEQ
 #define NYL_STAGE_SEQ 1
 #endif
+// block order of the stage kernel: XCD y slabs marched in z (1) or z slabs (0)
+#ifndef NYL_STAGE_YSLAB
+#define NYL_STAGE_YSLAB 1
+#endif
 #if NYL_STAGE_SEQ
 #define STAGE_SEQ(idx, dep) asm volatile("" : "+v"(idx) : "v"(dep))
 #else
@@ -79,23 +83,107 @@ __device__ __forceinline__ T tracer_axis(T d, const T *__restrict__ q_arr,
     return (d + fl) - fr;
 }
 
+// Periodic axes: the element offsets of the cells -3..+3 away along the
+// axis (wrapped), computed once per cell; every stencil load of that axis is
+// then q[c + off[..]].  The loads and the arithmetic are those of the
+// general path (face_flux's interior stencil), so the results are the same.
+struct PerLine {
+    int m3, m2, m1, z, p1, p2, p3;   // offsets of the cells -3..+3 away
+};
+
+__device__ __forceinline__ PerLine per_line(int m, int n, int stride) {
+    PerLine L;
+    L.m3 = (wrapi(m - 3, n) - m) * stride;
+    L.m2 = (wrapi(m - 2, n) - m) * stride;
+    L.m1 = (wrapi(m - 1, n) - m) * stride;
+    L.z = 0;
+    L.p1 = (wrapi(m + 1, n) - m) * stride;
+    L.p2 = (wrapi(m + 2, n) - m) * stride;
+    L.p3 = (wrapi(m + 3, n) - m) * stride;
+    return L;
+}
+
+// offset of relative position d (-3..3; a constant at every call site, so
+// the offsets stay in registers)
+__device__ __forceinline__ int lo(const PerLine &L, int d) {
+    switch (d) {
+    case -3: return L.m3;
+    case -2: return L.m2;
+    case -1: return L.m1;
+    case 0: return L.z;
+    case 1: return L.p1;
+    case 2: return L.p2;
+    default: return L.p3;
+    }
+}
+
+// u times the WENO5 reconstruction at the face between relative positions r
+// and r + 1, upwinded by the sign of u; the value at relative position p is
+// q[c + lo(L, p + qs)] (qs = -1: the vortex force's q(k) = vort(k-1)).
+template <typename T>
+__device__ __forceinline__ T per_face(T u, const T *__restrict__ q, int c, const PerLine &L,
+                                      int r, int qs) {
+    const int om2 = lo(L, r - 2 + qs), om1 = lo(L, r - 1 + qs), o0 = lo(L, r + qs);
+    const int op1 = lo(L, r + 1 + qs), op2 = lo(L, r + 2 + qs), op3 = lo(L, r + 3 + qs);
+    T a0, a1, a2, a3, a4;
+    if (u > T(0)) {
+        a0 = q[c + om2]; a1 = q[c + om1]; a2 = q[c + o0]; a3 = q[c + op1]; a4 = q[c + op2];
+    } else {
+        a0 = q[c + op3]; a1 = q[c + op2]; a2 = q[c + op1]; a3 = q[c + o0]; a4 = q[c + om1];
+    }
+    return u * weno5<T>(a0, a1, a2, a3, a4);
+}
+
+// tracer_axis on a periodic axis
+template <typename T>
+__device__ __forceinline__ T tracer_axis_per(T d, const T *__restrict__ q, const T *__restrict__ ud,
+                                             T cu, int c, const PerLine &L) {
+    const T fr = per_face<T>(ud[c] * cu, q, c, L, 0, 0);
+    STAGE_SEQ(c, fr);
+    const T fl = per_face<T>(ud[c + L.m1] * cu, q, c, L, -1, 0);
+    STAGE_SEQ(c, fl);
+    return (d + fl) - fr;
+}
+
+// vf_flux on a periodic axis
+template <typename T>
+__device__ __forceinline__ T vf_flux_per(const T *__restrict__ ud, T cu, const T *__restrict__ om,
+                                         int c, const PerLine &L, int oe) {
+    auto UU = [&](int p) -> T { return T(0.5) * (ud[p] * cu + ud[p + oe] * cu); };
+    const T uu_prev = UU(c + L.m1);
+    const T u1d = T(0.5) * (uu_prev + UU(c));
+    return per_face<T>(u1d, om, c, L, 0, -1);
+}
+
 // Fused LES.rhs (core/model_les.py:129-144) + RK3_SSP stage update
 // (core/timescheme.py:422-443) for one cell.
 template <typename T, int MODE, int PX, int PY, int PZ>
 __device__ __forceinline__ void stage_cell(const Geo &g, const StageArgs<T> &a, int i, int j,
                                            int k) {
     const int64_t c = (int64_t)k * g.sz + (int64_t)j * g.sy + i;
+    const int ci = (int)c;
+    PerLine Lx, Ly, Lz;
+    if constexpr (PX != 0) Lx = per_line(i, g.nx, 1);
+    if constexpr (PY != 0) Ly = per_line(j, g.ny, (int)g.sy);
+    if constexpr (PZ != 0) Lz = per_line(k, g.nz, (int)g.sz);
 
     // tracer b: rhstrac, directions i, j, k (core/tracer.py:63-72)
     const T cx = T(a.idx2), cy = T(a.idy2), cz = T(a.idz2);
     T db = T(0);
     int64_t bx = c - i, by = c - (int64_t)j * g.sy, bz = c - (int64_t)k * g.sz;
-    db = tracer_axis<T, PX>(db, a.b, a.u, cx, bx, 1, i, g.nx);
+    int cs = ci;
+    if constexpr (PX != 0) db = tracer_axis_per<T>(db, a.b, a.u, cx, cs, Lx);
+    else db = tracer_axis<T, 0>(db, a.b, a.u, cx, bx, 1, i, g.nx);
     STAGE_SEQ(by, db);
-    db = tracer_axis<T, PY>(db, a.b, a.v, cy, by, g.sy, j, g.ny);
+    STAGE_SEQ(cs, db);
+    if constexpr (PY != 0) db = tracer_axis_per<T>(db, a.b, a.v, cy, cs, Ly);
+    else db = tracer_axis<T, 0>(db, a.b, a.v, cy, by, g.sy, j, g.ny);
     STAGE_SEQ(bz, db);
-    db = tracer_axis<T, PZ>(db, a.b, a.w, cz, bz, g.sz, k, g.nz);
+    STAGE_SEQ(cs, db);
+    if constexpr (PZ != 0) db = tracer_axis_per<T>(db, a.b, a.w, cz, cs, Lz);
+    else db = tracer_axis<T, 0>(db, a.b, a.w, cz, bz, g.sz, k, g.nz);
     STAGE_SEQ(bx, db);
+    STAGE_SEQ(cs, db);
 
     // vortex force, triplets 'ikj', 'jik', 'kji' (core/vortex_force.py:69-81)
     const int64_t oex = PX ? (int64_t)(wrapi(i + 1, g.nx) - i) : 1;
@@ -105,17 +193,46 @@ __device__ __forceinline__ void stage_cell(const Geo &g, const StageArgs<T> &a, 
     const bool ey = PY || j < g.ny - 1;
     const bool ez = PZ || k < g.nz - 1;
     T du = T(0), dv = T(0), dw = T(0);
-    if (ey) dv = dv - vf_flux<T, PX>(a.u, cx, a.oz, bx, 1, i, g.nx, oey);      // direc(U_i, vor_k, du_j)
+    // direc(U_i, vor_k, du_j)
+    if (ey) {
+        if constexpr (PX != 0) dv = dv - vf_flux_per<T>(a.u, cx, a.oz, cs, Lx, (int)oey);
+        else dv = dv - vf_flux<T, 0>(a.u, cx, a.oz, bx, 1, i, g.nx, oey);
+    }
     STAGE_SEQ(by, dv);
-    if (ex) du = du + vf_flux<T, PY>(a.v, cy, a.oz, by, g.sy, j, g.ny, oex);   // flip (U_j, vor_k, du_i)
+    STAGE_SEQ(cs, dv);
+    // flip (U_j, vor_k, du_i)
+    if (ex) {
+        if constexpr (PY != 0) du = du + vf_flux_per<T>(a.v, cy, a.oz, cs, Ly, (int)oex);
+        else du = du + vf_flux<T, 0>(a.v, cy, a.oz, by, g.sy, j, g.ny, oex);
+    }
     STAGE_SEQ(by, du);
-    if (ez) dw = dw - vf_flux<T, PY>(a.v, cy, a.ox, by, g.sy, j, g.ny, oez);   // direc(U_j, vor_i, du_k)
+    STAGE_SEQ(cs, du);
+    // direc(U_j, vor_i, du_k)
+    if (ez) {
+        if constexpr (PY != 0) dw = dw - vf_flux_per<T>(a.v, cy, a.ox, cs, Ly, (int)oez);
+        else dw = dw - vf_flux<T, 0>(a.v, cy, a.ox, by, g.sy, j, g.ny, oez);
+    }
     STAGE_SEQ(bz, dw);
-    if (ey) dv = dv + vf_flux<T, PZ>(a.w, cz, a.ox, bz, g.sz, k, g.nz, oey);   // flip (U_k, vor_i, du_j)
+    STAGE_SEQ(cs, dw);
+    // flip (U_k, vor_i, du_j)
+    if (ey) {
+        if constexpr (PZ != 0) dv = dv + vf_flux_per<T>(a.w, cz, a.ox, cs, Lz, (int)oey);
+        else dv = dv + vf_flux<T, 0>(a.w, cz, a.ox, bz, g.sz, k, g.nz, oey);
+    }
     STAGE_SEQ(bz, dv);
-    if (ex) du = du - vf_flux<T, PZ>(a.w, cz, a.oy, bz, g.sz, k, g.nz, oex);   // direc(U_k, vor_j, du_i)
+    STAGE_SEQ(cs, dv);
+    // direc(U_k, vor_j, du_i)
+    if (ex) {
+        if constexpr (PZ != 0) du = du - vf_flux_per<T>(a.w, cz, a.oy, cs, Lz, (int)oex);
+        else du = du - vf_flux<T, 0>(a.w, cz, a.oy, bz, g.sz, k, g.nz, oex);
+    }
     STAGE_SEQ(bx, du);
-    if (ez) dw = dw + vf_flux<T, PX>(a.u, cx, a.oy, bx, 1, i, g.nx, oez);      // flip (U_i, vor_j, du_k)
+    STAGE_SEQ(cs, du);
+    // flip (U_i, vor_j, du_k)
+    if (ez) {
+        if constexpr (PX != 0) dw = dw + vf_flux_per<T>(a.u, cx, a.oy, cs, Lx, (int)oez);
+        else dw = dw + vf_flux<T, 0>(a.u, cx, a.oy, bx, 1, i, g.nx, oez);
+    }
 
     // bernoulli (core/bernoulli.py:14-23, core/fortran_bernoulli.f90:2-58)
     const T kec = a.ke[c];
@@ -170,7 +287,11 @@ __device__ __forceinline__ void stage_cell(const Geo &g, const StageArgs<T> &a, 
 template <typename T, int MODE, int PX, int PY, int PZ>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NYL_STAGE_WAVES)))
 void k_stage(Geo g, StageArgs<T> a) {
+#if NYL_STAGE_YSLAB
+    const Blk3 bk = xcd_block_yslab();
+#else
     const Blk3 bk = xcd_block();
+#endif
     const int i = bk.x * NYL_BX + threadIdx.x;
     const int j = bk.y * NYL_BY + threadIdx.y;
     const int k = bk.z;
