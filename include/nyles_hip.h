@@ -124,6 +124,10 @@ int nyl_ctx_destroy(nyl_ctx *ctx);
 int nyl_workspace_bytes(const nyl_ctx *ctx, uint64_t *bytes);
 int nyl_bind_workspace(nyl_ctx *ctx, void *dptr, uint64_t bytes);
 int nyl_set_stream(nyl_ctx *ctx, void *hip_stream);
+/* on (default) = 1: on a non-null stream, the V-cycle below the finest level
+ * is captured once into a hipGraph and replayed; 0: direct launches.  No
+ * counterpart in the reference. */
+int nyl_set_graphs(nyl_ctx *ctx, int on);
 
 /* Byte offset, element count and dtype of a field in the workspace.  The
  * state fields are double-buffered: query again after every step. */

@@ -40,7 +40,8 @@ NYL_OP_PRESSURE = 5
 # every symbol include/nyles_hip.h declares
 EXPORTED = [
     "nyl_abi_version", "nyl_last_error", "nyl_ctx_create", "nyl_ctx_destroy",
-    "nyl_workspace_bytes", "nyl_bind_workspace", "nyl_set_stream", "nyl_field_offset",
+    "nyl_workspace_bytes", "nyl_bind_workspace", "nyl_set_stream", "nyl_set_graphs",
+    "nyl_field_offset",
     "nyl_mg_levels", "nyl_mg_level_shape", "nyl_rhs", "nyl_rk3_stage", "nyl_diagnose",
     "nyl_step_rk3", "nyl_step_ef", "nyl_step_lfam3", "nyl_lfam3_reset", "nyl_mg_solve", "nyl_apply", "nyl_reduce", "nyl_prof_start",
     "nyl_prof_stop", "nyl_debug_mg", "nyl_reduce_last", "nyl_sync",
@@ -104,6 +105,7 @@ def load():
         "nyl_workspace_bytes": (ci, [vp, P(u64)]),
         "nyl_bind_workspace": (ci, [vp, vp, u64]),
         "nyl_set_stream": (ci, [vp, vp]),
+        "nyl_set_graphs": (ci, [vp, ci]),
         "nyl_field_offset": (ci, [vp, ci, P(u64), P(i64), P(ctypes.c_int32)]),
         "nyl_mg_levels": (ci, [vp, P(ctypes.c_int32)]),
         "nyl_mg_level_shape": (ci, [vp, ci, P(ctypes.c_int32), P(ctypes.c_int32),

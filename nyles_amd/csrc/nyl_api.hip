@@ -356,21 +356,68 @@ int restrict_level(nyl_ctx *c, int l) {
     return NYL_OK;
 }
 
-// core/mgfor/solvers.f90:35-55.  pre_done: the level-1 pre-smooth already
-// ran into the spare buffer.
-// addx: the mixed solve's x, which receives the level-1 result (x += e)
-int vcycle(nyl_ctx *c, bool pre_done, double *addx = nullptr) {
+// the part of a V-cycle below level 1: from level 2's smoothing (its first
+// sweep written by the restriction from level 1) down to the coarsest level
+// and back up to level 2's post-smoothing
+int coarse_cycle(nyl_ctx *c) {
     const int L = c->nlev;
     int rc;
-    for (int l = 0; l < L - 1; l++) {
-        // below level 1 the first sweep was written by the restriction
-        if ((rc = smooth_level(c, l, -1, l == 0 ? pre_done : true))) return rc;
+    for (int l = 1; l < L - 1; l++) {
+        if ((rc = smooth_level(c, l, -1, true))) return rc;
         if ((rc = restrict_level(c, l))) return rc;
     }
     if ((rc = smooth_level(c, L - 1, -1, true))) return rc;
-    for (int l = L - 2; l >= 0; l--)
-        if ((rc = smooth_level(c, l, l + 1, false, l == 0 ? addx : nullptr))) return rc;
+    for (int l = L - 2; l >= 1; l--)
+        if ((rc = smooth_level(c, l, l + 1))) return rc;
     return NYL_OK;
+}
+
+void drop_graphs(nyl_ctx *c) {
+    if (c->g_coarse) {
+        (void)hipGraphExecDestroy(c->g_coarse);
+        c->g_coarse = nullptr;
+    }
+}
+
+// The coarse part is the same launch sequence every V-cycle: it is captured
+// once into a hipGraph and replayed (the per-launch overhead dominates these
+// small levels).  Direct launches are used on the null stream, and while a
+// kernel class that includes coarse-level launches is being timed.
+int coarse_cycle_run(nyl_ctx *c) {
+    const int pc = c->prof_cls;
+    const bool timed = pc == NYL_K_MG_SWEEP || pc == NYL_K_MG_RESTRICT || pc == NYL_K_MG_PROLONG;
+    if (!c->use_graphs || !c->stream || timed) return coarse_cycle(c);
+    if (!c->g_coarse) {
+        hipGraph_t graph;
+        HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        const int prof = c->prof_cls;
+        c->prof_cls = 0;                 // no event records inside the graph
+        int rc = coarse_cycle(c);
+        c->prof_cls = prof;
+        hipError_t e = hipStreamEndCapture(c->stream, &graph);
+        if (rc) return rc;
+        HIPCHK(e);
+        HIPCHK(hipGraphInstantiate(&c->g_coarse, graph, nullptr, nullptr, 0));
+        HIPCHK(hipGraphDestroy(graph));
+    }
+    HIPCHK(hipGraphLaunch(c->g_coarse, c->stream));
+    return NYL_OK;
+}
+
+// core/mgfor/solvers.f90:35-55.  pre_done: the level-1 pre-smooth already
+// ran into the spare buffer.  addx: the mixed solve's x, which receives the
+// level-1 result (x += e).  Below level 1 the first sweep of each level is
+// written by the restriction.
+int vcycle(nyl_ctx *c, bool pre_done, double *addx = nullptr) {
+    int rc;
+    if ((rc = smooth_level(c, 0, -1, pre_done))) return rc;
+    if ((rc = restrict_level(c, 0))) return rc;
+    if (c->nlev > 2) {
+        if ((rc = coarse_cycle_run(c))) return rc;
+    } else {
+        if ((rc = smooth_level(c, 1, -1, true))) return rc;
+    }
+    return smooth_level(c, 0, 1, false, addx);
 }
 
 void fill_stats(nyl_ctx *c, nyl_mg_stats *st, int nite, double res, double normb) {
@@ -685,12 +732,15 @@ int nyl_ctx_create(const nyl_desc *desc, nyl_ctx **out) {
     build_tabs(c->tabs);
     build_layout(c);
     c->stream = nullptr;
+    c->use_graphs = 1;
+    c->g_coarse = nullptr;
     *out = c;
     return NYL_OK;
 }
 
 int nyl_ctx_destroy(nyl_ctx *c) {
     if (!c) return NYL_OK;
+    drop_graphs(c);
     for (int q = 0; q < c->prof_nev; q++) (void)hipEventDestroy(c->prof_ev[q]);
     if (c->host_out) (void)hipHostFree(c->host_out);
     delete c;
@@ -709,8 +759,16 @@ int nyl_set_stream(nyl_ctx *c, void *stream) {
     return NYL_OK;
 }
 
+int nyl_set_graphs(nyl_ctx *c, int on) {
+    if (!c) return fail(NYL_E_ARG, "null context");
+    c->use_graphs = on ? 1 : 0;
+    if (!on) drop_graphs(c);
+    return NYL_OK;
+}
+
 int nyl_bind_workspace(nyl_ctx *c, void *dptr, uint64_t bytes) {
     if (!c || !dptr) return fail(NYL_E_ARG, "null argument");
+    drop_graphs(c);
     if (bytes < c->ws_bytes)
         return fail(NYL_E_ARG, "workspace of %llu bytes is smaller than the %llu needed",
                     (unsigned long long)bytes, (unsigned long long)c->ws_bytes);

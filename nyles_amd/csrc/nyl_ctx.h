@@ -38,6 +38,9 @@ struct nyl_ctx {
     hipStream_t stream;
     double *host_out;        // pinned, 8 doubles
     int total_cycles;
+    // the V-cycle below level 1, replayed from a hipGraph (captured on first use)
+    int use_graphs;
+    hipGraphExec_t g_coarse;
     // HIP-event timing of one kernel class (nyl_prof_start / nyl_prof_stop)
     int prof_cls;
     int prof_used;

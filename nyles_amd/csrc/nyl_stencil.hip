@@ -40,6 +40,25 @@ static inline dim3 cell_grid(const Geo &g) {
     return dim3((g.nx + NYL_BX - 1) / NYL_BX, (g.ny + NYL_BY - 1) / NYL_BY, g.nz);
 }
 
+// The per-cell diagnostic kernels (div, correct, diag): each thread takes
+// NYL_CELL_KZ consecutive planes of its column (when nz is a multiple), in
+// the XCD y-slab order of the stage kernel.
+#ifndef NYL_CELL_KZ
+#define NYL_CELL_KZ 1
+#endif
+#ifndef NYL_CELL_YSLAB
+#define NYL_CELL_YSLAB 0
+#endif
+static inline int cell_kz(const Geo &g) { return g.nz % NYL_CELL_KZ == 0 ? NYL_CELL_KZ : 1; }
+static inline dim3 cell_grid_kz(const Geo &g) {
+    return dim3((g.nx + NYL_BX - 1) / NYL_BX, (g.ny + NYL_BY - 1) / NYL_BY, g.nz / cell_kz(g));
+}
+#if NYL_CELL_YSLAB
+#define CELL_BLOCK() xcd_block_yslab()
+#else
+#define CELL_BLOCK() xcd_block()
+#endif
+
 // Vortex-force flux of one term at a cell (core/fortran_vortex_force.f90
 // WENO branch, :66-81 and :145-159): the vorticity component `om` is
 // upwinded along axis d with the contravariant velocity U_d = ud*cu
@@ -386,17 +405,17 @@ __device__ __forceinline__ void diag_cell(const Geo &g, const DiagArgs<T> &a, in
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_diag(Geo g, DiagArgs<T> a, int what) {
-    const Blk3 bk = xcd_block();
+    const Blk3 bk = CELL_BLOCK();
     const int i = bk.x * NYL_BX + threadIdx.x;
     const int j = bk.y * NYL_BY + threadIdx.y;
-    const int k = bk.z;
     if (i >= g.nx || j >= g.ny) return;
-    diag_cell<T>(g, a, what, i, j, k);
+    const int kz = g.nz / (int)gridDim.z;
+    for (int k = bk.z * kz; k < (bk.z + 1) * kz; k++) diag_cell<T>(g, a, what, i, j, k);
 }
 
 template <typename T>
 hipError_t launch_diag(const Geo &g, const DiagArgs<T> &a, int what, hipStream_t s) {
-    hipLaunchKernelGGL((k_diag<T>), cell_grid(g), dim3(NYL_BX, NYL_BY, 1), 0, s, g, a, what);
+    hipLaunchKernelGGL((k_diag<T>), cell_grid_kz(g), dim3(NYL_BX, NYL_BY, 1), 0, s, g, a, what);
     return hipGetLastError();
 }
 
@@ -423,12 +442,12 @@ __device__ __forceinline__ void div_cell(const Geo &g, const DivArgs<T, TO> &a, 
 
 template <typename T, typename TO>
 __global__ __launch_bounds__(256) void k_div(Geo g, DivArgs<T, TO> a) {
-    const Blk3 bk = xcd_block();
+    const Blk3 bk = CELL_BLOCK();
     const int i = bk.x * NYL_BX + threadIdx.x;
     const int j = bk.y * NYL_BY + threadIdx.y;
-    const int k = bk.z;
     if (i >= g.nx || j >= g.ny) return;
-    div_cell<T, TO>(g, a, i, j, k);
+    const int kz = g.nz / (int)gridDim.z;
+    for (int k = bk.z * kz; k < (bk.z + 1) * kz; k++) div_cell<T, TO>(g, a, i, j, k);
 }
 
 template <typename T, typename TO>
@@ -436,7 +455,7 @@ hipError_t launch_div(const Geo &g, const T *u, const T *v, const T *w, double i
                       double idy2, double idz2, TO *out, hipStream_t s) {
     DivArgs<T, TO> a;
     a.u = u; a.v = v; a.w = w; a.idx2 = idx2; a.idy2 = idy2; a.idz2 = idz2; a.out = out;
-    hipLaunchKernelGGL((k_div<T, TO>), cell_grid(g), dim3(NYL_BX, NYL_BY, 1), 0, s, g, a);
+    hipLaunchKernelGGL((k_div<T, TO>), cell_grid_kz(g), dim3(NYL_BX, NYL_BY, 1), 0, s, g, a);
     return hipGetLastError();
 }
 
@@ -464,12 +483,12 @@ __device__ __forceinline__ void correct_cell(const Geo &g, const CorrectArgs<T, 
 
 template <typename T, typename TX>
 __global__ __launch_bounds__(256) void k_correct(Geo g, CorrectArgs<T, TX> a) {
-    const Blk3 bk = xcd_block();
+    const Blk3 bk = CELL_BLOCK();
     const int i = bk.x * NYL_BX + threadIdx.x;
     const int j = bk.y * NYL_BY + threadIdx.y;
-    const int k = bk.z;
     if (i >= g.nx || j >= g.ny) return;
-    correct_cell<T, TX>(g, a, i, j, k);
+    const int kz = g.nz / (int)gridDim.z;
+    for (int k = bk.z * kz; k < (bk.z + 1) * kz; k++) correct_cell<T, TX>(g, a, i, j, k);
 }
 
 template <typename T, typename TX>
@@ -477,7 +496,7 @@ hipError_t launch_correct(const Geo &g, T *u, T *v, T *w, const TX *x, double dx
                           hipStream_t s) {
     CorrectArgs<T, TX> a;
     a.u = u; a.v = v; a.w = w; a.x = x; a.dx2 = dx2;
-    hipLaunchKernelGGL((k_correct<T, TX>), cell_grid(g), dim3(NYL_BX, NYL_BY, 1), 0, s, g, a);
+    hipLaunchKernelGGL((k_correct<T, TX>), cell_grid_kz(g), dim3(NYL_BX, NYL_BY, 1), 0, s, g, a);
     return hipGetLastError();
 }
 

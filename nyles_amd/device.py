@@ -58,7 +58,11 @@ class DeviceModel:
         self.workspace_bytes = nbytes.value
         with torch.cuda.device(self.device):
             self.ws = torch.empty(self.workspace_bytes, dtype=torch.uint8, device=self.device)
-            self.stream = torch.cuda.current_stream(self.device)
+            # a stream of its own (default priority): the library captures the
+            # coarse multigrid levels into a hipGraph, which the null stream
+            # does not allow
+            self.stream = torch.cuda.Stream(device=self.device)
+        torch.cuda.current_stream(self.device).synchronize()   # ws allocated
         L.check(lib.nyl_set_stream(ctx, ctypes.c_void_p(self.stream.cuda_stream)))
         L.check(lib.nyl_bind_workspace(ctx, ctypes.c_void_p(self.ws.data_ptr()),
                                        self.workspace_bytes))
@@ -161,6 +165,10 @@ class DeviceModel:
 
     def sync(self):
         L.check(self.lib.nyl_sync(self.ctx))
+
+    def set_graphs(self, on):
+        """replay the coarse multigrid levels from a hipGraph (default) or not"""
+        L.check(self.lib.nyl_set_graphs(self.ctx, int(bool(on))))
 
     def mg_levels(self):
         n = ctypes.c_int32()
