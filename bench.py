@@ -17,8 +17,10 @@ velocity, buoyancy).
 
 Beside the headline, the same run measures an ensemble of 64 members of
 uneven sizes advanced by batched steps against the same members stepped one
-after another, and a serving session (Poisson job arrivals, continuous
-batching) reported as p50/p99 time-to-completion and throughput.
+after another, a serving session (Poisson job arrivals, continuous batching)
+reported as p50/p99 time-to-completion and throughput, and a Rayleigh-Taylor
+256 x 256 x 512 run with the projection converged to |r|/|b| <= 1e-10 in
+both precisions.
 
 Prints ONE JSON line.  ``roofline`` prices the dominant kernel of the
 headline path, the damped-Jacobi smoother sweep on the finest multigrid level
@@ -62,6 +64,9 @@ def parse():
     ap.add_argument("--serving-jobs", type=int, default=40,
                     help="jobs of the serving run (0: skip)")
     ap.add_argument("--serving-rate", type=float, default=10.0, help="job arrivals per second")
+    ap.add_argument("--rt-steps", type=int, default=30,
+                    help="steps of the Rayleigh-Taylor 256x256x512 run at |r|/|b| <= 1e-10 "
+                         "(0: skip)")
     ap.add_argument("--single", action="store_true",
                     help="time only the headline precision (short runs under a profiler)")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
@@ -129,6 +134,53 @@ def ensemble_run(members, steps, precision="mixed", seed=1000):
         out[mode + "_Mcell_steps_per_s"] = round(cells * steps / el / 1e6, 3)
     out["batching_speedup"] = round(out["batched_Mcell_steps_per_s"] /
                                     out["sequential_Mcell_steps_per_s"], 3)
+    return out
+
+
+def rt_run(steps, tol2=1e-20, maxite=50):
+    """Rayleigh-Taylor 256 x 256 x 512 (perio_xy, heavy fluid on top), the
+    projection converged to |r|/|b| <= 1e-10 (unsquared; tol 1e-20 on mgfor's
+    squared ratio): mixed (FP32 stencils, FP64 outer defect correction) and
+    FP64, timed, and the buoyancy of the two compared after the same steps."""
+    import numpy as np
+    import torch
+    from nyles_amd import _lib as L
+    from nyles_amd import workloads
+    from nyles_amd.device import DeviceModel
+    st, meta = workloads.rayleigh_taylor()
+    (Lx, Ly, Lz), (nz, ny, nx) = meta["L"], st["b"].shape
+    dx, dy, dz = Lx / nx, Ly / ny, Lz / nz
+    dt = 0.5 * dz / np.sqrt(1.0 * Lz)          # buoyancy jump 1, depth Lz
+    out = {"grid": [nx, ny, nz], "steps": steps, "dt": dt,
+           "tolerance": "|r|/|b| <= 1e-10 (unsquared), at most %d V-cycles" % maxite}
+    bs = {}
+    for prec, code in (("fp64", L.NYL_PREC_FP64), ("mixed", L.NYL_PREC_MIXED)):
+        m = DeviceModel(nx, ny, nz, (1, 1, 0), dx, dy, dz, idx2=1 / dx ** 2, idy2=1 / dy ** 2,
+                        idz2=1 / dz ** 2, dx2=dx ** 2, precision=code, mg_tol=tol2,
+                        mg_maxite=maxite)
+        for k, fid in (("b", L.NYL_F_B), ("u_i", L.NYL_F_U), ("u_j", L.NYL_F_V),
+                       ("u_k", L.NYL_F_W)):
+            m.upload(fid, st[k])
+        m.diagnose()
+        m.step_rk3(dt)
+        torch.cuda.synchronize()
+        nite, res = 0, 0.0
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            s = m.step_rk3(dt)
+            nite += s["nite"]
+            res = max(res, s["res"])
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        out[prec] = {"Mcell_steps_per_s": round(nx * ny * nz * steps / el / 1e6, 3),
+                     "vcycles_per_step": round(nite / steps, 2),
+                     "max_final_residual_unsquared": float(np.sqrt(res))}
+        bs[prec] = m.download(L.NYL_F_B)
+        m.close()
+    out["speedup_mixed_vs_fp64"] = round(out["mixed"]["Mcell_steps_per_s"] /
+                                         out["fp64"]["Mcell_steps_per_s"], 3)
+    out["b_l2_rel_err"] = float(np.sqrt(np.sum((bs["mixed"] - bs["fp64"]) ** 2) /
+                                        np.sum(bs["fp64"] ** 2)))
     return out
 
 
@@ -352,6 +404,9 @@ def main():
     if args.serving_jobs > 0:
         print("serving, %d jobs" % args.serving_jobs, file=sys.stderr, flush=True)
         out["serving"] = serving_run(args.serving_jobs, args.serving_rate)
+    if args.rt_steps > 0:
+        print("rayleigh-taylor, %d steps" % args.rt_steps, file=sys.stderr, flush=True)
+        out["rayleigh_taylor"] = rt_run(args.rt_steps)
     if not args.no_cpu_baseline:
         print("cpu baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(n, args.cpu_steps)

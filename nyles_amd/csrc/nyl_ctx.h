@@ -60,6 +60,13 @@ int set_error(int code, const char *fmt, ...);
 
 // Level l holds FP64 arrays in FP64 mode; in mixed mode the inner V-cycle
 // levels are FP32 and the coarsest level stays FP64.
-inline bool lev64(const nyl_ctx *c, int l) { return !c->mixed || l == c->nlev - 1; }
+// Mixed precision: levels 0 .. NYL_MIXED_F32_LEVELS-1 in FP32 (never the
+// coarsest), the others in FP64.
+#ifndef NYL_MIXED_F32_LEVELS
+#define NYL_MIXED_F32_LEVELS 64
+#endif
+inline bool lev64(const nyl_ctx *c, int l) {
+    return !c->mixed || l == c->nlev - 1 || l >= NYL_MIXED_F32_LEVELS;
+}
 
 }  // namespace nyl

@@ -22,8 +22,8 @@ def cmp():
 
 @pytest.mark.parametrize("geometry", GEOMETRIES)
 def test_defect_correction_matches_fp64_solve(cmp, geometry):
-    """One FP32 V-cycle per outer FP64 residual is the FP64 V-cycle iteration
-    to FP32 precision: same cycle count (+-1 at the stop test) and the same
+    """FP32 V-cycles inside an FP64 outer residual reproduce the FP64 V-cycle
+    iteration to FP32 precision: same cycle count (+-1 at the stop test) and the same
     solution to 1e-5 relative (up to the additive constant of the singular
     problem)."""
     from nyles_amd import _lib as L
@@ -152,5 +152,52 @@ def test_mixed_lock_exchange_buoyancy_error(cmp):
     print("  b rel err %.2e, b sum %.6e -> %.6e" % (e, s0, sm), flush=True)
     assert e < 1e-5
     assert abs(sm - s0) <= 1e-4 * float(np.sum(np.abs(b)))
+    d64.close()
+    dm.close()
+
+
+@pytest.mark.parametrize("batched", [False, True])
+def test_mixed_tight_tolerance_rayleigh_taylor(cmp, batched):
+    """Rayleigh-Taylor 64x64x128 (perio_xy) with the projection converged to
+    |r|/|b| <= 1e-10 (tol 1e-20 on the squared ratio, 60 V-cycles at most):
+    the mixed path's FP64 defect correction reaches the same tolerance as the
+    FP64 solve at every step, alone and in a batch, and after 20 steps the
+    buoyancy is within 1e-6 relative (L2) of FP64."""
+    from nyles_amd import _lib as L
+    from nyles_amd import workloads
+    from nyles_amd.device import DeviceBatch, DeviceModel
+    print("case mixed rt tight tol batched=%d" % batched, flush=True)
+    st, meta = workloads.rayleigh_taylor(64, 64, 128, nmodes=8)
+    Lx, Ly, Lz = meta["L"]
+    dx, dy, dz = Lx / 64, Ly / 64, Lz / 128
+    dt = 0.5 * dz / np.sqrt(Lz)
+    devs = {}
+    for prec in (L.NYL_PREC_FP64, L.NYL_PREC_MIXED):
+        d = DeviceModel(64, 64, 128, (1, 1, 0), dx, dy, dz, precision=prec,
+                        mg_tol=1e-20, mg_maxite=60)
+        for k, fid in (("b", L.NYL_F_B), ("u_i", L.NYL_F_U), ("u_j", L.NYL_F_V),
+                       ("u_k", L.NYL_F_W)):
+            d.upload(fid, st[k])
+        s = d.diagnose()
+        assert s["res"] < 1e-20, (prec, s)
+        devs[prec] = d
+    d64, dm = devs[L.NYL_PREC_FP64], devs[L.NYL_PREC_MIXED]
+    if batched:    # a batch holds members of one precision
+        bats = [DeviceBatch([d], autotune=False) for d in (d64, dm)]
+    nite = {0: 0, 1: 0}
+    for n in range(20):
+        if batched:
+            ss = [bt.step_rk3([dt])[0] for bt in bats]
+        else:
+            ss = [d64.step_rk3(dt), dm.step_rk3(dt)]
+        for q, s in enumerate(ss):
+            assert s["res"] < 1e-20 and s["nite"] < 3 * 60, (n, q, s)
+            nite[q] += s["nite"]
+    if batched:
+        for bt in bats:
+            bt.close()
+    e = cmp.rel_l2(dm.download(L.NYL_F_B), d64.download(L.NYL_F_B))
+    print("  V-cycles fp64 %d mixed %d, b rel err %.2e" % (nite[0], nite[1], e), flush=True)
+    assert e < 1e-6
     d64.close()
     dm.close()
