@@ -490,12 +490,11 @@ int mg_solve_fp64(nyl_ctx *c, nyl_mg_stats *st) {
 // One inner solve resolves a limited reduction of r: A e of a smooth e
 // cancels to ~eps32*(n/pi)^2 of its terms, so below that the FP32 residual
 // norm is rounding noise and the FP32 e stops improving.  Hence an outer step
-// whose target asks for more than kInnerRed2*|r|^2 runs kInnerCycles V-cycles
-// with no inner test, and no outer step runs more than kInnerMax V-cycles;
-// the FP64 residual decides.  At mgfor's tolerance (1e-6) neither limit is
-// normally reached and the count is that of the FP64 solve.
-constexpr double kInnerRed2 = 1e-6;   // squared reduction one inner test resolves
-constexpr int kInnerCycles = 3;
+// whose tolerance asks for more than kInnerRed2*|r|^2 tests the inner
+// residual against kInnerRed2*|r|^2 instead and runs at most kInnerMax
+// V-cycles; the FP64 residual decides.  At mgfor's tolerance (1e-6) the
+// tolerance is the target and the V-cycle count is that of the FP64 solve.
+constexpr double kInnerRed2 = 1e-6;   // squared reduction of one inner solve
 constexpr int kInnerMax = 8;
 int mg_solve_mixed(nyl_ctx *c, nyl_mg_stats *st) {
     Level &f = c->lev[0];
@@ -545,20 +544,16 @@ int mg_solve_mixed(nyl_ctx *c, nyl_mg_stats *st) {
     int nite = 0;
     while (!(res < c->d.mg_tol) && nite < c->d.mg_maxite) {
         bool pending = true;       // y holds the first sweep from e = 0
-        const bool fixed = res * kInnerRed2 > c->d.mg_tol;
+        const bool rel = res * kInnerRed2 > c->d.mg_tol;
+        const double target = rel ? res * kInnerRed2 : c->d.mg_tol;
         for (int k = 1;; k++) {
             if ((rc = vcycle(c, pending))) return rc;
             nite = nite + 1;
-            if (nite >= c->d.mg_maxite || k >= kInnerMax) break;
-            if (fixed) {
-                if (k >= kInnerCycles) break;
-                pending = false;
-                continue;
-            }
+            if (nite >= c->d.mg_maxite || (rel && k >= kInnerMax)) break;
             double rin;
             if ((rc = inner(&rin))) return rc;
             pending = spec;
-            if (rin < c->d.mg_tol) break;
+            if (rin < target) break;
         }
         LAUNCH(NYL_K_MG_UPDATE, (2 * sizeof(double) + sizeof(float)) * cells(f.g),
                launch_add_correction<float>(f.g, xo, e, c->stream));
