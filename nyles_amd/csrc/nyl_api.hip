@@ -240,6 +240,26 @@ void set_tendency_out(nyl_ctx *c, StageArgs<T> &a, int slot) {
 // residual of the mixed solve).
 // addx (level 0 of the mixed solve): the last sweep adds its result to the
 // FP64 solution instead of storing it (tiled levels), or an add follows.
+// Levels 1 .. nlev-2 run the fused level kernels (down: second pre-sweep +
+// restriction; up: prolongation + both post-sweeps), whose up-leg leaves the
+// level's correction in y.
+#ifndef NYL_FUSED_LEVELS
+#define NYL_FUSED_LEVELS 1
+#endif
+#ifndef NYL_FUSED_MAX_CELLS
+#define NYL_FUSED_MAX_CELLS 32768
+#endif
+bool fused_level(const nyl_ctx *c, int l) {
+    return NYL_FUSED_LEVELS && l >= 1 && l <= c->nlev - 2 &&
+           cells(c->lev[l].g) <= NYL_FUSED_MAX_CELLS &&
+           fused_fits(c->lev[l].g, c->lev[l + 1].g);
+}
+
+// offset of the buffer that holds level l's correction after its up-leg
+uint64_t sol_off(const nyl_ctx *c, int l) {
+    return fused_level(c, l) ? c->lev[l].y : c->lev[l].x;
+}
+
 template <typename Tf>
 int smooth_level_t(nyl_ctx *c, int l, int xc_level, bool pre_done, double *addx) {
     Level &f = c->lev[l];
@@ -250,13 +270,14 @@ int smooth_level_t(nyl_ctx *c, int l, int xc_level, bool pre_done, double *addx)
     const bool tiled = sweep25_fits(f.g);
     if (xc_level >= 0) {
         Level &q = c->lev[xc_level];
+        const uint64_t qx = sol_off(c, xc_level);
         const bool same = lev64(c, xc_level) == std::is_same<Tf, double>::value;
         if (NYL_FUSE_PROLONG && tiled && same) {
             // prolongation fused into the first sweep: x + P(xc) -> y, then y -> x
             const double bp = (3 + 0.125) * cells(f.g) * sizeof(Tf);
             LAUNCH(SWC, bp,
                    launch_mg_sweep25<Tf>(f.g, x, y, b, om, c->tabs, nullptr, nullptr, c->stream,
-                                         &q.g, F<Tf>(c, q.x)));
+                                         &q.g, F<Tf>(c, qx)));
             LAUNCH(SWC, by, launch_mg_sweep25<Tf>(f.g, y, x, b, om, c->tabs, nullptr,
                                                               nullptr, c->stream));
             return NYL_OK;
@@ -264,14 +285,14 @@ int smooth_level_t(nyl_ctx *c, int l, int xc_level, bool pre_done, double *addx)
         const double bp = 2.125 * cells(f.g) * sizeof(Tf);
         if (same && restrict25_fits(f.g)) {
             LAUNCH(NYL_K_MG_PROLONG, bp,
-                   launch_mg_prolong25<Tf, Tf>(f.g, q.g, x, F<Tf>(c, q.x), c->tabs, c->stream));
+                   launch_mg_prolong25<Tf, Tf>(f.g, q.g, x, F<Tf>(c, qx), c->tabs, c->stream));
         } else if (lev64(c, xc_level)) {
             LAUNCH(NYL_K_MG_PROLONG, bp,
-                   launch_mg_prolong<Tf, double>(f.g, q.g, x, F<double>(c, q.x), c->tabs,
+                   launch_mg_prolong<Tf, double>(f.g, q.g, x, F<double>(c, qx), c->tabs,
                                                  c->stream));
         } else if constexpr (std::is_same<Tf, float>::value) {
             LAUNCH(NYL_K_MG_PROLONG, bp,
-                   launch_mg_prolong<float, float>(f.g, q.g, x, F<float>(c, q.x), c->tabs,
+                   launch_mg_prolong<float, float>(f.g, q.g, x, F<float>(c, qx), c->tabs,
                                                    c->stream));
         } else {
             return fail(NYL_E_STATE, "FP64 level above an FP32 level");
@@ -356,6 +377,62 @@ int restrict_level(nyl_ctx *c, int l) {
     return NYL_OK;
 }
 
+// level l's second pre-sweep (y -> x) and the restriction into level l + 1
+// (b and the first sweep from zero into y) in one launch
+int fused_down(nyl_ctx *c, int l) {
+    Level &f = c->lev[l], &q = c->lev[l + 1];
+    const bool f64 = lev64(c, l), c64 = lev64(c, l + 1);
+    const double om = c->d.mg_omega;
+    const double es = f64 ? sizeof(double) : sizeof(float);
+    const double bytes = 3 * cells(f.g) * es + 2 * cells(q.g) * (c64 ? 8.0 : 4.0);
+    if (f64 && c64)
+        LAUNCH(NYL_K_MG_SWEEP, bytes,
+               launch_mg_down<double, double>(f.g, q.g, F<double>(c, f.y), F<double>(c, f.b),
+                                              F<double>(c, f.x), F<double>(c, q.b),
+                                              F<double>(c, q.y), om, c->tabs, c->stream));
+    else if (!f64 && c64)
+        LAUNCH(NYL_K_MG_SWEEP, bytes,
+               launch_mg_down<float, double>(f.g, q.g, F<float>(c, f.y), F<float>(c, f.b),
+                                             F<float>(c, f.x), F<double>(c, q.b),
+                                             F<double>(c, q.y), om, c->tabs, c->stream));
+    else if (!f64 && !c64)
+        LAUNCH(NYL_K_MG_SWEEP, bytes,
+               launch_mg_down<float, float>(f.g, q.g, F<float>(c, f.y), F<float>(c, f.b),
+                                            F<float>(c, f.x), F<float>(c, q.b),
+                                            F<float>(c, q.y), om, c->tabs, c->stream));
+    else
+        return fail(NYL_E_STATE, "FP64 level above an FP32 level");
+    return NYL_OK;
+}
+
+// level l's prolongation from level l + 1 and both post-sweeps, x -> y
+int fused_up(nyl_ctx *c, int l) {
+    Level &f = c->lev[l], &q = c->lev[l + 1];
+    const bool f64 = lev64(c, l), c64 = lev64(c, l + 1);
+    const double om = c->d.mg_omega;
+    const uint64_t qx = sol_off(c, l + 1);
+    const double es = f64 ? sizeof(double) : sizeof(float);
+    const double bytes = 3 * cells(f.g) * es + cells(q.g) * (c64 ? 8.0 : 4.0);
+    if (f64 && c64)
+        LAUNCH(NYL_K_MG_PROLONG, bytes,
+               launch_mg_up<double, double>(f.g, q.g, F<double>(c, f.x), F<double>(c, qx),
+                                            F<double>(c, f.b), F<double>(c, f.y), om, c->tabs,
+                                            c->stream));
+    else if (!f64 && c64)
+        LAUNCH(NYL_K_MG_PROLONG, bytes,
+               launch_mg_up<float, double>(f.g, q.g, F<float>(c, f.x), F<double>(c, qx),
+                                           F<float>(c, f.b), F<float>(c, f.y), om, c->tabs,
+                                           c->stream));
+    else if (!f64 && !c64)
+        LAUNCH(NYL_K_MG_PROLONG, bytes,
+               launch_mg_up<float, float>(f.g, q.g, F<float>(c, f.x), F<float>(c, qx),
+                                          F<float>(c, f.b), F<float>(c, f.y), om, c->tabs,
+                                          c->stream));
+    else
+        return fail(NYL_E_STATE, "FP64 level above an FP32 level");
+    return NYL_OK;
+}
+
 // the part of a V-cycle below level 1: from level 2's smoothing (its first
 // sweep written by the restriction from level 1) down to the coarsest level
 // and back up to level 2's post-smoothing
@@ -363,12 +440,21 @@ int coarse_cycle(nyl_ctx *c) {
     const int L = c->nlev;
     int rc;
     for (int l = 1; l < L - 1; l++) {
+        if (fused_level(c, l)) {
+            if ((rc = fused_down(c, l))) return rc;
+            continue;
+        }
         if ((rc = smooth_level(c, l, -1, true))) return rc;
         if ((rc = restrict_level(c, l))) return rc;
     }
     if ((rc = smooth_level(c, L - 1, -1, true))) return rc;
-    for (int l = L - 2; l >= 1; l--)
+    for (int l = L - 2; l >= 1; l--) {
+        if (fused_level(c, l)) {
+            if ((rc = fused_up(c, l))) return rc;
+            continue;
+        }
         if ((rc = smooth_level(c, l, l + 1))) return rc;
+    }
     return NYL_OK;
 }
 

@@ -80,6 +80,16 @@ struct MGTabs {
 template <typename T>
 hipError_t launch_mg_sweep(const Geo &g, const T *src, T *dst, const T *b, double omega,
                            const MGTabs &tab, hipStream_t s);
+// Fused level kernels (below the finest level).  down: x = sweep(y), then
+// the residual b - A x restricted into bc and the coarse first sweep from
+// zero into yc.  up: y = sweep(sweep(x + P xc)); x and xc are only read.
+bool fused_fits(const Geo &gf, const Geo &gc);
+template <typename Tf, typename Tc>
+hipError_t launch_mg_down(const Geo &gf, const Geo &gc, const Tf *y, const Tf *b, Tf *x, Tc *bc,
+                          Tc *yc, double omega, const MGTabs &tab, hipStream_t s);
+template <typename Tf, typename Tc>
+hipError_t launch_mg_up(const Geo &gf, const Geo &gc, const Tf *x, const Tc *xc, const Tf *b,
+                        Tf *y, double omega, const MGTabs &tab, hipStream_t s);
 // residual + restriction from a level of type Tf to the next of type Tc;
 // yc non-null: also the coarse level's first sweep from x = 0 into yc (coarse
 // x is then left as is, the second sweep overwrites it), else xc = 0

@@ -984,6 +984,271 @@ hipError_t launch_b_acc(int n, const BMG &mg, int mode, hipStream_t s) {
 
 #undef BT
 
+// ---------------------------------------------------------------------------
+// Fused level kernels for the levels below the finest, where a launch costs
+// more than its data: one launch does a level's second pre-smoothing sweep
+// and the residual restriction to the next level (k_mg_down), one does the
+// prolongation and both post-smoothing sweeps (k_mg_up).  A block owns an
+// FT^3 tile of fine cells and recomputes a halo in LDS (two cells for two
+// sweeps, one for sweep + residual); every value is computed by the
+// expression of the per-cell kernels (sweep_cell, residual_at,
+// restrict_cell, prolong_term), so the results are bitwise those of the
+// separate launches.  k_mg_up reads x and writes its result to y (other
+// blocks read x in their halos), so after it the level's correction is in y.
+
+#ifndef NYL_FUSED_TILE
+#define NYL_FUSED_TILE 4
+#endif
+#define FT NYL_FUSED_TILE    // fine tile edge (cells)
+#define FH2 (FT + 4)         // tile + two-cell halo
+#define FH1 (FT + 2)         // tile + one-cell halo
+#define FUSED_T 256
+
+// periodic wrap for any offset (tiles may exceed a small level)
+__device__ __forceinline__ int wrapg(int m, int n) {
+    const int r = m % n;
+    return r < 0 ? r + n : r;
+}
+
+// in-domain test of an unwrapped coordinate; w = the cell it stands for
+__device__ __forceinline__ bool axis_in(int m, int n, int per, int &w) {
+    if (per) {
+        w = wrapg(m, n);
+        return true;
+    }
+    w = m;
+    return m >= 0 && m < n;
+}
+
+// sweep_cell with the source in an E^3 LDS box; (a, bb, cc) the box
+// position of the cell, (wi, wj, wk) the cell (for the wall tests)
+template <typename T, int E>
+__device__ __forceinline__ T sweep_box(const Geo &g, const T *src, int a, int bb, int cc, int wi,
+                                       int wj, int wk, T bv, T om, T cff1, const MGTabs &tab) {
+    const int q = (cc * E + bb) * E + a;
+    int nw = 0;
+    T xm = T(0), xp = T(0), ym = T(0), yp = T(0), zm = T(0), zp = T(0);
+    if (g.px || wi > 0) { xm = src[q - 1]; nw++; }
+    if (g.px || wi < g.nx - 1) { xp = src[q + 1]; nw++; }
+    if (g.py || wj > 0) { ym = src[q - E]; nw++; }
+    if (g.py || wj < g.ny - 1) { yp = src[q + E]; nw++; }
+    if (g.pz || wk > 0) { zm = src[q - E * E]; nw++; }
+    if (g.pz || wk < g.nz - 1) { zp = src[q + E * E]; nw++; }
+    const T S = xm + xp + ym + yp + zm + zp;
+    return cff1 * src[q] + om * (S - bv) * T(tab.idiag[nw]);
+}
+
+// residual_at with x in an E^3 LDS box
+template <typename T, int E>
+__device__ __forceinline__ T resid_box(const Geo &g, const T *src, int a, int bb, int cc, int wi,
+                                       int wj, int wk, T bv, const MGTabs &tab) {
+    const int q = (cc * E + bb) * E + a;
+    int nw = 0;
+    T xm = T(0), xp = T(0), ym = T(0), yp = T(0), zm = T(0), zp = T(0);
+    if (g.px || wi > 0) { xm = src[q - 1]; nw++; }
+    if (g.px || wi < g.nx - 1) { xp = src[q + 1]; nw++; }
+    if (g.py || wj > 0) { ym = src[q - E]; nw++; }
+    if (g.py || wj < g.ny - 1) { yp = src[q + E]; nw++; }
+    if (g.pz || wk > 0) { zm = src[q - E * E]; nw++; }
+    if (g.pz || wk < g.nz - 1) { zp = src[q + E * E]; nw++; }
+    return bv + T(tab.diag[nw]) * src[q] - (xm + xp + ym + yp + zm + zp);
+}
+
+// Loads of an E^3 box of a level field at unwrapped offsets from (i0, j0,
+// k0), zero outside a wall axis: every load of a thread is issued before
+// the first is stored, so a block waits for one memory latency, not one per
+// round.
+template <typename T, int E>
+__device__ __forceinline__ void load_box(const Geo &g, const T *__restrict__ src, T *box, int i0,
+                                         int j0, int k0, int tid) {
+    constexpr int N = E * E * E, R = (N + FUSED_T - 1) / FUSED_T;
+    T v[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const int q = tid + r * FUSED_T;
+        const int a = q % E, t = q / E, bb = t % E, cc = t / E;
+        int wi, wj, wk;
+        const bool ii = axis_in(i0 + a, g.nx, g.px, wi);
+        const bool ij = axis_in(j0 + bb, g.ny, g.py, wj);
+        const bool ik = axis_in(k0 + cc, g.nz, g.pz, wk);
+        v[r] = (q < N && ii && ij && ik) ? src[(int64_t)wk * g.sz + (int64_t)wj * g.sy + wi] : T(0);
+    }
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const int q = tid + r * FUSED_T;
+        if (q < N) box[q] = v[r];
+    }
+}
+
+// x = sweep(y) on the level; b_c = restriction of b - A x; y_c = the
+// coarse level's first sweep from zero
+template <typename Tf, typename Tc>
+__global__ __launch_bounds__(FUSED_T) void k_mg_down(Geo gf, Geo gc, const Tf *__restrict__ y,
+                                                     const Tf *__restrict__ b,
+                                                     Tf *__restrict__ x, Tc *__restrict__ bc,
+                                                     Tc *__restrict__ yc, double omega,
+                                                     MGTabs tab) {
+    __shared__ Tf s2[FH2 * FH2 * FH2];
+    __shared__ Tf s1[FH1 * FH1 * FH1];
+    __shared__ Tf sb[FH1 * FH1 * FH1];
+    const int tid = threadIdx.x;
+    const int i0 = blockIdx.x * FT, j0 = blockIdx.y * FT, k0 = blockIdx.z * FT;
+    const Tf om = Tf(omega), cff1 = Tf(1.0 - omega);
+    load_box<Tf, FH2>(gf, y, s2, i0 - 2, j0 - 2, k0 - 2, tid);
+    load_box<Tf, FH1>(gf, b, sb, i0 - 1, j0 - 1, k0 - 1, tid);
+    __syncthreads();
+    for (int q = tid; q < FH1 * FH1 * FH1; q += FUSED_T) {
+        const int a = q % FH1, t = q / FH1, bb = t % FH1, cc = t / FH1;
+        const int gi = i0 - 1 + a, gj = j0 - 1 + bb, gk = k0 - 1 + cc;
+        int wi, wj, wk;
+        const bool ii = axis_in(gi, gf.nx, gf.px, wi);
+        const bool ij = axis_in(gj, gf.ny, gf.py, wj);
+        const bool ik = axis_in(gk, gf.nz, gf.pz, wk);
+        Tf v = Tf(0);
+        if (ii && ij && ik) {
+            v = sweep_box<Tf, FH2>(gf, s2, a + 1, bb + 1, cc + 1, wi, wj, wk, sb[q], om, cff1, tab);
+            const bool own = a >= 1 && a <= FT && bb >= 1 && bb <= FT && cc >= 1 && cc <= FT &&
+                             gi < gf.nx && gj < gf.ny && gk < gf.nz;
+            if (own) x[(int64_t)wk * gf.sz + (int64_t)wj * gf.sy + wi] = v;
+        }
+        s1[q] = v;
+    }
+    __syncthreads();
+    if (tid < (FT / 2) * (FT / 2) * (FT / 2)) {
+        const int ca = tid % (FT / 2), cb = (tid / (FT / 2)) % (FT / 2), cz = tid / ((FT / 2) * (FT / 2));
+        const int ic = i0 / 2 + ca, jc = j0 / 2 + cb, kc = k0 / 2 + cz;
+        if (ic < gc.nx && jc < gc.ny && kc < gc.nz) {
+            auto R = [&](int dx, int dy, int dz) -> Tf {
+                const int a = 1 + 2 * ca + dx, bb = 1 + 2 * cb + dy, cc = 1 + 2 * cz + dz;
+                return resid_box<Tf, FH1>(gf, s1, a, bb, cc, 2 * ic + dx, 2 * jc + dy, 2 * kc + dz,
+                                          sb[(cc * FH1 + bb) * FH1 + a], tab);
+            };
+            const Tf r000 = R(0, 0, 0), r100 = R(1, 0, 0), r010 = R(0, 1, 0), r110 = R(1, 1, 0);
+            const Tf r001 = R(0, 0, 1), r101 = R(1, 0, 1), r011 = R(0, 1, 1), r111 = R(1, 1, 1);
+            const int64_t ccell = (int64_t)kc * gc.sz + (int64_t)jc * gc.sy + ic;
+            const Tc bv = Tc(Tf(0.5) * (r000 + r100 + r010 + r110 + r001 + r101 + r011 + r111));
+            bc[ccell] = bv;
+            yc[ccell] = sweep_from_zero<Tc>(gc, bv, omega, tab, ic, jc, kc);
+        }
+    }
+}
+
+#define FC (FT / 2 + 4)      // coarse box edge of k_mg_up
+
+// prolong_term with the coarse values in an FC^3 LDS box whose origin is
+// the coarse cell (ci0, cj0, ck0); (gi, gj, gk) an unwrapped fine cell (on a
+// periodic axis the same coarse values, wrapped, as for the cell it stands
+// for; on a wall axis the cell itself)
+template <typename Tf, typename Tc>
+__device__ __forceinline__ Tf prolong_box(const Geo &gc, const Tc *cbx, int ci0, int cj0, int ck0,
+                                          int gi, int gj, int gk, const MGTabs &tab) {
+    const int ic = gi >> 1, jc = gj >> 1, kc = gk >> 1;
+    const int icn = (gi & 1) ? ic + 1 : ic - 1;
+    const int jcn = (gj & 1) ? jc + 1 : jc - 1;
+    const int kcn = (gk & 1) ? kc + 1 : kc - 1;
+    const int mx = (gc.px || (icn >= 0 && icn < gc.nx)) ? 1 : 0;
+    const int my = (gc.py || (jcn >= 0 && jcn < gc.ny)) ? 1 : 0;
+    const int mz = (gc.pz || (kcn >= 0 && kcn < gc.nz)) ? 1 : 0;
+    auto X = [&](int a, int bb, int cc, bool ok) -> Tf {
+        return ok ? Tf(cbx[((cc - ck0) * FC + (bb - cj0)) * FC + (a - ci0)]) : Tf(0);
+    };
+    auto A = [&](int kk, bool okz) -> Tf {
+        return Tf(9) * X(ic, jc, kk, okz) + Tf(3) * X(icn, jc, kk, okz && mx)
+             + Tf(3) * X(ic, jcn, kk, okz && my) + X(icn, jcn, kk, okz && mx && my);
+    };
+    const Tf bmid = A(kc, true);
+    const Tf aside = A(kcn, mz != 0);
+    return Tf(tab.pcoef[mx + 2 * my + 4 * mz]) * (Tf(3) * bmid + aside);
+}
+
+// y = sweep(sweep(x + P xc)) on the level (x and xc are not written)
+template <typename Tf, typename Tc>
+__global__ __launch_bounds__(FUSED_T) void k_mg_up(Geo gf, Geo gc, const Tf *__restrict__ x,
+                                                   const Tc *__restrict__ xc,
+                                                   const Tf *__restrict__ b, Tf *__restrict__ y,
+                                                   double omega, MGTabs tab) {
+    __shared__ Tf s2[FH2 * FH2 * FH2];
+    __shared__ Tf s1[FH1 * FH1 * FH1];
+    __shared__ Tf sb[FH1 * FH1 * FH1];
+    __shared__ Tc scx[FC * FC * FC];
+    const int tid = threadIdx.x;
+    const int i0 = blockIdx.x * FT, j0 = blockIdx.y * FT, k0 = blockIdx.z * FT;
+    const int ci0 = i0 / 2 - 2, cj0 = j0 / 2 - 2, ck0 = k0 / 2 - 2;
+    const Tf om = Tf(omega), cff1 = Tf(1.0 - omega);
+    load_box<Tf, FH2>(gf, x, s2, i0 - 2, j0 - 2, k0 - 2, tid);
+    load_box<Tf, FH1>(gf, b, sb, i0 - 1, j0 - 1, k0 - 1, tid);
+    load_box<Tc, FC>(gc, xc, scx, ci0, cj0, ck0, tid);
+    __syncthreads();
+    for (int q = tid; q < FH2 * FH2 * FH2; q += FUSED_T) {
+        const int a = q % FH2, t = q / FH2, bb = t % FH2, cc = t / FH2;
+        const int gi = i0 - 2 + a, gj = j0 - 2 + bb, gk = k0 - 2 + cc;
+        int wi, wj, wk;
+        const bool ii = axis_in(gi, gf.nx, gf.px, wi);
+        const bool ij = axis_in(gj, gf.ny, gf.py, wj);
+        const bool ik = axis_in(gk, gf.nz, gf.pz, wk);
+        if (ii && ij && ik) s2[q] = s2[q] + prolong_box<Tf, Tc>(gc, scx, ci0, cj0, ck0, gi, gj, gk, tab);
+    }
+    __syncthreads();
+    for (int q = tid; q < FH1 * FH1 * FH1; q += FUSED_T) {
+        const int a = q % FH1, t = q / FH1, bb = t % FH1, cc = t / FH1;
+        int wi, wj, wk;
+        const bool ii = axis_in(i0 - 1 + a, gf.nx, gf.px, wi);
+        const bool ij = axis_in(j0 - 1 + bb, gf.ny, gf.py, wj);
+        const bool ik = axis_in(k0 - 1 + cc, gf.nz, gf.pz, wk);
+        Tf v = Tf(0);
+        if (ii && ij && ik)
+            v = sweep_box<Tf, FH2>(gf, s2, a + 1, bb + 1, cc + 1, wi, wj, wk, sb[q], om, cff1, tab);
+        s1[q] = v;
+    }
+    __syncthreads();
+    for (int q = tid; q < FT * FT * FT; q += FUSED_T) {
+        const int a = q % FT, t = q / FT, bb = t % FT, cc = t / FT;
+        const int i = i0 + a, j = j0 + bb, k = k0 + cc;
+        if (i < gf.nx && j < gf.ny && k < gf.nz) {
+            const int64_t c = (int64_t)k * gf.sz + (int64_t)j * gf.sy + i;
+            const Tf bv = sb[((cc + 1) * FH1 + bb + 1) * FH1 + a + 1];
+            y[c] = sweep_box<Tf, FH1>(gf, s1, a + 1, bb + 1, cc + 1, i, j, k, bv, om, cff1, tab);
+        }
+    }
+}
+
+static inline dim3 fused_grid(const Geo &gf) {
+    return dim3((gf.nx + FT - 1) / FT, (gf.ny + FT - 1) / FT, (gf.nz + FT - 1) / FT);
+}
+
+bool fused_fits(const Geo &gf, const Geo &gc) {
+    return gf.nx == 2 * gc.nx && gf.ny == 2 * gc.ny && gf.nz == 2 * gc.nz && gc.nx >= 1 &&
+           gc.ny >= 1 && gc.nz >= 1;
+}
+
+template <typename Tf, typename Tc>
+hipError_t launch_mg_down(const Geo &gf, const Geo &gc, const Tf *y, const Tf *b, Tf *x, Tc *bc,
+                          Tc *yc, double omega, const MGTabs &tab, hipStream_t s) {
+    hipLaunchKernelGGL((k_mg_down<Tf, Tc>), fused_grid(gf), dim3(FUSED_T), 0, s, gf, gc, y, b, x,
+                       bc, yc, omega, tab);
+    return hipGetLastError();
+}
+
+template <typename Tf, typename Tc>
+hipError_t launch_mg_up(const Geo &gf, const Geo &gc, const Tf *x, const Tc *xc, const Tf *b,
+                        Tf *y, double omega, const MGTabs &tab, hipStream_t s) {
+    hipLaunchKernelGGL((k_mg_up<Tf, Tc>), fused_grid(gf), dim3(FUSED_T), 0, s, gf, gc, x, xc, b,
+                       y, omega, tab);
+    return hipGetLastError();
+}
+
+#define NYL_FUSED_INST(Tf, Tc)                                                                  \
+    template hipError_t launch_mg_down<Tf, Tc>(const Geo &, const Geo &, const Tf *,            \
+                                               const Tf *, Tf *, Tc *, Tc *, double,             \
+                                               const MGTabs &, hipStream_t);                     \
+    template hipError_t launch_mg_up<Tf, Tc>(const Geo &, const Geo &, const Tf *, const Tc *,  \
+                                             const Tf *, Tf *, double, const MGTabs &,          \
+                                             hipStream_t);
+NYL_FUSED_INST(double, double)
+NYL_FUSED_INST(float, float)
+NYL_FUSED_INST(float, double)
+#undef NYL_FUSED_INST
+
 bool sweep25_fits(const Geo &g) {
     return g.nx % SW_TX == 0 && g.ny % SW_TY == 0 && g.nx >= SW_TX;
 }
