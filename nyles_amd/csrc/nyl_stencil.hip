@@ -5,6 +5,7 @@
 // by operation (the library is compiled with -ffp-contract=off), so its
 // results are bit-identical to the oracle's.
 #include "nyl_internal.h"
+#include <initializer_list>
 
 namespace nyl {
 
@@ -349,6 +350,207 @@ hipError_t launch_stage(const Geo &g, const StageArgs<T> &a, int mode, hipStream
     return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// Four cells along x per thread: the per-cell kernels here are latency-bound
+// at one cell per thread (one 4-byte load per field and cell); the x4 forms
+// move the same bytes with 16-byte loads and stores, a quarter of the memory
+// instructions.  Each cell's arithmetic is that of div_cell, correct_cell or
+// diag_cell, term for term, so the results are the same bits.  They need
+// nx % 4 == 0 and 16-byte aligned fields (the workspace is 256-byte aligned).
+
+template <typename T>
+struct Q4 {
+    T v[4];
+};
+__device__ __forceinline__ Q4<float> ld4(const float *p) {
+    const float4 q = *reinterpret_cast<const float4 *>(p);
+    return {{q.x, q.y, q.z, q.w}};
+}
+__device__ __forceinline__ Q4<double> ld4(const double *p) {
+    const double2 a = *reinterpret_cast<const double2 *>(p);
+    const double2 b = *reinterpret_cast<const double2 *>(p + 2);
+    return {{a.x, a.y, b.x, b.y}};
+}
+__device__ __forceinline__ void st4(float *p, const Q4<float> &q) {
+    *reinterpret_cast<float4 *>(p) = make_float4(q.v[0], q.v[1], q.v[2], q.v[3]);
+}
+__device__ __forceinline__ void st4(double *p, const Q4<double> &q) {
+    *reinterpret_cast<double2 *>(p) = make_double2(q.v[0], q.v[1]);
+    *reinterpret_cast<double2 *>(p + 2) = make_double2(q.v[2], q.v[3]);
+}
+
+static inline dim3 cell_grid4(const Geo &g) {
+    return dim3((g.nx / 4 + NYL_BX - 1) / NYL_BX, (g.ny + NYL_BY - 1) / NYL_BY, g.nz / cell_kz(g));
+}
+
+static inline bool quad_ok(const Geo &g, std::initializer_list<const void *> ptrs) {
+    if (g.nx % 4 != 0) return false;
+    for (const void *p : ptrs)
+        if (p && (reinterpret_cast<uintptr_t>(p) & 15u) != 0) return false;
+    return true;
+}
+
+// diag_cell for cells i0 .. i0+3
+template <typename T>
+__global__ __launch_bounds__(256) void k_diag4(Geo g, DiagArgs<T> a, int what) {
+    const Blk3 bk = CELL_BLOCK();
+    const int i0 = (bk.x * NYL_BX + threadIdx.x) * 4;
+    const int j = bk.y * NYL_BY + threadIdx.y;
+    if (i0 >= g.nx || j >= g.ny) return;
+    const int kz = g.nz / (int)gridDim.z;
+    // x neighbours of the quad's end cells
+    const bool hxm = g.px || i0 > 0;                  // cell i0 has an x-1 neighbour
+    const int64_t oxm = g.px ? (int64_t)(i0 - wrapi(i0 - 1, g.nx)) : 1;
+    const bool ex3 = g.px || i0 + 3 < g.nx - 1;       // cell i0+3 has an x+1 neighbour
+    const int64_t oxp3 = g.px ? (int64_t)(wrapi(i0 + 4, g.nx) - (i0 + 3)) : 1;
+    const bool ey = g.py || j < g.ny - 1, hym = g.py || j > 0;
+    const int64_t oyp = g.py ? (int64_t)(wrapi(j + 1, g.ny) - j) * g.sy : g.sy;
+    const int64_t oym = g.py ? (int64_t)(j - wrapi(j - 1, g.ny)) * g.sy : g.sy;
+    for (int k = bk.z * kz; k < (bk.z + 1) * kz; k++) {
+        const int64_t c = (int64_t)k * g.sz + (int64_t)j * g.sy + i0;
+        const Q4<T> u = ld4(a.u + c), v = ld4(a.v + c), w = ld4(a.w + c);
+        if (what & 1) {
+            Q4<T> X, Y, Z;
+#pragma unroll
+            for (int m = 0; m < 4; m++) {
+                X.v[m] = u.v[m] * T(a.idx2);
+                Y.v[m] = v.v[m] * T(a.idy2);
+                Z.v[m] = w.v[m] * T(a.idz2);
+            }
+            st4(a.Ux + c, X); st4(a.Uy + c, Y); st4(a.Uz + c, Z);
+        }
+        const bool ez = g.pz || k < g.nz - 1, hzm = g.pz || k > 0;
+        const int64_t ozp = g.pz ? (int64_t)(wrapi(k + 1, g.nz) - k) * g.sz : g.sz;
+        const int64_t ozm = g.pz ? (int64_t)(k - wrapi(k - 1, g.nz)) * g.sz : g.sz;
+        if (what & 2) {
+            const T v4 = ex3 ? a.v[c + 3 + oxp3] : T(0);
+            const T w4 = ex3 ? a.w[c + 3 + oxp3] : T(0);
+            Q4<T> uyp{}, wyp{}, vzp{}, uzp{};
+            if (ey) { uyp = ld4(a.u + c + oyp); wyp = ld4(a.w + c + oyp); }
+            if (ez) { vzp = ld4(a.v + c + ozp); uzp = ld4(a.u + c + ozp); }
+            Q4<T> OX, OY, OZ;
+#pragma unroll
+            for (int m = 0; m < 4; m++) {
+                const bool ex = m < 3 || ex3;
+                const T vxp = m < 3 ? v.v[m + 1] : v4, wxp = m < 3 ? w.v[m + 1] : w4;
+                const T uc = u.v[m], vc = v.v[m], wc = w.v[m];
+                T oz = T(0);
+                if (ex && ey) {
+                    oz = vxp - vc - uyp.v[m] + uc;
+                    if (a.fpar > 0.0) oz = oz + T(a.fpar);
+                }
+                T ox = T(0);
+                if (ey && ez) ox = wyp.v[m] - wc - vzp.v[m] + vc;
+                T oy = T(0);
+                if (ez && ex) oy = uzp.v[m] - uc - wxp + wc;
+                OX.v[m] = ox; OY.v[m] = oy; OZ.v[m] = oz;
+            }
+            st4(a.ox + c, OX); st4(a.oy + c, OY); st4(a.oz + c, OZ);
+        }
+        if (what & 4) {
+            const T um0 = hxm ? a.u[c - oxm] : T(0);
+            Q4<T> vm{}, wm{};
+            if (hym) vm = ld4(a.v + c - oym);
+            if (hzm) wm = ld4(a.w + c - ozm);
+            Q4<T> KE;
+#pragma unroll
+            for (int m = 0; m < 4; m++) {
+                const T uc = u.v[m], vc = v.v[m], wc = w.v[m];
+                T ke = T(0);
+                if (m > 0 || hxm) {
+                    const T um = m > 0 ? u.v[m - 1] : um0;
+                    const T cff2 = T(0.5 * a.idx2);
+                    ke = ke + cff2 * T(0.5) * (uc * uc + um * um);
+                }
+                if (hym) {
+                    const T cff2 = T(0.5 * a.idy2);
+                    ke = ke + cff2 * T(0.5) * (vc * vc + vm.v[m] * vm.v[m]);
+                }
+                if (hzm) {
+                    const T cff2 = T(0.5 * a.idz2);
+                    ke = ke + cff2 * T(0.5) * (wc * wc + wm.v[m] * wm.v[m]);
+                }
+                KE.v[m] = ke;
+            }
+            st4(a.ke + c, KE);
+        }
+    }
+}
+
+// div_cell for cells i0 .. i0+3
+template <typename T, typename TO>
+__global__ __launch_bounds__(256) void k_div4(Geo g, DivArgs<T, TO> a) {
+    const Blk3 bk = CELL_BLOCK();
+    const int i0 = (bk.x * NYL_BX + threadIdx.x) * 4;
+    const int j = bk.y * NYL_BY + threadIdx.y;
+    if (i0 >= g.nx || j >= g.ny) return;
+    const int kz = g.nz / (int)gridDim.z;
+    const TO cx = TO(a.idx2), cy = TO(a.idy2), cz = TO(a.idz2);
+    const bool hxm = g.px || i0 > 0, hym = g.py || j > 0;
+    const int64_t oxm = g.px ? (int64_t)(i0 - wrapi(i0 - 1, g.nx)) : 1;
+    const int64_t oym = g.py ? (int64_t)(j - wrapi(j - 1, g.ny)) * g.sy : g.sy;
+    for (int k = bk.z * kz; k < (bk.z + 1) * kz; k++) {
+        const int64_t c = (int64_t)k * g.sz + (int64_t)j * g.sy + i0;
+        const bool hzm = g.pz || k > 0;
+        const int64_t ozm = g.pz ? (int64_t)(k - wrapi(k - 1, g.nz)) * g.sz : g.sz;
+        const Q4<T> u = ld4(a.u + c), v = ld4(a.v + c), w = ld4(a.w + c);
+        const T um0 = hxm ? a.u[c - oxm] : T(0);
+        Q4<T> vm{}, wm{};
+        if (hym) vm = ld4(a.v + c - oym);
+        if (hzm) wm = ld4(a.w + c - ozm);
+        Q4<TO> D;
+#pragma unroll
+        for (int m = 0; m < 4; m++) {
+            TO d;
+            const TO Uc = TO(u.v[m]) * cx;
+            if (m == 0 && !hxm) d = Uc;
+            else d = Uc - TO(m > 0 ? u.v[m - 1] : um0) * cx;
+            const TO Vc = TO(v.v[m]) * cy;
+            if (!hym) d = d + Vc;
+            else d = d + (Vc - TO(vm.v[m]) * cy);
+            const TO Wc = TO(w.v[m]) * cz;
+            if (!hzm) d = d + Wc;
+            else d = d + (Wc - TO(wm.v[m]) * cz);
+            D.v[m] = d;
+        }
+        st4(a.out + c, D);
+    }
+}
+
+// correct_cell for cells i0 .. i0+3 (a cell without the face keeps its value)
+template <typename T, typename TX>
+__global__ __launch_bounds__(256) void k_correct4(Geo g, CorrectArgs<T, TX> a) {
+    const Blk3 bk = CELL_BLOCK();
+    const int i0 = (bk.x * NYL_BX + threadIdx.x) * 4;
+    const int j = bk.y * NYL_BY + threadIdx.y;
+    if (i0 >= g.nx || j >= g.ny) return;
+    const int kz = g.nz / (int)gridDim.z;
+    const bool ex3 = g.px || i0 + 3 < g.nx - 1;
+    const int64_t oxp3 = g.px ? (int64_t)(wrapi(i0 + 4, g.nx) - (i0 + 3)) : 1;
+    const bool ey = g.py || j < g.ny - 1;
+    const int64_t oyp = g.py ? (int64_t)(wrapi(j + 1, g.ny) - j) * g.sy : g.sy;
+    const TX dx2 = TX(a.dx2);
+    for (int k = bk.z * kz; k < (bk.z + 1) * kz; k++) {
+        const int64_t c = (int64_t)k * g.sz + (int64_t)j * g.sy + i0;
+        const bool ez = g.pz || k < g.nz - 1;
+        const int64_t ozp = g.pz ? (int64_t)(wrapi(k + 1, g.nz) - k) * g.sz : g.sz;
+        const Q4<TX> x = ld4(a.x + c);
+        Q4<T> u = ld4(a.u + c), v = ld4(a.v + c), w = ld4(a.w + c);
+        const TX x4 = ex3 ? a.x[c + 3 + oxp3] : TX(0);
+        Q4<TX> xy{}, xz{};
+        if (ey) xy = ld4(a.x + c + oyp);
+        if (ez) xz = ld4(a.x + c + ozp);
+#pragma unroll
+        for (int m = 0; m < 4; m++) {
+            const TX pc = x.v[m] * dx2;
+            if (m < 3 || ex3) u.v[m] = T(TX(u.v[m]) - ((m < 3 ? x.v[m + 1] : x4) * dx2 - pc));
+            if (ey) v.v[m] = T(TX(v.v[m]) - (xy.v[m] * dx2 - pc));
+            if (ez) w.v[m] = T(TX(w.v[m]) - (xz.v[m] * dx2 - pc));
+        }
+        st4(a.u + c, u); st4(a.v + c, v); st4(a.w + c, w);
+    }
+}
+
 // U_from_u (core/cov_to_contra.py:15-20), vorticity (core/vorticity.py:25-34,
 // core/fortran_vorticity.f90:2-28) and kinenergy (core/kinenergy.py:13-24,
 // core/fortran_kinenergy.f90:145-153) from the covariant velocity.
@@ -415,7 +617,10 @@ __global__ __launch_bounds__(256) void k_diag(Geo g, DiagArgs<T> a, int what) {
 
 template <typename T>
 hipError_t launch_diag(const Geo &g, const DiagArgs<T> &a, int what, hipStream_t s) {
-    hipLaunchKernelGGL((k_diag<T>), cell_grid_kz(g), dim3(NYL_BX, NYL_BY, 1), 0, s, g, a, what);
+    if (quad_ok(g, {a.u, a.v, a.w, a.Ux, a.Uy, a.Uz, a.ox, a.oy, a.oz, a.ke}))
+        hipLaunchKernelGGL((k_diag4<T>), cell_grid4(g), dim3(NYL_BX, NYL_BY, 1), 0, s, g, a, what);
+    else
+        hipLaunchKernelGGL((k_diag<T>), cell_grid_kz(g), dim3(NYL_BX, NYL_BY, 1), 0, s, g, a, what);
     return hipGetLastError();
 }
 
@@ -455,7 +660,10 @@ hipError_t launch_div(const Geo &g, const T *u, const T *v, const T *w, double i
                       double idy2, double idz2, TO *out, hipStream_t s) {
     DivArgs<T, TO> a;
     a.u = u; a.v = v; a.w = w; a.idx2 = idx2; a.idy2 = idy2; a.idz2 = idz2; a.out = out;
-    hipLaunchKernelGGL((k_div<T, TO>), cell_grid_kz(g), dim3(NYL_BX, NYL_BY, 1), 0, s, g, a);
+    if (quad_ok(g, {u, v, w, out}))
+        hipLaunchKernelGGL((k_div4<T, TO>), cell_grid4(g), dim3(NYL_BX, NYL_BY, 1), 0, s, g, a);
+    else
+        hipLaunchKernelGGL((k_div<T, TO>), cell_grid_kz(g), dim3(NYL_BX, NYL_BY, 1), 0, s, g, a);
     return hipGetLastError();
 }
 
@@ -496,7 +704,10 @@ hipError_t launch_correct(const Geo &g, T *u, T *v, T *w, const TX *x, double dx
                           hipStream_t s) {
     CorrectArgs<T, TX> a;
     a.u = u; a.v = v; a.w = w; a.x = x; a.dx2 = dx2;
-    hipLaunchKernelGGL((k_correct<T, TX>), cell_grid_kz(g), dim3(NYL_BX, NYL_BY, 1), 0, s, g, a);
+    if (quad_ok(g, {u, v, w, x}))
+        hipLaunchKernelGGL((k_correct4<T, TX>), cell_grid4(g), dim3(NYL_BX, NYL_BY, 1), 0, s, g, a);
+    else
+        hipLaunchKernelGGL((k_correct<T, TX>), cell_grid_kz(g), dim3(NYL_BX, NYL_BY, 1), 0, s, g, a);
     return hipGetLastError();
 }
 
