@@ -9,6 +9,8 @@
 #include <vector>
 
 #include "nyl_ctx.h"
+#include <atomic>
+#include <chrono>
 
 using namespace nyl;
 
@@ -193,11 +195,29 @@ int timed_launch(nyl_ctx *c, int cls, double bytes, Fn fn) {
 
 double cells(const Geo &g) { return (double)g.nx * g.ny * g.nz; }
 
+// The results reach the host through coherent pinned memory: a one-thread
+// kernel stores them there and then a sequence number, and the host polls
+// that number instead of a stream synchronisation plus a copy (each host
+// round trip of the multigrid loop is on the critical path).  After a
+// bounded poll the host falls back to a blocking synchronisation, which also
+// reports a failed kernel.
 int read_results(nyl_ctx *c, double *out, int n) {
-    HIPCHK(hipMemcpyAsync(c->host_out, c->ws + c->result, n * sizeof(double),
-                          hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
-    for (int q = 0; q < n; q++) out[q] = c->host_out[q];
+    const uint32_t seq = ++c->pub_seq;
+    volatile uint32_t *flag = reinterpret_cast<volatile uint32_t *>(c->host_out + 8);
+    HIPCHK(launch_publish(reinterpret_cast<const double *>(c->ws + c->result), n, c->host_dev,
+                          reinterpret_cast<uint32_t *>(c->host_dev + 8), seq, c->stream));
+    const auto t0 = std::chrono::steady_clock::now();
+    for (uint64_t it = 1; *flag != seq; it++) {
+        if ((it & 1023) == 0 &&
+            std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(50)) {
+            HIPCHK(hipStreamSynchronize(c->stream));
+            if (*flag != seq) return fail(NYL_E_HIP, "result of the device reduction not published");
+            break;
+        }
+    }
+    std::atomic_thread_fence(std::memory_order_acquire);
+    const volatile double *v = c->host_out;
+    for (int q = 0; q < n; q++) out[q] = v[q];
     return NYL_OK;
 }
 
@@ -870,7 +890,13 @@ int nyl_bind_workspace(nyl_ctx *c, void *dptr, uint64_t bytes) {
         return fail(NYL_E_ARG, "workspace of %llu bytes is smaller than the %llu needed",
                     (unsigned long long)bytes, (unsigned long long)c->ws_bytes);
     if (((uintptr_t)dptr) & 255) return fail(NYL_E_ARG, "workspace must be 256-byte aligned");
-    if (!c->host_out) HIPCHK(hipHostMalloc((void **)&c->host_out, 8 * sizeof(double), 0));
+    if (!c->host_out) {
+        HIPCHK(hipHostMalloc((void **)&c->host_out, 16 * sizeof(double),
+                             hipHostMallocMapped | hipHostMallocCoherent));
+        HIPCHK(hipHostGetDevicePointer((void **)&c->host_dev, c->host_out, 0));
+        *reinterpret_cast<volatile uint32_t *>(c->host_out + 8) = 0;
+        c->pub_seq = 0;
+    }
     c->ws = (char *)dptr;
     HIPCHK(launch_fill((double *)c->ws, (int64_t)(c->ws_bytes / sizeof(double)), 0.0, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));

@@ -36,7 +36,9 @@ struct nyl_ctx {
     int cur;                 // which state buffer is current
     char *ws;
     hipStream_t stream;
-    double *host_out;        // pinned, 8 doubles
+    double *host_out;        // pinned host memory, coherent: 8 result doubles + a sequence flag
+    double *host_dev;        // host_out as the device addresses it
+    uint32_t pub_seq;        // last published result sequence number
     int total_cycles;
     // the V-cycle below level 1, replayed from a hipGraph (captured on first use)
     int use_graphs;

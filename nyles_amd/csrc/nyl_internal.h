@@ -80,6 +80,9 @@ struct MGTabs {
 template <typename T>
 hipError_t launch_mg_sweep(const Geo &g, const T *src, T *dst, const T *b, double omega,
                            const MGTabs &tab, hipStream_t s);
+// copy n doubles to coherent host memory, then store seq into flag
+hipError_t launch_publish(const double *src, int n, double *dst, uint32_t *flag, uint32_t seq,
+                          hipStream_t s);
 // Fused level kernels (below the finest level).  down: x = sweep(y), then
 // the residual b - A x restricted into bc and the coarse first sweep from
 // zero into yc.  up: y = sweep(sweep(x + P xc)); x and xc are only read.

@@ -698,6 +698,22 @@ hipError_t launch_sum_sumsq(const Geo &g, const double *v, double *partials, dou
     return hipGetLastError();
 }
 
+// n results from device memory to coherent host memory, then the sequence
+// number (after a system-scope fence, so the host sees the values first)
+__global__ __launch_bounds__(64) void k_publish(const double *__restrict__ src, int n,
+                                                double *dst, uint32_t *flag, uint32_t seq) {
+    if (threadIdx.x != 0) return;
+    for (int q = 0; q < n; q++) dst[q] = src[q];
+    __threadfence_system();
+    __hip_atomic_store(flag, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+hipError_t launch_publish(const double *src, int n, double *dst, uint32_t *flag, uint32_t seq,
+                          hipStream_t s) {
+    hipLaunchKernelGGL(k_publish, dim3(1), dim3(64), 0, s, src, n, dst, flag, seq);
+    return hipGetLastError();
+}
+
 // Sum of the kinetic energy field evaluated in FP64 from u, v, w with the
 // formula of core/fortran_kinenergy.f90:145-153 (as the diagnostic kernel):
 // the energy diagnostic of both precision modes.
