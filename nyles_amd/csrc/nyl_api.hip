@@ -585,8 +585,9 @@ int mg_solve_fp64(nyl_ctx *c, nyl_mg_stats *st) {
 }
 
 // Mixed precision: iterative refinement with the stop rule of mgfor's solve.
-// The right-hand side's mean is removed in FP64 (compatibility of the
-// singular Neumann/periodic problem).  An outer step computes r = b - A x in
+// As in mgfor the right-hand side is used as given (the discrete divergence
+// of the singular Neumann/periodic problem sums to zero up to rounding, far
+// below any tolerance).  An outer step computes r = b - A x in
 // FP64 and solves A e = r from e = 0 with FP32 V-cycles under mgfor's loop:
 // before each further V-cycle the squared residual r - A e = b - A(x + e),
 // evaluated in FP32 and summed in FP64, is tested against tol*|b|^2 (fused
@@ -609,20 +610,23 @@ int mg_solve_mixed(nyl_ctx *c, nyl_mg_stats *st) {
     float *e = F<float>(c, f.x), *r = F<float>(c, f.b), *y = F<float>(c, f.y);
     const bool spec = sweep25_fits(f.g) && sweep25_blocks(f.g) <= NYL_RED_BLOCKS;
     int rc;
-    double s2[2], rn, res;
-    HIPCHK(launch_sum_sumsq(f.g, bo, part, res_dev, c->stream));
-    if ((rc = read_results(c, s2, 2))) return rc;
-    const double n = cells(f.g);
-    const double shift = s2[0] / n;
-    double normb = s2[1] - n * shift * shift;
-    if (normb < 0.0) normb = 0.0;
+    double rn, res;
+    // |b|^2 with the first outer residual when its partial sums fit, else
+    // in a pass of its own
+    const bool fused_b = outer_bsq_fits(f.g);
+    const double shift = 0.0;
+    double normb = 0.0;
+    if (!fused_b) {
+        HIPCHK(launch_norm2<double>(f.g, bo, part, res_dev, c->stream));
+        if ((rc = read_result(c, &normb))) return rc;
+    }
     c->mg_shift = shift;
     // FP64 residual of x into r (and the first sweep from e = 0 into y)
     auto outer = [&](double *out) -> int {
         if (normb > 0.0) {
             LAUNCH(NYL_K_RESNORM, (2 * sizeof(double) + 2 * sizeof(float)) * cells(f.g),
                    launch_mg_outer_resid<float>(f.g, xo, bo, shift, r, y, c->d.mg_omega, c->tabs,
-                                                part, res_dev, c->stream));
+                                                part, res_dev, c->stream, false));
             int r2 = read_result(c, &rn);
             if (r2) return r2;
             *out = rn / normb;
@@ -646,7 +650,17 @@ int mg_solve_mixed(nyl_ctx *c, nyl_mg_stats *st) {
         *out = rn / normb;
         return NYL_OK;
     };
-    if ((rc = outer(&res))) return rc;
+    if (fused_b) {
+        LAUNCH(NYL_K_RESNORM, (2 * sizeof(double) + 2 * sizeof(float)) * cells(f.g),
+               launch_mg_outer_resid<float>(f.g, xo, bo, shift, r, y, c->d.mg_omega, c->tabs,
+                                            part, res_dev, c->stream, true));
+        double s2[2];
+        if ((rc = read_results(c, s2, 2))) return rc;
+        normb = s2[1];
+        res = normb > 0.0 ? s2[0] / normb : 0.0;
+    } else if ((rc = outer(&res))) {
+        return rc;
+    }
     int nite = 0;
     while (!(res < c->d.mg_tol) && nite < c->d.mg_maxite) {
         bool pending = true;       // y holds the first sweep from e = 0

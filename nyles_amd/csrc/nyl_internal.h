@@ -106,12 +106,14 @@ hipError_t launch_mg_prolong(const Geo &gf, const Geo &gc, Tf *xf, const Tc *xc,
 
 // mixed precision defect correction: outer FP64 residual of x (b - shift)
 // into the inner right-hand side r and the inner first sweep y, with the sum
-// of r**2 in dev_out[0]; x += e
+// of r**2 in dev_out[0] and, with_bsq (only where outer_bsq_fits), the sum of
+// b**2 in dev_out[1]; x += e
 bool outer_resid_fits(const Geo &g);
+bool outer_bsq_fits(const Geo &g);
 template <typename TI>
 hipError_t launch_mg_outer_resid(const Geo &g, const double *x, const double *b, double shift,
                                  TI *r, TI *y, double omega, const MGTabs &tab, double *partials,
-                                 double *dev_out, hipStream_t s);
+                                 double *dev_out, hipStream_t s, bool with_bsq);
 template <typename TI>
 hipError_t launch_add_correction(const Geo &g, double *x, const TI *e, hipStream_t s);
 // dev_out[0] = sum v, dev_out[1] = sum v**2

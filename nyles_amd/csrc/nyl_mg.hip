@@ -1315,7 +1315,8 @@ __global__ __launch_bounds__(256) void k_mg_outer25(Geo g, const double *__restr
                                                     TI *__restrict__ r_out,
                                                     TI *__restrict__ y_out, double omega,
                                                     MGTabs tab, int kz,
-                                                    double *__restrict__ partials) {
+                                                    double *__restrict__ partials,
+                                                    double *__restrict__ partials_b) {
     constexpr int LX = SW_TX + 2, LY = SW_TY + 2, NRING = 2 * LX + 2 * SW_TY;
     __shared__ double sp[3][LY][LX];
     const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * SW_TX + tx;
@@ -1357,7 +1358,7 @@ __global__ __launch_bounds__(256) void k_mg_outer25(Geo g, const double *__restr
         nxy[r] = (g.px || i > 0) + (g.px || i < g.nx - 1) + (g.py || j > 0) + (g.py || j < g.ny - 1);
     }
     double zm[SW_R], z0[SW_R], zp[SW_R];
-    double acc = 0.0;
+    double acc = 0.0, accb = 0.0;
     load_plane(k0 - 1, 0, zm);
     load_plane(k0, 1, z0);
     for (int k = k0; k < k0 + kz; k++) {
@@ -1373,8 +1374,10 @@ __global__ __launch_bounds__(256) void k_mg_outer25(Geo g, const double *__restr
                            + sp[cur][ly + 1][lx] + zm[r] + zp[r];
             const int64_t c = (int64_t)k * g.sz + (int64_t)j * g.sy + i;
             const int nw = nxy[r] + nz_;
-            const double rr = (b[c] - shift) + tab.diag[nw] * z0[r] - S;
+            const double bc = b[c];
+            const double rr = (bc - shift) + tab.diag[nw] * z0[r] - S;
             acc = acc + rr * rr;
+            accb = accb + bc * bc;
             const TI rf = TI(rr);
             r_out[c] = rf;
             y_out[c] = cff1 * TI(0) + om * (TI(0) - rf) * TI(tab.idiag[nw]);
@@ -1385,6 +1388,10 @@ __global__ __launch_bounds__(256) void k_mg_outer25(Geo g, const double *__restr
     __syncthreads();
     const int nb = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
     block_sum_store(acc, partials + nb, tid);
+    if (partials_b) {
+        __syncthreads();
+        block_sum_store(accb, partials_b + nb, tid);
+    }
 }
 
 // planes per block of the tiled outer residual: as many as divide nz up to 8
@@ -1400,19 +1407,30 @@ bool outer25_fits(const Geo &g) {
     return nb <= NYL_RED_BLOCKS;
 }
 
+bool outer_bsq_fits(const Geo &g) {
+    if (!sweep25_fits(g)) return false;
+    const int64_t nb = (int64_t)(g.nx / SW_TX) * (g.ny / SW_TY) * (g.nz / outer25_kz(g));
+    return 2 * nb <= NYL_RED_BLOCKS;
+}
+
 template <typename TI>
 hipError_t launch_mg_outer_resid(const Geo &g, const double *x, const double *b, double shift,
                                  TI *r, TI *y, double omega, const MGTabs &tab, double *partials,
-                                 double *dev_out, hipStream_t s) {
+                                 double *dev_out, hipStream_t s, bool with_bsq) {
     if (outer25_fits(g)) {
         const int kz = outer25_kz(g);
         const dim3 grd(g.nx / SW_TX, g.ny / SW_TY, g.nz / kz), blk(SW_TX, 4);
-        hipLaunchKernelGGL((k_mg_outer25<TI>), grd, blk, 0, s, g, x, b, shift, r, y, omega, tab,
-                           kz, partials);
         const int nb = grd.x * grd.y * grd.z;
+        if (with_bsq && 2 * nb > NYL_RED_BLOCKS) return hipErrorInvalidValue;
+        double *pb = with_bsq ? partials + nb : nullptr;
+        hipLaunchKernelGGL((k_mg_outer25<TI>), grd, blk, 0, s, g, x, b, shift, r, y, omega, tab,
+                           kz, partials, pb);
         hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(RED_T), 0, s, partials, nb, dev_out);
+        if (with_bsq)
+            hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(RED_T), 0, s, pb, nb, dev_out + 1);
         return hipGetLastError();
     }
+    if (with_bsq) return hipErrorInvalidValue;
     const dim3 grd = outer_grid(g);
     hipLaunchKernelGGL((k_mg_outer_resid<TI>), grd, dim3(MG_BX, MG_BY, 1), 0, s, g, x, b, shift,
                        r, y, omega, tab, partials);
@@ -1592,7 +1610,8 @@ template hipError_t launch_mg_prolong<float, double>(const Geo &, const Geo &, f
                                                      const double *, const MGTabs &, hipStream_t);
 template hipError_t launch_mg_outer_resid<float>(const Geo &, const double *, const double *,
                                                  double, float *, float *, double,
-                                                 const MGTabs &, double *, double *, hipStream_t);
+                                                 const MGTabs &, double *, double *, hipStream_t,
+                                                 bool);
 template hipError_t launch_add_correction<float>(const Geo &, double *, const float *,
                                                  hipStream_t);
 
