@@ -843,9 +843,13 @@ int nyl_ctx_create(const nyl_desc *desc, nyl_ctx **out) {
         if (p3[a] != 0 && p3[a] != 1) return fail(NYL_E_ARG, "periodic flags must be 0 or 1");
     }
     if (!(d.dx > 0 && d.dy > 0 && d.dz > 0)) return fail(NYL_E_ARG, "grid spacings must be > 0");
-    if ((int64_t)d.nx * d.ny * d.nz >= ((int64_t)1 << 31))
-        return fail(NYL_E_SIZE, "grid of %lld cells: at most 2^31 - 1 cells per model",
-                    (long long)d.nx * d.ny * d.nz);
+    // the stage kernel addresses a field with 32-bit byte offsets
+    const int64_t esz = d.precision == NYL_PREC_FP64 ? 8 : 4;
+    if ((int64_t)d.nx * d.ny * d.nz * esz >= ((int64_t)1 << 32))
+        return fail(NYL_E_SIZE, "grid of %lld cells: a field must stay below 4 GiB "
+                    "(%lld cells in FP64, %lld in mixed precision)",
+                    (long long)d.nx * d.ny * d.nz, ((long long)1 << 29) - 1,
+                    ((long long)1 << 30) - 1);
     if (d.mg_maxite < 0) return fail(NYL_E_ARG, "mg_maxite must be >= 0");
     nyl_ctx *c = new nyl_ctx();
     memset(c, 0, sizeof(*c));

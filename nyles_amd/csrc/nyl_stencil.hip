@@ -103,12 +103,26 @@ __device__ __forceinline__ T tracer_axis(T d, const T *__restrict__ q_arr,
     return (d + fl) - fr;
 }
 
-// Periodic axes: the element offsets of the cells -3..+3 away along the
-// axis (wrapped), computed once per cell; every stencil load of that axis is
-// then q[c + off[..]].  The loads and the arithmetic are those of the
-// general path (face_flux's interior stencil), so the results are the same.
+// Stage-kernel loads and stores at a 32-bit byte offset from the field's
+// base pointer (the base is uniform, so the load takes it from scalar
+// registers and the offset as one 32-bit register: no 64-bit address
+// arithmetic per load).  A field is below 4 GiB (checked at context
+// creation); offsets wrap modulo 2^32 to the right address.
+template <typename T>
+__device__ __forceinline__ T ldb(const T *__restrict__ p, uint32_t boff) {
+    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(p) + boff);
+}
+template <typename T>
+__device__ __forceinline__ void stb(T *__restrict__ p, uint32_t boff, T v) {
+    *reinterpret_cast<T *>(reinterpret_cast<char *>(p) + boff) = v;
+}
+
+// Periodic axes: the byte offsets of the cells -3..+3 away along the axis
+// (wrapped), computed once per cell; every stencil load of that axis is then
+// at cb + off[..].  The loads and the arithmetic are those of the general
+// path (face_flux's interior stencil), so the results are the same.
 struct PerLine {
-    int m3, m2, m1, z, p1, p2, p3;   // offsets of the cells -3..+3 away
+    int m3, m2, m1, z, p1, p2, p3;   // byte offsets of the cells -3..+3 away
 };
 
 __device__ __forceinline__ PerLine per_line(int m, int n, int stride) {
@@ -141,38 +155,37 @@ __device__ __forceinline__ int lo(const PerLine &L, int d) {
 // and r + 1, upwinded by the sign of u; the value at relative position p is
 // q[c + lo(L, p + qs)] (qs = -1: the vortex force's q(k) = vort(k-1)).
 template <typename T>
-__device__ __forceinline__ T per_face(T u, const T *__restrict__ q, int c, const PerLine &L,
+__device__ __forceinline__ T per_face(T u, const T *__restrict__ q, uint32_t cb, const PerLine &L,
                                       int r, int qs) {
-    const int om2 = lo(L, r - 2 + qs), om1 = lo(L, r - 1 + qs), o0 = lo(L, r + qs);
-    const int op1 = lo(L, r + 1 + qs), op2 = lo(L, r + 2 + qs), op3 = lo(L, r + 3 + qs);
-    T a0, a1, a2, a3, a4;
-    if (u > T(0)) {
-        a0 = q[c + om2]; a1 = q[c + om1]; a2 = q[c + o0]; a3 = q[c + op1]; a4 = q[c + op2];
-    } else {
-        a0 = q[c + op3]; a1 = q[c + op2]; a2 = q[c + op1]; a3 = q[c + o0]; a4 = q[c + om1];
-    }
-    return u * weno5<T>(a0, a1, a2, a3, a4);
+    const uint32_t om2 = cb + lo(L, r - 2 + qs), om1 = cb + lo(L, r - 1 + qs);
+    const uint32_t o0 = cb + lo(L, r + qs), op1 = cb + lo(L, r + 1 + qs);
+    const uint32_t op2 = cb + lo(L, r + 2 + qs), op3 = cb + lo(L, r + 3 + qs);
+    // upwind stencil: offsets selected, then one set of five loads
+    const bool pos = u > T(0);
+    const uint32_t s0 = pos ? om2 : op3, s1 = pos ? om1 : op2, s2 = pos ? o0 : op1;
+    const uint32_t s3 = pos ? op1 : o0, s4 = pos ? op2 : om1;
+    return u * weno5<T>(ldb(q, s0), ldb(q, s1), ldb(q, s2), ldb(q, s3), ldb(q, s4));
 }
 
-// tracer_axis on a periodic axis
+// tracer_axis on a periodic axis (cb: the cell's byte offset)
 template <typename T>
 __device__ __forceinline__ T tracer_axis_per(T d, const T *__restrict__ q, const T *__restrict__ ud,
-                                             T cu, int c, const PerLine &L) {
-    const T fr = per_face<T>(ud[c] * cu, q, c, L, 0, 0);
-    STAGE_SEQ(c, fr);
-    const T fl = per_face<T>(ud[c + L.m1] * cu, q, c, L, -1, 0);
-    STAGE_SEQ(c, fl);
+                                             T cu, uint32_t cb, const PerLine &L) {
+    const T fr = per_face<T>(ldb(ud, cb) * cu, q, cb, L, 0, 0);
+    STAGE_SEQ(cb, fr);
+    const T fl = per_face<T>(ldb(ud, cb + L.m1) * cu, q, cb, L, -1, 0);
+    STAGE_SEQ(cb, fl);
     return (d + fl) - fr;
 }
 
-// vf_flux on a periodic axis
+// vf_flux on a periodic axis (cb, oe: byte offsets)
 template <typename T>
 __device__ __forceinline__ T vf_flux_per(const T *__restrict__ ud, T cu, const T *__restrict__ om,
-                                         int c, const PerLine &L, int oe) {
-    auto UU = [&](int p) -> T { return T(0.5) * (ud[p] * cu + ud[p + oe] * cu); };
-    const T uu_prev = UU(c + L.m1);
-    const T u1d = T(0.5) * (uu_prev + UU(c));
-    return per_face<T>(u1d, om, c, L, 0, -1);
+                                         uint32_t cb, const PerLine &L, uint32_t oe) {
+    auto UU = [&](uint32_t p) -> T { return T(0.5) * (ldb(ud, p) * cu + ldb(ud, p + oe) * cu); };
+    const T uu_prev = UU(cb + L.m1);
+    const T u1d = T(0.5) * (uu_prev + UU(cb));
+    return per_face<T>(u1d, om, cb, L, 0, -1);
 }
 
 // Fused LES.rhs (core/model_les.py:129-144) + RK3_SSP stage update
@@ -181,17 +194,18 @@ template <typename T, int MODE, int PX, int PY, int PZ>
 __device__ __forceinline__ void stage_cell(const Geo &g, const StageArgs<T> &a, int i, int j,
                                            int k) {
     const int64_t c = (int64_t)k * g.sz + (int64_t)j * g.sy + i;
-    const int ci = (int)c;
+    constexpr int ES = (int)sizeof(T);
+    const uint32_t cb = (uint32_t)c * (uint32_t)ES;   // the cell's byte offset
     PerLine Lx, Ly, Lz;
-    if constexpr (PX != 0) Lx = per_line(i, g.nx, 1);
-    if constexpr (PY != 0) Ly = per_line(j, g.ny, (int)g.sy);
-    if constexpr (PZ != 0) Lz = per_line(k, g.nz, (int)g.sz);
+    if constexpr (PX != 0) Lx = per_line(i, g.nx, ES);
+    if constexpr (PY != 0) Ly = per_line(j, g.ny, (int)g.sy * ES);
+    if constexpr (PZ != 0) Lz = per_line(k, g.nz, (int)g.sz * ES);
 
     // tracer b: rhstrac, directions i, j, k (core/tracer.py:63-72)
     const T cx = T(a.idx2), cy = T(a.idy2), cz = T(a.idz2);
     T db = T(0);
     int64_t bx = c - i, by = c - (int64_t)j * g.sy, bz = c - (int64_t)k * g.sz;
-    int cs = ci;
+    uint32_t cs = cb;
     if constexpr (PX != 0) db = tracer_axis_per<T>(db, a.b, a.u, cx, cs, Lx);
     else db = tracer_axis<T, 0>(db, a.b, a.u, cx, bx, 1, i, g.nx);
     STAGE_SEQ(by, db);
@@ -209,98 +223,100 @@ __device__ __forceinline__ void stage_cell(const Geo &g, const StageArgs<T> &a, 
     const int64_t oex = PX ? (int64_t)(wrapi(i + 1, g.nx) - i) : 1;
     const int64_t oey = PY ? (int64_t)(wrapi(j + 1, g.ny) - j) * g.sy : g.sy;
     const int64_t oez = PZ ? (int64_t)(wrapi(k + 1, g.nz) - k) * g.sz : g.sz;
+    const uint32_t bex = (uint32_t)oex * (uint32_t)ES, bey = (uint32_t)oey * (uint32_t)ES;
+    const uint32_t bez = (uint32_t)oez * (uint32_t)ES;
     const bool ex = PX || i < g.nx - 1;   // loops stop at n-1 on a wall axis
     const bool ey = PY || j < g.ny - 1;
     const bool ez = PZ || k < g.nz - 1;
     T du = T(0), dv = T(0), dw = T(0);
     // direc(U_i, vor_k, du_j)
     if (ey) {
-        if constexpr (PX != 0) dv = dv - vf_flux_per<T>(a.u, cx, a.oz, cs, Lx, (int)oey);
+        if constexpr (PX != 0) dv = dv - vf_flux_per<T>(a.u, cx, a.oz, cs, Lx, bey);
         else dv = dv - vf_flux<T, 0>(a.u, cx, a.oz, bx, 1, i, g.nx, oey);
     }
     STAGE_SEQ(by, dv);
     STAGE_SEQ(cs, dv);
     // flip (U_j, vor_k, du_i)
     if (ex) {
-        if constexpr (PY != 0) du = du + vf_flux_per<T>(a.v, cy, a.oz, cs, Ly, (int)oex);
+        if constexpr (PY != 0) du = du + vf_flux_per<T>(a.v, cy, a.oz, cs, Ly, bex);
         else du = du + vf_flux<T, 0>(a.v, cy, a.oz, by, g.sy, j, g.ny, oex);
     }
     STAGE_SEQ(by, du);
     STAGE_SEQ(cs, du);
     // direc(U_j, vor_i, du_k)
     if (ez) {
-        if constexpr (PY != 0) dw = dw - vf_flux_per<T>(a.v, cy, a.ox, cs, Ly, (int)oez);
+        if constexpr (PY != 0) dw = dw - vf_flux_per<T>(a.v, cy, a.ox, cs, Ly, bez);
         else dw = dw - vf_flux<T, 0>(a.v, cy, a.ox, by, g.sy, j, g.ny, oez);
     }
     STAGE_SEQ(bz, dw);
     STAGE_SEQ(cs, dw);
     // flip (U_k, vor_i, du_j)
     if (ey) {
-        if constexpr (PZ != 0) dv = dv + vf_flux_per<T>(a.w, cz, a.ox, cs, Lz, (int)oey);
+        if constexpr (PZ != 0) dv = dv + vf_flux_per<T>(a.w, cz, a.ox, cs, Lz, bey);
         else dv = dv + vf_flux<T, 0>(a.w, cz, a.ox, bz, g.sz, k, g.nz, oey);
     }
     STAGE_SEQ(bz, dv);
     STAGE_SEQ(cs, dv);
     // direc(U_k, vor_j, du_i)
     if (ex) {
-        if constexpr (PZ != 0) du = du - vf_flux_per<T>(a.w, cz, a.oy, cs, Lz, (int)oex);
+        if constexpr (PZ != 0) du = du - vf_flux_per<T>(a.w, cz, a.oy, cs, Lz, bex);
         else du = du - vf_flux<T, 0>(a.w, cz, a.oy, bz, g.sz, k, g.nz, oex);
     }
     STAGE_SEQ(bx, du);
     STAGE_SEQ(cs, du);
     // flip (U_i, vor_j, du_k)
     if (ez) {
-        if constexpr (PX != 0) dw = dw + vf_flux_per<T>(a.u, cx, a.oy, cs, Lx, (int)oez);
+        if constexpr (PX != 0) dw = dw + vf_flux_per<T>(a.u, cx, a.oy, cs, Lx, bez);
         else dw = dw + vf_flux<T, 0>(a.u, cx, a.oy, bx, 1, i, g.nx, oez);
     }
 
     // bernoulli (core/bernoulli.py:14-23, core/fortran_bernoulli.f90:2-58)
-    const T kec = a.ke[c];
-    if (ex) du = du - (a.ke[c + oex] - kec);
-    if (ey) dv = dv - (a.ke[c + oey] - kec);
+    const T kec = ldb(a.ke, cb);
+    if (ex) du = du - (ldb(a.ke, cb + bex) - kec);
+    if (ey) dv = dv - (ldb(a.ke, cb + bey) - kec);
     if (ez) {
         const T cff = T(0.5 * a.dz);
-        dw = dw - (a.ke[c + oez] - kec) + cff * (a.b[c + oez] + a.b[c]);
+        dw = dw - (ldb(a.ke, cb + bez) - kec) + cff * (ldb(a.b, cb + bez) + ldb(a.b, cb));
     }
 
     if (MODE == 0 || MODE == 1 || MODE == 2) {
-        a.tb[c] = db; a.tu[c] = du; a.tv[c] = dv; a.tw[c] = dw;
+        stb(a.tb, cb, db); stb(a.tu, cb, du); stb(a.tv, cb, dv); stb(a.tw, cb, dw);
     }
     const T cc = T(a.c);
     if (MODE == 1) {            // s += dt*ds0
-        a.bn[c] = a.b[c] + cc * db;
-        a.un[c] = a.u[c] + cc * du;
-        a.vn[c] = a.v[c] + cc * dv;
-        a.wn[c] = a.w[c] + cc * dw;
+        stb(a.bn, cb, ldb(a.b, cb) + cc * db);
+        stb(a.un, cb, ldb(a.u, cb) + cc * du);
+        stb(a.vn, cb, ldb(a.v, cb) + cc * dv);
+        stb(a.wn, cb, ldb(a.w, cb) + cc * dw);
     } else if (MODE == 2) {     // s += (dt/4.)*(ds1-3*ds0)
-        a.bn[c] = a.b[c] + cc * (db - T(3) * a.d0b[c]);
-        a.un[c] = a.u[c] + cc * (du - T(3) * a.d0u[c]);
-        a.vn[c] = a.v[c] + cc * (dv - T(3) * a.d0v[c]);
-        a.wn[c] = a.w[c] + cc * (dw - T(3) * a.d0w[c]);
+        stb(a.bn, cb, ldb(a.b, cb) + cc * (db - T(3) * ldb(a.d0b, cb)));
+        stb(a.un, cb, ldb(a.u, cb) + cc * (du - T(3) * ldb(a.d0u, cb)));
+        stb(a.vn, cb, ldb(a.v, cb) + cc * (dv - T(3) * ldb(a.d0v, cb)));
+        stb(a.wn, cb, ldb(a.w, cb) + cc * (dw - T(3) * ldb(a.d0w, cb)));
     } else if (MODE == 3) {     // s += (dt/12.)*(8*ds2-ds0-ds1)
-        a.bn[c] = a.b[c] + cc * ((T(8) * db - a.d0b[c]) - a.d1b[c]);
-        a.un[c] = a.u[c] + cc * ((T(8) * du - a.d0u[c]) - a.d1u[c]);
-        a.vn[c] = a.v[c] + cc * ((T(8) * dv - a.d0v[c]) - a.d1v[c]);
-        a.wn[c] = a.w[c] + cc * ((T(8) * dw - a.d0w[c]) - a.d1w[c]);
+        stb(a.bn, cb, ldb(a.b, cb) + cc * ((T(8) * db - ldb(a.d0b, cb)) - ldb(a.d1b, cb)));
+        stb(a.un, cb, ldb(a.u, cb) + cc * ((T(8) * du - ldb(a.d0u, cb)) - ldb(a.d1u, cb)));
+        stb(a.vn, cb, ldb(a.v, cb) + cc * ((T(8) * dv - ldb(a.d0v, cb)) - ldb(a.d1v, cb)));
+        stb(a.wn, cb, ldb(a.w, cb) + cc * ((T(8) * dw - ldb(a.d0w, cb)) - ldb(a.d1w, cb)));
     } else if (MODE == 4) {     // LFAM3 predictor (core/timescheme.py:367-385)
         // s = sb + (2.*dt)*ds; s = (1./12.)*(5.*s+8.*sn-sb); sb = sn
         const T c2 = T(a.c2), c12 = T(1. / 12.);
         T sb, t1;
-        sb = a.sbb[c]; t1 = sb + c2 * db; a.bn[c] = c12 * (T(5) * t1 + T(8) * a.b[c] - sb); a.sbb[c] = a.b[c];
-        sb = a.sbu[c]; t1 = sb + c2 * du; a.un[c] = c12 * (T(5) * t1 + T(8) * a.u[c] - sb); a.sbu[c] = a.u[c];
-        sb = a.sbv[c]; t1 = sb + c2 * dv; a.vn[c] = c12 * (T(5) * t1 + T(8) * a.v[c] - sb); a.sbv[c] = a.v[c];
-        sb = a.sbw[c]; t1 = sb + c2 * dw; a.wn[c] = c12 * (T(5) * t1 + T(8) * a.w[c] - sb); a.sbw[c] = a.w[c];
+        sb = ldb(a.sbb, cb); t1 = sb + c2 * db; stb(a.bn, cb, c12 * (T(5) * t1 + T(8) * ldb(a.b, cb) - sb)); stb(a.sbb, cb, ldb(a.b, cb));
+        sb = ldb(a.sbu, cb); t1 = sb + c2 * du; stb(a.un, cb, c12 * (T(5) * t1 + T(8) * ldb(a.u, cb) - sb)); stb(a.sbu, cb, ldb(a.u, cb));
+        sb = ldb(a.sbv, cb); t1 = sb + c2 * dv; stb(a.vn, cb, c12 * (T(5) * t1 + T(8) * ldb(a.v, cb) - sb)); stb(a.sbv, cb, ldb(a.v, cb));
+        sb = ldb(a.sbw, cb); t1 = sb + c2 * dw; stb(a.wn, cb, c12 * (T(5) * t1 + T(8) * ldb(a.w, cb) - sb)); stb(a.sbw, cb, ldb(a.w, cb));
     } else if (MODE == 5) {     // LFAM3 corrector: s = sn + dt*ds (core/timescheme.py:393-398)
-        a.bn[c] = a.bn[c] + cc * db;
-        a.un[c] = a.un[c] + cc * du;
-        a.vn[c] = a.vn[c] + cc * dv;
-        a.wn[c] = a.wn[c] + cc * dw;
+        stb(a.bn, cb, ldb(a.bn, cb) + cc * db);
+        stb(a.un, cb, ldb(a.un, cb) + cc * du);
+        stb(a.vn, cb, ldb(a.vn, cb) + cc * dv);
+        stb(a.wn, cb, ldb(a.wn, cb) + cc * dw);
     } else if (MODE == 6) {     // first LFAM3 step: sn = s; sb = s; s += dt*ds (:352-364)
-        a.sbb[c] = a.b[c]; a.sbu[c] = a.u[c]; a.sbv[c] = a.v[c]; a.sbw[c] = a.w[c];
-        a.bn[c] = a.b[c] + cc * db;
-        a.un[c] = a.u[c] + cc * du;
-        a.vn[c] = a.v[c] + cc * dv;
-        a.wn[c] = a.w[c] + cc * dw;
+        stb(a.sbb, cb, ldb(a.b, cb)); stb(a.sbu, cb, ldb(a.u, cb)); stb(a.sbv, cb, ldb(a.v, cb)); stb(a.sbw, cb, ldb(a.w, cb));
+        stb(a.bn, cb, ldb(a.b, cb) + cc * db);
+        stb(a.un, cb, ldb(a.u, cb) + cc * du);
+        stb(a.vn, cb, ldb(a.v, cb) + cc * dv);
+        stb(a.wn, cb, ldb(a.w, cb) + cc * dw);
     }
 }
 
