@@ -63,3 +63,29 @@ def test_context_rejects_bad_sizes_without_gpu():
     # no workspace bound: compute entry points refuse
     assert L.nyl_rhs(ctx, 0, 0) == -2
     assert L.nyl_ctx_destroy(ctx) == 0
+
+
+def test_context_rejects_fields_of_4gib_without_gpu():
+    """The stage kernel addresses a field with 32-bit byte offsets: a grid
+    whose field reaches 4 GiB is refused (2^29 cells in FP64, 2^30 in mixed
+    precision), smaller grids are accepted."""
+    from nyles_amd import _lib
+    L = _lib.load()
+    d = _lib.NylDesc()
+    d.dx = d.dy = d.dz = 1.0
+    d.mg_maxite, d.mg_tol, d.mg_omega = 20, 1e-6, 0.9
+    ctx = ctypes.c_void_p()
+    d.nx, d.ny, d.nz = 1024, 1024, 512        # 2^29 cells
+    d.precision = _lib.NYL_PREC_FP64
+    assert L.nyl_ctx_create(ctypes.byref(d), ctypes.byref(ctx)) == -4
+    assert b"4 GiB" in L.nyl_last_error()
+    d.precision = _lib.NYL_PREC_MIXED          # 2 GiB per FP32 field
+    assert L.nyl_ctx_create(ctypes.byref(d), ctypes.byref(ctx)) == 0
+    assert L.nyl_ctx_destroy(ctx) == 0
+    d.nx, d.ny, d.nz = 1024, 1024, 1024        # 2^30 cells: 4 GiB in FP32
+    assert L.nyl_ctx_create(ctypes.byref(d), ctypes.byref(ctx)) == -4
+    d.nx, d.ny, d.nz = 1024, 1024, 256         # 2^28 cells: 2 GiB in FP64
+    d.precision = _lib.NYL_PREC_FP64
+    rc = L.nyl_ctx_create(ctypes.byref(d), ctypes.byref(ctx))
+    assert rc == 0, L.nyl_last_error()
+    assert L.nyl_ctx_destroy(ctx) == 0
