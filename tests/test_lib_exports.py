@@ -84,7 +84,7 @@ def test_context_rejects_fields_of_4gib_without_gpu():
     assert L.nyl_ctx_destroy(ctx) == 0
     d.nx, d.ny, d.nz = 1024, 1024, 1024        # 2^30 cells: 4 GiB in FP32
     assert L.nyl_ctx_create(ctypes.byref(d), ctypes.byref(ctx)) == -4
-    d.nx, d.ny, d.nz = 1024, 1024, 256         # 2^28 cells: 2 GiB in FP64
+    d.nx, d.ny, d.nz = 512, 512, 1024          # 2^28 cells: 2 GiB in FP64
     d.precision = _lib.NYL_PREC_FP64
     rc = L.nyl_ctx_create(ctypes.byref(d), ctypes.byref(ctx))
     assert rc == 0, L.nyl_last_error()
