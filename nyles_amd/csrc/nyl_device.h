@@ -178,12 +178,15 @@ __device__ __forceinline__ const I &find_item(const I *items, int n, int64_t blk
 }
 
 // cell (i, j, k) of flattened index q of a level, false past its last cell
+// (a level has fewer than 2^31 cells: the divisions run in 32 bits, far
+// cheaper than 64-bit ones on the GPU)
 __device__ __forceinline__ bool cell_of(const Geo &g, int64_t q, int &i, int &j, int &k) {
     if (q >= (int64_t)g.nx * g.ny * g.nz) return false;
-    i = (int)(q % g.nx);
-    const int64_t t = q / g.nx;
-    j = (int)(t % g.ny);
-    k = (int)(t / g.ny);
+    const uint32_t q32 = (uint32_t)q, nx = (uint32_t)g.nx, ny = (uint32_t)g.ny;
+    const uint32_t t = q32 / nx;
+    i = (int)(q32 - t * nx);
+    j = (int)(t % ny);
+    k = (int)(t / ny);
     return true;
 }
 
