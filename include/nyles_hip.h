@@ -211,6 +211,7 @@ int nyl_reduce(nyl_ctx *ctx, int what, double *out);
 #define NYL_K_RESNORM 8      /* residual norm (two launches); mixed mode: outer FP64 residual */
 #define NYL_K_MG_UPDATE 9    /* mixed mode: x += e of the defect correction */
 #define NYL_K_MG_SWEEP_FINE 10  /* the NYL_K_MG_SWEEP launches on the finest level only */
+#define NYL_K_MG_SWEEP2 11   /* two post-smoothing sweeps fused in one pass */
 int nyl_prof_start(nyl_ctx *ctx, int kernel_class);
 int nyl_prof_stop(nyl_ctx *ctx, double *total_ms, int64_t *launches, double *alg_bytes);
 

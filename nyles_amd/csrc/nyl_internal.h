@@ -143,6 +143,10 @@ hipError_t launch_mg_sweep25(const Geo &g, const T *src, T *dst, const T *b, dou
 bool restrict25_fits(const Geo &g);
 template <typename Tf, typename Tc>
 hipError_t launch_mg_prolong25(const Geo &gf, const Geo &gc, Tf *xf, const Tc *xc,
+                               const MGTabs &tab, hipStream_t s, Tf *yf = nullptr);
+// two consecutive tiled sweeps src -> (tmp) -> dst in one pass (src != dst)
+template <typename T>
+hipError_t launch_mg_sweep2x25(const Geo &g, const T *src, T *dst, const T *b, double omega,
                                const MGTabs &tab, hipStream_t s);
 template <typename Tf, typename Tc>
 hipError_t launch_mg_restrict25(const Geo &gf, const Geo &gc, const Tf *x, const Tf *b, Tc *bc,

@@ -1451,7 +1451,7 @@ hipError_t launch_mg_outer_resid(const Geo &g, const double *x, const double *b,
 template <typename Tf, typename Tc>
 __global__ __launch_bounds__(256) void k_mg_prolong25(Geo gf, Geo gc, Tf *__restrict__ xf,
                                                       const Tc *__restrict__ xc, MGTabs tab,
-                                                      int kz) {
+                                                      int kz, Tf *__restrict__ yf) {
     __shared__ Tf cp[3][PR_CY][PR_CX];
     const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * SW_TX + tx;
     const Blk3 bk = xcd_block();
@@ -1522,7 +1522,9 @@ __global__ __launch_bounds__(256) void k_mg_prolong25(Geo gf, Geo gc, Tf *__rest
                 const Tf term = Tf(tab.pcoef[mx + 2 * my[r] + 4 * mz]) * (Tf(3) * bmid + aside);
                 const int j = j0 + ty + 4 * r;
                 const int64_t c = (int64_t)k * gf.sz + (int64_t)j * gf.sy + i;
-                xf[c] = xf[c] + term;
+                const Tf v = xf[c] + term;
+                if (yf) yf[c] = v;
+                else xf[c] = v;
             }
         }
     }
@@ -1530,12 +1532,135 @@ __global__ __launch_bounds__(256) void k_mg_prolong25(Geo gf, Geo gc, Tf *__rest
 
 template <typename Tf, typename Tc>
 hipError_t launch_mg_prolong25(const Geo &gf, const Geo &gc, Tf *xf, const Tc *xc,
-                               const MGTabs &tab, hipStream_t s) {
+                               const MGTabs &tab, hipStream_t s, Tf *yf) {
     const int kz = sweep25_kz(gf, 2);
     dim3 grd(gf.nx / SW_TX, gf.ny / SW_TY, gf.nz / kz), blk(SW_TX, 4);
-    hipLaunchKernelGGL((k_mg_prolong25<Tf, Tc>), grd, blk, 0, s, gf, gc, xf, xc, tab, kz);
+    hipLaunchKernelGGL((k_mg_prolong25<Tf, Tc>), grd, blk, 0, s, gf, gc, xf, xc, tab, kz, yf);
     return hipGetLastError();
 }
+
+// Two consecutive smoothing sweeps of k_mg_sweep25 (src -> tmp -> dst) in
+// one 2.5-D march.  The first sweep is evaluated on the 64 x 8 tile and a
+// one-cell ring (its input held with a two-cell ring in LDS) and kept for
+// three planes; the second sweep is evaluated on the tile.  Each value has
+// the per-cell sweep's expression (zero for an absent neighbour), so dst is
+// bitwise that of the two separate sweeps.  src and dst must differ.
+#define S2_LX (SW_TX + 4)
+#define S2_LY (SW_TY + 4)
+#define S2_MX (SW_TX + 2)
+#define S2_MY (SW_TY + 2)
+template <typename T>
+__global__ __launch_bounds__(256) void k_mg_sweep2x25(Geo g, const T *__restrict__ src,
+                                                      T *__restrict__ dst,
+                                                      const T *__restrict__ b, double omega,
+                                                      MGTabs tab, int kz) {
+    __shared__ T pin[3][S2_LY * S2_LX];
+    __shared__ T mid[3][S2_MY * S2_MX];
+    const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * SW_TX + tx;
+    const Blk3 bk = xcd_block();
+    const int i0 = bk.x * SW_TX, j0 = bk.y * SW_TY, k0 = bk.z * kz;
+    const T om = T(omega), cff1 = T(1.0 - omega);
+    auto zin = [&](int m) { return g.pz || (m >= 0 && m < g.nz); };
+    auto zw = [&](int m) { return g.pz ? wrapg(m, g.nz) : m; };
+    auto slot = [&](int m) { return (m - k0 + 3) % 3; };   // m >= k0 - 2
+    // input plane m with a two-cell ring, zero outside a wall axis
+    auto load_in = [&](int m) {
+        T *P = pin[slot(m)];
+        const bool zi = zin(m);
+        const int64_t base = (int64_t)zw(m) * g.sz;
+        for (int q = tid; q < S2_LY * S2_LX; q += 256) {
+            const int lx = q % S2_LX, ly = q / S2_LX;
+            int gi = i0 - 2 + lx, gj = j0 - 2 + ly;
+            bool in = zi;
+            if (g.px) gi = wrapg(gi, g.nx); else in = in && gi >= 0 && gi < g.nx;
+            if (g.py) gj = wrapg(gj, g.ny); else in = in && gj >= 0 && gj < g.ny;
+            P[q] = in ? src[base + (int64_t)gj * g.sy + gi] : T(0);
+        }
+    };
+    // first sweep at plane m on the tile and its one-cell ring
+    auto comp_mid = [&](int m) {
+        T *M = mid[slot(m)];
+        const T *P0 = pin[slot(m)], *Pm = pin[slot(m - 1)], *Pp = pin[slot(m + 1)];
+        const bool zi = zin(m);
+        const int mw = zw(m);
+        const int nzw = (g.pz || mw > 0) + (g.pz || mw < g.nz - 1);
+        for (int q = tid; q < S2_MY * S2_MX; q += 256) {
+            const int lx = q % S2_MX, ly = q / S2_MX;
+            int gi = i0 - 1 + lx, gj = j0 - 1 + ly;
+            bool in = zi;
+            if (g.px) gi = wrapg(gi, g.nx); else in = in && gi >= 0 && gi < g.nx;
+            if (g.py) gj = wrapg(gj, g.ny); else in = in && gj >= 0 && gj < g.ny;
+            T v = T(0);
+            if (in) {
+                const int p = (ly + 1) * S2_LX + (lx + 1);
+                const T S = P0[p - 1] + P0[p + 1] + P0[p - S2_LX] + P0[p + S2_LX] + Pm[p] + Pp[p];
+                const int nw = (g.px || gi > 0) + (g.px || gi < g.nx - 1) + (g.py || gj > 0) +
+                               (g.py || gj < g.ny - 1) + nzw;
+                const T bv = b[(int64_t)mw * g.sz + (int64_t)gj * g.sy + gi];
+                v = cff1 * P0[p] + om * (S - bv) * T(tab.idiag[nw]);
+            }
+            M[q] = v;
+        }
+    };
+    // second sweep at plane k on the tile
+    auto comp_out = [&](int k) {
+        const T *M0 = mid[slot(k)], *Mm = mid[slot(k - 1)], *Mp = mid[slot(k + 1)];
+        const int nzw = (g.pz || k > 0) + (g.pz || k < g.nz - 1);
+        const int i = i0 + tx;
+#pragma unroll
+        for (int r = 0; r < SW_R; r++) {
+            const int j = j0 + ty + 4 * r;
+            const int p = (ty + 4 * r + 1) * S2_MX + (tx + 1);
+            const T S = M0[p - 1] + M0[p + 1] + M0[p - S2_MX] + M0[p + S2_MX] + Mm[p] + Mp[p];
+            const int nw = (g.px || i > 0) + (g.px || i < g.nx - 1) + (g.py || j > 0) +
+                           (g.py || j < g.ny - 1) + nzw;
+            const int64_t c = (int64_t)k * g.sz + (int64_t)j * g.sy + i;
+            dst[c] = cff1 * M0[p] + om * (S - b[c]) * T(tab.idiag[nw]);
+        }
+    };
+    load_in(k0 - 2);
+    load_in(k0 - 1);
+    load_in(k0);
+    __syncthreads();
+    comp_mid(k0 - 1);
+    __syncthreads();
+    load_in(k0 + 1);
+    __syncthreads();
+    comp_mid(k0);
+    for (int k = k0; k < k0 + kz; k++) {
+        __syncthreads();          // readers of the buffers reused below are done
+        load_in(k + 2);           // replaces plane k - 1 of the input
+        __syncthreads();
+        comp_mid(k + 1);          // replaces plane k - 2 of the first sweep
+        __syncthreads();
+        comp_out(k);
+    }
+}
+
+// planes per block of the two-sweep kernel: up to 16 dividing nz, keeping
+// at least 1024 blocks
+static int sweep2x_kz(const Geo &g) {
+    const int64_t cols = (int64_t)(g.nx / SW_TX) * (g.ny / SW_TY);
+    int kz = 16;
+    while (kz > 1 && (g.nz % kz != 0 || cols * (g.nz / kz) < 1024)) kz >>= 1;
+    return kz;
+}
+
+template <typename T>
+hipError_t launch_mg_sweep2x25(const Geo &g, const T *src, T *dst, const T *b, double omega,
+                               const MGTabs &tab, hipStream_t s) {
+    if (src == dst) return hipErrorInvalidValue;
+    const int kz = sweep2x_kz(g);
+    dim3 grd(g.nx / SW_TX, g.ny / SW_TY, g.nz / kz), blk(SW_TX, 4);
+    hipLaunchKernelGGL((k_mg_sweep2x25<T>), grd, blk, 0, s, g, src, dst, b, omega, tab, kz);
+    return hipGetLastError();
+}
+template hipError_t launch_mg_sweep2x25<double>(const Geo &, const double *, double *,
+                                                const double *, double, const MGTabs &,
+                                                hipStream_t);
+template hipError_t launch_mg_sweep2x25<float>(const Geo &, const float *, float *,
+                                               const float *, double, const MGTabs &,
+                                               hipStream_t);
 
 bool restrict25_fits(const Geo &g) {
     return sweep25_fits(g) && g.nz % 2 == 0;
@@ -1556,7 +1681,7 @@ hipError_t launch_mg_restrict25(const Geo &gf, const Geo &gc, const Tf *x, const
                                              const MGTabs &, double *, double *, hipStream_t,   \
                                              const Geo *, const T *, double *);                 \
     template hipError_t launch_mg_prolong25<T, T>(const Geo &, const Geo &, T *, const T *,     \
-                                                  const MGTabs &, hipStream_t);                 \
+                                                  const MGTabs &, hipStream_t, T *);            \
     template hipError_t launch_mg_restrict25<T, T>(const Geo &, const Geo &, const T *,         \
                                                    const T *, T *, T *, const MGTabs &,         \
                                                    hipStream_t, T *, double);                   \
