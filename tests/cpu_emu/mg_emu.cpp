@@ -1,0 +1,149 @@
+// Multigrid kernels: the tiled and fused forms against the per-cell ones,
+// bit for bit, on small grids of every periodicity.
+#include "../../nyles_amd/csrc/nyl_mg.hip"
+#include "emu.h"
+using namespace nyl;
+
+// tiled sweep, restriction (with the coarse first sweep), prolongation (in
+// place and out of place) and two-sweep pass against the per-cell kernels
+template <typename T>
+int check_tiled(int nx, int ny, int nz, int px, int py, int pz) {
+    Geo gf = mk(nx, ny, nz, px, py, pz), gc = mk(nx / 2, ny / 2, nz / 2, px, py, pz);
+    MGTabs tab = mktab();
+    const size_t nf = (size_t)nx * ny * nz, nc = nf / 8;
+    std::mt19937_64 rng(nx + 3 * ny + 7 * nz + px + 2 * py + 4 * pz);
+    auto x = rnd<T>(nf, rng), b = rnd<T>(nf, rng), xc = rnd<T>(nc, rng);
+    const double om = 0.9;
+    int bad = 0;
+    // sweep
+    std::vector<T> y1(nf), y2(nf);
+    run_blocks(mg_grid(gf), dim3(MG_BX, MG_BY, 1), [&]() {
+        k_mg_sweep<T>(gf, x.data(), y1.data(), b.data(), om, tab); }, false);
+    const int kzs = sweep25_kz(gf, 1);
+    const dim3 gs(nx / SW_TX, ny / SW_TY, nz / kzs);
+    run_blocks(gs, dim3(SW_TX, 4, 1), [&]() {
+        k_mg_sweep25<T, false, false, false>(gf, x.data(), y2.data(), b.data(), om, tab, kzs,
+                                             nullptr, gf, nullptr, nullptr); }, true);
+    bad += ndiff(y1, y2);
+    // restriction with the coarse level's first sweep
+    std::vector<T> bc1(nc), bc2(nc), yc1(nc), yc2(nc), xd(nc);
+    run_blocks(mg_grid(gc), dim3(MG_BX, MG_BY, 1), [&]() {
+        k_mg_resid_restrict<T, T>(gf, gc, x.data(), b.data(), bc1.data(), xd.data(), tab,
+                                  yc1.data(), om); }, false);
+    const int kzr = sweep25_kz(gf, 2);
+    run_blocks(dim3(nx / SW_TX, ny / SW_TY, nz / kzr), dim3(SW_TX, 4, 1), [&]() {
+        k_mg_restrict25<T, T>(gf, gc, x.data(), b.data(), bc2.data(), xd.data(), tab, kzr,
+                              yc2.data(), om); }, true);
+    bad += ndiff(bc1, bc2) + ndiff(yc1, yc2);
+    // prolongation, in place and into another array
+    std::vector<T> p1 = x, p2 = x, p3(nf), x0 = x;
+    run_blocks(mg_grid(gf), dim3(MG_BX, MG_BY, 1), [&]() {
+        k_mg_prolong<T, T>(gf, gc, p1.data(), xc.data(), tab); }, false);
+    run_blocks(dim3(nx / SW_TX, ny / SW_TY, nz / kzr), dim3(SW_TX, 4, 1), [&]() {
+        k_mg_prolong25<T, T>(gf, gc, p2.data(), xc.data(), tab, kzr, nullptr); }, true);
+    std::vector<T> p4 = x;
+    run_blocks(dim3(nx / SW_TX, ny / SW_TY, nz / kzr), dim3(SW_TX, 4, 1), [&]() {
+        k_mg_prolong25<T, T>(gf, gc, p4.data(), xc.data(), tab, kzr, p3.data()); }, true);
+    bad += ndiff(p1, p2) + ndiff(p1, p3) + ndiff(p4, x0);
+    // two sweeps in one pass
+    std::vector<T> z1(nf), z2(nf);
+    run_blocks(mg_grid(gf), dim3(MG_BX, MG_BY, 1), [&]() {
+        k_mg_sweep<T>(gf, y1.data(), z1.data(), b.data(), om, tab); }, false);
+    const int kz2 = nz % 4 == 0 ? 4 : 1;
+    run_blocks(dim3(nx / SW_TX, ny / SW_TY, nz / kz2), dim3(SW_TX, 4, 1), [&]() {
+        k_mg_sweep2x25<T>(gf, x.data(), z2.data(), b.data(), om, tab, kz2); }, true);
+    bad += ndiff(z1, z2);
+    printf("tiled<%s> %dx%dx%d p=%d%d%d: %d mismatches\n", sizeof(T) == 8 ? "f64" : "f32",
+           nx, ny, nz, px, py, pz, bad);
+    return bad != 0;
+}
+
+// fused level kernels against sweep + restriction and prolongation + two sweeps
+template <typename Tf, typename Tc>
+int check_fused(int nx, int ny, int nz, int px, int py, int pz) {
+    Geo gf = mk(nx, ny, nz, px, py, pz), gc = mk(nx / 2, ny / 2, nz / 2, px, py, pz);
+    if (!fused_fits(gf, gc)) return 1;
+    MGTabs tab = mktab();
+    const size_t nf = (size_t)nx * ny * nz, nc = nf / 8;
+    std::mt19937_64 rng(nx * 131 + ny * 17 + nz + px * 5 + py * 7 + pz * 11);
+    auto y = rnd<Tf>(nf, rng), b = rnd<Tf>(nf, rng), x0 = rnd<Tf>(nf, rng);
+    auto xc = rnd<Tc>(nc, rng);
+    std::vector<Tf> xr(nf, Tf(3)), xf(nf, Tf(3));
+    std::vector<Tc> bcr(nc, Tc(5)), bcf(nc, Tc(5)), ycr(nc, Tc(6)), ycf(nc, Tc(6)), xd(nc);
+    const double om = 0.9;
+    run_blocks(mg_grid(gf), dim3(MG_BX, MG_BY, 1), [&]() {
+        k_mg_sweep<Tf>(gf, y.data(), xr.data(), b.data(), om, tab); }, false);
+    run_blocks(mg_grid(gc), dim3(MG_BX, MG_BY, 1), [&]() {
+        k_mg_resid_restrict<Tf, Tc>(gf, gc, xr.data(), b.data(), bcr.data(), xd.data(), tab,
+                                    ycr.data(), om); }, false);
+    run_blocks(fused_grid(gf), dim3(FUSED_T, 1, 1), [&]() {
+        k_mg_down<Tf, Tc>(gf, gc, y.data(), b.data(), xf.data(), bcf.data(), ycf.data(), om,
+                          tab); }, true);
+    int bad = ndiff(xr, xf) + ndiff(bcr, bcf) + ndiff(ycr, ycf);
+    std::vector<Tf> xp = x0, t(nf), ur(nf), uf(nf, Tf(9)), x0c = x0;
+    run_blocks(mg_grid(gf), dim3(MG_BX, MG_BY, 1), [&]() {
+        k_mg_prolong<Tf, Tc>(gf, gc, xp.data(), xc.data(), tab); }, false);
+    run_blocks(mg_grid(gf), dim3(MG_BX, MG_BY, 1), [&]() {
+        k_mg_sweep<Tf>(gf, xp.data(), t.data(), b.data(), om, tab); }, false);
+    run_blocks(mg_grid(gf), dim3(MG_BX, MG_BY, 1), [&]() {
+        k_mg_sweep<Tf>(gf, t.data(), ur.data(), b.data(), om, tab); }, false);
+    run_blocks(fused_grid(gf), dim3(FUSED_T, 1, 1), [&]() {
+        k_mg_up<Tf, Tc>(gf, gc, x0.data(), xc.data(), b.data(), uf.data(), om, tab); }, true);
+    bad += ndiff(ur, uf) + ndiff(x0, x0c);
+    printf("fused<%s,%s> %dx%dx%d p=%d%d%d: %d mismatches\n", sizeof(Tf) == 8 ? "f64" : "f32",
+           sizeof(Tc) == 8 ? "f64" : "f32", nx, ny, nz, px, py, pz, bad);
+    return bad != 0;
+}
+
+// FP64 outer residual of the mixed solve: tiled (with and without the sum
+// of b**2) against per-cell; the partial sums add up to the same totals
+int check_outer(int nx, int ny, int nz, int px, int py, int pz) {
+    Geo g = mk(nx, ny, nz, px, py, pz);
+    MGTabs tab = mktab();
+    const size_t n = (size_t)nx * ny * nz;
+    std::mt19937_64 rng(7 + px + 2 * py + 4 * pz);
+    auto x = rnd<double>(n, rng), b = rnd<double>(n, rng);
+    std::vector<double> p1(NYL_RED_BLOCKS, 0.0), p2(NYL_RED_BLOCKS, 0.0), p3(NYL_RED_BLOCKS, 0.0),
+        pb(NYL_RED_BLOCKS, 0.0);
+    std::vector<float> r1(n), y1(n), r2(n), y2(n), r3(n), y3(n);
+    run_blocks(outer_grid(g), dim3(MG_BX, MG_BY, 1), [&]() {
+        k_mg_outer_resid<float>(g, x.data(), b.data(), 0.0, r1.data(), y1.data(), 0.9, tab,
+                                p1.data()); }, true);
+    const int kz = outer25_kz(g);
+    const dim3 grd(nx / SW_TX, ny / SW_TY, nz / kz);
+    run_blocks(grd, dim3(SW_TX, 4, 1), [&]() {
+        k_mg_outer25<float>(g, x.data(), b.data(), 0.0, r2.data(), y2.data(), 0.9, tab, kz,
+                            p2.data(), nullptr); }, true);
+    run_blocks(grd, dim3(SW_TX, 4, 1), [&]() {
+        k_mg_outer25<float>(g, x.data(), b.data(), 0.0, r3.data(), y3.data(), 0.9, tab, kz,
+                            p3.data(), pb.data()); }, true);
+    int bad = ndiff(r1, r2) + ndiff(y1, y2) + ndiff(r2, r3) + ndiff(y2, y3) + ndiff(p2, p3);
+    double s1 = 0, s2 = 0, sb = 0, sb2 = 0;
+    for (double v : p1) s1 += v;
+    for (double v : p2) s2 += v;
+    for (double v : pb) sb += v;
+    for (double v : b) sb2 += v * v;
+    bad += std::fabs(s1 - s2) > 1e-12 * s1 || std::fabs(sb - sb2) > 1e-12 * sb2;
+    printf("outer %dx%dx%d p=%d%d%d: %d mismatches\n", nx, ny, nz, px, py, pz, bad);
+    return bad != 0;
+}
+
+int main() {
+    int fails = 0;
+    for (int p = 0; p < 8; p++) {
+        const int px = p & 1, py = (p >> 1) & 1, pz = (p >> 2) & 1;
+        if (p == 0 || p == 3 || p == 4 || p == 7) {   // 000, 110, 001, 111
+            fails += check_tiled<double>(64, 8, 8, px, py, pz);
+            fails += check_tiled<float>(128, 16, 4, px, py, pz);
+            fails += check_outer(64, 8, 16, px, py, pz);
+        }
+        const int sizes[][3] = {{8, 8, 8}, {2, 2, 4}, {4, 2, 2}, {20, 12, 6}};
+        for (auto &s : sizes) {
+            fails += check_fused<double, double>(s[0], s[1], s[2], px, py, pz);
+            fails += check_fused<float, float>(s[0], s[1], s[2], px, py, pz);
+            fails += check_fused<float, double>(s[0], s[1], s[2], px, py, pz);
+        }
+    }
+    printf(fails ? "FAIL %d\n" : "ALL OK\n", fails);
+    return fails != 0;
+}
