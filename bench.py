@@ -331,8 +331,8 @@ def error_check(n, dt, steps):
 def main():
     args = parse()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(run with --gpus 1)", file=sys.stderr)
+        print("bench.py measures one GPU: the Nyles time step here runs one model per GPU "
+              "and has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
     import numpy as np
     import torch
