@@ -1554,55 +1554,88 @@ __global__ __launch_bounds__(256) void k_mg_sweep2x25(Geo g, const T *__restrict
                                                       T *__restrict__ dst,
                                                       const T *__restrict__ b, double omega,
                                                       MGTabs tab, int kz) {
-    __shared__ T pin[3][S2_LY * S2_LX];
-    __shared__ T mid[3][S2_MY * S2_MX];
+    constexpr int NIN = S2_LY * S2_LX, NMID = S2_MY * S2_MX;
+    constexpr int RIN = (NIN + 255) / 256, RMID = (NMID + 255) / 256;
+    __shared__ T pin[3][NIN];
+    __shared__ T mid[3][NMID];
     const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * SW_TX + tx;
     const Blk3 bk = xcd_block();
     const int i0 = bk.x * SW_TX, j0 = bk.y * SW_TY, k0 = bk.z * kz;
     const T om = T(omega), cff1 = T(1.0 - omega);
+    // per-thread positions, fixed for the march: input ring cells (in-plane
+    // offset, or -1 outside a wall axis) ...
+    int in_off[RIN];
+#pragma unroll
+    for (int r = 0; r < RIN; r++) {
+        const int q = tid + 256 * r;
+        const int lx = q % S2_LX, ly = q / S2_LX;
+        int gi = i0 - 2 + lx, gj = j0 - 2 + ly;
+        bool in = q < NIN;
+        if (g.px) gi = wrapg(gi, g.nx); else in = in && gi >= 0 && gi < g.nx;
+        if (g.py) gj = wrapg(gj, g.ny); else in = in && gj >= 0 && gj < g.ny;
+        in_off[r] = in ? gj * (int)g.sy + gi : -1;
+    }
+    // ... and first-sweep cells (in-plane offset or -1, in-plane neighbour count)
+    int mid_off[RMID], mid_nxy[RMID];
+#pragma unroll
+    for (int r = 0; r < RMID; r++) {
+        const int q = tid + 256 * r;
+        const int lx = q % S2_MX, ly = q / S2_MX;
+        int gi = i0 - 1 + lx, gj = j0 - 1 + ly;
+        bool in = q < NMID;
+        if (g.px) gi = wrapg(gi, g.nx); else in = in && gi >= 0 && gi < g.nx;
+        if (g.py) gj = wrapg(gj, g.ny); else in = in && gj >= 0 && gj < g.ny;
+        mid_off[r] = in ? gj * (int)g.sy + gi : -1;
+        mid_nxy[r] = (g.px || gi > 0) + (g.px || gi < g.nx - 1) + (g.py || gj > 0) +
+                     (g.py || gj < g.ny - 1);
+    }
+    int out_nxy[SW_R];
+#pragma unroll
+    for (int r = 0; r < SW_R; r++) {
+        const int i = i0 + tx, j = j0 + ty + 4 * r;
+        out_nxy[r] = (g.px || i > 0) + (g.px || i < g.nx - 1) + (g.py || j > 0) +
+                     (g.py || j < g.ny - 1);
+    }
     auto zin = [&](int m) { return g.pz || (m >= 0 && m < g.nz); };
     auto zw = [&](int m) { return g.pz ? wrapg(m, g.nz) : m; };
     auto slot = [&](int m) { return (m - k0 + 3) % 3; };   // m >= k0 - 2
-    // input plane m with a two-cell ring, zero outside a wall axis
     auto load_in = [&](int m) {
         T *P = pin[slot(m)];
         const bool zi = zin(m);
         const int64_t base = (int64_t)zw(m) * g.sz;
-        for (int q = tid; q < S2_LY * S2_LX; q += 256) {
-            const int lx = q % S2_LX, ly = q / S2_LX;
-            int gi = i0 - 2 + lx, gj = j0 - 2 + ly;
-            bool in = zi;
-            if (g.px) gi = wrapg(gi, g.nx); else in = in && gi >= 0 && gi < g.nx;
-            if (g.py) gj = wrapg(gj, g.ny); else in = in && gj >= 0 && gj < g.ny;
-            P[q] = in ? src[base + (int64_t)gj * g.sy + gi] : T(0);
+        T v[RIN];
+#pragma unroll
+        for (int r = 0; r < RIN; r++) v[r] = (zi && in_off[r] >= 0) ? src[base + in_off[r]] : T(0);
+#pragma unroll
+        for (int r = 0; r < RIN; r++) {
+            const int q = tid + 256 * r;
+            if (q < NIN) P[q] = v[r];
         }
     };
-    // first sweep at plane m on the tile and its one-cell ring
     auto comp_mid = [&](int m) {
         T *M = mid[slot(m)];
         const T *P0 = pin[slot(m)], *Pm = pin[slot(m - 1)], *Pp = pin[slot(m + 1)];
         const bool zi = zin(m);
         const int mw = zw(m);
         const int nzw = (g.pz || mw > 0) + (g.pz || mw < g.nz - 1);
-        for (int q = tid; q < S2_MY * S2_MX; q += 256) {
-            const int lx = q % S2_MX, ly = q / S2_MX;
-            int gi = i0 - 1 + lx, gj = j0 - 1 + ly;
-            bool in = zi;
-            if (g.px) gi = wrapg(gi, g.nx); else in = in && gi >= 0 && gi < g.nx;
-            if (g.py) gj = wrapg(gj, g.ny); else in = in && gj >= 0 && gj < g.ny;
+        const int64_t base = (int64_t)mw * g.sz;
+        T bv[RMID];
+#pragma unroll
+        for (int r = 0; r < RMID; r++) bv[r] = (zi && mid_off[r] >= 0) ? b[base + mid_off[r]] : T(0);
+#pragma unroll
+        for (int r = 0; r < RMID; r++) {
+            const int q = tid + 256 * r;
+            if (q >= NMID) continue;
             T v = T(0);
-            if (in) {
+            if (zi && mid_off[r] >= 0) {
+                const int lx = q % S2_MX, ly = q / S2_MX;
                 const int p = (ly + 1) * S2_LX + (lx + 1);
                 const T S = P0[p - 1] + P0[p + 1] + P0[p - S2_LX] + P0[p + S2_LX] + Pm[p] + Pp[p];
-                const int nw = (g.px || gi > 0) + (g.px || gi < g.nx - 1) + (g.py || gj > 0) +
-                               (g.py || gj < g.ny - 1) + nzw;
-                const T bv = b[(int64_t)mw * g.sz + (int64_t)gj * g.sy + gi];
-                v = cff1 * P0[p] + om * (S - bv) * T(tab.idiag[nw]);
+                v = cff1 * P0[p] + om * (S - bv[r]) * T(tab.idiag[mid_nxy[r] + nzw]);
             }
             M[q] = v;
         }
     };
-    // second sweep at plane k on the tile
     auto comp_out = [&](int k) {
         const T *M0 = mid[slot(k)], *Mm = mid[slot(k - 1)], *Mp = mid[slot(k + 1)];
         const int nzw = (g.pz || k > 0) + (g.pz || k < g.nz - 1);
@@ -1611,11 +1644,9 @@ __global__ __launch_bounds__(256) void k_mg_sweep2x25(Geo g, const T *__restrict
         for (int r = 0; r < SW_R; r++) {
             const int j = j0 + ty + 4 * r;
             const int p = (ty + 4 * r + 1) * S2_MX + (tx + 1);
-            const T S = M0[p - 1] + M0[p + 1] + M0[p - S2_MX] + M0[p + S2_MX] + Mm[p] + Mp[p];
-            const int nw = (g.px || i > 0) + (g.px || i < g.nx - 1) + (g.py || j > 0) +
-                           (g.py || j < g.ny - 1) + nzw;
             const int64_t c = (int64_t)k * g.sz + (int64_t)j * g.sy + i;
-            dst[c] = cff1 * M0[p] + om * (S - b[c]) * T(tab.idiag[nw]);
+            const T S = M0[p - 1] + M0[p + 1] + M0[p - S2_MX] + M0[p + S2_MX] + Mm[p] + Mp[p];
+            dst[c] = cff1 * M0[p] + om * (S - b[c]) * T(tab.idiag[out_nxy[r] + nzw]);
         }
     };
     load_in(k0 - 2);
