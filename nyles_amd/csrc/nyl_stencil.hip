@@ -60,49 +60,6 @@ static inline dim3 cell_grid_kz(const Geo &g) {
 #define CELL_BLOCK() xcd_block()
 #endif
 
-// Vortex-force flux of one term at a cell (core/fortran_vortex_force.f90
-// WENO branch, :66-81 and :145-159): the vorticity component `om` is
-// upwinded along axis d with the contravariant velocity U_d = ud*cu
-// averaged along the target axis e (UU) and then along d (u1d).  `based` is
-// the index of the cell's line start along d, `sd` the stride of d, `md` the
-// cell's position on that line, `oe` the offset to the e+1 neighbour.  U is
-// recomputed from the covariant u with the product U_from_u stores
-// (core/cov_to_contra.py:15-17), so it is the same double.
-template <typename T, int pd>
-__device__ __forceinline__ T vf_flux(const T *__restrict__ ud, T cu, const T *__restrict__ om,
-                                     int64_t based, int64_t sd, int md, int nd, int64_t oe) {
-    auto UU = [&](int m) -> T {
-        const int64_t p = based + (int64_t)m * sd;
-        return T(0.5) * (ud[p] * cu + ud[p + oe] * cu);
-    };
-    T uu_prev;
-    if (!pd && md == 0) uu_prev = T(0);   // UU_0 = 0. at the line start
-    else uu_prev = UU(pd ? wrapi(md - 1, nd) : md - 1);
-    const T u1d = T(0.5) * (uu_prev + UU(md));
-    auto q = [&](int m) -> T {            // q(1) = 0., q(k) = vort(k-1)
-        if (pd) return om[based + (int64_t)wrapi(m - 1, nd) * sd];
-        if (m == 0) return T(0);
-        return om[based + (int64_t)(m - 1) * sd];
-    };
-    return face_flux<T, pd>(md, nd, u1d, q);
-}
-
-// Tracer flux divergence along one axis (core/fortran_upwind.f90:228-234):
-// d = d + flux(m-1) - flux(m), or d - flux(1) at a wall start.
-// U_d = ud*cu as in vf_flux.
-template <typename T, int pd>
-__device__ __forceinline__ T tracer_axis(T d, const T *__restrict__ q_arr,
-                                         const T *__restrict__ ud, T cu, int64_t based,
-                                         int64_t sd, int md, int nd) {
-    auto q = [&](int m) -> T { return q_arr[based + (int64_t)(pd ? wrapi(m, nd) : m) * sd]; };
-    const T fr = face_flux<T, pd>(md, nd, ud[based + (int64_t)md * sd] * cu, q);
-    STAGE_SEQ(based, fr);
-    if (!pd && md == 0) return d - fr;
-    const int ml = pd ? wrapi(md - 1, nd) : md - 1;
-    const T fl = face_flux<T, pd>(ml, nd, ud[based + (int64_t)ml * sd] * cu, q);
-    return (d + fl) - fr;
-}
-
 // Stage-kernel loads and stores at a 32-bit byte offset from the field's
 // base pointer (the base is uniform, so the load takes it from scalar
 // registers and the offset as one 32-bit register: no 64-bit address
@@ -115,6 +72,56 @@ __device__ __forceinline__ T ldb(const T *__restrict__ p, uint32_t boff) {
 template <typename T>
 __device__ __forceinline__ void stb(T *__restrict__ p, uint32_t boff, T v) {
     *reinterpret_cast<T *>(reinterpret_cast<char *>(p) + boff) = v;
+}
+
+// Vortex-force flux of one term at a cell (core/fortran_vortex_force.f90
+// WENO branch, :66-81 and :145-159): the vorticity component `om` is
+// upwinded along axis d with the contravariant velocity U_d = ud*cu
+// averaged along the target axis e (UU) and then along d (u1d).  `based` is
+// the index of the cell's line start along d, `sd` the stride of d, `md` the
+// cell's position on that line, `oe` the offset to the e+1 neighbour.  U is
+// recomputed from the covariant u with the product U_from_u stores
+// (core/cov_to_contra.py:15-17), so it is the same double.
+template <typename T, int pd>
+__device__ __forceinline__ T vf_flux(const T *__restrict__ ud, T cu, const T *__restrict__ om,
+                                     int64_t based, int64_t sd, int md, int nd, int64_t oe) {
+    // 32-bit byte offsets from the field bases (a field is below 4 GiB)
+    constexpr uint32_t ES = sizeof(T);
+    const uint32_t bb = (uint32_t)based * ES, sb = (uint32_t)sd * ES, eb = (uint32_t)oe * ES;
+    auto UU = [&](int m) -> T {
+        const uint32_t p = bb + (uint32_t)m * sb;
+        return T(0.5) * (ldb(ud, p) * cu + ldb(ud, p + eb) * cu);
+    };
+    T uu_prev;
+    if (!pd && md == 0) uu_prev = T(0);   // UU_0 = 0. at the line start
+    else uu_prev = UU(pd ? wrapi(md - 1, nd) : md - 1);
+    const T u1d = T(0.5) * (uu_prev + UU(md));
+    auto q = [&](int m) -> T {            // q(1) = 0., q(k) = vort(k-1)
+        if (pd) return ldb(om, bb + (uint32_t)wrapi(m - 1, nd) * sb);
+        if (m == 0) return T(0);
+        return ldb(om, bb + (uint32_t)(m - 1) * sb);
+    };
+    return face_flux<T, pd>(md, nd, u1d, q);
+}
+
+// Tracer flux divergence along one axis (core/fortran_upwind.f90:228-234):
+// d = d + flux(m-1) - flux(m), or d - flux(1) at a wall start.
+// U_d = ud*cu as in vf_flux.
+template <typename T, int pd>
+__device__ __forceinline__ T tracer_axis(T d, const T *__restrict__ q_arr,
+                                         const T *__restrict__ ud, T cu, int64_t based,
+                                         int64_t sd, int md, int nd) {
+    // 32-bit byte offsets from the field bases (a field is below 4 GiB)
+    constexpr uint32_t ES = sizeof(T);
+    uint32_t bb = (uint32_t)based * ES;
+    const uint32_t sb = (uint32_t)sd * ES;
+    auto q = [&](int m) -> T { return ldb(q_arr, bb + (uint32_t)(pd ? wrapi(m, nd) : m) * sb); };
+    const T fr = face_flux<T, pd>(md, nd, ldb(ud, bb + (uint32_t)md * sb) * cu, q);
+    STAGE_SEQ(bb, fr);
+    if (!pd && md == 0) return d - fr;
+    const int ml = pd ? wrapi(md - 1, nd) : md - 1;
+    const T fl = face_flux<T, pd>(ml, nd, ldb(ud, bb + (uint32_t)ml * sb) * cu, q);
+    return (d + fl) - fr;
 }
 
 // Periodic axes: the byte offsets of the cells -3..+3 away along the axis
