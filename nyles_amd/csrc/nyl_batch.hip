@@ -33,7 +33,7 @@ uint64_t al256(uint64_t v) { return (v + 255) & ~uint64_t(255); }
 
 // one launch of a plan
 enum LKind {
-    L_STAGE, L_DIV, L_CORR, L_DIAG, L_SWEEP, L_RESTR, L_PROL, L_RED, L_OUTER, L_ADD, L_FINAL,
+    L_STAGE, L_DIV, L_CORR, L_DIAG, L_SWEEP, L_RESTR, L_PROL, L_RED, L_OUTER, L_FINAL,
     L_ACC
 };
 enum LType { T_D, T_F, T_FD };   // double, float, float fine over double coarse
@@ -259,7 +259,8 @@ bool b_lev64(const nyl_batch *b, int q, int l) { return lev64(b->m[q], l); }
 // whose level l has the type T and that have more than `below` levels below l
 template <typename T>
 void sweep_launch(nyl_batch *b, int l, bool first, bool second, bool t64,
-                  std::vector<Launch> &out, bool skip_first_l0, int below = 0) {
+                  std::vector<Launch> &out, bool skip_first_l0, int below = 0,
+                  bool addx_l0 = false) {
     for (int pass = 0; pass < 2; pass++) {
         if ((pass == 0 && !first) || (pass == 1 && !second)) continue;
         std::vector<SweepItem<T>> v;
@@ -275,6 +276,8 @@ void sweep_launch(nyl_batch *b, int l, bool first, bool second, bool t64,
             it.dst = F<T>(c, pass == 0 ? f.y : f.x);
             it.b = F<T>(c, f.b);
             it.omega = c->d.mg_omega;
+            // the mixed solve's last sweep of a V-cycle adds its result to x
+            it.xo = (addx_l0 && pass == 1 && l == 0) ? F<double>(c, c->mgx) : nullptr;
             v.push_back(it);
         }
         if (!v.empty()) out.push_back(add_table(b, L_SWEEP, t64 ? T_D : T_F, v, nb));
@@ -343,7 +346,7 @@ void build_vcycle(nyl_batch *b, std::vector<Launch> &out) {
         sweep_launch<double>(b, l, true, false, true, out, false, 1);
         sweep_launch<float>(b, l, true, false, false, out, false, 1);
         sweep_launch<double>(b, l, false, true, true, out, false, 1);
-        sweep_launch<float>(b, l, false, true, false, out, false, 1);
+        sweep_launch<float>(b, l, false, true, false, out, false, 1, b->mixed != 0);
     }
     std::vector<int> np(b->n);
     if (!b->mixed) {
@@ -361,16 +364,11 @@ void build_vcycle(nyl_batch *b, std::vector<Launch> &out) {
         }
         out.push_back(add_table(b, L_RED, T_D, v, nb, 1, 1));   // gated by active
     } else {
-        std::vector<AddItem> va;
+        // x += e was done by the last level-1 sweep (its result added to x)
         std::vector<OuterItem> vo;
-        int64_t nba = 0, nbo = 0;
+        int64_t nbo = 0;
         for (int q = 0; q < b->n; q++) {
             nyl_ctx *c = b->m[q];
-            AddItem a;
-            a.h = head(c->lev[0].g, q, gcells(c->lev[0].g), nba);
-            a.x = F<double>(c, c->mgx);
-            a.e = F<float>(c, c->lev[0].x);
-            va.push_back(a);
             OuterItem o;
             o.h = head(c->lev[0].g, q, gcells(c->lev[0].g), nbo, kRedCap);
             o.x = F<double>(c, c->mgx); o.b = F<double>(c, c->mgb);
@@ -380,7 +378,6 @@ void build_vcycle(nyl_batch *b, std::vector<Launch> &out) {
             np[q] = o.h.nblk;
             vo.push_back(o);
         }
-        out.push_back(add_table(b, L_ADD, T_D, va, nba));
         out.push_back(add_table(b, L_OUTER, T_D, vo, nbo, 0, 1));   // gated by active
     }
     out.push_back(final_table(b, np, NYL_FIN_STEP));
@@ -493,9 +490,6 @@ int run_launch(nyl_batch *b, const Launch &L) {
         // the initial residual runs for every member, the V-cycle's (arg2) for active ones
         BCHK(launch_b_outer_resid((const OuterItem *)tab, L.nitems, L.nblk, b->mg, b->tabs,
                                   L.arg2 ? act : nullptr, s));
-        break;
-    case L_ADD:
-        BCHK(launch_b_add((const AddItem *)tab, L.nitems, L.nblk, act, s));
         break;
     case L_FINAL:
         BCHK(launch_b_final((const FinalItem *)tab, L.nitems, L.arg, b->mg, s));

@@ -168,7 +168,9 @@ template <typename T> struct StageItem { BHead h; StageArgs<T> a; };
 template <typename T> struct DivItem { BHead h; DivArgs<T, double> a; };
 template <typename T> struct CorrItem { BHead h; CorrectArgs<T, double> a; };
 template <typename T> struct DiagItem { BHead h; DiagArgs<T> a; };
-template <typename T> struct SweepItem { BHead h; const T *src; T *dst; const T *b; double omega; };
+// xo non-null: the sweep result is added to xo (FP64) instead of stored in dst
+template <typename T> struct SweepItem { BHead h; const T *src; T *dst; const T *b; double omega;
+                                         double *xo; };
 template <typename Tf, typename Tc>
 struct RestrItem { BHead h; Geo gc; const Tf *x; const Tf *b; Tc *bc; Tc *xc; };
 template <typename Tf, typename Tc>
@@ -177,7 +179,6 @@ struct ProlItem { BHead h; Geo gc; Tf *xf; const Tc *xc; };
 template <typename T> struct RedItem { BHead h; const T *v; const T *b; double *partials; };
 struct OuterItem { BHead h; const double *x; const double *b; float *r; float *y; double omega;
                    double *partials; };
-struct AddItem { BHead h; double *x; const float *e; };
 
 // Device-side multigrid control of a batch: the stop rule of mgfor's solve
 // (core/mgfor/solvers.f90:17-30) evaluated per member on the device.
@@ -223,8 +224,6 @@ hipError_t launch_b_reduce(const RedItem<T> *items, int n, int64_t nblk, int kin
 // active null: every member (the initial residual of a solve)
 hipError_t launch_b_outer_resid(const OuterItem *items, int n, int64_t nblk, const BMG &mg,
                                 const MGTabs &tab, const int *active, hipStream_t s);
-hipError_t launch_b_add(const AddItem *items, int n, int64_t nblk, const int *active,
-                        hipStream_t s);
 hipError_t launch_b_final(const FinalItem *items, int n, int kind, const BMG &mg, hipStream_t s);
 // mode 0: acc = 0; mode 1: acc += nite
 hipError_t launch_b_acc(int n, const BMG &mg, int mode, hipStream_t s);

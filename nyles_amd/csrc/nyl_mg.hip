@@ -46,14 +46,21 @@ __device__ __forceinline__ Nbrs<T> neighbours(const Geo &g, const T *__restrict_
 // One damped-Jacobi sweep of fsmoother3d (core/mgfor/basicoperators.f90:363-400):
 // dst = cff1*src + omega*(sum6(src) - b)*idiag.
 template <typename T>
+__device__ __forceinline__ T sweep_value(const Geo &g, const T *__restrict__ src,
+                                         const T *__restrict__ b, double omega,
+                                         const MGTabs &tab, int i, int j, int k, int64_t c) {
+    const Nbrs<T> n = neighbours<T>(g, src, i, j, k, c);
+    const T S = n.xm + n.xp + n.ym + n.yp + n.zm + n.zp;
+    const T om = T(omega), cff1 = T(1.0 - omega);
+    return cff1 * src[c] + om * (S - b[c]) * T(tab.idiag[n.nwet]);
+}
+
+template <typename T>
 __device__ __forceinline__ void sweep_cell(const Geo &g, const T *__restrict__ src,
                                            T *__restrict__ dst, const T *__restrict__ b,
                                            double omega, const MGTabs &tab, int i, int j, int k) {
     const int64_t c = (int64_t)k * g.sz + (int64_t)j * g.sy + i;
-    const Nbrs<T> n = neighbours<T>(g, src, i, j, k, c);
-    const T S = n.xm + n.xp + n.ym + n.yp + n.zm + n.zp;
-    const T om = T(omega), cff1 = T(1.0 - omega);
-    dst[c] = cff1 * src[c] + om * (S - b[c]) * T(tab.idiag[n.nwet]);
+    dst[c] = sweep_value<T>(g, src, b, omega, tab, i, j, k, c);
 }
 
 template <typename T>
@@ -774,7 +781,10 @@ __global__ __launch_bounds__(BT) void k_b_sweep(const SweepItem<T> *__restrict__
     if (active && !active[it.h.member]) return;
     int i, j, k;
     if (!cell_of(it.h.g, ((int64_t)blockIdx.x - it.h.start) * BT + threadIdx.x, i, j, k)) return;
-    sweep_cell<T>(it.h.g, it.src, it.dst, it.b, it.omega, tab, i, j, k);
+    const int64_t c = (int64_t)k * it.h.g.sz + (int64_t)j * it.h.g.sy + i;
+    const T v = sweep_value<T>(it.h.g, it.src, it.b, it.omega, tab, i, j, k, c);
+    if (it.xo) it.xo[c] = it.xo[c] + (double)v;   // the mixed solve's x += e, e not stored
+    else it.dst[c] = v;
 }
 
 template <typename T>
@@ -900,20 +910,6 @@ hipError_t launch_b_outer_resid(const OuterItem *items, int n, int64_t nblk, con
                                 const MGTabs &tab, const int *active, hipStream_t s) {
     hipLaunchKernelGGL(k_b_outer_resid, dim3((unsigned)nblk), BT, 0, s, items, n, mg, tab,
                        active);
-    return hipGetLastError();
-}
-
-__global__ __launch_bounds__(BT) void k_b_add(const AddItem *__restrict__ items, int n,
-                                              const int *__restrict__ active) {
-    const AddItem &it = find_item(items, n, blockIdx.x);
-    if (!active[it.h.member]) return;
-    const int64_t q = ((int64_t)blockIdx.x - it.h.start) * BT + threadIdx.x;
-    if (q < (int64_t)it.h.g.nx * it.h.g.ny * it.h.g.nz) it.x[q] = it.x[q] + (double)it.e[q];
-}
-
-hipError_t launch_b_add(const AddItem *items, int n, int64_t nblk, const int *active,
-                        hipStream_t s) {
-    hipLaunchKernelGGL(k_b_add, dim3((unsigned)nblk), BT, 0, s, items, n, active);
     return hipGetLastError();
 }
 
