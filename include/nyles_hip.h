@@ -52,10 +52,12 @@ extern "C" {
 #define NYL_PREC_FP64 0     /* loop-faithful FP64, bit-compatible with the oracle */
 /* NYL_PREC_MIXED: model fields (state, tendencies, vor, ke) and their
  * stencils in FP32; the projection is iterative refinement: FP32 V-cycles
- * (FP64 coarsest level) on the correction inside an FP64 outer residual, the
- * right-hand side's mean removed in FP64, mgfor's stop rule and V-cycle
- * count; NYL_RED_* reductions accumulate in FP64.  No counterpart in the
- * reference (which is FP64 throughout). */
+ * (FP64 coarsest level) on the correction inside an FP64 outer residual,
+ * with mgfor's stop rule and V-cycle count (a tolerance tighter than one
+ * FP32 inner solve resolves, below 1e-6 of the current squared residual, is
+ * met through more outer steps of at most 8 V-cycles); NYL_RED_* reductions
+ * accumulate in FP64.  A field must stay below 4 GiB in either mode.  No
+ * counterpart in the reference (which is FP64 throughout). */
 #define NYL_PREC_MIXED 1
 
 typedef struct nyl_ctx nyl_ctx;
