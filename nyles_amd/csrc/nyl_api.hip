@@ -637,7 +637,6 @@ int mg_solve_mixed(nyl_ctx *c, nyl_mg_stats *st) {
         HIPCHK(launch_norm2<double>(f.g, bo, part, res_dev, c->stream));
         if ((rc = read_result(c, &normb))) return rc;
     }
-    c->mg_shift = shift;
     // FP64 residual of x into r (and the first sweep from e = 0 into y)
     auto outer = [&](double *out) -> int {
         if (normb > 0.0) {

@@ -30,7 +30,6 @@ struct nyl_ctx {
     int lf_first;            // next LFAM3 step is the first one
     uint64_t ke, vor[3], uc[3], p;
     uint64_t mgx, mgb;       // FP64 solution and right-hand side of the projection
-    double mg_shift;         // mean removed from the right-hand side (mixed mode)
     uint64_t partials, result;
     uint64_t ws_bytes;
     int cur;                 // which state buffer is current
