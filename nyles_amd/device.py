@@ -246,6 +246,10 @@ class DeviceBatch:
         L.check(self.lib.nyl_batch_set_tile(self.batch, int(tpb)))
         self.tile = int(tpb)
 
+    def set_graphs(self, on):
+        """replay the step plans from hipGraphs (captured on next use) or not"""
+        L.check(self.lib.nyl_batch_set_graphs(self.batch, int(bool(on))))
+
     def step_rk3(self, dts):
         """one RK3_SSP step of every member (member q with dts[q])"""
         n = len(self.members)

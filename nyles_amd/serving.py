@@ -137,16 +137,23 @@ class Server:
 
     max_slots: most jobs resident at once; hbm_fraction: share of the device
     memory the resident jobs may hold; tile: stage tile of the batches (None:
-    autotune whenever the number of resident jobs doubles or halves).
+    autotune whenever the number of resident jobs doubles or halves);
+    graphs: True/False, or "auto": a batch replays its plans from hipGraphs
+    once its membership has stayed the same for graph_after steps (capturing
+    the plans of a batch that changes every few steps costs more than the
+    replays save).
     """
 
-    def __init__(self, precision="mixed", max_slots=64, hbm_fraction=0.8, graphs=True,
-                 tile=None, keep_state=False):
+    def __init__(self, precision="mixed", max_slots=64, hbm_fraction=0.8, graphs="auto",
+                 tile=None, keep_state=False, graph_after=8):
         if precision not in PRECISION:
             raise ValueError("precision must be fp64 or mixed")
         self.precision = precision
         self.max_slots = max_slots
+        if graphs not in (True, False, "auto"):
+            raise ValueError('graphs must be True, False or "auto"')
         self.graphs = graphs
+        self.graph_after = graph_after
         self.tile = tile
         self.keep_state = keep_state
         props = torch.cuda.get_device_properties(0)
@@ -160,6 +167,7 @@ class Server:
         used = 0
         batch = None
         dirty = False
+        stable = 0
         tuned_for, tile = 0, self.tile
         rebuilds, steps, slot_steps, cellsteps = 0, 0, 0, 0
         peak = 0
@@ -192,15 +200,19 @@ class Server:
                     batch.close()
                 n = len(active)
                 retune = tile is None or n >= 2 * tuned_for or 2 * n <= tuned_for
+                g0 = self.graphs is True
                 if self.tile is None and retune:
-                    batch = DeviceBatch([j.model for j in active], graphs=self.graphs,
-                                        autotune=True)
+                    batch = DeviceBatch([j.model for j in active], graphs=g0, autotune=True)
                     tile, tuned_for = batch.tile, n
                 else:
-                    batch = DeviceBatch([j.model for j in active], graphs=self.graphs,
-                                        tile=tile)
+                    batch = DeviceBatch([j.model for j in active], graphs=g0, tile=tile)
                 rebuilds += 1
                 dirty = False
+                stable = 0
+            if self.graphs == "auto":
+                stable += 1
+                if stable == self.graph_after:
+                    batch.set_graphs(True)
             st = batch.step_rk3([j.dt for j in active])
             steps += 1
             slot_steps += len(active)
