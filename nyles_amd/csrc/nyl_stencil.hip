@@ -41,6 +41,18 @@ namespace nyl {
 #ifndef NYL_STAGE_SEQ_FACES
 #define NYL_STAGE_SEQ_FACES 0
 #endif
+// the sequencing points between the vortex-force terms in FP32: every
+// term (1), every second term (2) or none (0: FP32 stage 579 -> 533 us; the
+// FP32 operands are half the registers, so the overlap fits)
+#ifndef NYL_STAGE_SEQ_VF32
+#define NYL_STAGE_SEQ_VF32 0
+#endif
+#define STAGE_SEQ_V(idx, dep, odd)                                             \
+    do {                                                                       \
+        if constexpr (sizeof(T) == 8 || NYL_STAGE_SEQ_VF32 == 1 ||             \
+                      (NYL_STAGE_SEQ_VF32 == 2 && (odd)))                      \
+            STAGE_SEQ(idx, dep);                                               \
+    } while (0)
 // the sequencing points between the tracer axes: kept for FP64 (overlapping
 // them made the FP64 stage 2 % slower), dropped for FP32 (2 % faster)
 #define STAGE_SEQ_A(idx, dep)                      \
@@ -257,36 +269,36 @@ __device__ __forceinline__ void stage_cell(const Geo &g, const StageArgs<T> &a, 
         if constexpr (PX != 0) dv = dv - vf_flux_per<T>(a.u, cx, a.oz, cs, Lx, bey);
         else dv = dv - vf_flux<T, 0>(a.u, cx, a.oz, bx, 1, i, g.nx, oey);
     }
-    STAGE_SEQ(by, dv);
-    STAGE_SEQ(cs, dv);
+    STAGE_SEQ_V(by, dv, 0);
+    STAGE_SEQ_V(cs, dv, 0);
     // flip (U_j, vor_k, du_i)
     if (ex) {
         if constexpr (PY != 0) du = du + vf_flux_per<T>(a.v, cy, a.oz, cs, Ly, bex);
         else du = du + vf_flux<T, 0>(a.v, cy, a.oz, by, g.sy, j, g.ny, oex);
     }
-    STAGE_SEQ(by, du);
-    STAGE_SEQ(cs, du);
+    STAGE_SEQ_V(by, du, 1);
+    STAGE_SEQ_V(cs, du, 1);
     // direc(U_j, vor_i, du_k)
     if (ez) {
         if constexpr (PY != 0) dw = dw - vf_flux_per<T>(a.v, cy, a.ox, cs, Ly, bez);
         else dw = dw - vf_flux<T, 0>(a.v, cy, a.ox, by, g.sy, j, g.ny, oez);
     }
-    STAGE_SEQ(bz, dw);
-    STAGE_SEQ(cs, dw);
+    STAGE_SEQ_V(bz, dw, 0);
+    STAGE_SEQ_V(cs, dw, 0);
     // flip (U_k, vor_i, du_j)
     if (ey) {
         if constexpr (PZ != 0) dv = dv + vf_flux_per<T>(a.w, cz, a.ox, cs, Lz, bey);
         else dv = dv + vf_flux<T, 0>(a.w, cz, a.ox, bz, g.sz, k, g.nz, oey);
     }
-    STAGE_SEQ(bz, dv);
-    STAGE_SEQ(cs, dv);
+    STAGE_SEQ_V(bz, dv, 1);
+    STAGE_SEQ_V(cs, dv, 1);
     // direc(U_k, vor_j, du_i)
     if (ex) {
         if constexpr (PZ != 0) du = du - vf_flux_per<T>(a.w, cz, a.oy, cs, Lz, bex);
         else du = du - vf_flux<T, 0>(a.w, cz, a.oy, bz, g.sz, k, g.nz, oex);
     }
-    STAGE_SEQ(bx, du);
-    STAGE_SEQ(cs, du);
+    STAGE_SEQ_V(bx, du, 0);
+    STAGE_SEQ_V(cs, du, 0);
     // flip (U_i, vor_j, du_k)
     if (ez) {
         if constexpr (PX != 0) dw = dw + vf_flux_per<T>(a.u, cx, a.oy, cs, Lx, bez);
