@@ -6,9 +6,9 @@
  * calls into:
  *   - the f2py stencil modules fortran_upwind, fortran_vortex_force,
  *     fortran_vorticity, fortran_bernoulli, fortran_kinenergy
- *     (core/Makefile:1-12; signatures core/fortran_upwind.f90:156,
+ *     (core/Makefile:1-12; signatures core/fortran_upwind.f90:3,
  *     core/fortran_vortex_force.f90:10,87, core/fortran_vorticity.f90:2,
- *     core/fortran_bernoulli.f90:2,29,61, core/fortran_kinenergy.f90:101);
+ *     core/fortran_bernoulli.f90:2,29,61, core/fortran_kinenergy.f90:3);
  *   - the ctypes ABI of the mgfor multigrid library
  *     (core/mgfordriver.py:5-105 over core/build.py:262-314:
  *     get_ptrmg, solve, set_pyarray, get_pyarray, get_pyshape);
@@ -110,7 +110,7 @@ enum {
 /* what nyl_reduce computes */
 #define NYL_RED_MAXU2 0   /* max over cells of U^2+V^2+W^2 (core/nyles.py:249) */
 #define NYL_RED_KE 1      /* sum over the interior of ke evaluated in FP64 from the current
-                             u, v, w (core/fortran_kinenergy.f90:145-153) */
+                             u, v, w (core/fortran_kinenergy.f90:47-55) */
 #define NYL_RED_BSUM 2    /* sum of b over the interior */
 #define NYL_RED_NONFINITE 3  /* number of non-finite values in b,u,v,w */
 
@@ -147,7 +147,7 @@ int nyl_mg_level_shape(const nyl_ctx *ctx, int level, int32_t *nx, int32_t *ny, 
  * core/cov_to_contra.py:15-17); vor and ke are those of the last diagnose. */
 int nyl_rhs(nyl_ctx *ctx, int slot, int last);
 
-/* One RK3_SSP stage (core/timescheme.py:403-444) without the projection:
+/* One RK3_SSP stage (core/timescheme.py:180-221) without the projection:
  * rhs into the stage's slot and the stage update of b,u,v,w.
  * stage 1: s+=c*ds0; 2: s+=c*(ds1-3*ds0); 3: s+=c*(8*ds2-ds0-ds1), with
  * c = dt, dt/4., dt/12. computed by the caller. */
@@ -159,15 +159,15 @@ int nyl_rk3_stage(nyl_ctx *ctx, int stage, double c);
  * kernel recomputes it from u); nyl_apply(NYL_OP_U_FROM_U) fills NYL_F_UC*. */
 int nyl_diagnose(nyl_ctx *ctx, nyl_mg_stats *stats);
 
-/* Timescheme.RK3_SSP forward (core/timescheme.py:403-444) with three
+/* Timescheme.RK3_SSP forward (core/timescheme.py:180-221) with three
  * diagnose_var calls; stats accumulates the three solves (nite = sum). */
 int nyl_step_rk3(nyl_ctx *ctx, double dt, nyl_mg_stats *stats);
 
-/* Timescheme.EulerForward (core/timescheme.py:336-345): rhs, s += dt*ds,
+/* Timescheme.EulerForward (core/timescheme.py:113-122): rhs, s += dt*ds,
  * diagnose_var. */
 int nyl_step_ef(nyl_ctx *ctx, double dt, nyl_mg_stats *stats);
 
-/* Timescheme.LFAM3 (core/timescheme.py:348-400): the first call is an Euler
+/* Timescheme.LFAM3 (core/timescheme.py:125-177): the first call is an Euler
  * step that also stores the 'before' state; later calls are the
  * leap-frog / Adams-Moulton predictor, diagnose_var, the corrector at n+1/2
  * and diagnose_var (stats.nite sums both solves).  nyl_lfam3_reset makes the
@@ -214,6 +214,7 @@ int nyl_reduce(nyl_ctx *ctx, int what, double *out);
 #define NYL_K_MG_UPDATE 9    /* mixed mode: x += e of the defect correction */
 #define NYL_K_MG_SWEEP_FINE 10  /* the NYL_K_MG_SWEEP launches on the finest level only */
 #define NYL_K_MG_SWEEP2 11   /* two post-smoothing sweeps fused in one pass */
+#define NYL_K_MG_COARSE 12   /* direct solve of a coarsest level with an odd size */
 int nyl_prof_start(nyl_ctx *ctx, int kernel_class);
 int nyl_prof_stop(nyl_ctx *ctx, double *total_ms, int64_t *launches, double *alg_bytes);
 

@@ -1,11 +1,11 @@
 """Time schemes (interface of core/timescheme.py).
 
 ``Timescheme.forward`` is one of ``EulerForward``, ``LFAM3`` or ``RK3_SSP``
-(core/timescheme.py:285-322); each runs the whole step on the GPU through
+(core/timescheme.py:62-99); each runs the whole step on the GPU through
 the model's device context: the stage updates are fused into the
 right-hand-side kernel and every stage is followed by the model's
 diagnose_var (projection), exactly as the reference sequences them
-(core/timescheme.py:336-444).
+(core/timescheme.py:113-221).
 """
 
 

@@ -70,18 +70,33 @@ Geo make_geo(int nx, int ny, int nz, int px, int py, int pz) {
 }
 
 // core/mgfor/mg_setup.f90:225-307 on one rank: halve x and y until one of
-// them equals 2; z (interior) halves alongside.  Every level that is
-// restricted must have even sizes so that its cells pair up exactly.
+// them equals 2; z (interior) halves alongside.  A level is restricted only
+// when all its sizes are even, so that its cells pair up exactly
+// (core/mgfor/basicoperators.f90:32-60).  The reference has no rule for a
+// level with an odd size (its restriction drops a cell, and on one rank a
+// size 3 never reaches 2): here the hierarchy stops at that level, which is
+// then solved directly (nyl_coarse.hip).  Grids the reference halves down to
+// x or y = 2 get exactly the reference's hierarchy.
 int build_hierarchy(nyl_ctx *c) {
     int x = c->d.nx, y = c->d.ny, z = c->d.nz;
     int n = 0;
+    c->exact = 0;
     c->lev[n++].g = make_geo(x, y, z, c->g.px, c->g.py, c->g.pz);
     while (!(x == 2 || y == 2)) {
-        if ((x & 1) || (y & 1) || (z & 1) || x < 2 || y < 2 || z < 2)
-            return fail(NYL_E_SIZE,
-                        "grid %dx%dx%d: multigrid level %d has size %dx%dx%d that does not "
-                        "halve evenly (the mgfor hierarchy needs x or y to reach 2)",
-                        c->d.nx, c->d.ny, c->d.nz, n - 1, x, y, z);
+        if ((x & 1) || (y & 1) || (z & 1)) {
+            if (n == 1)
+                return fail(NYL_E_SIZE,
+                            "grid %dx%dx%d: the multigrid needs at least one halving "
+                            "(nx, ny and nz even)", c->d.nx, c->d.ny, c->d.nz);
+            const Geo &g = c->lev[n - 1].g;
+            if (!coarse_fits(g))
+                return fail(NYL_E_SIZE,
+                            "grid %dx%dx%d: the coarsest multigrid level %dx%dx%d has an odd "
+                            "size and more than %d cells along an axis (direct coarse solve)",
+                            c->d.nx, c->d.ny, c->d.nz, x, y, z, CS_MAX);
+            c->exact = 1;
+            break;
+        }
         x /= 2; y /= 2; z /= 2;
         if (n >= kMaxLevels) return fail(NYL_E_SIZE, "too many multigrid levels");
         c->lev[n++].g = make_geo(x, y, z, c->g.px, c->g.py, c->g.pz);
@@ -133,6 +148,7 @@ void build_layout(nyl_ctx *c) {
     }
     c->partials = take(NYL_RED_BLOCKS * sizeof(double));
     c->result = take(8 * sizeof(double));
+    c->cs_tab = c->exact ? take(coarse_table_doubles() * sizeof(double)) : 0;
     c->ws_bytes = off;
 }
 
@@ -469,6 +485,23 @@ int fused_up(nyl_ctx *c, int l) {
     return NYL_OK;
 }
 
+// the coarsest level's step of a V-cycle: mgfor's smooth() (two sweeps, the
+// first already written by the restriction), or the direct solve of a level
+// with an odd size
+int coarsest(nyl_ctx *c) {
+    const int L = c->nlev - 1;
+    if (!c->exact) return smooth_level(c, L, -1, true);
+    Level &f = c->lev[L];
+    CoarseArgs a;
+    a.g = f.g;
+    a.b = F<double>(c, f.b);
+    a.s = F<double>(c, f.y);
+    a.x = F<double>(c, f.x);
+    a.tab = F<double>(c, c->cs_tab);
+    LAUNCH(NYL_K_MG_COARSE, 4 * cells(f.g) * sizeof(double), launch_coarse_solve(a, c->stream));
+    return NYL_OK;
+}
+
 // the part of a V-cycle below level 1: from level 2's smoothing (its first
 // sweep written by the restriction from level 1) down to the coarsest level
 // and back up to level 2's post-smoothing
@@ -483,7 +516,7 @@ int coarse_cycle(nyl_ctx *c) {
         if ((rc = smooth_level(c, l, -1, true))) return rc;
         if ((rc = restrict_level(c, l))) return rc;
     }
-    if ((rc = smooth_level(c, L - 1, -1, true))) return rc;
+    if ((rc = coarsest(c))) return rc;
     for (int l = L - 2; l >= 1; l--) {
         if (fused_level(c, l)) {
             if ((rc = fused_up(c, l))) return rc;
@@ -508,7 +541,7 @@ void drop_graphs(nyl_ctx *c) {
 int coarse_cycle_run(nyl_ctx *c) {
     const int pc = c->prof_cls;
     const bool timed = pc == NYL_K_MG_SWEEP || pc == NYL_K_MG_RESTRICT || pc == NYL_K_MG_PROLONG ||
-                       pc == NYL_K_MG_SWEEP2;
+                       pc == NYL_K_MG_SWEEP2 || pc == NYL_K_MG_COARSE;
     if (!c->use_graphs || !c->stream || timed) return coarse_cycle(c);
     if (!c->g_coarse) {
         hipGraph_t graph;
@@ -538,7 +571,7 @@ int vcycle(nyl_ctx *c, bool pre_done, double *addx = nullptr) {
     if (c->nlev > 2) {
         if ((rc = coarse_cycle_run(c))) return rc;
     } else {
-        if ((rc = smooth_level(c, 1, -1, true))) return rc;
+        if ((rc = coarsest(c))) return rc;
     }
     return smooth_level(c, 0, 1, false, addx);
 }
@@ -677,15 +710,19 @@ int mg_solve_mixed(nyl_ctx *c, nyl_mg_stats *st) {
     } else if ((rc = outer(&res))) {
         return rc;
     }
+    // mgfor's loop shape (core/mgfor/solvers.f90:22-29): test, V-cycle,
+    // count, stop at maxite (so maxite = 0 still runs one V-cycle)
     int nite = 0;
-    while (!(res < c->d.mg_tol) && nite < c->d.mg_maxite) {
+    bool at_max = false;
+    while (!(res < c->d.mg_tol) && !at_max) {
         bool pending = true;       // y holds the first sweep from e = 0
         const bool rel = res * kInnerRed2 > c->d.mg_tol;
         const double target = rel ? res * kInnerRed2 : c->d.mg_tol;
         for (int k = 1;; k++) {
             if ((rc = vcycle(c, pending))) return rc;
             nite = nite + 1;
-            if (nite >= c->d.mg_maxite || (rel && k >= kInnerMax)) break;
+            at_max = nite >= c->d.mg_maxite;
+            if (at_max || (rel && k >= kInnerMax)) break;
             double rin;
             if ((rc = inner(&rin))) return rc;
             pending = spec;
@@ -933,6 +970,12 @@ int nyl_bind_workspace(nyl_ctx *c, void *dptr, uint64_t bytes) {
     }
     c->ws = (char *)dptr;
     HIPCHK(launch_fill((double *)c->ws, (int64_t)(c->ws_bytes / sizeof(double)), 0.0, c->stream));
+    if (c->exact) {
+        std::vector<double> tab(coarse_table_doubles());
+        coarse_tables(c->lev[c->nlev - 1].g, tab.data());
+        HIPCHK(hipMemcpyAsync(c->ws + c->cs_tab, tab.data(), tab.size() * sizeof(double),
+                              hipMemcpyHostToDevice, c->stream));
+    }
     HIPCHK(hipStreamSynchronize(c->stream));
     c->cur = 0;
     c->total_cycles = 0;

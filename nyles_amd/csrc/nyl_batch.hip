@@ -34,7 +34,7 @@ uint64_t al256(uint64_t v) { return (v + 255) & ~uint64_t(255); }
 // one launch of a plan
 enum LKind {
     L_STAGE, L_DIV, L_CORR, L_DIAG, L_SWEEP, L_RESTR, L_PROL, L_RED, L_OUTER, L_FINAL,
-    L_ACC
+    L_ACC, L_COARSE
 };
 enum LType { T_D, T_F, T_FD };   // double, float, float fine over double coarse
 
@@ -269,6 +269,7 @@ void sweep_launch(nyl_batch *b, int l, bool first, bool second, bool t64,
             nyl_ctx *c = b->m[q];
             if (l + below >= c->nlev || b_lev64(b, q, l) != t64) continue;
             if (pass == 0 && l == 0 && skip_first_l0) continue;
+            if (c->exact && l == c->nlev - 1) continue;   // solved directly
             const Level &f = c->lev[l];
             SweepItem<T> it;
             it.h = head(f.g, q, gcells(f.g), nb);
@@ -289,6 +290,33 @@ void smooth_launches(nyl_batch *b, int l, std::vector<Launch> &out, bool skip_fi
     sweep_launch<float>(b, l, true, false, false, out, skip_first_l0);
     sweep_launch<double>(b, l, false, true, true, out, skip_first_l0);
     sweep_launch<float>(b, l, false, true, false, out, skip_first_l0);
+}
+
+// the direct solve of the members whose coarsest level is l and has an odd
+// size (three passes: planes, rows, planes)
+void coarse_launches(nyl_batch *b, int l, std::vector<Launch> &out) {
+    for (int pass = 0; pass < 3; pass++) {
+        std::vector<CoarseItem> v;
+        int64_t nb = 0;
+        for (int q = 0; q < b->n; q++) {
+            nyl_ctx *c = b->m[q];
+            if (!c->exact || l != c->nlev - 1) continue;
+            const Level &f = c->lev[l];
+            CoarseItem it;
+            it.h.g = f.g;
+            it.h.member = q;
+            it.h.nblk = pass == 1 ? f.g.ny : f.g.nz;
+            it.h.start = nb;
+            nb += it.h.nblk;
+            it.a.g = f.g;
+            it.a.b = F<double>(c, f.b);
+            it.a.s = F<double>(c, f.y);
+            it.a.x = F<double>(c, f.x);
+            it.a.tab = F<double>(c, c->cs_tab);
+            v.push_back(it);
+        }
+        if (!v.empty()) out.push_back(add_table(b, L_COARSE, T_D, v, nb, pass));
+    }
 }
 
 template <typename Tf, typename Tc>
@@ -334,6 +362,7 @@ void build_vcycle(nyl_batch *b, std::vector<Launch> &out) {
     const bool skip = b->mixed != 0;   // mixed: the first level-1 sweep comes from the outer residual
     for (int l = 0; l < b->lmax; l++) {
         smooth_launches(b, l, out, skip);
+        coarse_launches(b, l, out);
         restrict_launch<double, double>(b, l, T_D, out);
         restrict_launch<float, float>(b, l, T_F, out);
         restrict_launch<float, double>(b, l, T_FD, out);
@@ -496,6 +525,9 @@ int run_launch(nyl_batch *b, const Launch &L) {
         break;
     case L_ACC:
         BCHK(launch_b_acc(b->n, b->mg, L.arg, s));
+        break;
+    case L_COARSE:
+        BCHK(launch_b_coarse((const CoarseItem *)tab, L.nitems, L.nblk, L.arg, act, s));
         break;
     }
     return NYL_OK;

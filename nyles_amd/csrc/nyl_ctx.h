@@ -22,6 +22,8 @@ struct nyl_ctx {
     int64_t ncell;
     int nlev;
     nyl::Level lev[nyl::kMaxLevels];
+    int exact;               // the coarsest level has an odd size: solved directly
+    uint64_t cs_tab;         // its eigenvector tables (byte offset)
     nyl::MGTabs tabs;
     // byte offsets of the fields
     uint64_t state[2][4];

@@ -153,6 +153,16 @@ hipError_t launch_mg_restrict25(const Geo &gf, const Geo &gc, const Tf *x, const
                                 Tc *xc, const MGTabs &tab, hipStream_t s, Tc *yc = nullptr,
                                 double omega = 0.0);
 
+// ---- direct solve of a coarsest level that cannot be halved (odd size) ------
+// x = A^-1 b with the constant mode removed, by per-axis eigenvector
+// transforms (s: scratch of the level's size; tab: coarse_tables()).
+#define CS_MAX 32
+struct CoarseArgs { Geo g; const double *b; double *s; double *x; const double *tab; };
+bool coarse_fits(const Geo &g);               // every axis <= CS_MAX cells
+int coarse_table_doubles();
+void coarse_tables(const Geo &g, double *tab);   // host: eigenvectors and eigenvalues
+hipError_t launch_coarse_solve(const CoarseArgs &a, hipStream_t s);
+
 // ---- ensemble batches -------------------------------------------------------
 // One member's share of a batched launch: the geometry of the level the
 // launch works on, the member's index in the batch, its number of blocks and
@@ -179,6 +189,8 @@ struct ProlItem { BHead h; Geo gc; Tf *xf; const Tc *xc; };
 template <typename T> struct RedItem { BHead h; const T *v; const T *b; double *partials; };
 struct OuterItem { BHead h; const double *x; const double *b; float *r; float *y; double omega;
                    double *partials; };
+// h.nblk = nz (passes 0, 2) or ny (pass 1) of the member's coarsest level
+struct CoarseItem { BHead h; CoarseArgs a; };
 
 // Device-side multigrid control of a batch: the stop rule of mgfor's solve
 // (core/mgfor/solvers.f90:17-30) evaluated per member on the device.
@@ -191,7 +203,7 @@ struct BMG {
 // one member's final reduction: partials[0..nparts) of the member
 struct FinalItem { const double *partials; int nparts; int member; double ncell; };
 #define NYL_FIN_NORMB 0   // normb = sum
-#define NYL_FIN_INIT 1    // res = sum/normb, active = !(res < tol) && maxite > 0, nite = 0
+#define NYL_FIN_INIT 1    // res = sum/normb, active = !(res < tol), nite = 0
 #define NYL_FIN_STEP 2    // nite++; active = nite < maxite && !(res = sum/normb < tol)
 #define NYL_FIN_SHIFT 3   // shift = sum/n, normb = sumsq - n shift^2 (two partial sets)
 
@@ -225,6 +237,8 @@ hipError_t launch_b_reduce(const RedItem<T> *items, int n, int64_t nblk, int kin
 hipError_t launch_b_outer_resid(const OuterItem *items, int n, int64_t nblk, const BMG &mg,
                                 const MGTabs &tab, const int *active, hipStream_t s);
 hipError_t launch_b_final(const FinalItem *items, int n, int kind, const BMG &mg, hipStream_t s);
+hipError_t launch_b_coarse(const CoarseItem *items, int n, int64_t nblk, int pass,
+                           const int *active, hipStream_t s);
 // mode 0: acc = 0; mode 1: acc += nite
 hipError_t launch_b_acc(int n, const BMG &mg, int mode, hipStream_t s);
 

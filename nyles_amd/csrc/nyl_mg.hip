@@ -722,7 +722,7 @@ hipError_t launch_publish(const double *src, int n, double *dst, uint32_t *flag,
 }
 
 // Sum of the kinetic energy field evaluated in FP64 from u, v, w with the
-// formula of core/fortran_kinenergy.f90:145-153 (as the diagnostic kernel):
+// formula of core/fortran_kinenergy.f90:47-55 (as the diagnostic kernel):
 // the energy diagnostic of both precision modes.
 template <typename T>
 __global__ __launch_bounds__(RED_T) void k_kesum(Geo g, const T *__restrict__ u,
@@ -948,7 +948,7 @@ __global__ __launch_bounds__(RED_T) void k_b_final(const FinalItem *__restrict__
         const double res = nb > 0.0 ? s1 / nb : 0.0;
         mg.res[m] = res;
         mg.nite[m] = 0;
-        mg.active[m] = (!(res < mg.tol[m]) && mg.maxite[m] > 0) ? 1 : 0;
+        mg.active[m] = !(res < mg.tol[m]) ? 1 : 0;   // mgfor runs one V-cycle even if maxite = 0
     } else {   // NYL_FIN_STEP
         const int nite = mg.nite[m] + 1;
         mg.nite[m] = nite;

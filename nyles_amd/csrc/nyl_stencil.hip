@@ -138,7 +138,7 @@ __device__ __forceinline__ T vf_flux(const T *__restrict__ ud, T cu, const T *__
     return face_flux<T, pd>(md, nd, u1d, q);
 }
 
-// Tracer flux divergence along one axis (core/fortran_upwind.f90:228-234):
+// Tracer flux divergence along one axis (core/fortran_upwind.f90:77-81):
 // d = d + flux(m-1) - flux(m), or d - flux(1) at a wall start.
 // U_d = ud*cu as in vf_flux.
 template <typename T, int pd>
@@ -230,7 +230,7 @@ __device__ __forceinline__ T vf_flux_per(const T *__restrict__ ud, T cu, const T
 }
 
 // Fused LES.rhs (core/model_les.py:129-144) + RK3_SSP stage update
-// (core/timescheme.py:422-443) for one cell.
+// (core/timescheme.py:199-220) for one cell.
 template <typename T, int MODE, int PX, int PY, int PZ>
 __device__ __forceinline__ void stage_cell(const Geo &g, const StageArgs<T> &a, int i, int j,
                                            int k) {
@@ -339,7 +339,7 @@ __device__ __forceinline__ void stage_cell(const Geo &g, const StageArgs<T> &a, 
         stb(a.un, cb, ldb(a.u, cb) + cc * ((T(8) * du - ldb(a.d0u, cb)) - ldb(a.d1u, cb)));
         stb(a.vn, cb, ldb(a.v, cb) + cc * ((T(8) * dv - ldb(a.d0v, cb)) - ldb(a.d1v, cb)));
         stb(a.wn, cb, ldb(a.w, cb) + cc * ((T(8) * dw - ldb(a.d0w, cb)) - ldb(a.d1w, cb)));
-    } else if (MODE == 4) {     // LFAM3 predictor (core/timescheme.py:367-385)
+    } else if (MODE == 4) {     // LFAM3 predictor (core/timescheme.py:144-162)
         // s = sb + (2.*dt)*ds; s = (1./12.)*(5.*s+8.*sn-sb); sb = sn
         const T c2 = T(a.c2), c12 = T(1. / 12.);
         T sb, t1;
@@ -347,7 +347,7 @@ __device__ __forceinline__ void stage_cell(const Geo &g, const StageArgs<T> &a, 
         sb = ldb(a.sbu, cb); t1 = sb + c2 * du; stb(a.un, cb, c12 * (T(5) * t1 + T(8) * ldb(a.u, cb) - sb)); stb(a.sbu, cb, ldb(a.u, cb));
         sb = ldb(a.sbv, cb); t1 = sb + c2 * dv; stb(a.vn, cb, c12 * (T(5) * t1 + T(8) * ldb(a.v, cb) - sb)); stb(a.sbv, cb, ldb(a.v, cb));
         sb = ldb(a.sbw, cb); t1 = sb + c2 * dw; stb(a.wn, cb, c12 * (T(5) * t1 + T(8) * ldb(a.w, cb) - sb)); stb(a.sbw, cb, ldb(a.w, cb));
-    } else if (MODE == 5) {     // LFAM3 corrector: s = sn + dt*ds (core/timescheme.py:393-398)
+    } else if (MODE == 5) {     // LFAM3 corrector: s = sn + dt*ds (core/timescheme.py:170-175)
         stb(a.bn, cb, ldb(a.bn, cb) + cc * db);
         stb(a.un, cb, ldb(a.un, cb) + cc * du);
         stb(a.vn, cb, ldb(a.vn, cb) + cc * dv);
@@ -610,7 +610,7 @@ __global__ __launch_bounds__(256) void k_correct4(Geo g, CorrectArgs<T, TX> a) {
 
 // U_from_u (core/cov_to_contra.py:15-20), vorticity (core/vorticity.py:25-34,
 // core/fortran_vorticity.f90:2-28) and kinenergy (core/kinenergy.py:13-24,
-// core/fortran_kinenergy.f90:145-153) from the covariant velocity.
+// core/fortran_kinenergy.f90:47-55) from the covariant velocity.
 template <typename T>
 __device__ __forceinline__ void diag_cell(const Geo &g, const DiagArgs<T> &a, int what, int i,
                                           int j, int k) {

@@ -54,6 +54,8 @@ class Job:
     vcycles: int = 0
     result: dict = field(default_factory=dict)
     model: object = None
+    failed: str = ""
+    bypassed: int = 0
 
     @property
     def latency(self):
@@ -132,6 +134,55 @@ def final_result(model, keep_state):
     return r
 
 
+class Admission:
+    """Which arrived jobs to admit between two batched steps (host logic).
+
+    Jobs are considered in arrival order.  A job is admitted when a slot is
+    free, the resident jobs' bytes plus its own stay within the budget and
+    it fits in the device's free memory.  A job that does not fit is passed
+    over by later, smaller jobs at most ``max_bypass`` times; after that no
+    later job is admitted before it (the memory that frees up is kept for
+    it).  A job that can never fit (more bytes than the budget, or more than
+    the free memory while none of the server's jobs is resident) is
+    rejected instead of blocking the queue.
+    """
+
+    def __init__(self, budget, max_slots, max_bypass=4):
+        self.budget = int(budget)
+        self.max_slots = int(max_slots)
+        self.max_bypass = int(max_bypass)
+
+    def select(self, arrived, n_active, used, free, need):
+        """arrived: jobs that have arrived and wait, in arrival order;
+        n_active: resident jobs; used: bytes they hold; free: free device
+        bytes; need(job): bytes the job would hold.
+        Returns (admit, reject): lists of jobs, in arrival order."""
+        admit, reject, waiting = [], [], []
+        for j in arrived:
+            nb = need(j)
+            if nb > self.budget:
+                j.failed = "needs %d bytes, above the HBM budget of %d" % (nb, self.budget)
+                reject.append(j)
+                continue
+            if nb > free and n_active == 0 and not admit:
+                j.failed = "needs %d bytes, above the free device memory (%d) " \
+                           "with no job resident" % (nb, free)
+                reject.append(j)
+                continue
+            if (n_active + len(admit) < self.max_slots and used + nb <= self.budget
+                    and nb <= free):
+                admit.append(j)
+                used += nb
+                free -= nb
+                for w in waiting:
+                    w.bypassed += 1
+                continue
+            waiting.append(j)
+            if j.bypassed >= self.max_bypass or n_active + len(admit) >= self.max_slots:
+                break
+        return admit, reject
+
+
 class Server:
     """Continuous-batching request loop on one GPU.
 
@@ -158,12 +209,20 @@ class Server:
         self.keep_state = keep_state
         props = torch.cuda.get_device_properties(0)
         self.budget = int(hbm_fraction * props.total_memory)
+        self.max_bypass = 4
 
     def run(self, jobs, log=None):
         for q, j in enumerate(jobs):
             j.id = q
         pending = sorted(jobs, key=lambda j: j.arrival)
-        active, done = [], []
+        active, done, rejected = [], [], []
+        policy = Admission(self.budget, self.max_slots, self.max_bypass)
+        needs = {}
+
+        def need(j):
+            if j.id not in needs:
+                needs[j.id] = workspace_bytes(j, self.precision)
+            return needs[j.id]
         used = 0
         batch = None
         dirty = False
@@ -174,20 +233,27 @@ class Server:
         t0 = time.perf_counter()
         while pending or active:
             now = time.perf_counter() - t0
-            while pending and pending[0].arrival <= now and len(active) < self.max_slots:
-                j = pending[0]
-                need = workspace_bytes(j, self.precision)
+            arrived = [j for j in pending if j.arrival <= now]
+            admit, reject = [], []
+            if arrived:
                 free, _ = torch.cuda.mem_get_info()
                 free += torch.cuda.memory_reserved() - torch.cuda.memory_allocated()
-                if used + need > self.budget or need > free:
-                    break                   # wait for resident jobs to finish
-                pending.pop(0)
+                admit, reject = policy.select(arrived, len(active), used, free, need)
+            taken = set(id(j) for j in admit + reject)
+            if taken:
+                pending = [j for j in pending if id(j) not in taken]
+            for j in reject:
+                j.done_at = now
+                rejected.append(j)
+                if log:
+                    log("job %d %s n=%d rejected: %s" % (j.id, j.kind, j.n, j.failed))
+            for j in admit:
                 j.model, j.dt = make_model(j, self.precision)
-                j.ws_bytes = need
+                j.ws_bytes = need(j)
                 nx, ny, nz, _, _ = job_grid(j)
                 j.cells = nx * ny * nz
                 j.admitted_at = time.perf_counter() - t0
-                used += need
+                used += j.ws_bytes
                 peak = max(peak, used)
                 active.append(j)
                 dirty = True
@@ -238,9 +304,10 @@ class Server:
         if batch is not None:
             batch.close()
         elapsed = time.perf_counter() - t0
-        lat = np.array([j.latency for j in done])
+        lat = np.array([j.latency for j in done]) if done else np.array([math.nan])
         return {
-            "jobs": len(done), "precision": self.precision, "elapsed_s": elapsed,
+            "jobs": len(done), "rejected": len(rejected), "precision": self.precision,
+            "elapsed_s": elapsed,
             "p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
             "mean_s": float(lat.mean()),
             "Mcell_steps_per_s": cellsteps / elapsed / 1e6,

@@ -134,7 +134,7 @@ void orc_flux1d(const double *u, const double *q, double *flux, int n)
 
 #define AT(p, a, b, c) (p)[(i64)(a) * s0 + (i64)(b) * s1 + (i64)(c) * s2]
 
-/* core/fortran_upwind.f90:156-240 (WENO branch :221-235).
+/* core/fortran_upwind.f90:3-87 (WENO branch :221-235).
  * trac,u,dtrac: dims (l,m,n); lines run along the third index. */
 void orc_upwind(const double *trac, const double *u, double *dtrac,
                 int l, int m, int n, i64 s0, i64 s1, i64 s2)
@@ -268,7 +268,7 @@ void orc_div(double *d, const double *u, int iflag, int l, int m, int n, i64 s0,
         }
 }
 
-/* core/fortran_kinenergy.f90:101-154: ke += (cff2*0.5)*(u(i)**2+u(i-1)**2), i>=2 */
+/* core/fortran_kinenergy.f90:3-56: ke += (cff2*0.5)*(u(i)**2+u(i-1)**2), i>=2 */
 void orc_kin(const double *u, double *ke, double ds2, int l, int m, int n, i64 s0, i64 s1, i64 s2)
 {
     const double cff2 = 0.5 * ds2;

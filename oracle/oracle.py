@@ -9,7 +9,7 @@ of the Fortran kernels (``nyles_oracle.c``):
 
 * ``OracleLES.rhs``            core/model_les.py:129-144
 * ``OracleLES.diagnose_var``   core/model_les.py:97-127
-* ``OracleLES.rk3_ssp``        core/timescheme.py:403-444
+* ``OracleLES.rk3_ssp``        core/timescheme.py:180-221
 * ``OracleLES.compute_dt``     core/nyles.py:227-260
 * ``halo_fill``                core/mpi/halo.py:123-188 (one rank)
 * ``OracleMG.solve_directly``  core/mgfordriver.py:66-78
@@ -136,7 +136,7 @@ def flux1d(u, q):
 
 
 def upwind(trac, u, dtrac):
-    """core/fortran_upwind.f90:156 upwind(trac, u, dtrac, order)."""
+    """core/fortran_upwind.f90:3 upwind(trac, u, dtrac, order)."""
     _same_layout(trac, u, dtrac)
     lib().orc_upwind(_vptr(trac), _vptr(u), _vptr(dtrac), *_view_args(dtrac))
 
@@ -183,7 +183,7 @@ def div(d, u, iflag):
 
 
 def kin(u, ke, ds2):
-    """core/fortran_kinenergy.f90:101 kin(u, v, ke, ds2, order) (v unused)."""
+    """core/fortran_kinenergy.f90:3 kin(u, v, ke, ds2, order) (v unused)."""
     _same_layout(u, ke)
     lib().orc_kin(_vptr(u), _vptr(ke), float(ds2), *_view_args(ke))
 
@@ -457,7 +457,7 @@ class OracleLES:
             self.fill(self.vor[d])
         self.fill(self.ke)
 
-    # -- core/timescheme.py:403-444 ----------------------------------------
+    # -- core/timescheme.py:180-221 ----------------------------------------
     def _pairs(self, *ds):
         out = []
         for name in self.PROGNOSTIC:
@@ -479,7 +479,7 @@ class OracleLES:
             s += (dt / 12.) * (8 * d2 - d0 - d1)
         self.diagnose_var()
 
-    # -- core/timescheme.py:336-345 ----------------------------------------
+    # -- core/timescheme.py:113-122 ----------------------------------------
     def euler_forward(self, dt):
         ds = self.ds[0]
         self.rhs(ds, last=True)
@@ -487,7 +487,7 @@ class OracleLES:
             s += dt * d
         self.diagnose_var()
 
-    # -- core/timescheme.py:348-400 ----------------------------------------
+    # -- core/timescheme.py:125-177 ----------------------------------------
     def lfam3(self, dt):
         if not hasattr(self, "lf_first"):
             self.lf_first = True

@@ -1,0 +1,155 @@
+"""Multigrid on grids that do not halve down to 2 (GPU): the ensemble sizes
+40..96 (steps of 8) and the served sizes up to 256, whose hierarchy stops at
+an odd coarsest level that is solved directly.
+
+The reference has no rule for these sizes (its restriction drops cells at an
+odd level), so there is no oracle: the solution is checked against a host
+spectral Poisson solve (scipy.fft: DCT-II on wall axes, FFT on periodic
+ones) of the same 7-point operator, to the solver's tolerance.  Tolerances:
+|r|/|b| <= 1e-10 gives the solution to 1e-8 relative (L2, constant mode
+removed); at mgfor's tolerance (squared ratio 1e-6) the V-cycle count stays
+that of a power-of-two grid of similar size (within 3).
+"""
+import numpy as np
+import pytest
+import scipy.fft as sfft
+
+pytestmark = pytest.mark.gpu
+
+PERIODIC = {"closed": (0, 0, 0), "perio_x": (1, 0, 0), "perio_xy": (1, 1, 0),
+            "perio_xyz": (1, 1, 1)}
+
+
+def spectral_solve(b, periodic_xyz):
+    """x with sum(neighbours) - diag*x = b - mean(b), zero mean"""
+    per = periodic_xyz[::-1]                 # array axes are (z, y, x)
+    bh = b.astype(np.complex128)
+    lam = np.zeros(b.shape)
+    for ax, p in enumerate(per):
+        n = b.shape[ax]
+        k = np.arange(n)
+        sh = [1, 1, 1]
+        sh[ax] = n
+        if p:
+            bh = sfft.fft(bh, axis=ax)
+            lam = lam + (2 * np.cos(2 * np.pi * k / n) - 2).reshape(sh)
+        else:
+            bh = sfft.dct(bh.real, type=2, norm="ortho", axis=ax) + \
+                1j * sfft.dct(bh.imag, type=2, norm="ortho", axis=ax)
+            lam = lam + (2 * np.cos(np.pi * k / n) - 2).reshape(sh)
+    lam[0, 0, 0] = 1.0
+    xh = bh / lam
+    xh[0, 0, 0] = 0.0
+    for ax in reversed(range(3)):
+        if per[ax]:
+            xh = sfft.ifft(xh, axis=ax)
+        else:
+            xh = sfft.idct(xh.real, type=2, norm="ortho", axis=ax) + \
+                1j * sfft.idct(xh.imag, type=2, norm="ortho", axis=ax)
+    return xh.real
+
+
+def _model(n3, geometry, precision, tol=1e-6, maxite=20):
+    from nyles_amd.device import DeviceModel
+    nx, ny, nz = n3
+    return DeviceModel(nx, ny, nz, PERIODIC[geometry], 1.0 / nx, 1.0 / nx, 1.0 / nx,
+                       precision=precision, mg_tol=tol, mg_maxite=maxite)
+
+
+def _levels(dev):
+    import ctypes
+    n = ctypes.c_int32()
+    dev.lib.nyl_mg_levels(dev.ctx, ctypes.byref(n))
+    out = []
+    for l in range(n.value):
+        a, b, c = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        dev.lib.nyl_mg_level_shape(dev.ctx, l, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c))
+        out.append((a.value, b.value, c.value))
+    return out
+
+
+CASES = [((40, 40, 40), "perio_xy"), ((48, 48, 48), "perio_xy"), ((56, 56, 56), "perio_xy"),
+         ((72, 72, 72), "perio_xy"), ((80, 80, 80), "perio_xy"), ((88, 88, 88), "perio_xy"),
+         ((96, 96, 96), "perio_xy"), ((88, 44, 44), "perio_x"), ((56, 56, 56), "closed"),
+         ((72, 72, 72), "perio_xyz"), ((136, 136, 136), "perio_xyz"),
+         ((248, 248, 248), "perio_xy")]
+
+
+@pytest.mark.parametrize("n3,geometry", CASES, ids=["%dx%dx%d-%s" % (c[0] + (c[1],)) for c in CASES])
+@pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "mixed"])
+def test_solution_matches_spectral_solve(n3, geometry, precision):
+    from nyles_amd import _lib as L
+    print("case mg sizes", n3, geometry, precision, flush=True)
+    nx, ny, nz = n3
+    dev = _model(n3, geometry, precision, tol=1e-20, maxite=60)
+    lev = _levels(dev)
+    assert lev[-1][0] % 2 == 1 or lev[-1][1] % 2 == 1 or lev[-1][2] % 2 == 1, lev
+    rng = np.random.default_rng(nx + ny + nz)
+    b = rng.standard_normal((nz, ny, nx))
+    b -= b.mean()
+    dev.upload(L.NYL_F_MGB, b)
+    st = dev.mg_solve()
+    x = dev.download(L.NYL_F_MGX)
+    xe = spectral_solve(b, PERIODIC[geometry])
+    err = np.linalg.norm((x - x.mean()) - xe) / np.linalg.norm(xe)
+    print("  levels", lev, "nite", st["nite"], "res %.2e" % st["res"], "x err %.2e" % err,
+          flush=True)
+    assert st["res"] < 1e-20
+    assert err < 1e-8
+    dev.close()
+
+
+@pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "mixed"])
+def test_cycle_count_at_mgfor_tolerance(precision):
+    """At mgfor's tolerance the odd-coarsest hierarchies converge in about as
+    many V-cycles as the power-of-two grids around them."""
+    from nyles_amd import _lib as L
+    print("case mg sizes cycle count", precision, flush=True)
+    counts = {}
+    for n in (32, 40, 48, 56, 64, 72, 80, 88, 96, 128, 136):
+        dev = _model((n, n, n), "perio_xy", precision)
+        rng = np.random.default_rng(n)
+        b = rng.standard_normal((n, n, n))
+        b -= b.mean()
+        dev.upload(L.NYL_F_MGB, b)
+        st = dev.mg_solve()
+        assert st["res"] < 1e-6, (n, st)
+        counts[n] = st["nite"]
+        dev.close()
+    print("  V-cycles", counts, flush=True)
+    pow2 = max(counts[32], counts[64], counts[128])
+    for n, c in counts.items():
+        assert c <= pow2 + 3, (n, counts)
+
+
+def test_odd_size_steps_mixed_close_to_fp64():
+    """Shear-layer members of ensemble sizes: 20 RK3 steps in mixed precision
+    stay within 1e-5 relative of the FP64 path (KE, b and u in L2)."""
+    from nyles_amd import _lib as L
+    from nyles_amd import workloads
+    print("case odd-size steps", flush=True)
+    for n in (40, 88):
+        st, meta = workloads.shear_layer(n, seed=1000 + n)
+        devs = []
+        for prec in (L.NYL_PREC_FP64, L.NYL_PREC_MIXED):
+            d = _model((n, n, n), "perio_xy", prec)
+            for k, fid in (("b", L.NYL_F_B), ("u_i", L.NYL_F_U), ("u_j", L.NYL_F_V),
+                           ("u_k", L.NYL_F_W)):
+                d.upload(fid, st[k])
+            d.diagnose()
+            devs.append(d)
+        dt = 0.5 / float(np.sqrt(devs[0].reduce(L.NYL_RED_MAXU2)))
+        for _ in range(20):
+            s64 = devs[0].step_rk3(dt)
+            sm = devs[1].step_rk3(dt)
+            assert s64["res"] < 1e-6 and sm["res"] < 1e-6
+        ke64, kem = devs[0].reduce(L.NYL_RED_KE), devs[1].reduce(L.NYL_RED_KE)
+        errs = {"ke": abs(kem - ke64) / abs(ke64)}
+        for k, fid in (("b", L.NYL_F_B), ("u", L.NYL_F_U), ("w", L.NYL_F_W)):
+            a, r = devs[1].download(fid), devs[0].download(fid)
+            errs[k] = np.linalg.norm(a - r) / np.linalg.norm(r)
+        print("  n", n, {k: "%.2e" % v for k, v in errs.items()}, flush=True)
+        for k, v in errs.items():
+            assert v < 1e-5, (n, k, v)
+        for d in devs:
+            d.close()

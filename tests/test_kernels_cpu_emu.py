@@ -19,7 +19,7 @@ EMU = os.path.join(HERE, "cpu_emu")
 INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 
 
-@pytest.mark.parametrize("name", ["cell_emu", "mg_emu"])
+@pytest.mark.parametrize("name", ["cell_emu", "mg_emu", "coarse_emu"])
 def test_kernels_match_per_cell_forms_bitwise(name, tmp_path):
     gxx = shutil.which("g++")
     if gxx is None:

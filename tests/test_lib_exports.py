@@ -42,26 +42,80 @@ def test_descriptor_layout_matches_header():
     assert ctypes.sizeof(_lib.NylMGStats) == 24
 
 
+def _desc(n3, periodic=(1, 1, 0), precision=0):
+    from nyles_amd import _lib
+    d = _lib.NylDesc()
+    d.nx, d.ny, d.nz = n3
+    d.periodic_x, d.periodic_y, d.periodic_z = periodic
+    d.dx = d.dy = d.dz = 1.0
+    d.precision = precision
+    d.mg_maxite, d.mg_tol, d.mg_omega = 20, 1e-6, 0.9
+    return d
+
+
+def _levels(L, ctx):
+    n = ctypes.c_int32()
+    assert L.nyl_mg_levels(ctx, ctypes.byref(n)) == 0
+    out = []
+    for l in range(n.value):
+        a, b, c = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        assert L.nyl_mg_level_shape(ctx, l, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)) == 0
+        out.append((a.value, b.value, c.value))
+    return out
+
+
 def test_context_rejects_bad_sizes_without_gpu():
     """Argument checks happen before any HIP call."""
     from nyles_amd import _lib
     L = _lib.load()
-    d = _lib.NylDesc()
-    d.nx, d.ny, d.nz = 48, 48, 48          # never halves down to 2
-    d.dx = d.dy = d.dz = 1.0
-    d.mg_maxite, d.mg_tol, d.mg_omega = 20, 1e-6, 0.9
     ctx = ctypes.c_void_p()
-    rc = L.nyl_ctx_create(ctypes.byref(d), ctypes.byref(ctx))
-    assert rc == -4 and b"halve" in L.nyl_last_error()
-    d.nx = d.ny = d.nz = 4
+    d = _desc((4, 4, 4))
+    assert L.nyl_ctx_create(ctypes.byref(d), ctypes.byref(ctx)) == -4     # fewer than 8 cells
+    d = _desc((40, 40, 41))                                                # no halving possible
     assert L.nyl_ctx_create(ctypes.byref(d), ctypes.byref(ctx)) == -4
-    d.nx = d.ny = d.nz = 32
+    assert b"at least one halving" in L.nyl_last_error()
+    d = _desc((1024, 1024, 24))            # coarsest 128x128x3: odd, too wide to solve directly
+    assert L.nyl_ctx_create(ctypes.byref(d), ctypes.byref(ctx)) == -4
+    assert b"direct coarse solve" in L.nyl_last_error()
+    d = _desc((32, 32, 32))
     rc = L.nyl_ctx_create(ctypes.byref(d), ctypes.byref(ctx))
     assert rc == 0
     nb = ctypes.c_uint64()
     assert L.nyl_workspace_bytes(ctx, ctypes.byref(nb)) == 0 and nb.value > 0
     # no workspace bound: compute entry points refuse
     assert L.nyl_rhs(ctx, 0, 0) == -2
+    assert L.nyl_ctx_destroy(ctx) == 0
+
+
+def test_hierarchy_of_grids_that_do_not_halve_to_2():
+    """The reference's hierarchy (halve x, y and z until x or y is 2,
+    core/mgfor/mg_setup.f90:262-303) where every level halves evenly; a grid
+    whose halving reaches an odd size stops there (the level is solved
+    directly), e.g. the ensemble sizes 40..96 in steps of 8 and the served
+    sizes up to 256, in both precisions."""
+    from nyles_amd import _lib
+    L = _lib.load()
+    ctx = ctypes.c_void_p()
+    want = {64: [(64, 64, 64), (32, 32, 32), (16, 16, 16), (8, 8, 8), (4, 4, 4), (2, 2, 2)],
+            48: [(48, 48, 48), (24, 24, 24), (12, 12, 12), (6, 6, 6), (3, 3, 3)],
+            40: [(40, 40, 40), (20, 20, 20), (10, 10, 10), (5, 5, 5)],
+            88: [(88, 88, 88), (44, 44, 44), (22, 22, 22), (11, 11, 11)],
+            248: [(248, 248, 248), (124, 124, 124), (62, 62, 62), (31, 31, 31)]}
+    for n, lev in want.items():
+        for prec in (0, 1):
+            d = _desc((n, n, n), precision=prec)
+            assert L.nyl_ctx_create(ctypes.byref(d), ctypes.byref(ctx)) == 0, L.nyl_last_error()
+            assert _levels(L, ctx) == lev
+            assert L.nyl_ctx_destroy(ctx) == 0
+    for n in range(32, 257, 8):
+        for prec in (0, 1):
+            d = _desc((n, n, n), precision=prec)
+            assert L.nyl_ctx_create(ctypes.byref(d), ctypes.byref(ctx)) == 0, (n, L.nyl_last_error())
+            assert L.nyl_ctx_destroy(ctx) == 0
+    # the lock-exchange shape keeps the reference's hierarchy down to y = 2
+    d = _desc((64, 32, 32), periodic=(1, 0, 0))
+    assert L.nyl_ctx_create(ctypes.byref(d), ctypes.byref(ctx)) == 0
+    assert _levels(L, ctx) == [(64, 32, 32), (32, 16, 16), (16, 8, 8), (8, 4, 4), (4, 2, 2)]
     assert L.nyl_ctx_destroy(ctx) == 0
 
 

@@ -204,3 +204,129 @@ def test_uniform_tracer_in_divergence_free_flow(oracle_mod):
     ds = m.ds[0]
     m.rhs(ds)
     assert np.max(np.abs(m.interior(ds["b"]))) < 1e-6 * np.max(np.abs(Ux))
+
+
+def _flux_line(oracle_mod, UU1, q_shifted):
+    """flux1d of a line whose face velocity UU1 is constant along the line:
+    the velocity the vortex force upwinds with is the staggered average
+    0.5*(UU_{i-1} + UU_i) with UU_0 = 0, i.e. UU1/2 at the first point and
+    UU1 elsewhere."""
+    n = len(q_shifted)
+    u1d = np.full(n, 0.5 * (UU1 + UU1))
+    u1d[0] = 0.5 * (0.0 + UU1)
+    return oracle_mod.flux1d(u1d, q_shifted)
+
+
+def _only(m, U=None, vor=None):
+    for d in "ijk":
+        m.U[d][:] = 0.0
+        m.vor[d][:] = 0.0
+    for d, a in (U or {}).items():
+        m.U[d][:] = a
+    for d, a in (vor or {}).items():
+        m.vor[d][:] = a
+    m.ke[:] = 0.0
+    m.b[:] = 0.0
+
+
+def test_vortex_force_flip_term_with_sheared_flow(oracle_mod):
+    """core/fortran_vortex_force.f90:145-159 through the 'kji' triplet of
+    core/vortex_force.py:69-81: with U^x = f(z) only and vor_j (omega_y) only,
+    du_k += flux1d along x of U^x averaged in z onto the omega point,
+    upwinding q(i) = omega_y(i-1), q(1) = 0; the last z face is untouched and
+    du_i = du_j = 0.  A wrong averaging axis (x instead of z) or an unshifted
+    q(i) = omega(i) changes du_k on most cells."""
+    m = model(oracle_mod, "closed", n=(12, 10, 14), L=(1.0, 1.0, 1.0))
+    rng = np.random.default_rng(61)
+    nz, ny, nx = m.shape
+    f = rng.uniform(-1.0, 1.0, nz)
+    f[3] = 0.0                                   # exact zero: the u > 0 split
+    om = rng.standard_normal(m.shape)
+    _only(m, U={"i": f[:, None, None] * np.ones(m.shape)}, vor={"j": om})
+    ds = m.ds[0]
+    m.rhs(ds)
+    assert np.all(ds["u"]["i"] == 0.0) and np.all(ds["u"]["j"] == 0.0)
+    got = ds["u"]["k"]
+    for k in range(nz - 1):
+        UU1 = 0.5 * (f[k] + f[k + 1])
+        for j in range(ny):
+            q = np.concatenate(([0.0], om[k, j, :-1]))
+            np.testing.assert_array_equal(got[k, j, :], 0.0 + _flux_line(oracle_mod, UU1, q))
+    assert np.all(got[nz - 1] == 0.0)
+    # the wrong axis or the unshifted q would differ
+    k, j = 2, 4
+    wrong_axis = _flux_line(oracle_mod, f[k], np.concatenate(([0.0], om[k, j, :-1])))
+    unshifted = _flux_line(oracle_mod, 0.5 * (f[k] + f[k + 1]), om[k, j, :])
+    assert np.count_nonzero(wrong_axis != got[k, j]) > nx // 2
+    assert np.count_nonzero(unshifted != got[k, j]) > nx // 2
+
+
+def test_vortex_force_flip_term_uniform_flow_periodic(oracle_mod):
+    """'ikj' flip term on a periodic y line: uniform U^y = U0 > 0 with
+    omega_z only gives du_x = +U0 * WENO5(omega_z(j-3 .. j+1)) at every
+    interior cell, and du_y = du_z = 0."""
+    m = model(oracle_mod, "perio_xyz", n=(8, 16, 8), L=(1.0, 1.0, 1.0))
+    rng = np.random.default_rng(62)
+    U0 = 0.61
+    w = rng.standard_normal(m.shape)
+    _only(m, U={"j": U0}, vor={"k": w})
+    ds = m.ds[0]
+    m.rhs(ds)
+    assert np.all(ds["u"]["j"] == 0.0) and np.all(ds["u"]["k"] == 0.0)
+    nh = m.nh
+    k, i = nh + 1, nh + 5
+    for j in range(nh, nh + m.ny):
+        expect = 0.0 + U0 * oracle_mod.weno5(*w[k, j - 3:j + 2, i])
+        assert ds["u"]["i"][k, j, i] == expect
+
+
+def test_vortex_force_direc_term_with_sheared_flow(oracle_mod):
+    """core/fortran_vortex_force.f90:66-81 through the 'ikj' triplet: with
+    U^x = f(y) only and omega_z only, du_j -= flux1d along x of U^x averaged
+    in y onto the omega point; the last y face is untouched; du_i = du_k = 0."""
+    m = model(oracle_mod, "closed", n=(14, 10, 8), L=(1.0, 1.0, 1.0))
+    rng = np.random.default_rng(63)
+    nz, ny, nx = m.shape
+    f = rng.uniform(-1.0, 1.0, ny)
+    om = rng.standard_normal(m.shape)
+    _only(m, U={"i": f[None, :, None] * np.ones(m.shape)}, vor={"k": om})
+    ds = m.ds[0]
+    m.rhs(ds)
+    assert np.all(ds["u"]["i"] == 0.0) and np.all(ds["u"]["k"] == 0.0)
+    got = ds["u"]["j"]
+    for j in range(ny - 1):
+        UU1 = 0.5 * (f[j] + f[j + 1])
+        for k in range(nz):
+            q = np.concatenate(([0.0], om[k, j, :-1]))
+            np.testing.assert_array_equal(got[k, j, :], 0.0 - _flux_line(oracle_mod, UU1, q))
+    assert np.all(got[:, ny - 1, :] == 0.0)
+
+
+@pytest.mark.parametrize("geometry", ["closed", "perio_x", "perio_y", "perio_xy", "perio_xyz"])
+def test_halo_fill_is_periodic_image(oracle_mod, geometry):
+    """core/mpi/halo.py:123-188 on one rank: after a fill every halo cell
+    equals its periodic image (index wrapped into the interior), the
+    interior is unchanged, and wall axes carry no halo
+    (core/mpi/topology.py:253-304)."""
+    nh = 3
+    m = model(oracle_mod, geometry, n=(10, 9, 8))
+    perio = oracle_mod.GEOMETRY_PERIODIC[geometry]
+    for a, (n, p) in enumerate(zip((m.nz, m.ny, m.nx), perio)):
+        assert m.shape[a] == (n + 2 * nh if p else n)
+    rng = np.random.default_rng(64)
+    x = rng.standard_normal(m.shape)
+    before = x.copy()
+    oracle_mod.halo_fill(x, geometry, nh)
+    idx = []
+    inner = []
+    for a, (n, p) in enumerate(zip((m.nz, m.ny, m.nx), perio)):
+        if p:
+            h = np.arange(n + 2 * nh)
+            idx.append(nh + (h - nh) % n)
+            inner.append(slice(nh, nh + n))
+        else:
+            idx.append(np.arange(n))
+            inner.append(slice(0, n))
+    image = before[np.ix_(*idx)]
+    np.testing.assert_array_equal(x, image)
+    np.testing.assert_array_equal(x[tuple(inner)], before[tuple(inner)])

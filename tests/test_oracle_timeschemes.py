@@ -1,6 +1,6 @@
 """The oracle's time schemes on a linear test equation ds/dt = lam*s (CPU):
 one step must multiply s by the scheme's amplification factor
-(core/timescheme.py:336-444)."""
+(core/timescheme.py:113-221)."""
 import numpy as np
 import pytest
 
