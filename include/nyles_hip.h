@@ -197,6 +197,21 @@ int nyl_apply(nyl_ctx *ctx, int op);
 /* Reductions (synchronous). */
 int nyl_reduce(nyl_ctx *ctx, int what, double *out);
 
+/* nyl_reduce on caller-owned fields, with no context, no workspace and no
+ * device allocation.  No counterpart in the reference: it is for callers
+ * that cannot allocate device memory (code under hipGraph capture, tight
+ * serving loops).  desc gives the grid (nx, ny, nz) and idx2, idy2, idz2;
+ * b, u, v, w are device fields of nx*ny*nz elements of dtype NYL_DT_F64 or
+ * NYL_DT_F32 (b only for NYL_RED_BSUM, u, v, w only for NYL_RED_MAXU2 and
+ * NYL_RED_KE); the result is written to dev_out[0] (device memory) on
+ * `stream`, asynchronously.  The partial sums live in a static device array
+ * of 16384 doubles (128 KiB per GPU, held until the process ends),
+ * so calls that overlap on one GPU race.  A field must stay below 4 GiB
+ * (larger grids are refused with NYL_E_SIZE). */
+int nyl_reduce_noalloc(const nyl_desc *desc, int32_t dtype, const void *b, const void *u,
+                       const void *v, const void *w, int32_t what, double *dev_out,
+                       void *stream);
+
 /* Kernel-class timing with HIP events on the context stream: every launch
  * of the class between start and stop is bracketed by two events; stop
  * synchronises and returns the summed kernel time, the launch count and the

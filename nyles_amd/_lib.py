@@ -48,6 +48,7 @@ EXPORTED = [
     "nyl_batch_create", "nyl_batch_destroy", "nyl_batch_workspace_bytes",
     "nyl_batch_bind_workspace", "nyl_batch_set_stream", "nyl_batch_set_graphs",
     "nyl_batch_autotune", "nyl_batch_set_tile", "nyl_batch_step_rk3",
+    "nyl_reduce_noalloc",
 ]
 
 # kernel classes of nyl_prof_start
@@ -134,6 +135,8 @@ def load():
         "nyl_batch_autotune": (ci, [vp, P(ctypes.c_int32), P(dbl)]),
         "nyl_batch_set_tile": (ci, [vp, ctypes.c_int32]),
         "nyl_batch_step_rk3": (ci, [vp, P(dbl), P(NylMGStats)]),
+        "nyl_reduce_noalloc": (ci, [P(NylDesc), ctypes.c_int32, vp, vp, vp, vp, ctypes.c_int32,
+                                    vp, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
