@@ -56,15 +56,22 @@ def parse():
     ap.add_argument("--err-steps", type=int, default=200,
                     help="steps of the mixed-vs-FP64 error check (0: skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=2,
+                    help="RK3 steps of the CPU baseline on all host cores")
+    ap.add_argument("--cpu-steps-1thread", type=int, default=1,
+                    help="RK3 steps of the CPU baseline on one thread (0: skip)")
     ap.add_argument("--breakdown", action="store_true",
                     help="report the per-class kernel times of both paths")
     ap.add_argument("--ensemble-members", type=int, default=64,
                     help="members of the ensemble run (0: skip)")
-    ap.add_argument("--serving-jobs", type=int, default=40,
-                    help="jobs of the serving run (0: skip)")
-    ap.add_argument("--serving-rate", type=float, default=10.0, help="job arrivals per second")
-    ap.add_argument("--rt-steps", type=int, default=30,
+    ap.add_argument("--ensemble-steps", type=int, default=300,
+                    help="batched steps of the ensemble run")
+    ap.add_argument("--ensemble-seq-steps", type=int, default=10,
+                    help="steps of the one-member-after-another reference point")
+    ap.add_argument("--serving-jobs", type=int, default=48,
+                    help="jobs of the serving run (the first arrivals of the stream; 0: skip)")
+    ap.add_argument("--serving-rate", type=float, default=1.0, help="job arrivals per second")
+    ap.add_argument("--rt-steps", type=int, default=200,
                     help="steps of the Rayleigh-Taylor 256x256x512 run at |r|/|b| <= 1e-10 "
                          "(0: skip)")
     ap.add_argument("--single", action="store_true",
@@ -87,61 +94,100 @@ def copy_calibration(mb):
                       "launches": 5}), flush=True)
 
 
-def ensemble_run(members, steps, precision="mixed", seed=1000):
-    """Ensemble of stratified shear layers (perio_xy) of uneven sizes: members
-    stepped one after another through the single-model API, then the same
-    ensemble advanced by batched steps (one set of launches per RK stage)."""
+def c4_sizes(members, seed):
+    """N log-uniform in [32, 96], rounded to a multiple of 8"""
     import numpy as np
+    rng = np.random.default_rng(seed)
+    return [int(8 * round(v / 8)) for v in np.exp(rng.uniform(np.log(32), np.log(96), members))]
+
+
+def ensemble_run(members, steps, seq_steps, precision="mixed", seed=1000):
+    """Ensemble of stratified shear layers (perio_xy) with N^3 cells, N
+    log-uniform in [32, 96] rounded to a multiple of 8: the members advanced
+    by batched steps (one set of launches per RK stage, hipGraph replays,
+    autotuned stage tile), and the same members stepped one after another
+    through the single-model API (over seq_steps steps) as the reference
+    point."""
     import torch
     from nyles_amd import serving
     from nyles_amd.device import DeviceBatch
-    rng = np.random.default_rng(seed)
-    # N log-uniform in [32, 96] restricted to the sizes the multigrid
-    # hierarchy accepts (it halves x and y down to 2): 32 and 64
-    sizes = [int(rng.choice([32, 64])) for _ in range(members)]
+    sizes = c4_sizes(members, seed)
     jobs = [serving.Job("shear", n, steps, seed=seed + q) for q, n in enumerate(sizes)]
     cells = sum(n ** 3 for n in sizes)
-    out = {"members": members, "sizes": {"32": sizes.count(32), "64": sizes.count(64)},
-           "precision": precision, "steps": steps, "workload": "stratified shear layer "
-           "perio_xy (tanh jet, N^2 = 0.1, noise 1e-3), fixed dt at CFL 0.5"}
+    out = {"members": members,
+           "sizes": {str(n): sizes.count(n) for n in sorted(set(sizes))},
+           "cells": cells, "precision": precision, "steps": steps,
+           "sequential_steps": seq_steps,
+           "workload": "stratified shear layer perio_xy (tanh jet, N^2 = 0.1, noise 1e-3), "
+                       "N log-uniform in [32, 96] rounded to 8, fixed dt at CFL 0.5"}
     for mode in ("sequential", "batched"):
         ms = [serving.make_model(j, precision) for j in jobs]
         models = [m for m, _ in ms]
         dts = [dt for _, dt in ms]
+        nsteps = steps if mode == "batched" else seq_steps
+        nite = 0
         if mode == "batched":
             b = DeviceBatch(models)
             out["tile"], out["tune_ms"] = b.tile, b.tune_ms
             b.step_rk3(dts)
             torch.cuda.synchronize()
             t0 = time.perf_counter()
-            for _ in range(steps):
-                b.step_rk3(dts)
+            for _ in range(nsteps):
+                nite += sum(s["nite"] for s in b.step_rk3(dts))
             torch.cuda.synchronize()
             el = time.perf_counter() - t0
             b.close()
+            out["vcycles_per_member_step"] = round(nite / (nsteps * members), 3)
         else:
             for m, dt in zip(models, dts):
                 m.step_rk3(dt)
             torch.cuda.synchronize()
             t0 = time.perf_counter()
-            for _ in range(steps):
+            for _ in range(nsteps):
                 for m, dt in zip(models, dts):
                     m.step_rk3(dt)
             torch.cuda.synchronize()
             el = time.perf_counter() - t0
         for m in models:
             m.close()
-        out[mode + "_Mcell_steps_per_s"] = round(cells * steps / el / 1e6, 3)
+        out[mode + "_Mcell_steps_per_s"] = round(cells * nsteps / el / 1e6, 3)
+        out[mode + "_s"] = round(el, 3)
     out["batching_speedup"] = round(out["batched_Mcell_steps_per_s"] /
                                     out["sequential_Mcell_steps_per_s"], 3)
     return out
+
+
+def state_errors(m, a):
+    """errors of the mixed model m against the FP64 model a: KE (evaluated in
+    FP64 by both), velocity and buoyancy (relative L2)"""
+    import numpy as np
+    from nyles_amd import _lib as L
+    fm = {k: m.download(f) for k, f in (("b", L.NYL_F_B), ("u", L.NYL_F_U), ("v", L.NYL_F_V),
+                                        ("w", L.NYL_F_W))}
+    fa = {k: a.download(f) for k, f in (("b", L.NYL_F_B), ("u", L.NYL_F_U), ("v", L.NYL_F_V),
+                                        ("w", L.NYL_F_W))}
+    kem, kea = m.reduce(L.NYL_RED_KE), a.reduce(L.NYL_RED_KE)
+    du2 = sum(float(np.sum((fm[k] - fa[k]) ** 2)) for k in ("u", "v", "w"))
+    u2 = sum(float(np.sum(fa[k] ** 2)) for k in ("u", "v", "w"))
+    bn = float(np.sqrt(np.sum(fa["b"] ** 2)))
+    return {
+        "ke_fp64": kea, "ke_mixed": kem,
+        "ke_rel_err": abs(kem - kea) / abs(kea) if kea != 0 else None,
+        "u_l2_rel_err": (du2 / u2) ** 0.5 if u2 > 0 else None,
+        "b_max_abs_err": float(np.max(np.abs(fm["b"] - fa["b"]))),
+        "b_l2_rel_err": (float(np.sqrt(np.sum((fm["b"] - fa["b"]) ** 2))) / bn) if bn > 0
+        else None,
+        "nonfinite_mixed": m.reduce(L.NYL_RED_NONFINITE),
+    }
 
 
 def rt_run(steps, tol2=1e-20, maxite=50):
     """Rayleigh-Taylor 256 x 256 x 512 (perio_xy, heavy fluid on top), the
     projection converged to |r|/|b| <= 1e-10 (unsquared; tol 1e-20 on mgfor's
     squared ratio): mixed (FP32 stencils, FP64 outer defect correction) and
-    FP64, timed, and the buoyancy of the two compared after the same steps."""
+    FP64 side by side from the same state, each timed over `steps` steps (one
+    untimed step first), then the mixed state compared with FP64 after the
+    same steps."""
     import numpy as np
     import torch
     from nyles_amd import _lib as L
@@ -151,9 +197,9 @@ def rt_run(steps, tol2=1e-20, maxite=50):
     (Lx, Ly, Lz), (nz, ny, nx) = meta["L"], st["b"].shape
     dx, dy, dz = Lx / nx, Ly / ny, Lz / nz
     dt = 0.5 * dz / np.sqrt(1.0 * Lz)          # buoyancy jump 1, depth Lz
-    out = {"grid": [nx, ny, nz], "steps": steps, "dt": dt,
+    out = {"grid": [nx, ny, nz], "steps": steps + 1, "timed_steps": steps, "dt": dt,
            "tolerance": "|r|/|b| <= 1e-10 (unsquared), at most %d V-cycles" % maxite}
-    bs = {}
+    models = {}
     for prec, code in (("fp64", L.NYL_PREC_FP64), ("mixed", L.NYL_PREC_MIXED)):
         m = DeviceModel(nx, ny, nz, (1, 1, 0), dx, dy, dz, idx2=1 / dx ** 2, idy2=1 / dy ** 2,
                         idz2=1 / dz ** 2, dx2=dx ** 2, precision=code, mg_tol=tol2,
@@ -175,49 +221,92 @@ def rt_run(steps, tol2=1e-20, maxite=50):
         out[prec] = {"Mcell_steps_per_s": round(nx * ny * nz * steps / el / 1e6, 3),
                      "vcycles_per_step": round(nite / steps, 2),
                      "max_final_residual_unsquared": float(np.sqrt(res))}
-        bs[prec] = m.download(L.NYL_F_B)
-        m.close()
+        models[prec] = m
     out["speedup_mixed_vs_fp64"] = round(out["mixed"]["Mcell_steps_per_s"] /
                                          out["fp64"]["Mcell_steps_per_s"], 3)
-    out["b_l2_rel_err"] = float(np.sqrt(np.sum((bs["mixed"] - bs["fp64"]) ** 2) /
-                                        np.sum(bs["fp64"] ** 2)))
+    out["error_vs_fp64"] = state_errors(models["mixed"], models["fp64"])
+    for m in models.values():
+        m.close()
+    return out
+
+
+def lock_exchange_error(steps):
+    """Lock exchange 64 x 32 x 32 (perio_x, walls in y and z): mixed and FP64
+    advanced `steps` RK3 steps at dt = 0.5 dx / sqrt(db Lz) from the same
+    state; errors of the mixed state against FP64."""
+    import numpy as np
+    from nyles_amd import _lib as L
+    from nyles_amd import workloads
+    from nyles_amd.device import DeviceModel
+    st, meta = workloads.lock_exchange()
+    (Lx, Ly, Lz), (nz, ny, nx) = meta["L"], st["b"].shape
+    dx, dy, dz = Lx / nx, Ly / ny, Lz / nz
+    dt = 0.5 * dx / np.sqrt(1.0 * Lz)
+    models = {}
+    for prec, code in (("fp64", L.NYL_PREC_FP64), ("mixed", L.NYL_PREC_MIXED)):
+        m = DeviceModel(nx, ny, nz, (1, 0, 0), dx, dy, dz, idx2=1 / dx ** 2, idy2=1 / dy ** 2,
+                        idz2=1 / dz ** 2, dx2=dx ** 2, precision=code)
+        for k, fid in (("b", L.NYL_F_B), ("u_i", L.NYL_F_U), ("u_j", L.NYL_F_V),
+                       ("u_k", L.NYL_F_W)):
+            m.upload(fid, st[k])
+        m.diagnose()
+        for _ in range(steps):
+            m.step_rk3(dt)
+        models[prec] = m
+    out = {"grid": [nx, ny, nz], "steps": steps, "dt": dt}
+    out.update(state_errors(models["mixed"], models["fp64"]))
+    for m in models.values():
+        m.close()
     return out
 
 
 def serving_run(count, rate, precision="mixed"):
-    """Poisson arrivals of lock-exchange, Taylor-Green and shear-layer jobs,
-    n in {32, 64, 128}, 50-200 steps each, served with continuous batching."""
+    """Poisson arrivals of lock-exchange, Taylor-Green and shear-layer jobs
+    (kind uniform), n log-uniform in [32, 256] rounded to a multiple of 8,
+    steps uniform in [50, 500], served with continuous batching: the first
+    `count` arrivals of the serving stream."""
     from nyles_amd import serving
-    jobs = serving.poisson_jobs(count, rate)
+    jobs = serving.poisson_jobs(count, rate, size_range=(32, 256), steps=(50, 500))
     rep = serving.Server(precision=precision).run(jobs)
     rep["arrival_rate_per_s"] = rate
-    rep["workload"] = ("%d jobs, Poisson arrivals at %.1f/s; kind uniform over lock exchange "
-                       "(n x n/2 x n/2), Taylor-Green (n^3), shear layer (n^3); n uniform over "
-                       "{32, 64, 128}; steps uniform in [50, 200]" % (count, rate))
+    rep["sizes"] = sorted(j.n for j in jobs)
+    rep["cell_steps_requested"] = int(sum(j.steps * j.cells for j in jobs if j.cells))
+    rep["workload"] = ("the first %d arrivals of a Poisson stream at %.2f/s; kind uniform over "
+                       "lock exchange (n x n/2 x n/2), Taylor-Green (n^3), shear layer (n^3); "
+                       "n log-uniform in [32, 256] rounded to 8; steps uniform in [50, 500]"
+                       % (count, rate))
     return {k: (round(v, 4) if isinstance(v, float) else v) for k, v in rep.items()}
 
 
 def measured_traffic(kclass, precision, n):
     """HBM bytes per launch of the roofline kernel, from the committed
     rocprofv3 --pmc summary of the same kernel and workload (FETCH_SIZE and
-    WRITE_SIZE passes scaled by a copy of known size); None when no summary
-    covers this kernel and workload."""
+    WRITE_SIZE passes scaled by a copy of known size), when that summary was
+    measured on the current kernel sources (its recorded source digest
+    matches); None otherwise."""
+    from nyles_amd import profiling
     if (kclass, precision, n) != ("mg_sweep_fine", "mixed", 256):
         return None
     path = os.path.join(ROOT, "profiles", "traffic_mg_sweep_fine_taylor_green_256_mixed.md")
+    found, digest = None, None
     try:
         with open(path) as f:
             for line in f:
                 if line.startswith("traffic_bytes_per_launch"):
-                    return int(line.split("=")[1])
+                    found = int(line.split("=")[1])
+                elif line.startswith("source_digest"):
+                    digest = line.split("=")[1].strip()
     except OSError:
         return None
-    return None
+    if digest != profiling.source_digest():
+        return None
+    return found
 
 
-def cpu_baseline(n, steps):
+def cpu_baseline(n, steps, steps1):
     """FP64 oracle on the host cores: initial projection untimed, then `steps`
-    RK3 steps timed."""
+    RK3 steps timed on all the cores of the process, then `steps1` steps on
+    one thread."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import numpy as np
     import oracle as O
@@ -228,7 +317,6 @@ def cpu_baseline(n, steps):
     except AttributeError:
         cores = os.cpu_count() or 1
     cores = min(cores, int(os.environ.get("OMP_NUM_THREADS", cores)))
-    O.set_threads(cores)
     st, meta = workloads.taylor_green(n, seed=2)
     L = meta["L"]
     orc = O.OracleLES(n, n, n, L[0], L[1], L[2], "perio_xyz")
@@ -236,6 +324,7 @@ def cpu_baseline(n, steps):
     orc.set_interior(orc.u["i"], st["u_i"])
     orc.set_interior(orc.u["j"], st["u_j"])
     orc.set_interior(orc.u["k"], st["u_k"])
+    O.set_threads(cores)
     orc.diagnose_var()
     U = orc.interior(orc.U["i"]), orc.interior(orc.U["j"]), orc.interior(orc.U["k"])
     dt = 0.5 / float(np.sqrt(np.max(U[0] ** 2 + U[1] ** 2 + U[2] ** 2)))
@@ -243,10 +332,23 @@ def cpu_baseline(n, steps):
     for _ in range(steps):
         orc.rk3_ssp(dt)
     el = time.perf_counter() - t0
-    return {"value": round(n ** 3 * steps / el / 1e6, 4), "unit": "Mcell-steps/s",
-            "cores": O.num_threads(), "kind": "port",
-            "sample": "taylor-green %d^3 fp64, %d RK3 step(s) after the initial projection, "
-                      "%.1f s wall on %d OpenMP threads" % (n, steps, el, O.num_threads())}
+    threads = O.num_threads()
+    out = {"value": round(n ** 3 * steps / el / 1e6, 4), "unit": "Mcell-steps/s",
+           "cores": threads, "kind": "port",
+           "sample": "taylor-green %d^3 fp64, %d RK3 step(s) after the initial projection, "
+                     "%.1f s wall on %d OpenMP threads" % (n, steps, el, threads)}
+    if steps1 > 0:
+        O.set_threads(1)
+        t0 = time.perf_counter()
+        for _ in range(steps1):
+            orc.rk3_ssp(dt)
+        el1 = time.perf_counter() - t0
+        O.set_threads(cores)
+        out["single_thread"] = {
+            "value": round(n ** 3 * steps1 / el1 / 1e6, 4), "unit": "Mcell-steps/s", "cores": 1,
+            "sample": "the same run, %d further RK3 step(s) on 1 thread, %.1f s wall"
+                      % (steps1, el1)}
+    return out
 
 
 def make_model(n, precision):
@@ -300,32 +402,15 @@ def timed(dev, dt, steps, warmup, kclass):
 def error_check(n, dt, steps):
     """Advance both paths `steps` steps from the same state; errors of the
     mixed path relative to FP64 (bit-identical to the oracle)."""
-    import numpy as np
-    from nyles_amd import _lib as L
-    res = {}
-    fields = {}
-    for prec in ("fp64", "mixed"):
-        dev = make_model(n, prec)
+    models = {p: make_model(n, p) for p in ("fp64", "mixed")}
+    for m in models.values():
         for _ in range(steps):
-            dev.step_rk3(dt)
-        res[prec] = {"ke": dev.reduce(L.NYL_RED_KE), "bsum": dev.reduce(L.NYL_RED_BSUM),
-                     "nonfinite": dev.reduce(L.NYL_RED_NONFINITE)}
-        fields[prec] = {k: dev.download(f) for k, f in
-                        (("b", L.NYL_F_B), ("u", L.NYL_F_U), ("v", L.NYL_F_V), ("w", L.NYL_F_W))}
-        dev.close()
-    a, m = fields["fp64"], fields["mixed"]
-    du2 = sum(float(np.sum((m[k] - a[k]) ** 2)) for k in ("u", "v", "w"))
-    u2 = sum(float(np.sum(a[k] ** 2)) for k in ("u", "v", "w"))
-    bn = float(np.sqrt(np.sum(a["b"] ** 2)))
-    return {
-        "steps": steps, "ke_fp64": res["fp64"]["ke"], "ke_mixed": res["mixed"]["ke"],
-        "ke_rel_err": abs(res["mixed"]["ke"] - res["fp64"]["ke"]) / abs(res["fp64"]["ke"]),
-        "u_l2_rel_err": (du2 / u2) ** 0.5 if u2 > 0 else None,
-        "b_max_abs_err": float(np.max(np.abs(m["b"] - a["b"]))),
-        "b_l2_rel_err": (float(np.sqrt(np.sum((m["b"] - a["b"]) ** 2))) / bn) if bn > 0 else None,
-        "nonfinite_mixed": res["mixed"]["nonfinite"],
-        "tolerance": "KE and b within 1e-5 relative of FP64",
-    }
+            m.step_rk3(dt)
+    out = {"steps": steps}
+    out.update(state_errors(models["mixed"], models["fp64"]))
+    for m in models.values():
+        m.close()
+    return out
 
 
 def main():
@@ -397,19 +482,27 @@ def main():
         out["breakdown"] = {p: runs[p]["breakdown"] for p in runs}
     if args.err_steps > 0:
         print("error check, %d steps" % args.err_steps, file=sys.stderr, flush=True)
-        out["error_vs_fp64"] = error_check(n, dt, args.err_steps)
+        err = {"tolerance": "KE, velocity and buoyancy within 1e-5 relative of FP64",
+               "taylor_green_%d" % n: error_check(n, dt, args.err_steps),
+               "lock_exchange_64x32x32": lock_exchange_error(args.err_steps)}
+        out["error_vs_fp64"] = err
     if args.ensemble_members > 0:
         print("ensemble, %d members" % args.ensemble_members, file=sys.stderr, flush=True)
-        out["ensemble"] = ensemble_run(args.ensemble_members, 10)
+        out["ensemble"] = ensemble_run(args.ensemble_members, args.ensemble_steps,
+                                       args.ensemble_seq_steps)
     if args.serving_jobs > 0:
         print("serving, %d jobs" % args.serving_jobs, file=sys.stderr, flush=True)
         out["serving"] = serving_run(args.serving_jobs, args.serving_rate)
     if args.rt_steps > 0:
         print("rayleigh-taylor, %d steps" % args.rt_steps, file=sys.stderr, flush=True)
-        out["rayleigh_taylor"] = rt_run(args.rt_steps)
+        out["rayleigh_taylor"] = rt_run(max(args.rt_steps - 1, 1))
+        if "error_vs_fp64" in out:
+            e = dict(out["rayleigh_taylor"]["error_vs_fp64"])
+            e["steps"] = out["rayleigh_taylor"]["steps"]
+            out["error_vs_fp64"]["rayleigh_taylor_256x256x512"] = e
     if not args.no_cpu_baseline:
         print("cpu baseline", file=sys.stderr, flush=True)
-        out["cpu_baseline"] = cpu_baseline(n, args.cpu_steps)
+        out["cpu_baseline"] = cpu_baseline(n, args.cpu_steps, args.cpu_steps_1thread)
     print(json.dumps(out), flush=True)
 
 

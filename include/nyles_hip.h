@@ -39,7 +39,7 @@
 extern "C" {
 #endif
 
-#define NYL_ABI_VERSION 1
+#define NYL_ABI_VERSION 2
 
 /* status codes */
 #define NYL_OK 0
@@ -52,7 +52,8 @@ extern "C" {
 #define NYL_PREC_FP64 0     /* loop-faithful FP64, bit-compatible with the oracle */
 /* NYL_PREC_MIXED: model fields (state, tendencies, vor, ke) and their
  * stencils in FP32; the projection is iterative refinement: FP32 V-cycles
- * (FP64 coarsest level) on the correction inside an FP64 outer residual,
+ * (FP64 coarsest level) on the correction inside an FP64 outer residual of
+ * the right-hand side with its mean removed (summed in FP64),
  * with mgfor's stop rule and V-cycle count (a tolerance tighter than one
  * FP32 inner solve resolves, below 1e-6 of the current squared residual, is
  * met through more outer steps of at most 8 V-cycles); NYL_RED_* reductions
@@ -76,7 +77,14 @@ typedef struct nyl_desc {
     int32_t mg_maxite;           /* 20 (core/mgfor/mg_types.f90:20) */
     double mg_tol;               /* 1e-6 on the squared ratio (core/mgfor/mg_types.f90:21) */
     double mg_omega;             /* 0.9 (core/mgfor/mg_types.f90:22) */
+    double viscosity;            /* diff_coef['u'] (core/model_les.py:54-57), 0: none; added
+                                    at the last stage (core/model_les.py:140-141) */
+    int32_t n_tracers;           /* passive tracers t0.. advected like b
+                                    (core/model_les.py:36-43), 0..NYL_MAX_TRACERS */
+    int32_t reserved;            /* 0 */
 } nyl_desc;
+
+#define NYL_MAX_TRACERS 8
 
 /* Per-solve statistics (core/mgfor/mg_types.f90:46-49 MG_Stats, plus normb). */
 typedef struct nyl_mg_stats {
@@ -101,6 +109,10 @@ enum {
     NYL_F_MGY,                                       /* MG level-1 smoother scratch y */
     NYL_NFIELDS
 };
+/* passive tracer q (0 <= q < n_tracers): its state, and its tendency in
+ * RK slot 0..2 */
+#define NYL_F_TRACER(q) (64 + (q))
+#define NYL_F_DS_TRACER(slot, q) (80 + NYL_MAX_TRACERS * (slot) + (q))
 
 /* dtype codes returned by nyl_field_offset: model fields are F32 in mixed
  * mode; NYL_F_P, NYL_F_DIV, NYL_F_MGX and NYL_F_MGB are always F64 */
@@ -141,8 +153,10 @@ int nyl_field_offset(const nyl_ctx *ctx, int field, uint64_t *offset_bytes,
 int nyl_mg_levels(const nyl_ctx *ctx, int32_t *nlevels);
 int nyl_mg_level_shape(const nyl_ctx *ctx, int level, int32_t *nx, int32_t *ny, int32_t *nz);
 
-/* LES.rhs (core/model_les.py:129-144): tracer advection, vortex force and
- * Bernoulli tendencies of the current state into tendency slot 0..2.  The
+/* LES.rhs (core/model_les.py:129-144): tracer advection (b and the passive
+ * tracers), vortex force and Bernoulli tendencies of the current state, and
+ * with last != 0 the viscosity (core/model_les.py:140-141), into tendency
+ * slot 0..2.  The
  * contravariant U is evaluated as u*idx2 ... (the product U_from_u stores,
  * core/cov_to_contra.py:15-17); vor and ke are those of the last diagnose. */
 int nyl_rhs(nyl_ctx *ctx, int slot, int last);
@@ -152,6 +166,17 @@ int nyl_rhs(nyl_ctx *ctx, int slot, int last);
  * stage 1: s+=c*ds0; 2: s+=c*(ds1-3*ds0); 3: s+=c*(8*ds2-ds0-ds1), with
  * c = dt, dt/4., dt/12. computed by the caller. */
 int nyl_rk3_stage(nyl_ctx *ctx, int stage, double c);
+
+/* The stage update of a time scheme from the stored tendencies of every
+ * prognostic field (b, u, v, w and the tracers), without recomputing the
+ * right-hand side: the update loops of core/timescheme.py:113-221 after
+ * LES.rhs and the user forcing (core/model_les.py:143-144) have written the
+ * tendencies (nyl_rhs, then the caller's changes).  mode 1..3: RK3_SSP
+ * stage k with c = dt, dt/4., dt/12. (tendency slots 0..k-1), mode 1 is
+ * also Euler forward; LFAM3: 6 first step (Euler, also saves the state), 4
+ * predictor (c2 = 2.*dt), 5 corrector (c = dt): slot 0.  The updated state
+ * becomes current. */
+int nyl_stage_update(nyl_ctx *ctx, int mode, double c, double c2);
 
 /* LES.diagnose_var (core/model_les.py:97-127): div, MG projection
  * (core/projection.py:38-87, core/mgfordriver.py:66-78), u -= grad p,

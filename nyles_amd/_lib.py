@@ -25,6 +25,17 @@ NYL_PREC_MIXED = 1
 NYL_DT_F64 = 0
 NYL_DT_F32 = 1
 
+NYL_MAX_TRACERS = 8
+
+
+def NYL_F_TRACER(q):
+    return 64 + q
+
+
+def NYL_F_DS_TRACER(slot, q):
+    return 80 + NYL_MAX_TRACERS * slot + q
+
+
 NYL_RED_MAXU2 = 0
 NYL_RED_KE = 1
 NYL_RED_BSUM = 2
@@ -42,7 +53,8 @@ EXPORTED = [
     "nyl_abi_version", "nyl_last_error", "nyl_ctx_create", "nyl_ctx_destroy",
     "nyl_workspace_bytes", "nyl_bind_workspace", "nyl_set_stream", "nyl_set_graphs",
     "nyl_field_offset",
-    "nyl_mg_levels", "nyl_mg_level_shape", "nyl_rhs", "nyl_rk3_stage", "nyl_diagnose",
+    "nyl_mg_levels", "nyl_mg_level_shape", "nyl_rhs", "nyl_rk3_stage", "nyl_stage_update",
+    "nyl_diagnose",
     "nyl_step_rk3", "nyl_step_ef", "nyl_step_lfam3", "nyl_lfam3_reset", "nyl_mg_solve", "nyl_apply", "nyl_reduce", "nyl_prof_start",
     "nyl_prof_stop", "nyl_debug_mg", "nyl_reduce_last", "nyl_sync",
     "nyl_batch_create", "nyl_batch_destroy", "nyl_batch_workspace_bytes",
@@ -70,6 +82,8 @@ class NylDesc(ctypes.Structure):
         ("dx2", ctypes.c_double), ("fparameter", ctypes.c_double),
         ("precision", ctypes.c_int32), ("mg_maxite", ctypes.c_int32),
         ("mg_tol", ctypes.c_double), ("mg_omega", ctypes.c_double),
+        ("viscosity", ctypes.c_double), ("n_tracers", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
     ]
 
 
@@ -113,6 +127,7 @@ def load():
                                     P(ctypes.c_int32)]),
         "nyl_rhs": (ci, [vp, ci, ci]),
         "nyl_rk3_stage": (ci, [vp, ci, dbl]),
+        "nyl_stage_update": (ci, [vp, ci, dbl, dbl]),
         "nyl_diagnose": (ci, [vp, P(NylMGStats)]),
         "nyl_step_rk3": (ci, [vp, dbl, P(NylMGStats)]),
         "nyl_step_ef": (ci, [vp, dbl, P(NylMGStats)]),

@@ -18,7 +18,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libnyles_hip.so")
 OBJDIR = os.path.join(HERE, "_build")
 SOURCES = ["nyl_api.hip", "nyl_stencil.hip", "nyl_mg.hip", "nyl_batch.hip", "nyl_coarse.hip",
-           "nyl_noalloc.hip"]
+           "nyl_noalloc.hip", "nyl_mgfor.hip"]
 ARCH = os.environ.get("NYL_OFFLOAD_ARCH", "gfx950")
 
 FLAGS = [

@@ -5,11 +5,25 @@ context holding the same fields, and the time scheme bound to it.
 ``rhs``, ``diagnose_var`` and ``forward`` run in the HIP library; the
 host mirrors are synchronised lazily (see variables.py).
 
+Physics hooks, as in the reference: ``diff_coef['u']`` is a viscosity added
+at the last stage of a step (core/model_les.py:140-141); ``n_tracers``
+passive tracers t0, t1, ... are advected like b (core/model_les.py:36-43);
+with ``forced`` the script attaches ``model.forcing``, whose
+``add(state, dstate, t)`` runs after each right-hand side on host views of
+the state and the tendencies (core/model_les.py:143-144), so the step runs
+rhs, forcing and stage update as separate device calls.
+
 Differences from the reference that a script can see:
 * geometries perio_x and perio_y are accepted (the reference asserts,
   core/model_les.py:80-88, although its multigrid supports them);
-* viscosity/diffusion (diff_coef), forcing, passive tracers and the linear
-  model are not available on the device path and raise NotImplementedError;
+* a passive tracer's periodic halo is refilled with the periodic image like
+  b's (the reference refills only b and u, core/model_les.py:100-101, so its
+  tracers read stale halo values on periodic axes); on wall axes there is no
+  halo and the results are the reference's;
+* diffusion coefficients of b and the tracers are accepted and, as in the
+  reference's LES (rhstrac is called without ``last``, core/model_les.py:133),
+  not applied;
+* the linear model is not available on the device path (NotImplementedError);
 * ``forward`` returns True when b or u holds a non-finite value (the
   reference's blow-up flag is never set, core/mgfordriver.py:32).
 """
@@ -45,17 +59,15 @@ class LES(object):
     def __init__(self, param, grid, linear=False):
         if linear:
             raise NotImplementedError("the linear model is not available on the GPU path")
-        if param["n_tracers"] > 0:
-            raise NotImplementedError("passive tracers are not available on the GPU path")
-        if param["forced"]:
-            raise NotImplementedError("user forcing is not available on the GPU path")
-        if len(param["diff_coef"]) > 0:
-            raise NotImplementedError("viscosity / diffusion is not available on the GPU path")
+        n_tracers = param["n_tracers"]
+        if not 0 <= n_tracers <= L.NYL_MAX_TRACERS:
+            raise ValueError("n_tracers must be 0..%d on the GPU path" % L.NYL_MAX_TRACERS)
         self.nonlinear = True
         self.grid = grid
         self.param = param
-        self.traclist = ["b"]
-        self.state = var.get_state(param)
+        self.traclist = ["b"] + ["t%d" % q for q in range(n_tracers)]
+        self.state = var.get_state(param, [("t%d" % q, "tracer%d" % q)
+                                           for q in range(n_tracers)])
         self.halo = halo.set_halo(param, self.state)
         self.neighbours = param["neighbours"]
         self.timescheme = ts.Timescheme(param, self.state)
@@ -66,25 +78,36 @@ class LES(object):
         self.rotating = param["rotating"]
         self.forced = param["forced"]
         self.diff_coef = param["diff_coef"]
-        self.add_viscosity = False
+        self.add_viscosity = "u" in self.diff_coef.keys()
+        self.viscosity = self.diff_coef["u"] if self.add_viscosity else 0.0
+        self.forcing = None
         if self.rotating:
             self.fparameter = param["coriolis"] * (grid.dx * grid.dy)
         else:
             self.fparameter = 0.0
         precision = {"fp64": L.NYL_PREC_FP64, "mixed": L.NYL_PREC_MIXED}[param["precision"]]
+        # the multigrid stop test: mgfor's parameters (the reference's mgfor
+        # ignores the multigrid block, core/mgfor/mg_types.f90:15-26), or
+        # the multigrid block (tol on |r|/|b|, squared for the test)
+        if param["mg_params"] == "multigrid":
+            mg_tol, mg_maxite, mg_omega = param["tol"] ** 2, param["maxite"], param["omega"]
+        else:
+            mg_tol, mg_maxite, mg_omega = param["mg_tol"], param["mg_maxite"], param["mg_omega"]
         self.blowup_check = param["blowup_check"]
         self.device = DeviceModel(
             param["nx"], param["ny"], param["nz"], GEOMETRY_PERIODIC_XYZ[param["geometry"]],
             grid.dx, grid.dy, grid.dz, idx2=grid.idx2, idy2=grid.idy2, idz2=grid.idz2,
             dx2=grid.dx ** 2, fparameter=self.fparameter, precision=precision,
-            mg_maxite=param["mg_maxite"], mg_tol=param["mg_tol"], mg_omega=param["mg_omega"])
-        self.timescheme.bind(self.device)
+            mg_maxite=mg_maxite, mg_tol=mg_tol, mg_omega=mg_omega,
+            viscosity=float(self.viscosity), n_tracers=n_tracers)
+        self.timescheme.bind(self.device, self)
         self.mg = MGInfo()
         st = self.state
         pairs = [(st.b, L.NYL_F_B), (st.u["i"], L.NYL_F_U), (st.u["j"], L.NYL_F_V),
                  (st.u["k"], L.NYL_F_W), (st.ke, L.NYL_F_KE), (st.vor["i"], L.NYL_F_VORX),
                  (st.vor["j"], L.NYL_F_VORY), (st.vor["k"], L.NYL_F_VORZ),
                  (st.div, L.NYL_F_DIV)]
+        pairs += [(st.get("t%d" % q), L.NYL_F_TRACER(q)) for q in range(n_tracers)]
         for scalar, fid in pairs:
             scalar.attach(self.device, fid)
         # U is recomputed inside the stage kernel, not stored: fill it on demand
@@ -93,20 +116,26 @@ class LES(object):
                            before_pull=lambda: self.device.apply(L.NYL_OP_U_FROM_U))
         st.p.attach(self.device, L.NYL_F_P,
                     before_pull=lambda: self.device.apply(L.NYL_OP_PRESSURE))
-        if self.timescheme.timestepping == "RK3_SSP":
-            for slot, ds in enumerate((self.timescheme.ds0, self.timescheme.ds1,
-                                       self.timescheme.ds2)):
-                base = L.NYL_F_DS0_B + 4 * slot
-                for k, name in enumerate(("b", "u_i", "u_j", "u_k")):
-                    ds.get(name).attach(self.device, base + k)
+        for slot, ds in enumerate(self._tendency_states()):
+            base = L.NYL_F_DS0_B + 4 * slot
+            for k, name in enumerate(("b", "u_i", "u_j", "u_k")):
+                ds.get(name).attach(self.device, base + k)
+            for q in range(n_tracers):
+                ds.get("t%d" % q).attach(self.device, L.NYL_F_DS_TRACER(slot, q))
         self.stats = []
 
     # -- host <-> device ------------------------------------------------------
+    def _tendency_states(self):
+        """the host States of the device's tendency slots 0.."""
+        ts_ = self.timescheme
+        if ts_.timestepping == "RK3_SSP":
+            return [ts_.ds0, ts_.ds1, ts_.ds2]
+        return [ts_.dstate]
+
     def _attached(self):
         out = self.state.scalars()
-        if self.timescheme.timestepping == "RK3_SSP":
-            for ds in (self.timescheme.ds0, self.timescheme.ds1, self.timescheme.ds2):
-                out += ds.scalars()
+        for ds in self._tendency_states():
+            out += [s for s in ds.scalars() if s._dev is not None]
         return out
 
     def push(self):
@@ -132,8 +161,9 @@ class LES(object):
 
     @timing
     def rhs(self, state, t, dstate, last=False):
-        """Tendencies of the current state; into the RK3 slot that ``dstate``
-        is, slot 0 otherwise."""
+        """Tendencies of the current state (viscosity when last); into the RK3
+        slot that ``dstate`` is, slot 0 otherwise; then the user forcing on
+        the host views (core/model_les.py:129-144)."""
         slot = 0
         for k, name in enumerate(("ds0", "ds1", "ds2")):
             if getattr(self.timescheme, name, None) is dstate:
@@ -141,6 +171,10 @@ class LES(object):
         self.push()
         self.device.rhs(slot, last)
         self.mark_device_newer()
+        if self.forced:
+            if self.forcing is None:
+                raise RuntimeError("forced = True: attach the forcing to model.forcing")
+            self.forcing.add(state, dstate, t)
 
     @timing
     def forward(self, t, dt):

@@ -260,9 +260,14 @@ class State(object):
         return out
 
 
-def get_state(param):
+def get_state(param, tracers=()):
+    """The model State; `tracers` adds prognostic scalars after the model
+    variables (core/model_les.py:36-43 adds them to modelvar)."""
     listvar = []
-    for nickname, var in modelvar.items():
+    table = dict(modelvar)
+    for nickname, name in tracers:
+        table[nickname] = ModelVariable("scalar", name, "", prognostic=True)
+    for nickname, var in table.items():
         if var.type == "scalar":
             listvar.append(Scalar(param, var.name, nickname, var.dimension, var.prognostic))
         else:

@@ -92,8 +92,9 @@ int build_hierarchy(nyl_ctx *c) {
             if (!coarse_fits(g))
                 return fail(NYL_E_SIZE,
                             "grid %dx%dx%d: the coarsest multigrid level %dx%dx%d has an odd "
-                            "size and more than %d cells along an axis (direct coarse solve)",
-                            c->d.nx, c->d.ny, c->d.nz, x, y, z, CS_MAX);
+                            "size and more than %d cells along x or %d along y or z (direct "
+                            "coarse solve)",
+                            c->d.nx, c->d.ny, c->d.nz, x, y, z, CS_MAXX, CS_MAX);
             c->exact = 1;
             break;
         }
@@ -149,6 +150,12 @@ void build_layout(nyl_ctx *c) {
     c->partials = take(NYL_RED_BLOCKS * sizeof(double));
     c->result = take(8 * sizeof(double));
     c->cs_tab = c->exact ? take(coarse_table_doubles() * sizeof(double)) : 0;
+    for (int q = 0; q < c->ntr; q++) {
+        c->tst[0][q] = take(fb);
+        c->tst[1][q] = take(fb);
+        for (int k = 0; k < 3; k++) c->tds[k][q] = take(fb);
+        c->tsb[q] = take(fb);
+    }
     c->ws_bytes = off;
 }
 
@@ -260,6 +267,8 @@ StageArgs<T> stage_args(nyl_ctx *c) {
     a.dz = c->d.dz;
     a.c = 0.0;
     a.c2 = 0.0;
+    a.vcx = a.vcy = a.vcz = 0.0;
+    a.visc = 0;
     return a;
 }
 
@@ -635,9 +644,12 @@ int mg_solve_fp64(nyl_ctx *c, nyl_mg_stats *st) {
 }
 
 // Mixed precision: iterative refinement with the stop rule of mgfor's solve.
-// As in mgfor the right-hand side is used as given (the discrete divergence
-// of the singular Neumann/periodic problem sums to zero up to rounding, far
-// below any tolerance).  An outer step computes r = b - A x in
+// The compatibility of the singular Neumann/periodic problem is enforced in
+// FP64 first: one pass sums b and b**2, the mean is removed from b inside
+// every outer residual and |b - mean|**2 is the norm of the stop test (the
+// discrete divergence sums to zero up to rounding, so this changes nothing
+// measurable, but a right-hand side with a mean could not converge).  An
+// outer step computes r = (b - mean) - A x in
 // FP64 and solves A e = r from e = 0 with FP32 V-cycles under mgfor's loop:
 // before each further V-cycle the squared residual r - A e = b - A(x + e),
 // evaluated in FP32 and summed in FP64, is tested against tol*|b|^2 (fused
@@ -661,15 +673,14 @@ int mg_solve_mixed(nyl_ctx *c, nyl_mg_stats *st) {
     const bool spec = sweep25_fits(f.g) && sweep25_blocks(f.g) <= NYL_RED_BLOCKS;
     int rc;
     double rn, res;
-    // |b|^2 with the first outer residual when its partial sums fit, else
-    // in a pass of its own
-    const bool fused_b = outer_bsq_fits(f.g);
-    const double shift = 0.0;
-    double normb = 0.0;
-    if (!fused_b) {
-        HIPCHK(launch_norm2<double>(f.g, bo, part, res_dev, c->stream));
-        if ((rc = read_result(c, &normb))) return rc;
-    }
+    // sum of b and of b**2 in FP64: the mean to remove and the norm of b - mean
+    double s2[2];
+    HIPCHK(launch_sum_sumsq(f.g, bo, part, res_dev, c->stream));
+    if ((rc = read_results(c, s2, 2))) return rc;
+    const double ncell = cells(f.g);
+    const double shift = s2[0] / ncell;
+    double normb = s2[1] - ncell * shift * shift;
+    if (normb < 0.0) normb = 0.0;
     // FP64 residual of x into r (and the first sweep from e = 0 into y)
     auto outer = [&](double *out) -> int {
         if (normb > 0.0) {
@@ -699,17 +710,7 @@ int mg_solve_mixed(nyl_ctx *c, nyl_mg_stats *st) {
         *out = rn / normb;
         return NYL_OK;
     };
-    if (fused_b) {
-        LAUNCH(NYL_K_RESNORM, (2 * sizeof(double) + 2 * sizeof(float)) * cells(f.g),
-               launch_mg_outer_resid<float>(f.g, xo, bo, shift, r, y, c->d.mg_omega, c->tabs,
-                                            part, res_dev, c->stream, true));
-        double s2[2];
-        if ((rc = read_results(c, s2, 2))) return rc;
-        normb = s2[1];
-        res = normb > 0.0 ? s2[0] / normb : 0.0;
-    } else if ((rc = outer(&res))) {
-        return rc;
-    }
+    if ((rc = outer(&res))) return rc;
     // mgfor's loop shape (core/mgfor/solvers.f90:22-29): test, V-cycle,
     // count, stop at maxite (so maxite = 0 still runs one V-cycle)
     int nite = 0;
@@ -827,20 +828,77 @@ double stage_words(int mode) {
 
 // mode st of the stage kernel; slot >= 0 also writes the tendencies there
 template <typename T>
-int stage_t(nyl_ctx *c, int st, int slot, double coef, double coef2) {
+int stage_t(nyl_ctx *c, int st, int slot, double coef, double coef2, bool last) {
     StageArgs<T> a = stage_args<T>(c);
     if (slot >= 0) set_tendency_out<T>(c, a, slot);
     a.c = coef;
     a.c2 = coef2;
+    if (last && c->d.viscosity != 0.0) {   // core/model_les.py:140-141
+        a.visc = 1;
+        a.vcx = c->d.viscosity * c->d.idx2;
+        a.vcy = c->d.viscosity * c->d.idy2;
+        a.vcz = c->d.viscosity * c->d.idz2;
+    }
     const double words = stage_words(st);
     LAUNCH(NYL_K_STAGE, words * cells(c->g) * sizeof(T), launch_stage<T>(c->g, a, st, c->stream));
+    // passive tracers: the same right-hand side and update, one launch each
+    const int s = c->cur, t = 1 - c->cur;
+    for (int q = 0; q < c->ntr; q++) {
+        TracerArgs<T> ta;
+        ta.q = F<T>(c, c->tst[s][q]);
+        ta.qn = F<T>(c, c->tst[t][q]);
+        ta.u = a.u; ta.v = a.v; ta.w = a.w;
+        ta.d0 = F<T>(c, c->tds[0][q]);
+        ta.d1 = F<T>(c, c->tds[1][q]);
+        ta.tout = slot >= 0 ? F<T>(c, c->tds[slot][q]) : nullptr;
+        ta.sb = F<T>(c, c->tsb[q]);
+        ta.idx2 = c->d.idx2; ta.idy2 = c->d.idy2; ta.idz2 = c->d.idz2;
+        ta.c = coef;
+        ta.c2 = coef2;
+        LAUNCH(NYL_K_STAGE, (words / 4.0) * cells(c->g) * sizeof(T),
+               launch_tracer_stage<T>(c->g, ta, st, c->stream));
+    }
     return NYL_OK;
 }
 
-int stage(nyl_ctx *c, int st, double coef, double coef2 = 0.0) {
+// the stage combination of mode st from the stored tendencies of every
+// prognostic field (b, u, v, w, tracers): slot 0 for modes 1 and 4..6, slots
+// 0..1 for mode 2 and 0..2 for mode 3
+template <typename T>
+int update_t(nyl_ctx *c, int st, double coef, double coef2) {
+    const int s = c->cur, t = 1 - c->cur;
+    auto one = [&](uint64_t cur, uint64_t nxt, uint64_t d0, uint64_t d1, uint64_t d2,
+                   uint64_t sb) -> int {
+        UpdateArgs<T> a;
+        a.s = F<T>(c, cur);
+        a.sn = F<T>(c, nxt);
+        a.ds0 = F<T>(c, d0);
+        a.ds1 = F<T>(c, d1);
+        a.ds2 = F<T>(c, d2);
+        a.sb = F<T>(c, sb);
+        a.c = coef;
+        a.c2 = coef2;
+        HIPCHK(launch_update<T>(c->ncell, a, st, c->stream));
+        return NYL_OK;
+    };
+    int rc;
+    for (int f = 0; f < 4; f++)
+        if ((rc = one(c->state[s][f], c->state[t][f], c->ds[0][f], c->ds[1][f], c->ds[2][f],
+                      c->sb[f])))
+            return rc;
+    for (int q = 0; q < c->ntr; q++)
+        if ((rc = one(c->tst[s][q], c->tst[t][q], c->tds[0][q], c->tds[1][q], c->tds[2][q],
+                      c->tsb[q])))
+            return rc;
+    return NYL_OK;
+}
+
+// last: the time scheme's rhs call with last = True (RK3 stage 3, Euler
+// forward, the LFAM3 corrector), which adds the viscosity
+int stage(nyl_ctx *c, int st, double coef, double coef2, bool last) {
     const int slot = (st == 1 || st == 2) ? st - 1 : -1;
-    int rc = c->mixed ? stage_t<float>(c, st, slot, coef, coef2)
-                      : stage_t<double>(c, st, slot, coef, coef2);
+    int rc = c->mixed ? stage_t<float>(c, st, slot, coef, coef2, last)
+                      : stage_t<double>(c, st, slot, coef, coef2, last);
     if (rc) return rc;
     if (st != 5) c->cur = 1 - c->cur;
     return NYL_OK;
@@ -904,9 +962,13 @@ int nyl_ctx_create(const nyl_desc *desc, nyl_ctx **out) {
                     (long long)d.nx * d.ny * d.nz, ((long long)1 << 29) - 1,
                     ((long long)1 << 30) - 1);
     if (d.mg_maxite < 0) return fail(NYL_E_ARG, "mg_maxite must be >= 0");
+    if (d.n_tracers < 0 || d.n_tracers > NYL_MAX_TRACERS)
+        return fail(NYL_E_ARG, "n_tracers must be 0..%d", NYL_MAX_TRACERS);
+    if (!(d.viscosity >= 0.0)) return fail(NYL_E_ARG, "viscosity must be >= 0");
     nyl_ctx *c = new nyl_ctx();
     memset(c, 0, sizeof(*c));
     c->d = d;
+    c->ntr = d.n_tracers;
     c->mixed = d.precision == NYL_PREC_MIXED;
     c->g = make_geo(d.nx, d.ny, d.nz, d.periodic_x, d.periodic_y, d.periodic_z);
     if (c->mixed && !outer_resid_fits(c->g)) {
@@ -1004,6 +1066,16 @@ int nyl_field_offset(const nyl_ctx *c, int field, uint64_t *off, int64_t *nelem,
             o = c->ds[q / 4][q % 4];
             break;
         }
+        if (field >= NYL_F_TRACER(0) && field < NYL_F_TRACER(c->ntr)) {
+            o = c->tst[s][field - NYL_F_TRACER(0)];
+            break;
+        }
+        if (field >= NYL_F_DS_TRACER(0, 0) && field < NYL_F_DS_TRACER(3, 0) &&
+            (field - NYL_F_DS_TRACER(0, 0)) % NYL_MAX_TRACERS < c->ntr) {
+            const int q = field - NYL_F_DS_TRACER(0, 0);
+            o = c->tds[q / NYL_MAX_TRACERS][q % NYL_MAX_TRACERS];
+            break;
+        }
         return fail(NYL_E_ARG, "unknown field %d", field);
     }
     *off = o;
@@ -1028,16 +1100,26 @@ int nyl_mg_level_shape(const nyl_ctx *c, int level, int32_t *nx, int32_t *ny, in
 int nyl_rhs(nyl_ctx *c, int slot, int last) {
     int rc = check_ready(c);
     if (rc) return rc;
-    (void)last;   // viscosity at the last stage is not part of this model path
     if (slot < 0 || slot > 2) return fail(NYL_E_ARG, "slot must be 0..2");
-    return c->mixed ? stage_t<float>(c, 0, slot, 0.0, 0.0) : stage_t<double>(c, 0, slot, 0.0, 0.0);
+    return c->mixed ? stage_t<float>(c, 0, slot, 0.0, 0.0, last != 0)
+                    : stage_t<double>(c, 0, slot, 0.0, 0.0, last != 0);
+}
+
+int nyl_stage_update(nyl_ctx *c, int mode, double coef, double coef2) {
+    int rc = check_ready(c);
+    if (rc) return rc;
+    if (mode < 1 || mode > 6) return fail(NYL_E_ARG, "mode must be 1..6");
+    rc = c->mixed ? update_t<float>(c, mode, coef, coef2) : update_t<double>(c, mode, coef, coef2);
+    if (rc) return rc;
+    c->cur = 1 - c->cur;   // mode 5 wrote over the saved 'now' state, which becomes current
+    return NYL_OK;
 }
 
 int nyl_rk3_stage(nyl_ctx *c, int st, double coef) {
     int rc = check_ready(c);
     if (rc) return rc;
     if (st < 1 || st > 3) return fail(NYL_E_ARG, "stage must be 1..3");
-    return stage(c, st, coef);
+    return stage(c, st, coef, 0.0, st == 3);
 }
 
 int nyl_diagnose(nyl_ctx *c, nyl_mg_stats *st) {
@@ -1053,7 +1135,7 @@ int nyl_step_rk3(nyl_ctx *c, double dt, nyl_mg_stats *st) {
     memset(&acc, 0, sizeof(acc));
     const double coef[3] = {dt, dt / 4., dt / 12.};
     for (int k = 1; k <= 3; k++) {
-        if ((rc = stage(c, k, coef[k - 1]))) return rc;
+        if ((rc = stage(c, k, coef[k - 1], 0.0, k == 3))) return rc;
         if ((rc = diagnose(c, &s1))) return rc;
         acc.nite += s1.nite;
         acc.res = s1.res;
@@ -1067,7 +1149,7 @@ int nyl_step_rk3(nyl_ctx *c, double dt, nyl_mg_stats *st) {
 int nyl_step_ef(nyl_ctx *c, double dt, nyl_mg_stats *st) {
     int rc = check_ready(c);
     if (rc) return rc;
-    if ((rc = stage(c, 1, dt))) return rc;
+    if ((rc = stage(c, 1, dt, 0.0, true))) return rc;
     return diagnose(c, st);
 }
 
@@ -1075,16 +1157,16 @@ int nyl_step_lfam3(nyl_ctx *c, double dt, nyl_mg_stats *st) {
     int rc = check_ready(c);
     if (rc) return rc;
     if (c->lf_first) {
-        if ((rc = stage(c, 6, dt))) return rc;
+        if ((rc = stage(c, 6, dt, 0.0, false))) return rc;
         c->lf_first = 0;
         return diagnose(c, st);
     }
     nyl_mg_stats s1, s2;
     // predictor: the current buffer keeps sn, the next one receives s
-    if ((rc = stage(c, 4, 0.0, 2. * dt))) return rc;
+    if ((rc = stage(c, 4, 0.0, 2. * dt, false))) return rc;
     if ((rc = diagnose(c, &s1))) return rc;
     // corrector: rhs at n+1/2, then s = sn + dt*ds written over sn
-    if ((rc = stage(c, 5, dt))) return rc;
+    if ((rc = stage(c, 5, dt, 0.0, true))) return rc;
     c->cur = 1 - c->cur;
     if ((rc = diagnose(c, &s2))) return rc;
     if (st) {

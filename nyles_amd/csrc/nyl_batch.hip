@@ -143,6 +143,12 @@ std::vector<StageItem<T>> stage_items(nyl_batch *b, int p, int st, int perio, in
         }
         a.idx2 = c->d.idx2; a.idy2 = c->d.idy2; a.idz2 = c->d.idz2;
         a.dz = c->d.dz;
+        if (st == 3 && c->d.viscosity != 0.0) {   // viscosity at the last RK stage
+            a.visc = 1;
+            a.vcx = c->d.viscosity * c->d.idx2;
+            a.vcy = c->d.viscosity * c->d.idy2;
+            a.vcz = c->d.viscosity * c->d.idz2;
+        }
         v.push_back(it);
     }
     return v;
@@ -664,6 +670,8 @@ int nyl_batch_create(nyl_ctx *const *members, int32_t n, nyl_batch **out) {
         if (!members[q]->ws) return set_error(NYL_E_STATE, "member %d has no workspace bound", q);
         if (members[q]->mixed != members[0]->mixed)
             return set_error(NYL_E_ARG, "members must share the precision mode");
+        if (members[q]->ntr > 0)
+            return set_error(NYL_E_ARG, "member %d has passive tracers: step it on its own", q);
         for (int r = 0; r < q; r++)
             if (members[r] == members[q]) return set_error(NYL_E_ARG, "member %d appears twice", q);
     }

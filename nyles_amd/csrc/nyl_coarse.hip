@@ -19,7 +19,9 @@
 // operator) is set to zero.  Three launches: the x and y transforms of each
 // z plane, the z transform + scaling + inverse z transform of each y row,
 // the inverse y and x transforms of each z plane; every pass holds its plane
-// and its two matrices in LDS.  Sizes are limited to CS_MAX cells per axis.
+// and its y or z matrix in LDS (the x matrix is read from the table in
+// memory).  Sizes are limited to CS_MAXX cells along x and CS_MAX along y
+// and z (a lock-exchange box n x n/2 x n/2 stops at 2m x m x m, m odd).
 #include <math.h>
 
 #include "nyl_internal.h"
@@ -32,20 +34,20 @@ namespace nyl {
 // scaling; 2: inverse y, x)
 template <int PASS>
 __device__ __forceinline__ void cs_pass(const CoarseArgs &a, int p) {
-    __shared__ double P[CS_MAX * CS_MAX];
-    __shared__ double T[CS_MAX * CS_MAX];
+    __shared__ double P[CS_MAXX * CS_MAX];
+    __shared__ double T[CS_MAXX * CS_MAX];
     __shared__ double E1[CS_MAX * CS_MAX];
-    __shared__ double E2[CS_MAX * CS_MAX];
     const int nx = a.g.nx, ny = a.g.ny, nz = a.g.nz;
     const int tid = threadIdx.x;
-    const double *ex = a.tab, *ey = a.tab + CS_MAX * CS_MAX, *ez = a.tab + 2 * CS_MAX * CS_MAX;
-    const double *mux = a.tab + 3 * CS_MAX * CS_MAX, *muy = mux + CS_MAX, *muz = muy + CS_MAX;
+    const double *ex = a.tab, *ey = a.tab + CS_MAXX * CS_MAXX;
+    const double *ez = ey + CS_MAX * CS_MAX;
+    const double *mux = ez + CS_MAX * CS_MAX, *muy = mux + CS_MAXX, *muz = muy + CS_MAX;
+    double *E2 = E1;   // the y matrix in the plane passes
     if (PASS != 1) {
         // plane z = p: P[y*nx + i]
         const int64_t base = (int64_t)p * a.g.sz;
         const double *src = PASS == 0 ? a.b : a.s;
         for (int t = tid; t < nx * ny; t += CS_T) P[t] = src[base + t];
-        for (int t = tid; t < nx * nx; t += CS_T) E1[t] = ex[t];
         for (int t = tid; t < ny * ny; t += CS_T) E2[t] = ey[t];
         __syncthreads();
         if (PASS == 0) {
@@ -53,7 +55,7 @@ __device__ __forceinline__ void cs_pass(const CoarseArgs &a, int p) {
             for (int t = tid; t < nx * ny; t += CS_T) {
                 const int y = t / nx, k = t - y * nx;
                 double acc = 0.0;
-                for (int i = 0; i < nx; i++) acc = fma(E1[k * nx + i], P[y * nx + i], acc);
+                for (int i = 0; i < nx; i++) acc = fma(ex[k * nx + i], P[y * nx + i], acc);
                 T[t] = acc;
             }
             __syncthreads();
@@ -77,7 +79,7 @@ __device__ __forceinline__ void cs_pass(const CoarseArgs &a, int p) {
             for (int t = tid; t < nx * ny; t += CS_T) {
                 const int y = t / nx, i = t - y * nx;
                 double acc = 0.0;
-                for (int k = 0; k < nx; k++) acc = fma(E1[k * nx + i], T[y * nx + k], acc);
+                for (int k = 0; k < nx; k++) acc = fma(ex[k * nx + i], T[y * nx + k], acc);
                 a.x[base + t] = acc;
             }
         }
@@ -115,7 +117,7 @@ __global__ __launch_bounds__(CS_T) void k_cs(CoarseArgs a) {
 }
 
 bool coarse_fits(const Geo &g) {
-    return g.nx >= 1 && g.ny >= 1 && g.nz >= 1 && g.nx <= CS_MAX && g.ny <= CS_MAX &&
+    return g.nx >= 1 && g.ny >= 1 && g.nz >= 1 && g.nx <= CS_MAXX && g.ny <= CS_MAX &&
            g.nz <= CS_MAX;
 }
 
@@ -171,13 +173,19 @@ static void axis_tables(int n, int periodic, double *E, double *mu) {
     }
 }
 
+// table layout: Ex (CS_MAXX^2), Ey, Ez (CS_MAX^2 each), mu_x (CS_MAXX),
+// mu_y, mu_z (CS_MAX each)
 void coarse_tables(const Geo &g, double *tab) {
     for (int q = 0; q < coarse_table_doubles(); q++) tab[q] = 0.0;
-    const int n3[3] = {g.nx, g.ny, g.nz}, p3[3] = {g.px, g.py, g.pz};
-    for (int a = 0; a < 3; a++)
-        axis_tables(n3[a], p3[a], tab + a * CS_MAX * CS_MAX, tab + 3 * CS_MAX * CS_MAX + a * CS_MAX);
+    double *ey = tab + CS_MAXX * CS_MAXX, *ez = ey + CS_MAX * CS_MAX;
+    double *mux = ez + CS_MAX * CS_MAX, *muy = mux + CS_MAXX, *muz = muy + CS_MAX;
+    axis_tables(g.nx, g.px, tab, mux);
+    axis_tables(g.ny, g.py, ey, muy);
+    axis_tables(g.nz, g.pz, ez, muz);
 }
 
-int coarse_table_doubles() { return 3 * CS_MAX * CS_MAX + 3 * CS_MAX; }
+int coarse_table_doubles() {
+    return CS_MAXX * CS_MAXX + 2 * CS_MAX * CS_MAX + CS_MAXX + 2 * CS_MAX;
+}
 
 }  // namespace nyl

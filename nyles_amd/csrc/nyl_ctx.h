@@ -29,6 +29,9 @@ struct nyl_ctx {
     uint64_t state[2][4];
     uint64_t ds[3][4];
     uint64_t sb[4];          // LFAM3 'before' state
+    // passive tracers t0.. (state double-buffered with b, u, v, w)
+    int ntr;
+    uint64_t tst[2][NYL_MAX_TRACERS], tds[3][NYL_MAX_TRACERS], tsb[NYL_MAX_TRACERS];
     int lf_first;            // next LFAM3 step is the first one
     uint64_t ke, vor[3], uc[3], p;
     uint64_t mgx, mgb;       // FP64 solution and right-hand side of the projection

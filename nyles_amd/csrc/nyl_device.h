@@ -163,6 +163,48 @@ __device__ __forceinline__ T face_flux(int f, int n, T u, const QF &q) {
     return u * w;
 }
 
+// face_flux with the stencil given relative to the face: qr(d) = q(f + d),
+// d in [-2, 3] (the positions every branch of face_flux reads).  For values
+// held in registers: each branch reads constant offsets.  Same values,
+// same operations, same results as face_flux.
+template <typename T, int PER, typename QR>
+__device__ __forceinline__ T face_flux_rel(int f, int n, T u, const QR &qr) {
+    const bool pos = u > T(0);
+    // every position read up front with a constant offset: the selections
+    // below then pick among values, never among positions (a position picked
+    // at run time would index the register window dynamically)
+    const T m2 = qr(-2), m1 = qr(-1), z0 = qr(0), p1 = qr(1), p2 = qr(2), p3 = qr(3);
+    if (PER || (f >= 2 && f <= n - 4)) {
+        T a0, a1, a2, a3, a4;
+        if (pos) { a0 = m2; a1 = m1; a2 = z0; a3 = p1; a4 = p2; }
+        else     { a0 = p3; a1 = p2; a2 = p1; a3 = z0; a4 = m1; }
+        return u * weno5<T>(a0, a1, a2, a3, a4);
+    }
+    int kind;
+    T a0 = T(0), a1 = T(0), a2 = T(0), a3 = T(0), a4 = T(0);
+    if (f == n - 1) {
+        kind = 0;
+    } else if (f == n - 2) {
+        if (pos) { kind = 3; a0 = m1; a1 = z0; a2 = p1; }
+        else     { kind = 1; a0 = p1; }
+    } else if (f == n - 3) {
+        if (pos) { kind = 5; a0 = m2; a1 = m1; a2 = z0; a3 = p1; a4 = p2; }
+        else     { kind = 3; a0 = p2; a1 = p1; a2 = z0; }
+    } else if (f == 0) {
+        if (pos) { kind = 1; a0 = z0; }
+        else     { kind = 3; a0 = p2; a1 = p1; a2 = z0; }
+    } else {  // f == 1
+        if (pos) { kind = 3; a0 = m1; a1 = z0; a2 = p1; }
+        else     { kind = 5; a0 = p3; a1 = p2; a2 = p1; a3 = z0; a4 = m1; }
+    }
+    T w;
+    if (kind == 5) w = weno5<T>(a0, a1, a2, a3, a4);
+    else if (kind == 3) w = weno3<T>(a0, a1, a2);
+    else if (kind == 1) w = a0;
+    else return T(0);
+    return u * w;
+}
+
 // ---------------------------------------------------------------------------
 // Ensemble batches: the item (one per member) owning flattened block `blk`
 // (items sorted by h.start, h.start of the first item is 0).

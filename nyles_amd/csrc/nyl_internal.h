@@ -22,6 +22,11 @@ struct StageArgs {
     double dz;                       // for cff = 0.5*dz (core/fortran_bernoulli.f90:47)
     double c;                        // RK coefficient dt, dt/4., dt/12.
     double c2;                       // LFAM3: 2.*dt (mode 4)
+    // viscosity at the last stage (core/model_les.py:140-141): when visc is
+    // set, du += vc_d * delta_d[delta_d[u]] for d = x, y, z in turn, with
+    // vc_d = viscosity * ids2[d] (core/viscosity.py:3-9)
+    double vcx, vcy, vcz;
+    int visc;
 };
 
 // mode 0: tendencies only; 1..3: RK3_SSP stage update;
@@ -29,6 +34,39 @@ struct StageArgs {
 // 6: Euler step that also copies the state into the LFAM3 'before' state
 template <typename T>
 hipError_t launch_stage(const Geo &g, const StageArgs<T> &a, int mode, hipStream_t s);
+
+// One passive tracer (core/model_les.py:36-43): its flux divergence with the
+// current U (core/tracer.py:59-72, as for b) and the stage update of the
+// launch's mode, the same arithmetic as b's in the stage kernel.
+template <typename T>
+struct TracerArgs {
+    const T *q;                      // current tracer
+    T *qn;                           // next (stage update target)
+    const T *u, *v, *w;              // current velocity (U = u*idx2 ...)
+    const T *d0, *d1;                // tendency slots 0, 1 (stages 2, 3)
+    T *tout;                         // tendency output (modes 0, 1, 2)
+    T *sb;                           // LFAM3 'before' state (modes 4, 6)
+    double idx2, idy2, idz2;
+    double c, c2;
+};
+template <typename T>
+hipError_t launch_tracer_stage(const Geo &g, const TracerArgs<T> &a, int mode, hipStream_t s);
+
+// The stage combination of a time scheme from stored tendencies, for one
+// field (core/timescheme.py:113-221): used when the tendencies are modified
+// between the right-hand side and the update (user forcing).  ds0 is the
+// stage's own tendency for modes 1 and 4..6; mode 2 reads ds0, ds1 and mode
+// 3 ds0, ds1, ds2 (RK3 slots).
+template <typename T>
+struct UpdateArgs {
+    const T *s;                      // current
+    T *sn;                           // next
+    const T *ds0, *ds1, *ds2;
+    T *sb;                           // LFAM3 'before'
+    double c, c2;
+};
+template <typename T>
+hipError_t launch_update(int64_t n, const UpdateArgs<T> &a, int mode, hipStream_t s);
 
 template <typename T>
 struct DiagArgs {
@@ -156,9 +194,10 @@ hipError_t launch_mg_restrict25(const Geo &gf, const Geo &gc, const Tf *x, const
 // ---- direct solve of a coarsest level that cannot be halved (odd size) ------
 // x = A^-1 b with the constant mode removed, by per-axis eigenvector
 // transforms (s: scratch of the level's size; tab: coarse_tables()).
-#define CS_MAX 32
+#define CS_MAX 32     // cells along y and z
+#define CS_MAXX 64    // cells along x
 struct CoarseArgs { Geo g; const double *b; double *s; double *x; const double *tab; };
-bool coarse_fits(const Geo &g);               // every axis <= CS_MAX cells
+bool coarse_fits(const Geo &g);               // nx <= CS_MAXX, ny and nz <= CS_MAX
 int coarse_table_doubles();
 void coarse_tables(const Geo &g, double *tab);   // host: eigenvectors and eigenvalues
 hipError_t launch_coarse_solve(const CoarseArgs &a, hipStream_t s);

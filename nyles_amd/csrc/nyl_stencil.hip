@@ -229,6 +229,44 @@ __device__ __forceinline__ T vf_flux_per(const T *__restrict__ ud, T cu, const T
     return per_face<T>(u1d, om, cb, L, 0, -1);
 }
 
+// One direction of core/fortran_dissipation.f90:2-34 add_laplacian at a
+// cell: d + coef*(fx - fxm), fx = f(m+1) - f(m), fxm = f(m) - f(m-1); on a
+// wall axis fxm = 0 at the first cell and fx = 0 at the last (no flux
+// through the ends of the line).  om, op: byte offsets of the m-1 and m+1
+// neighbours (wrapped on a periodic axis).
+template <typename T, int P>
+__device__ __forceinline__ T lap_axis(T d, const T *__restrict__ f, uint32_t cb, int m, int n,
+                                      int32_t om, int32_t op, T coef) {
+    const T c0 = ldb(f, cb);
+    const T fx = (P || m < n - 1) ? ldb(f, cb + (uint32_t)op) - c0 : T(0);
+    const T fxm = (P || m > 0) ? c0 - ldb(f, cb + (uint32_t)om) : T(0);
+    return d + coef * (fx - fxm);
+}
+
+// add_viscosity (core/viscosity.py:3-9) for the three components of a cell:
+// for each component, the x, y and z directions in that order
+template <typename T, int PX, int PY, int PZ>
+__device__ __forceinline__ void viscosity_cell(const Geo &g, const StageArgs<T> &a, uint32_t cb,
+                                               int i, int j, int k, T &du, T &dv, T &dw) {
+    constexpr int ES = (int)sizeof(T);
+    const int32_t sx = ES, sy = (int32_t)g.sy * ES, sz = (int32_t)g.sz * ES;
+    const int32_t oxm = PX ? (wrapi(i - 1, g.nx) - i) * sx : -sx;
+    const int32_t oxp = PX ? (wrapi(i + 1, g.nx) - i) * sx : sx;
+    const int32_t oym = PY ? (wrapi(j - 1, g.ny) - j) * sy : -sy;
+    const int32_t oyp = PY ? (wrapi(j + 1, g.ny) - j) * sy : sy;
+    const int32_t ozm = PZ ? (wrapi(k - 1, g.nz) - k) * sz : -sz;
+    const int32_t ozp = PZ ? (wrapi(k + 1, g.nz) - k) * sz : sz;
+    const T cx = T(a.vcx), cy = T(a.vcy), cz = T(a.vcz);
+    auto lap3 = [&](T d, const T *__restrict__ f) -> T {
+        d = lap_axis<T, PX>(d, f, cb, i, g.nx, oxm, oxp, cx);
+        d = lap_axis<T, PY>(d, f, cb, j, g.ny, oym, oyp, cy);
+        return lap_axis<T, PZ>(d, f, cb, k, g.nz, ozm, ozp, cz);
+    };
+    du = lap3(du, a.u);
+    dv = lap3(dv, a.v);
+    dw = lap3(dw, a.w);
+}
+
 // Fused LES.rhs (core/model_les.py:129-144) + RK3_SSP stage update
 // (core/timescheme.py:199-220) for one cell.
 template <typename T, int MODE, int PX, int PY, int PZ>
@@ -319,6 +357,8 @@ __device__ __forceinline__ void stage_cell(const Geo &g, const StageArgs<T> &a, 
         const T cff = T(0.5 * a.dz);
         dw = dw - (ldb(a.ke, cb + bez) - kec) + cff * (ldb(a.b, cb + bez) + ldb(a.b, cb));
     }
+    // viscosity at the last stage (core/model_les.py:140-141)
+    if (a.visc) viscosity_cell<T, PX, PY, PZ>(g, a, cb, i, j, k, du, dv, dw);
 
     if (MODE == 0 || MODE == 1 || MODE == 2) {
         stb(a.tb, cb, db); stb(a.tu, cb, du); stb(a.tv, cb, dv); stb(a.tw, cb, dw);
@@ -391,18 +431,560 @@ static void launch_stage_p(const Geo &g, const StageArgs<T> &a, int mode, hipStr
     }
 }
 
+
+// ---------------------------------------------------------------------------
+// The stage kernel as a 2.5-D march (k_stage25).  A block owns a tile of
+// S25_TX x S25_TY columns and walks a range of z planes.  Per plane:
+//  * the fields that the in-plane (x, y) stencils read -- b, the three
+//    vorticity components, u, v, w and ke of plane k, u and v of plane
+//    k+1 -- are staged in LDS, each element loaded once, with the halo each
+//    stencil reaches (3 cells for the WENO stencils, 1 for the averages);
+//  * the x and y tracer face fluxes of the tile are evaluated once each into
+//    LDS (the per-cell kernel evaluates every face twice, once for each
+//    cell beside it);
+//  * the z stencils (the tracer and vortex-force terms 4 and 5) read
+//    per-thread register windows of the column (b over k-3..k+3, vor_i and
+//    vor_j over k-3..k+2), the z face flux of the tracer and the z-averaged
+//    velocities of terms 4 and 5 are carried to the next plane.
+// Per cell that is about 9.5 reconstructions instead of 12, and no stencil
+// value is gathered from L1/L2.  Every value is the per-cell kernel's
+// expression in the same order (stage_cell, vf_flux, tracer_axis), so the
+// results are the same bits (checked on the CPU against k_stage).
+#define S25_TX 64
+#define S25_TY 4
+#define S25_NT (S25_TX * S25_TY)
+#ifndef NYL_STAGE25
+#define NYL_STAGE25 0
+#endif
+// planes per block (the grid needs many blocks to fill the chip)
+#ifndef NYL_S25_BLOCKS
+#define NYL_S25_BLOCKS 4096
+#endif
+
+constexpr int S25_BW = S25_TX + 6, S25_BH = S25_TY + 6;
+constexpr int S25_W1 = S25_TX + 1, S25_H1 = S25_TY + 1;
+
+// Sequencing of the per-cell terms of k_stage25: the LDS column index of
+// the next term passes through an empty asm that takes the previous term's
+// result, so the next term's loads are not hoisted above it and each term
+// holds only its own operands (no instruction is emitted).
+#if NYL_STAGE_SEQ
+#define S25_SEQ(idx, dep) asm volatile("" : "+v"(idx) : "v"(dep))
+#else
+#define S25_SEQ(idx, dep) ((void)0)
+#endif
+
+template <typename T>
+struct S25Lds {
+    T b[S25_BH * S25_BW];     // b(k):   x in [-3, TX+3), y in [-3, TY+3)
+    T oz[S25_BH * S25_BW];    // vor_k:  same box
+    T ox[S25_BH * S25_TX];    // vor_i:  x in [0, TX),    y in [-3, TY+3)
+    T oy[S25_TY * S25_BW];    // vor_j:  x in [-3, TX+3), y in [0, TY)
+    T u0[S25_H1 * S25_W1];    // u(k):   x in [-1, TX),   y in [0, TY+1)
+    T u1[S25_TY * S25_W1];    // u(k+1): x in [-1, TX),   y in [0, TY)
+    T v0[S25_H1 * S25_W1];    // v(k):   x in [0, TX+1),  y in [-1, TY)
+    T v1[S25_H1 * S25_TX];    // v(k+1): x in [0, TX),    y in [-1, TY)
+    T w0[S25_H1 * S25_W1];    // w(k):   x in [0, TX+1),  y in [0, TY+1)
+    T ke[S25_H1 * S25_W1];    // ke(k):  x in [0, TX+1),  y in [0, TY+1)
+    T fx[S25_TY * S25_W1];    // x-face tracer fluxes, faces -1 .. TX-1 of each row
+    T fy[S25_H1 * S25_TX];    // y-face tracer fluxes, faces -1 .. TY-1 of each column
+};
+
+// Columns of the LDS boxes for one lane: x = i0 + X0 + lane (lo) and
+// x = i0 + X0 + 64 + lane (hi, the lanes past 64 of a wide box), wrapped on
+// a periodic axis, -1 off the grid; X0 = -3, -1, 0.  Plane-invariant.
+struct S25Cols {
+    int lo[3], hi[3];
+};
+template <int X0>
+__device__ constexpr int s25_xi() { return X0 == -3 ? 0 : (X0 == -1 ? 1 : 2); }
+
+// One LDS box: field f of plane offset pl (elements) over x in
+// [i0+X0, i0+X0+W), y in [j0+Y0, j0+Y0+H), row-major: wave w fills rows w,
+// w+TY, ... (the row's offset is wave-uniform), each lane one or two
+// columns.  Periodic positions wrap; positions off a wall axis (or past a
+// partial tile) store 0: the closures never read them.
+template <typename T, int PY, int X0, int Y0, int W, int H>
+__device__ __forceinline__ void s25_box(T *__restrict__ dst, const T *__restrict__ f,
+                                        const Geo &g, int j0, int64_t pl, int lane, int wave,
+                                        const S25Cols &cc) {
+    constexpr uint32_t ES = sizeof(T);
+    constexpr int XI = s25_xi<X0>();
+    const int lo = cc.lo[XI], hi = cc.hi[XI];
+    for (int r = wave; r < H; r += S25_TY) {
+        int gj = j0 + Y0 + r;
+        if (PY) gj = gj < 0 ? gj + g.ny : (gj >= g.ny ? gj - g.ny : gj);
+        const bool rok = gj >= 0 && gj < g.ny;
+        const T *__restrict__ row = f + (rok ? pl + (int64_t)gj * g.sy : 0);
+        dst[r * W + lane] = (rok && lo >= 0) ? ldb(row, (uint32_t)lo * ES) : T(0);
+        if (W > S25_TX && lane < W - S25_TX)
+            dst[r * W + S25_TX + lane] = (rok && hi >= 0) ? ldb(row, (uint32_t)hi * ES) : T(0);
+    }
+}
+
+// A window of n values of the thread's column along z, held in scalars (a
+// register array indexed by a selected position would be placed in scratch
+// memory); at(d) with a constant d.
+template <typename T>
+struct S25Win7 {
+    T v0, v1, v2, v3, v4, v5, v6;
+    __device__ __forceinline__ T at(int d) const {
+        switch (d) {
+        case 0: return v0; case 1: return v1; case 2: return v2; case 3: return v3;
+        case 4: return v4; case 5: return v5; default: return v6;
+        }
+    }
+    __device__ __forceinline__ void shift() { v0 = v1; v1 = v2; v2 = v3; v3 = v4; v4 = v5; v5 = v6; }
+};
+
+// vf_flux on staged values: UU(dm) is the averaged contravariant velocity at
+// line position md + dm (dm = -1, 0), om(m) the vorticity at line position m
+template <typename T, int PER, typename UF, typename OF>
+__device__ __forceinline__ T s25_vf(int md, int nd, const UF &UU, const OF &om) {
+    T uu_prev;
+    if (!PER && md == 0) uu_prev = T(0);   // UU_0 = 0. at the line start
+    else uu_prev = UU(-1);
+    const T u1d = T(0.5) * (uu_prev + UU(0));
+    auto q = [&](int m) -> T {             // q(1) = 0., q(k) = vort(k-1)
+        if (!PER && m == 0) return T(0);
+        return om(m - 1);
+    };
+    return face_flux<T, PER>(md, nd, u1d, q);
+}
+
+template <typename T, int MODE, int PX, int PY, int PZ>
+__global__ __launch_bounds__(S25_NT) void k_stage25(Geo g, StageArgs<T> a, int kz) {
+    __shared__ S25Lds<T> L;
+    constexpr uint32_t ES = sizeof(T);
+    const Blk3 bk = xcd_block();
+    const int tid = threadIdx.x;
+    const int tx = tid % S25_TX, ty = tid / S25_TX;
+    const int i0 = bk.x * S25_TX, j0 = bk.y * S25_TY;
+    const int i = i0 + tx, j = j0 + ty;
+    const int nz = g.nz;
+    const int k0 = bk.z * kz, k1 = k0 + kz < nz ? k0 + kz : nz;
+    const bool live = i < g.nx && j < g.ny;
+    // the thread's column (threads past a partial tile use column 0, and
+    // compute nothing from it)
+    const int ic = live ? i : 0, jc = live ? j : 0;
+    const int64_t col = (int64_t)jc * g.sy + ic;
+    const T cx = T(a.idx2), cy = T(a.idy2), cz = T(a.idz2);
+    auto zw = [&](int kk) -> int { return PZ ? (kk < 0 ? kk + nz : (kk >= nz ? kk - nz : kk)) : kk; };
+    auto zok = [&](int kk) -> bool { return PZ || (kk >= 0 && kk < nz); };
+    // field f at the thread's column (offset dc within the plane), plane kk
+    auto colv = [&](const T *__restrict__ f, int kk, int64_t dc) -> T {
+        return zok(kk) ? ldb(f, (uint32_t)((int64_t)zw(kk) * g.sz + col + dc) * ES) : T(0);
+    };
+    // offsets of the x+1 and y+1 neighbours of the column (0 when absent)
+    const int64_t dxp = (PX || ic + 1 < g.nx) ? (int64_t)((PX && ic + 1 >= g.nx) ? 1 - g.nx : 1)
+                                              : 0;
+    const int64_t dyp = (PY || jc + 1 < g.ny) ? ((PY && jc + 1 >= g.ny) ? (1 - g.ny) * g.sy : g.sy)
+                                              : 0;
+    // register windows along z: b over k-3 .. k+3, vor_i and vor_j over
+    // k-3 .. k+2 (the last slot of those unused)
+    S25Win7<T> bw, xw, yw;
+    bw.v0 = colv(a.b, k0 - 3, 0); bw.v1 = colv(a.b, k0 - 2, 0); bw.v2 = colv(a.b, k0 - 1, 0);
+    bw.v3 = colv(a.b, k0, 0); bw.v4 = colv(a.b, k0 + 1, 0); bw.v5 = colv(a.b, k0 + 2, 0);
+    xw.v0 = colv(a.ox, k0 - 3, 0); xw.v1 = colv(a.ox, k0 - 2, 0); xw.v2 = colv(a.ox, k0 - 1, 0);
+    xw.v3 = colv(a.ox, k0, 0); xw.v4 = colv(a.ox, k0 + 1, 0);
+    yw.v0 = colv(a.oy, k0 - 3, 0); yw.v1 = colv(a.oy, k0 - 2, 0); yw.v2 = colv(a.oy, k0 - 1, 0);
+    yw.v3 = colv(a.oy, k0, 0); yw.v4 = colv(a.oy, k0 + 1, 0);
+    bw.v6 = xw.v5 = xw.v6 = yw.v5 = yw.v6 = T(0);
+    // carried from the plane below: the tracer flux through face k-1 and the
+    // z-averaged U_k of vortex-force terms 4 (averaged in y) and 5 (in x)
+    T fzl = T(0), uu4p = T(0), uu5p = T(0);
+    if (PZ || k0 > 0) {
+        const T wm = colv(a.w, k0 - 1, 0);
+        fzl = face_flux_rel<T, PZ>(zw(k0 - 1), nz, wm * cz,
+                                   [&](int d) -> T { return bw.at(d + 2); });
+        uu4p = T(0.5) * (wm * cz + colv(a.w, k0 - 1, dyp) * cz);
+        uu5p = T(0.5) * (wm * cz + colv(a.w, k0 - 1, dxp) * cz);
+    }
+    // the lane's box columns
+    S25Cols cc;
+    {
+        const int x0s[3] = {-3, -1, 0};
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            int lo = i0 + x0s[q] + tx, hi = lo + S25_TX;
+            if (PX) {
+                lo = lo < 0 ? lo + g.nx : (lo >= g.nx ? lo - g.nx : lo);
+                hi = hi >= g.nx ? hi - g.nx : hi;
+            }
+            cc.lo[q] = (lo >= 0 && lo < g.nx) ? lo : -1;
+            cc.hi[q] = (hi >= 0 && hi < g.nx) ? hi : -1;
+        }
+    }
+    for (int k = k0; k < k1; k++) {
+        const int64_t pl = (int64_t)k * g.sz;
+        const bool has_kp = zok(k + 1);
+        const int64_t plp = (int64_t)zw(k + 1) * g.sz;
+        bw.v6 = colv(a.b, k + 3, 0);
+        xw.v5 = colv(a.ox, k + 2, 0);
+        yw.v5 = colv(a.oy, k + 2, 0);
+        const T ke_up = has_kp ? ldb(a.ke, (uint32_t)(plp + col) * ES) : T(0);
+        s25_box<T, PY, -3, -3, S25_BW, S25_BH>(L.b, a.b, g, j0, pl, tx, ty, cc);
+        s25_box<T, PY, -3, -3, S25_BW, S25_BH>(L.oz, a.oz, g, j0, pl, tx, ty, cc);
+        s25_box<T, PY, 0, -3, S25_TX, S25_BH>(L.ox, a.ox, g, j0, pl, tx, ty, cc);
+        s25_box<T, PY, -3, 0, S25_BW, S25_TY>(L.oy, a.oy, g, j0, pl, tx, ty, cc);
+        s25_box<T, PY, -1, 0, S25_W1, S25_H1>(L.u0, a.u, g, j0, pl, tx, ty, cc);
+        s25_box<T, PY, 0, -1, S25_W1, S25_H1>(L.v0, a.v, g, j0, pl, tx, ty, cc);
+        s25_box<T, PY, 0, 0, S25_W1, S25_H1>(L.w0, a.w, g, j0, pl, tx, ty, cc);
+        s25_box<T, PY, 0, 0, S25_W1, S25_H1>(L.ke, a.ke, g, j0, pl, tx, ty, cc);
+        if (has_kp) {
+            s25_box<T, PY, -1, 0, S25_W1, S25_TY>(L.u1, a.u, g, j0, plp, tx, ty, cc);
+            s25_box<T, PY, 0, -1, S25_TX, S25_H1>(L.v1, a.v, g, j0, plp, tx, ty, cc);
+        }
+        __syncthreads();
+        // tracer face fluxes of the tile: x faces i0-1 .. i0+TX-1 of each
+        // row, y faces j0-1 .. j0+TY-1 of each column.  Wave w: the x faces
+        // i0-1 .. i0+TX-2 of row w and the y faces of row j0+w-1; then wave 0
+        // the y faces of row j0+TY-1 and wave 1 the last x face of each row.
+        {
+            auto xface = [&](int r, int fr) -> T {
+                const int f = i0 + fr;
+                if (!((PX || f >= 0) && f < g.nx && j0 + r < g.ny)) return T(0);
+                auto q = [&](int m) -> T { return L.b[(r + 3) * S25_BW + (m - i0 + 3)]; };
+                return face_flux<T, PX>(f, g.nx, L.u0[r * S25_W1 + fr + 1] * cx, q);
+            };
+            auto yface = [&](int c, int fr) -> T {
+                const int f = j0 + fr;
+                if (!((PY || f >= 0) && f < g.ny && i0 + c < g.nx)) return T(0);
+                auto q = [&](int m) -> T { return L.b[(m - j0 + 3) * S25_BW + (c + 3)]; };
+                return face_flux<T, PY>(f, g.ny, L.v0[(fr + 1) * S25_W1 + c] * cy, q);
+            };
+            const T fxv = xface(ty, tx - 1);
+            L.fx[ty * S25_W1 + tx] = fxv;
+            int cq = tx;
+            S25_SEQ(cq, fxv);
+            const T fyv = yface(cq, ty - 1);
+            L.fy[ty * S25_TX + tx] = fyv;
+            S25_SEQ(cq, fyv);
+            if (ty == 0) L.fy[S25_TY * S25_TX + tx] = yface(cq, S25_TY - 1);
+            else if (ty == 1 && tx < S25_TY) L.fx[tx * S25_W1 + S25_TX] = xface(cq, S25_TX - 1);
+        }
+        __syncthreads();
+        if (live) {
+            const uint32_t cb = (uint32_t)(pl + col) * ES;
+            // tracer b: rhstrac, directions i, j, k (core/tracer.py:63-72)
+            T db = T(0);
+            {
+                const T fl = L.fx[ty * S25_W1 + tx], fr = L.fx[ty * S25_W1 + tx + 1];
+                if (!PX && i == 0) db = db - fr;
+                else db = (db + fl) - fr;
+            }
+            {
+                const T fl = L.fy[ty * S25_TX + tx], fr = L.fy[(ty + 1) * S25_TX + tx];
+                if (!PY && j == 0) db = db - fr;
+                else db = (db + fl) - fr;
+            }
+            const T wc = L.w0[ty * S25_W1 + tx];
+            {
+                const T fr = face_flux_rel<T, PZ>(k, nz, wc * cz,
+                                                  [&](int d) -> T { return bw.at(d + 3); });
+                if (!PZ && k == 0) db = db - fr;
+                else db = (db + fzl) - fr;
+                fzl = fr;
+            }
+            // vortex force, triplets 'ikj', 'jik', 'kji' (core/vortex_force.py:69-81)
+            const bool ex = PX || i < g.nx - 1;
+            const bool ey = PY || j < g.ny - 1;
+            const bool ez = PZ || k < nz - 1;
+            int lx = tx;             // the LDS column, sequenced between the terms
+            S25_SEQ(lx, db);
+            const T uu4 = T(0.5) * (wc * cz + L.w0[(ty + 1) * S25_W1 + lx] * cz);
+            const T uu5 = T(0.5) * (wc * cz + L.w0[ty * S25_W1 + lx + 1] * cz);
+            T du = T(0), dv = T(0), dw = T(0);
+            // direc(U_i, vor_k, du_j): along x, U_i averaged in y
+            if (ey)
+                dv = dv - s25_vf<T, PX>(i, g.nx,
+                    [&](int dm) -> T {
+                        return T(0.5) * (L.u0[ty * S25_W1 + lx + 1 + dm] * cx +
+                                         L.u0[(ty + 1) * S25_W1 + lx + 1 + dm] * cx);
+                    },
+                    [&](int m) -> T { return L.oz[(ty + 3) * S25_BW + (m - i + lx + 3)]; });
+            S25_SEQ(lx, dv);
+            // flip(U_j, vor_k, du_i): along y, U_j averaged in x
+            if (ex)
+                du = du + s25_vf<T, PY>(j, g.ny,
+                    [&](int dm) -> T {
+                        return T(0.5) * (L.v0[(ty + 1 + dm) * S25_W1 + lx] * cy +
+                                         L.v0[(ty + 1 + dm) * S25_W1 + lx + 1] * cy);
+                    },
+                    [&](int m) -> T { return L.oz[(m - j + ty + 3) * S25_BW + (lx + 3)]; });
+            S25_SEQ(lx, du);
+            // direc(U_j, vor_i, du_k): along y, U_j averaged in z
+            if (ez)
+                dw = dw - s25_vf<T, PY>(j, g.ny,
+                    [&](int dm) -> T {
+                        return T(0.5) * (L.v0[(ty + 1 + dm) * S25_W1 + lx] * cy +
+                                         L.v1[(ty + 1 + dm) * S25_TX + lx] * cy);
+                    },
+                    [&](int m) -> T { return L.ox[(m - j + ty + 3) * S25_TX + lx]; });
+            S25_SEQ(lx, dw);
+            // flip(U_k, vor_i, du_j): along z, U_k averaged in y
+            if (ey) {
+                const T uu_prev = (!PZ && k == 0) ? T(0) : uu4p;
+                const T u1d = T(0.5) * (uu_prev + uu4);
+                dv = dv + face_flux_rel<T, PZ>(k, nz, u1d, [&](int d) -> T {
+                    if (!PZ && k + d == 0) return T(0);
+                    return xw.at(d + 2);
+                });
+            }
+            // direc(U_k, vor_j, du_i): along z, U_k averaged in x
+            if (ex) {
+                const T uu_prev = (!PZ && k == 0) ? T(0) : uu5p;
+                const T u1d = T(0.5) * (uu_prev + uu5);
+                du = du - face_flux_rel<T, PZ>(k, nz, u1d, [&](int d) -> T {
+                    if (!PZ && k + d == 0) return T(0);
+                    return yw.at(d + 2);
+                });
+            }
+            uu4p = uu4;
+            uu5p = uu5;
+            S25_SEQ(lx, du);
+            // flip(U_i, vor_j, du_k): along x, U_i averaged in z
+            if (ez)
+                dw = dw + s25_vf<T, PX>(i, g.nx,
+                    [&](int dm) -> T {
+                        return T(0.5) * (L.u0[ty * S25_W1 + lx + 1 + dm] * cx +
+                                         L.u1[ty * S25_W1 + lx + 1 + dm] * cx);
+                    },
+                    [&](int m) -> T { return L.oy[ty * S25_BW + (m - i + lx + 3)]; });
+
+            // bernoulli (core/bernoulli.py:14-23, core/fortran_bernoulli.f90:2-58)
+            const T kec = L.ke[ty * S25_W1 + tx];
+            if (ex) du = du - (L.ke[ty * S25_W1 + tx + 1] - kec);
+            if (ey) dv = dv - (L.ke[(ty + 1) * S25_W1 + tx] - kec);
+            if (ez) {
+                const T cff = T(0.5 * a.dz);
+                dw = dw - (ke_up - kec) + cff * (bw.v4 + bw.v3);
+            }
+            // viscosity at the last stage (core/model_les.py:140-141)
+            if (a.visc) viscosity_cell<T, PX, PY, PZ>(g, a, cb, i, j, k, du, dv, dw);
+
+            const T b0 = bw.v3, u0 = L.u0[ty * S25_W1 + tx + 1];
+            const T v0 = L.v0[(ty + 1) * S25_W1 + tx], w0 = wc;
+            if (MODE == 0 || MODE == 1 || MODE == 2) {
+                stb(a.tb, cb, db); stb(a.tu, cb, du); stb(a.tv, cb, dv); stb(a.tw, cb, dw);
+            }
+            const T cc = T(a.c);
+            if (MODE == 1) {            // s += dt*ds0
+                stb(a.bn, cb, b0 + cc * db);
+                stb(a.un, cb, u0 + cc * du);
+                stb(a.vn, cb, v0 + cc * dv);
+                stb(a.wn, cb, w0 + cc * dw);
+            } else if (MODE == 2) {     // s += (dt/4.)*(ds1-3*ds0)
+                stb(a.bn, cb, b0 + cc * (db - T(3) * ldb(a.d0b, cb)));
+                stb(a.un, cb, u0 + cc * (du - T(3) * ldb(a.d0u, cb)));
+                stb(a.vn, cb, v0 + cc * (dv - T(3) * ldb(a.d0v, cb)));
+                stb(a.wn, cb, w0 + cc * (dw - T(3) * ldb(a.d0w, cb)));
+            } else if (MODE == 3) {     // s += (dt/12.)*(8*ds2-ds0-ds1)
+                stb(a.bn, cb, b0 + cc * ((T(8) * db - ldb(a.d0b, cb)) - ldb(a.d1b, cb)));
+                stb(a.un, cb, u0 + cc * ((T(8) * du - ldb(a.d0u, cb)) - ldb(a.d1u, cb)));
+                stb(a.vn, cb, v0 + cc * ((T(8) * dv - ldb(a.d0v, cb)) - ldb(a.d1v, cb)));
+                stb(a.wn, cb, w0 + cc * ((T(8) * dw - ldb(a.d0w, cb)) - ldb(a.d1w, cb)));
+            } else if (MODE == 4) {     // LFAM3 predictor (core/timescheme.py:144-162)
+                const T c2 = T(a.c2), c12 = T(1. / 12.);
+                T sb, t1;
+                sb = ldb(a.sbb, cb); t1 = sb + c2 * db; stb(a.bn, cb, c12 * (T(5) * t1 + T(8) * b0 - sb)); stb(a.sbb, cb, b0);
+                sb = ldb(a.sbu, cb); t1 = sb + c2 * du; stb(a.un, cb, c12 * (T(5) * t1 + T(8) * u0 - sb)); stb(a.sbu, cb, u0);
+                sb = ldb(a.sbv, cb); t1 = sb + c2 * dv; stb(a.vn, cb, c12 * (T(5) * t1 + T(8) * v0 - sb)); stb(a.sbv, cb, v0);
+                sb = ldb(a.sbw, cb); t1 = sb + c2 * dw; stb(a.wn, cb, c12 * (T(5) * t1 + T(8) * w0 - sb)); stb(a.sbw, cb, w0);
+            } else if (MODE == 5) {     // LFAM3 corrector: s = sn + dt*ds
+                stb(a.bn, cb, ldb(a.bn, cb) + cc * db);
+                stb(a.un, cb, ldb(a.un, cb) + cc * du);
+                stb(a.vn, cb, ldb(a.vn, cb) + cc * dv);
+                stb(a.wn, cb, ldb(a.wn, cb) + cc * dw);
+            } else if (MODE == 6) {     // first LFAM3 step: sn = s; sb = s; s += dt*ds
+                stb(a.sbb, cb, b0); stb(a.sbu, cb, u0); stb(a.sbv, cb, v0); stb(a.sbw, cb, w0);
+                stb(a.bn, cb, b0 + cc * db);
+                stb(a.un, cb, u0 + cc * du);
+                stb(a.vn, cb, v0 + cc * dv);
+                stb(a.wn, cb, w0 + cc * dw);
+            }
+        }
+        bw.shift();
+        xw.shift();
+        yw.shift();
+        __syncthreads();
+    }
+}
+
+static inline int s25_kz(const Geo &g) {
+    const int64_t tiles = (int64_t)((g.nx + S25_TX - 1) / S25_TX) * ((g.ny + S25_TY - 1) / S25_TY);
+    int64_t chunks = (NYL_S25_BLOCKS + tiles - 1) / tiles;
+    if (chunks > g.nz) chunks = g.nz;
+    if (chunks < 1) chunks = 1;
+    int kz = (int)((g.nz + chunks - 1) / chunks);
+    return kz < 1 ? 1 : kz;
+}
+
+// the 2.5-D form needs a full-width tile row (narrower grids keep the
+// per-cell kernel)
+static inline bool s25_fits(const Geo &g) { return NYL_STAGE25 && g.nx >= S25_TX; }
+
+template <typename T, int PX, int PY, int PZ>
+static void launch_stage25_p(const Geo &g, const StageArgs<T> &a, int mode, hipStream_t s) {
+    const int kz = s25_kz(g);
+    dim3 blk(S25_NT), grd((g.nx + S25_TX - 1) / S25_TX, (g.ny + S25_TY - 1) / S25_TY,
+                          (g.nz + kz - 1) / kz);
+    switch (mode) {
+    case 0: hipLaunchKernelGGL((k_stage25<T, 0, PX, PY, PZ>), grd, blk, 0, s, g, a, kz); break;
+    case 1: hipLaunchKernelGGL((k_stage25<T, 1, PX, PY, PZ>), grd, blk, 0, s, g, a, kz); break;
+    case 2: hipLaunchKernelGGL((k_stage25<T, 2, PX, PY, PZ>), grd, blk, 0, s, g, a, kz); break;
+    case 3: hipLaunchKernelGGL((k_stage25<T, 3, PX, PY, PZ>), grd, blk, 0, s, g, a, kz); break;
+    case 4: hipLaunchKernelGGL((k_stage25<T, 4, PX, PY, PZ>), grd, blk, 0, s, g, a, kz); break;
+    case 5: hipLaunchKernelGGL((k_stage25<T, 5, PX, PY, PZ>), grd, blk, 0, s, g, a, kz); break;
+    case 6: hipLaunchKernelGGL((k_stage25<T, 6, PX, PY, PZ>), grd, blk, 0, s, g, a, kz); break;
+    default: break;
+    }
+}
+
 template <typename T>
 hipError_t launch_stage(const Geo &g, const StageArgs<T> &a, int mode, hipStream_t s) {
     if (mode < 0 || mode > 6) return hipErrorInvalidValue;
+    const bool t25 = s25_fits(g);
     switch (g.px + 2 * g.py + 4 * g.pz) {
-    case 0: launch_stage_p<T, 0, 0, 0>(g, a, mode, s); break;
-    case 1: launch_stage_p<T, 1, 0, 0>(g, a, mode, s); break;
-    case 2: launch_stage_p<T, 0, 1, 0>(g, a, mode, s); break;
-    case 3: launch_stage_p<T, 1, 1, 0>(g, a, mode, s); break;
-    case 4: launch_stage_p<T, 0, 0, 1>(g, a, mode, s); break;
-    case 5: launch_stage_p<T, 1, 0, 1>(g, a, mode, s); break;
-    case 6: launch_stage_p<T, 0, 1, 1>(g, a, mode, s); break;
-    default: launch_stage_p<T, 1, 1, 1>(g, a, mode, s); break;
+#define NYL_STAGE_CASE(c, X, Y, Z)                                                   \
+    case c:                                                                          \
+        if (t25) launch_stage25_p<T, X, Y, Z>(g, a, mode, s);                        \
+        else launch_stage_p<T, X, Y, Z>(g, a, mode, s);                              \
+        break;
+    NYL_STAGE_CASE(0, 0, 0, 0)
+    NYL_STAGE_CASE(1, 1, 0, 0)
+    NYL_STAGE_CASE(2, 0, 1, 0)
+    NYL_STAGE_CASE(3, 1, 1, 0)
+    NYL_STAGE_CASE(4, 0, 0, 1)
+    NYL_STAGE_CASE(5, 1, 0, 1)
+    NYL_STAGE_CASE(6, 0, 1, 1)
+    default:
+        if (t25) launch_stage25_p<T, 1, 1, 1>(g, a, mode, s);
+        else launch_stage_p<T, 1, 1, 1>(g, a, mode, s);
+        break;
+#undef NYL_STAGE_CASE
+    }
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Passive tracers: the b part of stage_cell for another scalar, with the
+// stage update of the mode (same expressions as b's).
+template <typename T, int MODE, int PX, int PY, int PZ>
+__device__ __forceinline__ void tracer_cell(const Geo &g, const TracerArgs<T> &a, int i, int j,
+                                            int k) {
+    const int64_t c = (int64_t)k * g.sz + (int64_t)j * g.sy + i;
+    constexpr int ES = (int)sizeof(T);
+    const uint32_t cb = (uint32_t)c * (uint32_t)ES;
+    const T cx = T(a.idx2), cy = T(a.idy2), cz = T(a.idz2);
+    T dq = T(0);
+    const int64_t bx = c - i, by = c - (int64_t)j * g.sy, bz = c - (int64_t)k * g.sz;
+    if constexpr (PX != 0) dq = tracer_axis_per<T>(dq, a.q, a.u, cx, cb, per_line(i, g.nx, ES));
+    else dq = tracer_axis<T, 0>(dq, a.q, a.u, cx, bx, 1, i, g.nx);
+    if constexpr (PY != 0)
+        dq = tracer_axis_per<T>(dq, a.q, a.v, cy, cb, per_line(j, g.ny, (int)g.sy * ES));
+    else dq = tracer_axis<T, 0>(dq, a.q, a.v, cy, by, g.sy, j, g.ny);
+    if constexpr (PZ != 0)
+        dq = tracer_axis_per<T>(dq, a.q, a.w, cz, cb, per_line(k, g.nz, (int)g.sz * ES));
+    else dq = tracer_axis<T, 0>(dq, a.q, a.w, cz, bz, g.sz, k, g.nz);
+
+    if (MODE == 0 || MODE == 1 || MODE == 2) stb(a.tout, cb, dq);
+    const T cc = T(a.c);
+    if (MODE == 1) {
+        stb(a.qn, cb, ldb(a.q, cb) + cc * dq);
+    } else if (MODE == 2) {
+        stb(a.qn, cb, ldb(a.q, cb) + cc * (dq - T(3) * ldb(a.d0, cb)));
+    } else if (MODE == 3) {
+        stb(a.qn, cb, ldb(a.q, cb) + cc * ((T(8) * dq - ldb(a.d0, cb)) - ldb(a.d1, cb)));
+    } else if (MODE == 4) {
+        const T c2 = T(a.c2), c12 = T(1. / 12.);
+        const T sb = ldb(a.sb, cb), t1 = sb + c2 * dq;
+        stb(a.qn, cb, c12 * (T(5) * t1 + T(8) * ldb(a.q, cb) - sb));
+        stb(a.sb, cb, ldb(a.q, cb));
+    } else if (MODE == 5) {
+        stb(a.qn, cb, ldb(a.qn, cb) + cc * dq);
+    } else if (MODE == 6) {
+        stb(a.sb, cb, ldb(a.q, cb));
+        stb(a.qn, cb, ldb(a.q, cb) + cc * dq);
+    }
+}
+
+template <typename T, int MODE, int PX, int PY, int PZ>
+__global__ __launch_bounds__(256) void k_tracer_stage(Geo g, TracerArgs<T> a) {
+    const Blk3 bk = xcd_block();
+    const int i = bk.x * NYL_BX + threadIdx.x;
+    const int j = bk.y * NYL_BY + threadIdx.y;
+    if (i >= g.nx || j >= g.ny) return;
+    tracer_cell<T, MODE, PX, PY, PZ>(g, a, i, j, bk.z);
+}
+
+template <typename T, int PX, int PY, int PZ>
+static void launch_tracer_p(const Geo &g, const TracerArgs<T> &a, int mode, hipStream_t s) {
+    dim3 blk(NYL_BX, NYL_BY, 1), grd = cell_grid(g);
+    switch (mode) {
+    case 0: hipLaunchKernelGGL((k_tracer_stage<T, 0, PX, PY, PZ>), grd, blk, 0, s, g, a); break;
+    case 1: hipLaunchKernelGGL((k_tracer_stage<T, 1, PX, PY, PZ>), grd, blk, 0, s, g, a); break;
+    case 2: hipLaunchKernelGGL((k_tracer_stage<T, 2, PX, PY, PZ>), grd, blk, 0, s, g, a); break;
+    case 3: hipLaunchKernelGGL((k_tracer_stage<T, 3, PX, PY, PZ>), grd, blk, 0, s, g, a); break;
+    case 4: hipLaunchKernelGGL((k_tracer_stage<T, 4, PX, PY, PZ>), grd, blk, 0, s, g, a); break;
+    case 5: hipLaunchKernelGGL((k_tracer_stage<T, 5, PX, PY, PZ>), grd, blk, 0, s, g, a); break;
+    case 6: hipLaunchKernelGGL((k_tracer_stage<T, 6, PX, PY, PZ>), grd, blk, 0, s, g, a); break;
+    default: break;
+    }
+}
+
+template <typename T>
+hipError_t launch_tracer_stage(const Geo &g, const TracerArgs<T> &a, int mode, hipStream_t s) {
+    if (mode < 0 || mode > 6) return hipErrorInvalidValue;
+    switch (g.px + 2 * g.py + 4 * g.pz) {
+    case 0: launch_tracer_p<T, 0, 0, 0>(g, a, mode, s); break;
+    case 1: launch_tracer_p<T, 1, 0, 0>(g, a, mode, s); break;
+    case 2: launch_tracer_p<T, 0, 1, 0>(g, a, mode, s); break;
+    case 3: launch_tracer_p<T, 1, 1, 0>(g, a, mode, s); break;
+    case 4: launch_tracer_p<T, 0, 0, 1>(g, a, mode, s); break;
+    case 5: launch_tracer_p<T, 1, 0, 1>(g, a, mode, s); break;
+    case 6: launch_tracer_p<T, 0, 1, 1>(g, a, mode, s); break;
+    default: launch_tracer_p<T, 1, 1, 1>(g, a, mode, s); break;
+    }
+    return hipGetLastError();
+}
+
+// The stage combinations of stage_cell with the tendency read from memory
+template <typename T, int MODE>
+__global__ __launch_bounds__(256) void k_update(int64_t n, UpdateArgs<T> a) {
+    const int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (q >= n) return;
+    const T cc = T(a.c);
+    if (MODE == 1) {
+        a.sn[q] = a.s[q] + cc * a.ds0[q];
+    } else if (MODE == 2) {
+        a.sn[q] = a.s[q] + cc * (a.ds1[q] - T(3) * a.ds0[q]);
+    } else if (MODE == 3) {
+        a.sn[q] = a.s[q] + cc * ((T(8) * a.ds2[q] - a.ds0[q]) - a.ds1[q]);
+    } else if (MODE == 4) {
+        const T c2 = T(a.c2), c12 = T(1. / 12.);
+        const T sb = a.sb[q], t1 = sb + c2 * a.ds0[q];
+        a.sn[q] = c12 * (T(5) * t1 + T(8) * a.s[q] - sb);
+        a.sb[q] = a.s[q];
+    } else if (MODE == 5) {
+        a.sn[q] = a.sn[q] + cc * a.ds0[q];
+    } else if (MODE == 6) {
+        a.sb[q] = a.s[q];
+        a.sn[q] = a.s[q] + cc * a.ds0[q];
+    }
+}
+
+template <typename T>
+hipError_t launch_update(int64_t n, const UpdateArgs<T> &a, int mode, hipStream_t s) {
+    const dim3 grd((unsigned)((n + 255) / 256));
+    switch (mode) {
+    case 1: hipLaunchKernelGGL((k_update<T, 1>), grd, dim3(256), 0, s, n, a); break;
+    case 2: hipLaunchKernelGGL((k_update<T, 2>), grd, dim3(256), 0, s, n, a); break;
+    case 3: hipLaunchKernelGGL((k_update<T, 3>), grd, dim3(256), 0, s, n, a); break;
+    case 4: hipLaunchKernelGGL((k_update<T, 4>), grd, dim3(256), 0, s, n, a); break;
+    case 5: hipLaunchKernelGGL((k_update<T, 5>), grd, dim3(256), 0, s, n, a); break;
+    case 6: hipLaunchKernelGGL((k_update<T, 6>), grd, dim3(256), 0, s, n, a); break;
+    default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
@@ -916,6 +1498,12 @@ NYL_B_STENCIL_INST(float)
 
 template hipError_t launch_stage<double>(const Geo &, const StageArgs<double> &, int, hipStream_t);
 template hipError_t launch_stage<float>(const Geo &, const StageArgs<float> &, int, hipStream_t);
+template hipError_t launch_tracer_stage<double>(const Geo &, const TracerArgs<double> &, int,
+                                                hipStream_t);
+template hipError_t launch_tracer_stage<float>(const Geo &, const TracerArgs<float> &, int,
+                                               hipStream_t);
+template hipError_t launch_update<double>(int64_t, const UpdateArgs<double> &, int, hipStream_t);
+template hipError_t launch_update<float>(int64_t, const UpdateArgs<float> &, int, hipStream_t);
 template hipError_t launch_diag<double>(const Geo &, const DiagArgs<double> &, int, hipStream_t);
 template hipError_t launch_diag<float>(const Geo &, const DiagArgs<float> &, int, hipStream_t);
 template hipError_t launch_div<double, double>(const Geo &, const double *, const double *,

@@ -30,7 +30,7 @@ class DeviceModel:
 
     def __init__(self, nx, ny, nz, periodic, dx, dy, dz, idx2=None, idy2=None, idz2=None,
                  dx2=None, fparameter=0.0, precision=L.NYL_PREC_FP64, mg_maxite=20,
-                 mg_tol=1e-6, mg_omega=0.9, device=None):
+                 mg_tol=1e-6, mg_omega=0.9, device=None, viscosity=0.0, n_tracers=0):
         lib = L.load()
         if not torch.cuda.is_available():
             raise L.NylesHipError("no GPU visible: the Nyles time step runs only on the GPU")
@@ -49,6 +49,9 @@ class DeviceModel:
         d.mg_maxite = mg_maxite
         d.mg_tol = mg_tol
         d.mg_omega = mg_omega
+        d.viscosity = viscosity
+        d.n_tracers = n_tracers
+        self.n_tracers = n_tracers
         self.desc = d
         ctx = ctypes.c_void_p()
         L.check(lib.nyl_ctx_create(ctypes.byref(d), ctypes.byref(ctx)))
@@ -101,6 +104,11 @@ class DeviceModel:
 
     def rk3_stage(self, stage, coef):
         L.check(self.lib.nyl_rk3_stage(self.ctx, stage, float(coef)))
+
+    def stage_update(self, mode, c, c2=0.0):
+        """stage combination of mode 1..6 from the stored tendencies
+        (nyl_stage_update)"""
+        L.check(self.lib.nyl_stage_update(self.ctx, int(mode), float(c), float(c2)))
 
     def diagnose(self):
         st = L.NylMGStats()

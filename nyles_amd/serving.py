@@ -318,15 +318,22 @@ class Server:
 
 
 def poisson_jobs(count, rate, seed=5, sizes=(32, 64, 128), steps=(50, 200),
-                 kinds=("lock", "tgv", "shear")):
+                 kinds=("lock", "tgv", "shear"), size_range=None):
     """`count` jobs arriving as a Poisson process of `rate` per second; kind
-    uniform, n log-uniform over `sizes`, steps uniform in [steps[0], steps[1]]"""
+    uniform, steps uniform in [steps[0], steps[1]]; n uniform over `sizes`,
+    or, with size_range = (lo, hi), log-uniform in [lo, hi] rounded to a
+    multiple of 8"""
     rng = np.random.default_rng(seed)
     t = 0.0
     jobs = []
     for q in range(count):
         t += rng.exponential(1.0 / rate)
-        jobs.append(Job(kind=str(rng.choice(kinds)), n=int(rng.choice(sizes)),
-                        steps=int(rng.integers(steps[0], steps[1] + 1)),
+        kind = str(rng.choice(kinds))
+        if size_range is None:
+            n = int(rng.choice(sizes))
+        else:
+            v = math.exp(rng.uniform(math.log(size_range[0]), math.log(size_range[1])))
+            n = int(8 * round(v / 8))
+        jobs.append(Job(kind=kind, n=n, steps=int(rng.integers(steps[0], steps[1] + 1)),
                         seed=int(rng.integers(0, 2 ** 31)), arrival=t))
     return jobs

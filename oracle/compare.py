@@ -11,14 +11,17 @@ GEOMETRY_XYZ = {g: tuple(reversed(p)) for g, p in O.GEOMETRY_PERIODIC.items()}
 
 
 def make_pair(nx, ny, nz, geometry, Lx=1.0, Ly=1.0, Lz=1.0, rotating=False, coriolis=1.0,
-              mg_maxite=20, mg_tol=1e-6, mg_omega=0.9, precision=0):
+              mg_maxite=20, mg_tol=1e-6, mg_omega=0.9, precision=0, viscosity=0.0,
+              n_tracers=0):
     from nyles_amd.device import DeviceModel
     orc = O.OracleLES(nx, ny, nz, Lx, Ly, Lz, geometry, rotating=rotating, coriolis=coriolis,
-                      mg_maxite=mg_maxite, mg_tol=mg_tol, mg_omega=mg_omega)
+                      mg_maxite=mg_maxite, mg_tol=mg_tol, mg_omega=mg_omega,
+                      viscosity=viscosity, n_tracers=n_tracers)
     dev = DeviceModel(nx, ny, nz, GEOMETRY_XYZ[geometry], orc.dx, orc.dy, orc.dz,
                       idx2=orc.idx2, idy2=orc.idy2, idz2=orc.idz2, dx2=orc.dx ** 2,
                       fparameter=orc.fparameter, mg_maxite=mg_maxite, mg_tol=mg_tol,
-                      mg_omega=mg_omega, precision=precision)
+                      mg_omega=mg_omega, precision=precision, viscosity=viscosity,
+                      n_tracers=n_tracers)
     return orc, dev
 
 
@@ -43,6 +46,11 @@ def random_state(orc, seed, amp_u=1e-2, amp_b=1.0):
 
 def load_state(orc, dev, st):
     from nyles_amd import _lib as L
+    for q in range(len(orc.tracers)):
+        name = "t%d" % q
+        if name in st:
+            orc.set_interior(orc.tracers[name], st[name])
+            dev.upload(L.NYL_F_TRACER(q), st[name])
     orc.set_interior(orc.b, st["b"])
     orc.set_interior(orc.u["i"], st["u_i"])
     orc.set_interior(orc.u["j"], st["u_j"])
@@ -55,10 +63,13 @@ def load_state(orc, dev, st):
 
 def oracle_fields(orc):
     I = orc.interior
-    return {"b": I(orc.b), "u_i": I(orc.u["i"]), "u_j": I(orc.u["j"]), "u_k": I(orc.u["k"]),
-            "ke": I(orc.ke), "vor_i": I(orc.vor["i"]), "vor_j": I(orc.vor["j"]),
-            "vor_k": I(orc.vor["k"]), "U_i": I(orc.U["i"]), "U_j": I(orc.U["j"]),
-            "U_k": I(orc.U["k"])}
+    out = {"b": I(orc.b), "u_i": I(orc.u["i"]), "u_j": I(orc.u["j"]), "u_k": I(orc.u["k"]),
+           "ke": I(orc.ke), "vor_i": I(orc.vor["i"]), "vor_j": I(orc.vor["j"]),
+           "vor_k": I(orc.vor["k"]), "U_i": I(orc.U["i"]), "U_j": I(orc.U["j"]),
+           "U_k": I(orc.U["k"])}
+    for name, t in orc.tracers.items():
+        out[name] = I(t)
+    return out
 
 
 def device_fields(dev):
@@ -67,6 +78,8 @@ def device_fields(dev):
     ids = {"b": L.NYL_F_B, "u_i": L.NYL_F_U, "u_j": L.NYL_F_V, "u_k": L.NYL_F_W,
            "ke": L.NYL_F_KE, "vor_i": L.NYL_F_VORX, "vor_j": L.NYL_F_VORY,
            "vor_k": L.NYL_F_VORZ, "U_i": L.NYL_F_UCX, "U_j": L.NYL_F_UCY, "U_k": L.NYL_F_UCZ}
+    for q in range(getattr(dev, "n_tracers", 0)):
+        ids["t%d" % q] = L.NYL_F_TRACER(q)
     return {k: dev.download(v) for k, v in ids.items()}
 
 

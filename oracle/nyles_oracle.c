@@ -13,6 +13,7 @@
  *   core/fortran_vorticity.f90    vorticity
  *   core/fortran_bernoulli.f90    gradke, gradkeandb, div
  *   core/fortran_kinenergy.f90    kin
+ *   core/fortran_dissipation.f90  add_laplacian (viscosity)
  *   core/mgfor/*.f90              the multigrid (setup, solve, vcycle,
  *                                 smoother, residual, restriction,
  *                                 prolongation, norm, halo fill)
@@ -278,6 +279,25 @@ void orc_kin(const double *u, double *ke, double ds2, int l, int m, int n, i64 s
             for (int i = 1; i < n; i++)
                 AT(ke, k, j, i) = AT(ke, k, j, i)
                     + cff2 * 0.5 * (AT(u, k, j, i) * AT(u, k, j, i) + AT(u, k, j, i - 1) * AT(u, k, j, i - 1));
+}
+/* core/fortran_dissipation.f90:2-34 add_laplacian: dphi += coef*delta[delta[phi]]
+ * along the third index, with no flux through either end of a line (fxm = 0
+ * at the first cell, fx = 0 at the last). */
+void orc_add_laplacian(const double *phi, double *dphi, double coef,
+                       int l, int m, int n, i64 s0, i64 s1, i64 s2)
+{
+    #pragma omp parallel for collapse(2) schedule(static)
+    for (int k = 0; k < l; k++)
+        for (int j = 0; j < m; j++) {
+            double fxm = 0.0, fx;
+            for (int i = 0; i < n - 1; i++) {
+                fx = AT(phi, k, j, i + 1) - AT(phi, k, j, i);
+                AT(dphi, k, j, i) = AT(dphi, k, j, i) + coef * (fx - fxm);
+                fxm = fx;
+            }
+            fx = 0.0;
+            AT(dphi, k, j, n - 1) = AT(dphi, k, j, n - 1) + coef * (fx - fxm);
+        }
 }
 #undef AT
 

@@ -66,6 +66,7 @@ def lib():
         L.orc_gradkeandb.argtypes = [dp, dp, dp, ctypes.c_double, ci, ci, ci, i64, i64, i64]
         L.orc_div.argtypes = [dp, dp, ci, ci, ci, ci, i64, i64, i64]
         L.orc_kin.argtypes = [dp, dp, ctypes.c_double, ci, ci, ci, i64, i64, i64]
+        L.orc_add_laplacian.argtypes = [dp, dp, ctypes.c_double, ci, ci, ci, i64, i64, i64]
         L.orc_mg_create.restype = ctypes.c_void_p
         L.orc_mg_create.argtypes = [ci, ci, ci, ci, ci, ctypes.c_double, ctypes.c_double]
         L.orc_mg_destroy.argtypes = [ctypes.c_void_p]
@@ -186,6 +187,12 @@ def kin(u, ke, ds2):
     """core/fortran_kinenergy.f90:3 kin(u, v, ke, ds2, order) (v unused)."""
     _same_layout(u, ke)
     lib().orc_kin(_vptr(u), _vptr(ke), float(ds2), *_view_args(ke))
+
+
+def add_laplacian(phi, dphi, coef):
+    """core/fortran_dissipation.f90:2 add_laplacian(phi, dphi, coef)."""
+    _same_layout(phi, dphi)
+    lib().orc_add_laplacian(_vptr(phi), _vptr(dphi), float(coef), *_view_args(dphi))
 
 
 # --------------------------------------------------------------------------
@@ -330,8 +337,14 @@ class OracleLES:
     PROGNOSTIC = ("b", "u_i", "u_j", "u_k")
 
     def __init__(self, nx, ny, nz, Lx=1.0, Ly=1.0, Lz=1.0, geometry="closed", nh=3,
-                 rotating=False, coriolis=1.0, mg_maxite=20, mg_tol=1e-6, mg_omega=0.9):
+                 rotating=False, coriolis=1.0, mg_maxite=20, mg_tol=1e-6, mg_omega=0.9,
+                 viscosity=0.0, n_tracers=0):
         self.nx, self.ny, self.nz, self.nh = nx, ny, nz, nh
+        # core/model_les.py:36-43: passive tracers t0, t1, ... after b
+        self.traclist = ["b"] + ["t%d" % q for q in range(n_tracers)]
+        self.PROGNOSTIC = tuple(self.traclist) + ("u_i", "u_j", "u_k")
+        # core/model_les.py:54-57: diff_coef['u'] is the viscosity
+        self.viscosity = viscosity
         self.geometry = geometry
         perio = GEOMETRY_PERIODIC[geometry]
         self.perio = perio
@@ -348,42 +361,53 @@ class OracleLES:
         self.fparameter = coriolis * (self.dx * self.dy) if rotating else 0.0
         z = lambda: np.zeros(self.shape)
         self.b, self.p, self.ke, self.div = z(), z(), z(), z()
+        self.tracers = {t: z() for t in self.traclist[1:]}
         self.u = {d: z() for d in "ijk"}
         self.U = {d: z() for d in "ijk"}
         self.vor = {d: z() for d in "ijk"}
         self.ds = [self._new_tendency() for _ in range(3)]
         self.mg = OracleMG(nx, ny, nz, geometry, nh, mg_maxite, mg_tol, mg_omega)
         self.nite_log = []
+        # user forcing: forcing(model, dstate, t) adds to the tendencies, as
+        # forcing.add(state, dstate, t) does in core/model_les.py:143-144
+        self.forcing = None
 
     def _new_tendency(self):
-        return {"b": np.zeros(self.shape), "u": {d: np.zeros(self.shape) for d in "ijk"}}
+        t = {name: np.zeros(self.shape) for name in self.traclist}
+        t["u"] = {d: np.zeros(self.shape) for d in "ijk"}
+        return t
 
     # -- state access ----------------------------------------------------
     def get(self, name, ds=None):
         if ds is None:
             if name == "b":
                 return self.b
+            if name in self.tracers:
+                return self.tracers[name]
             return self.u[name[-1]]
-        if name == "b":
-            return ds["b"]
+        if name in self.traclist:
+            return ds[name]
         return ds["u"][name[-1]]
 
     def fill(self, x):
         halo_fill(x, self.geometry, self.nh)
 
     # -- core/model_les.py:129-144 ----------------------------------------
-    def rhs(self, ds, last=False):
-        ds["b"][...] = 0.0                       # reset_state, core/model_les.py:167-175
+    def rhs(self, ds, last=False, forcing=None, t=0.0):
+        for name in self.traclist:               # reset_state, core/model_les.py:167-175
+            ds[name][...] = 0.0
         for d in "ijk":
             ds["u"][d][...] = 0.0
-        # Tracer_numerics.rhstrac, core/tracer.py:59-72
-        for d in "ijk":
-            velocity = view(self.U[d], d)
-            field = view(self.b, d)
-            dfield = view(ds["b"], d)
-            if d == "i":
-                dfield[...] = 0.0
-            upwind(field, velocity, dfield)
+        # Tracer_numerics.rhstrac, core/tracer.py:59-72 (no diffusion: LES calls
+        # it without `last`, core/model_les.py:133)
+        for name in self.traclist:
+            for d in "ijk":
+                velocity = view(self.U[d], d)
+                field = view(self.get(name), d)
+                dfield = view(ds[name], d)
+                if d == "i":
+                    dfield[...] = 0.0
+                upwind(field, velocity, dfield)
         # vortex_force, core/vortex_force.py:69-81
         for k, j, i in ["ikj", "jik", "kji"]:
             u_i = flipview(ds["u"][i], j)
@@ -401,6 +425,16 @@ class OracleLES:
                 gradke(ke, du)
             else:
                 gradkeandb(ke, view(self.b, d), du, self.dz)
+        # viscosity at the last stage, core/model_les.py:140-141 with
+        # core/viscosity.py:3-9
+        if last and self.viscosity != 0.0:
+            for d in "ijk":
+                coef = self.viscosity * self.ids2[d]
+                for comp in "ijk":
+                    add_laplacian(view(self.u[comp], d), view(ds["u"][comp], d), coef)
+        # user forcing, core/model_les.py:143-144
+        if forcing is not None:
+            forcing(self, ds, t)
 
     # -- core/cov_to_contra.py:4-20 ----------------------------------------
     def U_from_u(self):
@@ -444,6 +478,11 @@ class OracleLES:
     # -- core/model_les.py:97-127 ------------------------------------------
     def diagnose_var(self):
         self.fill(self.b)
+        # the reference refills only b and u here (core/model_les.py:100-101),
+        # so a passive tracer's periodic halo is never refreshed; the oracle
+        # refills it like b's (the periodic image), the device's behaviour
+        for t in self.tracers.values():
+            self.fill(t)
         for d in "ijk":
             self.fill(self.u[d])
         self.U_from_u()
@@ -464,37 +503,37 @@ class OracleLES:
             out.append((self.get(name),) + tuple(self.get(name, d) for d in ds))
         return out
 
-    def rk3_ssp(self, dt):
+    def rk3_ssp(self, dt, t=0.0):
         ds0, ds1, ds2 = self.ds
-        self.rhs(ds0, last=False)
+        self.rhs(ds0, last=False, forcing=self.forcing, t=t)
         for s, d0 in self._pairs(ds0):
             s += dt * d0
         self.diagnose_var()
-        self.rhs(ds1, last=False)
+        self.rhs(ds1, last=False, forcing=self.forcing, t=t + dt)
         for s, d0, d1 in self._pairs(ds0, ds1):
             s += (dt / 4.) * (d1 - 3 * d0)
         self.diagnose_var()
-        self.rhs(ds2, last=True)
+        self.rhs(ds2, last=True, forcing=self.forcing, t=t + dt * 0.5)
         for s, d0, d1, d2 in self._pairs(ds0, ds1, ds2):
             s += (dt / 12.) * (8 * d2 - d0 - d1)
         self.diagnose_var()
 
     # -- core/timescheme.py:113-122 ----------------------------------------
-    def euler_forward(self, dt):
+    def euler_forward(self, dt, t=0.0):
         ds = self.ds[0]
-        self.rhs(ds, last=True)
+        self.rhs(ds, last=True, forcing=self.forcing, t=t)
         for s, d in self._pairs(ds):
             s += dt * d
         self.diagnose_var()
 
     # -- core/timescheme.py:125-177 ----------------------------------------
-    def lfam3(self, dt):
+    def lfam3(self, dt, t=0.0):
         if not hasattr(self, "lf_first"):
             self.lf_first = True
             self.stateb = self._new_tendency()
             self.staten = self._new_tendency()
         ds = self.ds[0]
-        self.rhs(ds)
+        self.rhs(ds, forcing=self.forcing, t=t)
         if self.lf_first:
             for s, d, sb, sn in self._pairs(ds, self.stateb, self.staten):
                 sn[:] = s
@@ -509,7 +548,7 @@ class OracleLES:
                 s[:] = (1. / 12.) * (5. * s + 8. * sn - sb)
                 sb[:] = sn
             self.diagnose_var()
-            self.rhs(ds, last=True)
+            self.rhs(ds, last=True, forcing=self.forcing, t=t + dt * .5)
             for s, d, sn in self._pairs(ds, self.staten):
                 s[:] = sn + dt * d
             self.diagnose_var()

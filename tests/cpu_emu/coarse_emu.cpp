@@ -71,7 +71,8 @@ static CoarseArgs args(Case &c) {
 int main() {
     std::mt19937_64 rng(11);
     const int sizes[][3] = {{5, 5, 5}, {3, 3, 3}, {7, 6, 5}, {1, 4, 3}, {2, 2, 2}, {9, 1, 11},
-                            {31, 31, 31}, {32, 8, 3}, {1, 1, 1}};
+                            {31, 31, 31}, {32, 8, 3}, {1, 1, 1}, {34, 17, 17}, {62, 31, 31},
+                            {64, 32, 32}};
     int fails = 0;
     for (auto &sz : sizes) {
         for (int p = 0; p < 8; p++) {

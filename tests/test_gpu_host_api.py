@@ -119,3 +119,33 @@ def test_blowup_stops_the_run(tmp_path, capsys):
     out = capsys.readouterr().out
     assert "BLOW UP!" in out and "Job is aborted" in out
     assert nyl.n == 1
+
+
+@pytest.mark.parametrize("which", ["mgfor", "multigrid"])
+def test_multigrid_parameters_source(tmp_path, which):
+    """device.mg_params = "multigrid" takes the stop test from the multigrid
+    block (tol on |r|/|b|, squared; maxite; omega), which the reference's
+    mgfor ignores; "mgfor" (default) keeps mgfor's 1e-6 / 20 / 0.9."""
+    if CORE not in sys.path:
+        sys.path.insert(0, CORE)
+    import parameters
+    import nyles as nyles_module
+    print("case mg params", which, flush=True)
+    parameters.InextensibleDict.frozen = False
+    param = parameters.UserParameters()
+    param.IO["datadir"] = str(tmp_path)
+    param.discretization["global_nx"] = 32
+    param.discretization["global_ny"] = 32
+    param.discretization["global_nz"] = 32
+    param.device["mg_params"] = which
+    param.multigrid["tol"] = 1e-4
+    param.multigrid["maxite"] = 7
+    param.multigrid["omega"] = 0.8
+    nyl = nyles_module.Nyles(param)
+    parameters.InextensibleDict.frozen = False
+    d = nyl.model.device.desc
+    if which == "multigrid":
+        assert (d.mg_tol, d.mg_maxite, d.mg_omega) == (1e-4 ** 2, 7, 0.8)
+    else:
+        assert (d.mg_tol, d.mg_maxite, d.mg_omega) == (1e-6, 20, 0.9)
+    nyl.model.device.close()

@@ -1,7 +1,7 @@
 """Kernel equivalence on the CPU (no GPU).
 
-The tiled, fused and four-cells-per-thread kernels must give the same bits
-as the per-cell kernels they replace (the FP64 path is bit-identical to the
+The tiled, fused and four-cells-per-thread kernels (and the 2.5-D stage
+kernel) must give the same bits as the per-cell kernels they replace (the FP64 path is bit-identical to the
 oracle through the per-cell arithmetic).  The kernel sources are compiled
 for the host with g++ against a CPU stand-in of the HIP header
 (tests/cpu_emu/hip/hip_runtime.h: one host thread per work-item with a real
@@ -19,7 +19,7 @@ EMU = os.path.join(HERE, "cpu_emu")
 INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 
 
-@pytest.mark.parametrize("name", ["cell_emu", "mg_emu", "coarse_emu"])
+@pytest.mark.parametrize("name", ["cell_emu", "mg_emu", "coarse_emu", "stage_emu"])
 def test_kernels_match_per_cell_forms_bitwise(name, tmp_path):
     gxx = shutil.which("g++")
     if gxx is None:

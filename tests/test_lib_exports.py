@@ -28,17 +28,20 @@ def test_library_exports_every_declared_symbol():
 def test_binding_sets_signatures_and_abi_version():
     from nyles_amd import _lib
     L = _lib.load()
-    assert L.nyl_abi_version() == 1
+    assert L.nyl_abi_version() == 2
     assert L.nyl_last_error() is not None
 
 
 def test_descriptor_layout_matches_header():
-    """nyl_desc: six int32, eight doubles, two int32, two doubles (112 bytes)."""
+    """nyl_desc: six int32, eight doubles, two int32, three doubles, two
+    int32 (128 bytes)."""
     from nyles_amd import _lib
-    assert ctypes.sizeof(_lib.NylDesc) == 112
+    assert ctypes.sizeof(_lib.NylDesc) == 128
     assert _lib.NylDesc.dx.offset == 24
     assert _lib.NylDesc.precision.offset == 88
     assert _lib.NylDesc.mg_tol.offset == 96
+    assert _lib.NylDesc.viscosity.offset == 112
+    assert _lib.NylDesc.n_tracers.offset == 120
     assert ctypes.sizeof(_lib.NylMGStats) == 24
 
 
@@ -112,6 +115,17 @@ def test_hierarchy_of_grids_that_do_not_halve_to_2():
             d = _desc((n, n, n), precision=prec)
             assert L.nyl_ctx_create(ctypes.byref(d), ctypes.byref(ctx)) == 0, (n, L.nyl_last_error())
             assert L.nyl_ctx_destroy(ctx) == 0
+    # every served shape: the lock exchange n x n/2 x n/2 (perio_x) as well,
+    # whose coarsest odd level is 2m x m x m (x up to 62 cells)
+    for n in range(32, 257, 8):
+        for prec in (0, 1):
+            d = _desc((n, n // 2, n // 2), periodic=(1, 0, 0), precision=prec)
+            assert L.nyl_ctx_create(ctypes.byref(d), ctypes.byref(ctx)) == 0, (n, L.nyl_last_error())
+            assert L.nyl_ctx_destroy(ctx) == 0
+    d = _desc((248, 124, 124), periodic=(1, 0, 0))
+    assert L.nyl_ctx_create(ctypes.byref(d), ctypes.byref(ctx)) == 0
+    assert _levels(L, ctx)[-1] == (62, 31, 31)
+    assert L.nyl_ctx_destroy(ctx) == 0
     # the lock-exchange shape keeps the reference's hierarchy down to y = 2
     d = _desc((64, 32, 32), periodic=(1, 0, 0))
     assert L.nyl_ctx_create(ctypes.byref(d), ctypes.byref(ctx)) == 0
@@ -143,3 +157,22 @@ def test_context_rejects_fields_of_4gib_without_gpu():
     rc = L.nyl_ctx_create(ctypes.byref(d), ctypes.byref(ctx))
     assert rc == 0, L.nyl_last_error()
     assert L.nyl_ctx_destroy(ctx) == 0
+
+
+def test_mgfor_shim_symbols_are_found_as_the_reference_finds_them():
+    """core/build.py:107-121 looks each mgfor function up by the substring
+    "MOD_" + name in the library's symbols and needs exactly one match."""
+    import shutil
+    import subprocess
+    from nyles_amd import _lib
+    nm = shutil.which("nm")
+    if nm is None:
+        import pytest
+        pytest.skip("nm is not available")
+    syms = subprocess.check_output([nm, "-D", "--defined-only", _lib.LIB_PATH]).decode()
+    names = [l.split()[-1] for l in syms.split("\n") if l.strip()]
+    for f in ("get_ptrmg", "print_mginfos", "solve", "get_pyshape", "set_pyarray",
+              "get_pyarray", "get_param"):
+        hits = [n for n in names if "MOD_" + f in n]
+        assert len(hits) == 1, (f, hits)
+

@@ -8,7 +8,7 @@ import pytest
 def _linear_model(oracle_mod, lam):
     m = oracle_mod.OracleLES(8, 8, 8, 1.0, 1.0, 1.0, "perio_xyz")
 
-    def rhs(ds, last=False):
+    def rhs(ds, last=False, forcing=None, t=0.0):
         ds["b"][...] = lam * m.b
         for d in "ijk":
             ds["u"][d][...] = lam * m.u[d]
