@@ -110,7 +110,7 @@ def ensemble_run(members, steps, seq_steps, precision="mixed", seed=1000):
     point."""
     import torch
     from nyles_amd import serving
-    from nyles_amd.device import DeviceBatch
+    from nyles_amd.device import DeviceBatch, tile_name
     sizes = c4_sizes(members, seed)
     jobs = [serving.Job("shear", n, steps, seed=seed + q) for q, n in enumerate(sizes)]
     cells = sum(n ** 3 for n in sizes)
@@ -128,7 +128,7 @@ def ensemble_run(members, steps, seq_steps, precision="mixed", seed=1000):
         nite = 0
         if mode == "batched":
             b = DeviceBatch(models)
-            out["tile"], out["tune_ms"] = b.tile, b.tune_ms
+            out["tile"], out["tune_ms"] = tile_name(b.tile), b.tune_ms
             b.step_rk3(dts)
             torch.cuda.synchronize()
             t0 = time.perf_counter()
@@ -399,6 +399,47 @@ def timed(dev, dt, steps, warmup, kclass):
     return el, nite, (kms, klaunch, kbytes)
 
 
+def measured_valu(precision, n):
+    """VALU issue of the stage kernel as a fraction of the SIMD's ceiling
+    (0.25 wave64 instructions per cycle: FP32 unpacked and FP64 alike), mean
+    over RK modes 1-3, from the committed rocprofv3 --pmc summary of the same
+    workload when it was measured on the current kernel sources; None
+    otherwise."""
+    from nyles_amd import profiling
+    if n != 256:
+        return None
+    path = os.path.join(ROOT, "profiles", "valu_stage_taylor_green_256.md")
+    found, digest = None, None
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("valu_issue_frac_%s " % precision):
+                    found = float(line.split("=")[1])
+                elif line.startswith("source_digest"):
+                    digest = line.split("=")[1].strip()
+    except OSError:
+        return None
+    return found if digest == profiling.source_digest() else None
+
+
+def stage_report(dev, dt, precision, n, steps=2):
+    """the stage kernel over `steps` steps: each launch's own duration (HIP
+    events in the dispatch), its algorithmic bytes against HBM peak, and its
+    VALU issue fraction (the kernel's other bound)"""
+    dev.prof_start("stage")
+    for _ in range(steps):
+        dev.step_rk3(dt)
+    ms, nl, by = dev.prof_stop()
+    if not nl:
+        return None
+    us = ms / nl * 1e3
+    gbs = by / nl / (us * 1e-6) / 1e9
+    return {"kernel": "k_stage (RK3 modes 1-3)", "launches": nl, "mean_launch_us": round(us, 2),
+            "alg_bytes_per_launch": by / nl, "achieved": round(gbs, 2), "unit": "GB/s",
+            "hbm_frac": round(gbs / HBM_PEAK_GBS, 4),
+            "valu_issue_frac": measured_valu(precision, n)}
+
+
 def error_check(n, dt, steps):
     """Advance both paths `steps` steps from the same state; errors of the
     mixed path relative to FP64 (bit-identical to the oracle)."""
@@ -440,7 +481,8 @@ def main():
         br = class_times(dev, dt) if args.breakdown else {}
         kclass = args.roofline_kernel
         el, nite, prof = timed(dev, dt, args.steps, 0, kclass)
-        runs[prec] = {"el": el, "nite": nite, "prof": prof, "kclass": kclass, "breakdown": br}
+        runs[prec] = {"el": el, "nite": nite, "prof": prof, "kclass": kclass, "breakdown": br,
+                      "stage": stage_report(dev, dt, prec, n)}
         dev.close()
         torch.cuda.synchronize()
 
@@ -471,11 +513,13 @@ def main():
             "alg_bytes_per_launch": kbytes / klaunch if klaunch else None,
         },
     }
+    out["stage_kernel"] = head["stage"]
     if oth is not None:
         out[other] = {
             "value": round(n ** 3 * args.steps / oth["el"] / 1e6, 3),
             "ms_per_step": round(oth["el"] / args.steps * 1e3, 4),
             "mg_vcycles_per_step": round(oth["nite"] / args.steps, 3),
+            "stage_kernel": oth["stage"],
         }
         out["speedup_mixed_vs_fp64"] = round(runs["fp64"]["el"] / runs["mixed"]["el"], 3)
     if args.breakdown:

@@ -81,10 +81,19 @@ typedef struct nyl_desc {
                                     at the last stage (core/model_les.py:140-141) */
     int32_t n_tracers;           /* passive tracers t0.. advected like b
                                     (core/model_les.py:36-43), 0..NYL_MAX_TRACERS */
-    int32_t reserved;            /* 0 */
+    int32_t model;               /* NYL_MODEL_*: param.model["modelname"] (core/nyles.py:92-100) */
 } nyl_desc;
 
 #define NYL_MAX_TRACERS 8
+
+/* model variants (core/nyles.py:92-100); the variants run each stage as
+ * tendencies + stage combination (nyl_rhs, nyl_stage_update) internally */
+#define NYL_MODEL_LES 0      /* core/model_les.py */
+#define NYL_MODEL_LINEAR 1   /* LES(linear=True): no vortex force, vorticity and ke
+                                not diagnosed (core/model_les.py:33-34,122-127,135-136) */
+#define NYL_MODEL_EULER 2    /* core/model_les_euler.py: b and the tracers are not
+                                advected (:131), Bernoulli without buoyancy
+                                (core/bernoulli.py:26-32) */
 
 /* Per-solve statistics (core/mgfor/mg_types.f90:46-49 MG_Stats, plus normb). */
 typedef struct nyl_mg_stats {
@@ -291,12 +300,19 @@ int nyl_batch_bind_workspace(nyl_batch *batch, void *dptr, uint64_t bytes);
 int nyl_batch_set_stream(nyl_batch *batch, void *hip_stream);
 /* on = 0: launch the plans directly instead of from hipGraphs */
 int nyl_batch_set_graphs(nyl_batch *batch, int on);
-/* Times RK stage 1 of the batch for stage tiles of 128, 256 and 512 cells
- * (ms_out[3], may be NULL), keeps the fastest (tpb_out).  It overwrites the
- * members' RK3 scratch (next state buffer, tendency slot 0) only. */
-int nyl_batch_autotune(nyl_batch *batch, int32_t *tpb_out, double *ms_out);
-/* Sets the stage tile (128, 256 or 512 cells) without timing. */
-int nyl_batch_set_tile(nyl_batch *batch, int32_t tpb);
+/* Stage-kernel tiles: {cells, tx, ty}; tx = ty = 0 is a row tile of
+ * `cells` consecutive cells of a member (x fastest), otherwise a brick of
+ * tx x ty x (cells / (tx ty)) cells.  NYL_TILE_CANDIDATES shapes of 128,
+ * 256 and 512 cells, listed by nyl_batch_tile_candidate(index, tile[3]). */
+#define NYL_TILE_CANDIDATES 15
+int nyl_batch_tile_candidate(int32_t index, int32_t *tile_out);
+/* Times RK stage 1 of the batch for every candidate tile (ms_out in
+ * candidate order, NYL_TILE_CANDIDATES entries, may be NULL) and keeps the
+ * fastest (tile_out[3], may be NULL).  It overwrites the members' RK3
+ * scratch (next state buffer, tendency slot 0) only. */
+int nyl_batch_autotune(nyl_batch *batch, int32_t *tile_out, double *ms_out);
+/* Sets the stage tile (one of the candidates) without timing. */
+int nyl_batch_set_tile(nyl_batch *batch, int32_t cells, int32_t tx, int32_t ty);
 /* One RK3_SSP step of every member, member q with time step dt[q];
  * stats[q] (may be NULL): V-cycles of the step, last residual and normb.
  * Work queued on the members' own streams must be complete before the call;

@@ -12,6 +12,7 @@ LIB_PATH = os.path.join(HERE, "libnyles_hip.so")
 
 # status codes, precision modes, field ids, reductions and ops (mirror the header)
 NYL_OK = 0
+NYL_E_ARG, NYL_E_STATE, NYL_E_HIP, NYL_E_SIZE = -1, -2, -3, -4
 NYL_PREC_FP64 = 0
 NYL_PREC_MIXED = 1
 
@@ -26,6 +27,8 @@ NYL_DT_F64 = 0
 NYL_DT_F32 = 1
 
 NYL_MAX_TRACERS = 8
+NYL_TILE_CANDIDATES = 15
+NYL_MODEL_LES, NYL_MODEL_LINEAR, NYL_MODEL_EULER = 0, 1, 2
 
 
 def NYL_F_TRACER(q):
@@ -59,7 +62,8 @@ EXPORTED = [
     "nyl_prof_stop", "nyl_debug_mg", "nyl_reduce_last", "nyl_sync",
     "nyl_batch_create", "nyl_batch_destroy", "nyl_batch_workspace_bytes",
     "nyl_batch_bind_workspace", "nyl_batch_set_stream", "nyl_batch_set_graphs",
-    "nyl_batch_autotune", "nyl_batch_set_tile", "nyl_batch_step_rk3",
+    "nyl_batch_tile_candidate", "nyl_batch_autotune", "nyl_batch_set_tile",
+    "nyl_batch_step_rk3",
     "nyl_reduce_noalloc",
 ]
 
@@ -83,7 +87,7 @@ class NylDesc(ctypes.Structure):
         ("precision", ctypes.c_int32), ("mg_maxite", ctypes.c_int32),
         ("mg_tol", ctypes.c_double), ("mg_omega", ctypes.c_double),
         ("viscosity", ctypes.c_double), ("n_tracers", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("model", ctypes.c_int32),
     ]
 
 
@@ -147,8 +151,9 @@ def load():
         "nyl_batch_bind_workspace": (ci, [vp, vp, u64]),
         "nyl_batch_set_stream": (ci, [vp, vp]),
         "nyl_batch_set_graphs": (ci, [vp, ci]),
+        "nyl_batch_tile_candidate": (ci, [ctypes.c_int32, P(ctypes.c_int32)]),
         "nyl_batch_autotune": (ci, [vp, P(ctypes.c_int32), P(dbl)]),
-        "nyl_batch_set_tile": (ci, [vp, ctypes.c_int32]),
+        "nyl_batch_set_tile": (ci, [vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]),
         "nyl_batch_step_rk3": (ci, [vp, P(dbl), P(NylMGStats)]),
         "nyl_reduce_noalloc": (ci, [P(NylDesc), ctypes.c_int32, vp, vp, vp, vp, ctypes.c_int32,
                                     vp, vp]),

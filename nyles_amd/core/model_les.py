@@ -23,7 +23,13 @@ Differences from the reference that a script can see:
 * diffusion coefficients of b and the tracers are accepted and, as in the
   reference's LES (rhstrac is called without ``last``, core/model_les.py:133),
   not applied;
-* the linear model is not available on the device path (NotImplementedError);
+* the Euler3d variant (``LES(..., euler=True)``, core/model_les_euler.py)
+  skips the halo fill of b and u before the projection
+  (core/model_les_euler.py:98-99), so in the reference the divergence at the
+  first interior cell of a periodic axis reads the halo left by the previous
+  diagnosis; here the wrapped neighbour is read, as in the LES.  Its missing
+  perio_xy case in the multigrid topology (core/model_les_euler.py:84-86
+  against core/model_les.py:85-86) is not reproduced either;
 * ``forward`` returns True when b or u holds a non-finite value (the
   reference's blow-up flag is never set, core/mgfordriver.py:32).
 """
@@ -56,16 +62,21 @@ class MGInfo(object):
 
 
 class LES(object):
-    def __init__(self, param, grid, linear=False):
-        if linear:
-            raise NotImplementedError("the linear model is not available on the GPU path")
+    """linear=True: LES(linear=True) of core/model_les.py (no vortex force,
+    vorticity and ke not diagnosed); euler=True: the LES of
+    core/model_les_euler.py (b and the tracers not advected, Bernoulli
+    without buoyancy)."""
+
+    def __init__(self, param, grid, linear=False, euler=False):
         n_tracers = param["n_tracers"]
         if not 0 <= n_tracers <= L.NYL_MAX_TRACERS:
             raise ValueError("n_tracers must be 0..%d on the GPU path" % L.NYL_MAX_TRACERS)
-        self.nonlinear = True
+        self.nonlinear = not linear
+        self.euler = euler
         self.grid = grid
         self.param = param
-        self.traclist = ["b"] + ["t%d" % q for q in range(n_tracers)]
+        # core/model_les_euler.py:36 keeps '' in place of b
+        self.traclist = ["" if euler else "b"] + ["t%d" % q for q in range(n_tracers)]
         self.state = var.get_state(param, [("t%d" % q, "tracer%d" % q)
                                            for q in range(n_tracers)])
         self.halo = halo.set_halo(param, self.state)
@@ -99,7 +110,9 @@ class LES(object):
             grid.dx, grid.dy, grid.dz, idx2=grid.idx2, idy2=grid.idy2, idz2=grid.idz2,
             dx2=grid.dx ** 2, fparameter=self.fparameter, precision=precision,
             mg_maxite=mg_maxite, mg_tol=mg_tol, mg_omega=mg_omega,
-            viscosity=float(self.viscosity), n_tracers=n_tracers)
+            viscosity=float(self.viscosity), n_tracers=n_tracers,
+            model=L.NYL_MODEL_EULER if euler else (L.NYL_MODEL_LINEAR if linear
+                                                   else L.NYL_MODEL_LES))
         self.timescheme.bind(self.device, self)
         self.mg = MGInfo()
         st = self.state

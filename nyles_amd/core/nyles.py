@@ -68,14 +68,15 @@ class Nyles(object):
         if name == "LES":
             self.model = model_les.LES(param, self.grid)
         elif name == "Euler3d":
-            # Euler3d is LES without buoyancy (identical when b = 0); the
-            # reference variant also skips the pre-projection halo fill.
-            print("Euler3d runs as the LES model (buoyancy is advected; b = 0 gives Euler)")
-            self.model = model_les.LES(param, self.grid)
+            self.model = model_les.LES(param, self.grid, euler=True)
         elif name == "linear":
             self.model = model_les.LES(param, self.grid, linear=True)
         else:
-            raise NotImplementedError("model %r is not available on the GPU path" % name)
+            # "advection" (core/model_advection.py): its constructor passes the
+            # grid where Tracer_numerics expects param (core/model_advection.py:29-30
+            # against core/tracer.py:10), so the reference cannot build it
+            raise NotImplementedError("model %r is not available (the reference's "
+                                      "Advection model cannot be constructed)" % name)
         self.diag = VFwork(self.model, self.grid)
         self.tend = param["tend"]
         self.auto_dt = param["auto_dt"]

@@ -177,8 +177,9 @@ int prof_drain(nyl_ctx *c) {
     return NYL_OK;
 }
 
-// Launch through `fn`; when the kernel class is being profiled, bracket the
-// launch with events on the context stream and count its algorithmic bytes.
+// Launch through `fn`; when the kernel class is being profiled, the first
+// kernel `fn` launches records its start and end on a pair of events (the
+// kernel's own duration) and its algorithmic bytes are counted.
 constexpr int kFineLevel = 0x100;   // launch works on the finest multigrid level
 
 template <typename Fn>
@@ -198,11 +199,13 @@ int timed_launch(nyl_ctx *c, int cls, double bytes, Fn fn) {
                 if (r) return r;
             }
         }
-        HIPCHK(hipEventRecord(c->prof_ev[2 * c->prof_used], c->stream));
+        g_launch_events = {c->prof_ev[2 * c->prof_used], c->prof_ev[2 * c->prof_used + 1]};
     }
-    HIPCHK(fn());
-    if (on) {
-        HIPCHK(hipEventRecord(c->prof_ev[2 * c->prof_used + 1], c->stream));
+    const hipError_t e = fn();
+    const bool timed = on && g_launch_events.start == nullptr;
+    g_launch_events = {nullptr, nullptr};
+    HIPCHK(e);
+    if (timed) {
         c->prof_used++;
         c->prof_launches++;
         c->prof_bytes += bytes;
@@ -269,6 +272,7 @@ StageArgs<T> stage_args(nyl_ctx *c) {
     a.c2 = 0.0;
     a.vcx = a.vcy = a.vcz = 0.0;
     a.visc = 0;
+    a.flags = 0;
     return a;
 }
 
@@ -800,10 +804,12 @@ int diagnose_t(nyl_ctx *c, nyl_mg_stats *st) {
     if ((rc = apply_op_t<T>(c, NYL_OP_DIV))) return rc;
     if ((rc = mg_solve(c, st))) return rc;
     if ((rc = apply_op_t<T>(c, NYL_OP_CORRECT))) return rc;
-    // vorticity and ke; U is not stored (the stage kernel recomputes it,
-    // NYL_OP_U_FROM_U fills it on demand)
-    LAUNCH(NYL_K_DIAG, 7 * cells(c->g) * sizeof(T),
-           launch_diag<T>(c->g, diag_args<T>(c), 6, c->stream));
+    // vorticity and ke (not in the linear model, core/model_les.py:122-127);
+    // U is not stored (the stage kernel recomputes it, NYL_OP_U_FROM_U fills
+    // it on demand)
+    if (c->d.model != NYL_MODEL_LINEAR)
+        LAUNCH(NYL_K_DIAG, 7 * cells(c->g) * sizeof(T),
+               launch_diag<T>(c->g, diag_args<T>(c), 6, c->stream));
     return NYL_OK;
 }
 
@@ -839,11 +845,18 @@ int stage_t(nyl_ctx *c, int st, int slot, double coef, double coef2, bool last) 
         a.vcy = c->d.viscosity * c->d.idy2;
         a.vcz = c->d.viscosity * c->d.idz2;
     }
+    if (c->d.model == NYL_MODEL_LINEAR) a.flags |= NYL_SF_NO_VF;
+    if (c->d.model == NYL_MODEL_EULER) a.flags |= NYL_SF_NO_B;
     const double words = stage_words(st);
     LAUNCH(NYL_K_STAGE, words * cells(c->g) * sizeof(T), launch_stage<T>(c->g, a, st, c->stream));
-    // passive tracers: the same right-hand side and update, one launch each
+    // passive tracers: the same right-hand side and update, one launch each;
+    // Euler3d advects none of them (zero tendencies)
     const int s = c->cur, t = 1 - c->cur;
     for (int q = 0; q < c->ntr; q++) {
+        if (c->d.model == NYL_MODEL_EULER) {
+            HIPCHK(launch_zero<T>(F<T>(c, c->tds[slot][q]), c->ncell, c->stream));
+            continue;
+        }
         TracerArgs<T> ta;
         ta.q = F<T>(c, c->tst[s][q]);
         ta.qn = F<T>(c, c->tst[t][q]);
@@ -896,9 +909,23 @@ int update_t(nyl_ctx *c, int st, double coef, double coef2) {
 // last: the time scheme's rhs call with last = True (RK3 stage 3, Euler
 // forward, the LFAM3 corrector), which adds the viscosity
 int stage(nyl_ctx *c, int st, double coef, double coef2, bool last) {
+    int rc;
+    if (c->d.model != NYL_MODEL_LES) {
+        // model variants: the tendencies into the slot the stage combination
+        // reads last (0 for modes 1 and 4..6, 1 for mode 2, 2 for mode 3),
+        // then the combination
+        const int slot = st == 2 ? 1 : (st == 3 ? 2 : 0);
+        rc = c->mixed ? stage_t<float>(c, 0, slot, 0.0, 0.0, last)
+                      : stage_t<double>(c, 0, slot, 0.0, 0.0, last);
+        if (rc) return rc;
+        rc = c->mixed ? update_t<float>(c, st, coef, coef2) : update_t<double>(c, st, coef, coef2);
+        if (rc) return rc;
+        if (st != 5) c->cur = 1 - c->cur;
+        return NYL_OK;
+    }
     const int slot = (st == 1 || st == 2) ? st - 1 : -1;
-    int rc = c->mixed ? stage_t<float>(c, st, slot, coef, coef2, last)
-                      : stage_t<double>(c, st, slot, coef, coef2, last);
+    rc = c->mixed ? stage_t<float>(c, st, slot, coef, coef2, last)
+                  : stage_t<double>(c, st, slot, coef, coef2, last);
     if (rc) return rc;
     if (st != 5) c->cur = 1 - c->cur;
     return NYL_OK;
@@ -965,6 +992,8 @@ int nyl_ctx_create(const nyl_desc *desc, nyl_ctx **out) {
     if (d.n_tracers < 0 || d.n_tracers > NYL_MAX_TRACERS)
         return fail(NYL_E_ARG, "n_tracers must be 0..%d", NYL_MAX_TRACERS);
     if (!(d.viscosity >= 0.0)) return fail(NYL_E_ARG, "viscosity must be >= 0");
+    if (d.model < NYL_MODEL_LES || d.model > NYL_MODEL_EULER)
+        return fail(NYL_E_ARG, "model must be NYL_MODEL_LES, _LINEAR or _EULER");
     nyl_ctx *c = new nyl_ctx();
     memset(c, 0, sizeof(*c));
     c->d = d;

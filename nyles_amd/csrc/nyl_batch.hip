@@ -56,7 +56,7 @@ struct nyl_batch {
     std::vector<int> cur0;      // members' state buffer at table build
     int mixed = 0;
     int lmax = 0;
-    int tpb = 256;              // stage kernel tile (threads = cells per tile)
+    StageTile tile = {256, -1, 0};   // stage kernel tile (threads = cells per tile)
     int use_graphs = 1;
     int phase = 0;              // member cur = cur0 ^ phase
     char *ws = nullptr;
@@ -70,7 +70,8 @@ struct nyl_batch {
     std::vector<Launch> pre[2][3], vcyc, post[2];
     std::vector<char> blob;     // host image of the item tables
     hipGraphExec_t g_pre[2][3] = {}, g_vcyc = nullptr, g_post[2] = {};
-    int *host_any = nullptr;    // pinned
+    int *host_any = nullptr;    // pinned: members still iterating, most V-cycles so far
+    int vcap[3] = {0, 0, 0};    // V-cycles launched without a poll, per RK stage
     double *host_dt = nullptr;  // pinned, n
     void *host_stats = nullptr; // pinned: nite_acc (int n), res (double n), normb (double n)
 };
@@ -108,11 +109,11 @@ Launch add_table(nyl_batch *b, LKind kind, LType type, const std::vector<I> &ite
     return L;
 }
 
-// stage-kernel items of tile size tpb for members whose periodicity class is
-// `perio`, at phase p and RK stage st (1..3)
+// stage-kernel items of tile shape `tile` for members whose periodicity
+// class is `perio`, at phase p and RK stage st (1..3)
 template <typename T>
-std::vector<StageItem<T>> stage_items(nyl_batch *b, int p, int st, int perio, int tpb,
-                                      int64_t &nblk) {
+std::vector<StageItem<T>> stage_items(nyl_batch *b, int p, int st, int perio,
+                                      const StageTile &tile, int64_t &nblk) {
     std::vector<StageItem<T>> v;
     nblk = 0;
     for (int q = 0; q < b->n; q++) {
@@ -123,7 +124,7 @@ std::vector<StageItem<T>> stage_items(nyl_batch *b, int p, int st, int perio, in
         memset(&it, 0, sizeof(it));
         it.h.g = c->g;
         it.h.member = q;
-        it.h.nblk = (int)((gcells(c->g) + tpb - 1) / tpb);
+        it.h.nblk = (int)stage_tiles(c->g, tile);
         it.h.start = nblk;
         nblk += it.h.nblk;
         StageArgs<T> &a = it.a;
@@ -158,7 +159,7 @@ template <typename T>
 void build_stage(nyl_batch *b, int p, int st, std::vector<Launch> &out) {
     for (int perio = 0; perio < 8; perio++) {
         int64_t nblk;
-        auto v = stage_items<T>(b, p, st, perio, b->tpb, nblk);
+        auto v = stage_items<T>(b, p, st, perio, b->tile, nblk);
         if (!v.empty()) out.push_back(add_table(b, L_STAGE, T_D, v, nblk, st, perio));
     }
 }
@@ -487,8 +488,8 @@ int run_launch(nyl_batch *b, const Launch &L) {
     const int *act = b->mg.active;
     switch (L.kind) {
     case L_STAGE:
-        if (b->mixed) BCHK(launch_b_stage<float>((const StageItem<float> *)tab, L.nitems, L.nblk, L.arg, L.arg2, b->tpb, b->mg.dt, s));
-        else BCHK(launch_b_stage<double>((const StageItem<double> *)tab, L.nitems, L.nblk, L.arg, L.arg2, b->tpb, b->mg.dt, s));
+        if (b->mixed) BCHK(launch_b_stage<float>((const StageItem<float> *)tab, L.nitems, L.nblk, L.arg, L.arg2, b->tile, b->mg.dt, s));
+        else BCHK(launch_b_stage<double>((const StageItem<double> *)tab, L.nitems, L.nblk, L.arg, L.arg2, b->tile, b->mg.dt, s));
         break;
     case L_DIV:
         if (L.type == T_F) BCHK(launch_b_div<float>((const DivItem<float> *)tab, L.nitems, L.nblk, s));
@@ -564,7 +565,8 @@ int run_plan(nyl_batch *b, std::vector<Launch> &plan, hipGraphExec_t *g) {
 }
 
 int read_any(nyl_batch *b, int *any) {
-    BCHK(hipMemcpyAsync(b->host_any, b->mg.any, sizeof(int), hipMemcpyDeviceToHost, b->stream));
+    BCHK(hipMemcpyAsync(b->host_any, b->mg.any, 2 * sizeof(int), hipMemcpyDeviceToHost,
+                        b->stream));
     BCHK(hipStreamSynchronize(b->stream));
     *any = b->host_any[0];
     return NYL_OK;
@@ -595,13 +597,12 @@ int rebuild(nyl_batch *b) {
 
 // Time RK stage 1 of the batch (it writes only the next state buffer and
 // tendency slot 0, which stage 1 of the next step overwrites) for each tile
-// size and keep the fastest.
+// shape of kStageTiles and keep the fastest.
 template <typename T>
 int autotune_t(nyl_batch *b, double *ms_out) {
-    const int cand[3] = {128, 256, 512};
     const uint64_t tune_off = tables_off(b) + al256(b->blob.size());
     double best = 1e300;
-    int best_tpb = b->tpb;
+    StageTile best_tile = b->tile;
     hipEvent_t e0, e1;
     BCHK(hipEventCreate(&e0));
     BCHK(hipEventCreate(&e1));
@@ -609,15 +610,15 @@ int autotune_t(nyl_batch *b, double *ms_out) {
     for (int q = 0; q < b->n; q++) b->host_dt[q] = 0.0;
     BCHK(hipMemcpyAsync(b->ws + b->o_dt, b->host_dt, sizeof(double) * b->n,
                         hipMemcpyHostToDevice, b->stream));
-    for (int ci = 0; ci < 3; ci++) {
-        const int tpb = cand[ci];
+    for (int ci = 0; ci < NYL_N_TILES; ci++) {
+        const StageTile &tile = kStageTiles[ci];
         std::vector<std::vector<char>> tabs;
         std::vector<std::pair<int64_t, int>> meta;   // nblk, perio
         std::vector<char> img;
         std::vector<uint64_t> offs;
         for (int perio = 0; perio < 8; perio++) {
             int64_t nblk;
-            auto v = stage_items<T>(b, b->phase, 1, perio, tpb, nblk);
+            auto v = stage_items<T>(b, b->phase, 1, perio, tile, nblk);
             if (v.empty()) continue;
             const uint64_t o = al256(img.size());
             img.resize(o + sizeof(StageItem<T>) * v.size());
@@ -638,7 +639,7 @@ int autotune_t(nyl_batch *b, double *ms_out) {
                     if (c->g.px + 2 * c->g.py + 4 * c->g.pz == meta[g].second) nitems++;
                 }
                 BCHK(launch_b_stage<T>((const StageItem<T> *)(b->ws + tune_off + offs[g]), nitems,
-                                       meta[g].first, 1, meta[g].second, tpb, b->mg.dt,
+                                       meta[g].first, 1, meta[g].second, tile, b->mg.dt,
                                        b->stream));
             }
             BCHK(hipEventRecord(e1, b->stream));
@@ -649,11 +650,11 @@ int autotune_t(nyl_batch *b, double *ms_out) {
         }
         const double ms = ms_sum / 3.0;
         if (ms_out) ms_out[ci] = ms;
-        if (ms < best) { best = ms; best_tpb = tpb; }
+        if (ms < best) { best = ms; best_tile = tile; }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    b->tpb = best_tpb;
+    b->tile = best_tile;
     // tables for the chosen tile, for the members' current buffers
     return rebuild(b);
 }
@@ -672,6 +673,8 @@ int nyl_batch_create(nyl_ctx *const *members, int32_t n, nyl_batch **out) {
             return set_error(NYL_E_ARG, "members must share the precision mode");
         if (members[q]->ntr > 0)
             return set_error(NYL_E_ARG, "member %d has passive tracers: step it on its own", q);
+        if (members[q]->d.model != NYL_MODEL_LES)
+            return set_error(NYL_E_ARG, "member %d is not an LES model: step it on its own", q);
         for (int r = 0; r < q; r++)
             if (members[r] == members[q]) return set_error(NYL_E_ARG, "member %d appears twice", q);
     }
@@ -687,7 +690,7 @@ int nyl_batch_create(nyl_ctx *const *members, int32_t n, nyl_batch **out) {
     b->o_normb = take(8 * n); b->o_res = take(8 * n); b->o_shift = take(8 * n);
     b->o_tol = take(8 * n); b->o_dt = take(8 * n);
     b->o_nite = take(4 * n); b->o_maxite = take(4 * n); b->o_active = take(4 * n);
-    b->o_any = take(4); b->o_acc = take(4 * n);
+    b->o_any = take(8); b->o_acc = take(4 * n);
     if (b->mixed) build_plans<float>(b);
     else build_plans<double>(b);
     b->ws_bytes = tables_off(b) + al256(b->blob.size()) + tune_bytes(b);
@@ -725,7 +728,7 @@ int nyl_batch_bind_workspace(nyl_batch *b, void *dptr, uint64_t bytes) {
     if (((uintptr_t)dptr) & 255) return set_error(NYL_E_ARG, "workspace must be 256-byte aligned");
     const int n = b->n;
     if (!b->host_any) {
-        BCHK(hipHostMalloc((void **)&b->host_any, sizeof(int), 0));
+        BCHK(hipHostMalloc((void **)&b->host_any, 2 * sizeof(int), 0));
         BCHK(hipHostMalloc((void **)&b->host_dt, sizeof(double) * n, 0));
         BCHK(hipHostMalloc(&b->host_stats, (sizeof(int) + 2 * sizeof(double)) * n, 0));
     }
@@ -756,20 +759,43 @@ int nyl_batch_set_graphs(nyl_batch *b, int on) {
     return NYL_OK;
 }
 
-int nyl_batch_autotune(nyl_batch *b, int32_t *tpb_out, double *ms_out) {
+int nyl_batch_autotune(nyl_batch *b, int32_t *tile_out, double *ms_out) {
     if (!b || !b->ws) return set_error(NYL_E_STATE, "batch workspace not bound");
     int rc = b->mixed ? autotune_t<float>(b, ms_out) : autotune_t<double>(b, ms_out);
     if (rc) return rc;
-    if (tpb_out) *tpb_out = b->tpb;
+    if (tile_out) {
+        const StageTile &t = b->tile;
+        tile_out[0] = t.tpb;
+        tile_out[1] = t.lx < 0 ? 0 : 1 << t.lx;
+        tile_out[2] = t.lx < 0 ? 0 : 1 << t.ly;
+    }
     return NYL_OK;
 }
 
-int nyl_batch_set_tile(nyl_batch *b, int32_t tpb) {
+int nyl_batch_tile_candidate(int32_t index, int32_t *tile_out) {
+    if (index < 0 || index >= NYL_N_TILES || !tile_out)
+        return set_error(NYL_E_ARG, "tile candidate index out of range");
+    const StageTile &t = kStageTiles[index];
+    tile_out[0] = t.tpb;
+    tile_out[1] = t.lx < 0 ? 0 : 1 << t.lx;
+    tile_out[2] = t.lx < 0 ? 0 : 1 << t.ly;
+    return NYL_OK;
+}
+
+int nyl_batch_set_tile(nyl_batch *b, int32_t cells, int32_t tx, int32_t ty) {
     if (!b || !b->ws) return set_error(NYL_E_STATE, "batch workspace not bound");
-    if (tpb != 128 && tpb != 256 && tpb != 512)
-        return set_error(NYL_E_ARG, "stage tile must be 128, 256 or 512 cells");
-    if (tpb == b->tpb) return NYL_OK;
-    b->tpb = tpb;
+    int found = -1;
+    for (int q = 0; q < NYL_N_TILES; q++) {
+        const StageTile &t = kStageTiles[q];
+        const int x = t.lx < 0 ? 0 : 1 << t.lx, y = t.lx < 0 ? 0 : 1 << t.ly;
+        if (t.tpb == cells && x == tx && y == ty) found = q;
+    }
+    if (found < 0)
+        return set_error(NYL_E_ARG, "stage tile %d cells %dx%d is not one of the candidates",
+                         cells, tx, ty);
+    const StageTile &t = kStageTiles[found];
+    if (t.tpb == b->tile.tpb && t.lx == b->tile.lx && t.ly == b->tile.ly) return NYL_OK;
+    b->tile = t;
     return rebuild(b);
 }
 
@@ -792,12 +818,20 @@ int nyl_batch_step_rk3(nyl_batch *b, const double *dt, nyl_mg_stats *stats) {
         const int p = b->phase;
         int rc = run_plan(b, b->pre[p][st - 1], &b->g_pre[p][st - 1]);
         if (rc) return rc;
-        int any = 0;
+        // V-cycles: as many as this stage needed last step are launched
+        // back to back with no host read in between (a V-cycle plan is a
+        // no-op for members the device stop rule has retired), then one
+        // read of the active count; further V-cycles, if any member still
+        // iterates, are launched one at a time with a read after each
+        int any = 0, it = 0;
+        for (; it < std::min(b->vcap[st - 1], maxite + 1); it++)
+            if ((rc = run_plan(b, b->vcyc, &b->g_vcyc))) return rc;
         if ((rc = read_any(b, &any))) return rc;
-        for (int it = 0; any > 0 && it <= maxite; it++) {
+        for (; any > 0 && it <= maxite; it++) {
             if ((rc = run_plan(b, b->vcyc, &b->g_vcyc))) return rc;
             if ((rc = read_any(b, &any))) return rc;
         }
+        b->vcap[st - 1] = b->host_any[1];
         if ((rc = run_plan(b, b->post[p], &b->g_post[p]))) return rc;
         b->phase ^= 1;
     }

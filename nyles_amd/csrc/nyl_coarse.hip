@@ -122,9 +122,9 @@ bool coarse_fits(const Geo &g) {
 }
 
 hipError_t launch_coarse_solve(const CoarseArgs &a, hipStream_t s) {
-    hipLaunchKernelGGL((k_cs<0>), dim3(a.g.nz), dim3(CS_T), 0, s, a);
-    hipLaunchKernelGGL((k_cs<1>), dim3(a.g.ny), dim3(CS_T), 0, s, a);
-    hipLaunchKernelGGL((k_cs<2>), dim3(a.g.nz), dim3(CS_T), 0, s, a);
+    NYL_LAUNCH((k_cs<0>), dim3(a.g.nz), dim3(CS_T), 0, s, a);
+    NYL_LAUNCH((k_cs<1>), dim3(a.g.ny), dim3(CS_T), 0, s, a);
+    NYL_LAUNCH((k_cs<2>), dim3(a.g.nz), dim3(CS_T), 0, s, a);
     return hipGetLastError();
 }
 
@@ -140,9 +140,9 @@ __global__ __launch_bounds__(CS_T) void k_b_cs(const CoarseItem *__restrict__ it
 hipError_t launch_b_coarse(const CoarseItem *items, int n, int64_t nblk, int pass,
                            const int *active, hipStream_t s) {
     switch (pass) {
-    case 0: hipLaunchKernelGGL((k_b_cs<0>), dim3((unsigned)nblk), dim3(CS_T), 0, s, items, n, active); break;
-    case 1: hipLaunchKernelGGL((k_b_cs<1>), dim3((unsigned)nblk), dim3(CS_T), 0, s, items, n, active); break;
-    default: hipLaunchKernelGGL((k_b_cs<2>), dim3((unsigned)nblk), dim3(CS_T), 0, s, items, n, active); break;
+    case 0: NYL_LAUNCH((k_b_cs<0>), dim3((unsigned)nblk), dim3(CS_T), 0, s, items, n, active); break;
+    case 1: NYL_LAUNCH((k_b_cs<1>), dim3((unsigned)nblk), dim3(CS_T), 0, s, items, n, active); break;
+    default: NYL_LAUNCH((k_b_cs<2>), dim3((unsigned)nblk), dim3(CS_T), 0, s, items, n, active); break;
     }
     return hipGetLastError();
 }

@@ -93,9 +93,37 @@ __device__ __forceinline__ T weno3(T qm, T q0, T qp) {
     return wdiv(w1 * qi1 + w2 * qi2, w1 + w2);
 }
 
+// The FP32 reconstruction of the mixed-precision path (no bitwise target):
+// core/weno.f90:25-54 with the products and sums contracted into fused
+// multiply-adds and the weights' divisions by the hardware reciprocal, about
+// 40 % fewer instructions than the operation-by-operation form (the FP32
+// stage kernel issues VALU instructions at the ceiling).  Same literals and
+// weights.
+__device__ __forceinline__ float weno5_f32(float qmm, float qm, float q0, float qp, float qpp) {
+    const float eps = 1e-16f;
+    const float c13 = 1.0f / 3.0f, c76 = 7.0f / 6.0f, c116 = 11.0f / 6.0f;
+    const float c16 = 1.0f / 6.0f, c56 = 5.0f / 6.0f;
+    const float k1 = 13.0f / 12.0f, k2 = 0.25f;
+    const float qi1 = fmaf(c13, qmm, fmaf(-c76, qm, c116 * q0));
+    const float qi2 = fmaf(c56, q0, fmaf(c13, qp, -(c16 * qm)));
+    const float qi3 = fmaf(c13, q0, fmaf(c56, qp, -(c16 * qpp)));
+    const float t1 = fmaf(-2.0f, qm, qmm + q0), t2 = fmaf(-4.0f, qm, fmaf(3.0f, q0, qmm));
+    const float t3 = fmaf(-2.0f, q0, qm + qp), t4 = qm - qp;
+    const float t5 = fmaf(-2.0f, qp, q0 + qpp), t6 = fmaf(-4.0f, qp, fmaf(3.0f, q0, qpp));
+    const float beta1 = fmaf(k1 * t1, t1, (k2 * t2) * t2);
+    const float beta2 = fmaf(k1 * t3, t3, (k2 * t4) * t4);
+    const float beta3 = fmaf(k1 * t5, t5, (k2 * t6) * t6);
+    const float tau5 = fabsf(beta1 - beta3);
+    const float w1 = fmaf(tau5, __builtin_amdgcn_rcpf(beta1 + eps), 1.0f);
+    const float w2 = fmaf(6.0f * tau5, __builtin_amdgcn_rcpf(beta2 + eps), 6.0f);
+    const float w3 = fmaf(3.0f * tau5, __builtin_amdgcn_rcpf(beta3 + eps), 3.0f);
+    return fmaf(w1, qi1, fmaf(w2, qi2, w3 * qi3)) * __builtin_amdgcn_rcpf(w1 + w2 + w3);
+}
+
 // core/weno.f90:25-54
 template <typename T>
 __device__ __forceinline__ T weno5(T qmm, T qm, T q0, T qp, T qpp) {
+    if constexpr (sizeof(T) == 4) return weno5_f32(qmm, qm, q0, qp, qpp);
     const T eps = T(1e-16f);
     const T c13 = T(1.0f / 3.0f), c76 = T(7.0f / 6.0f), c116 = T(11.0f / 6.0f);
     const T c16 = T(1.0f / 6.0f), c56 = T(5.0f / 6.0f);
@@ -203,6 +231,15 @@ __device__ __forceinline__ T face_flux_rel(int f, int n, T u, const QR &qr) {
     else if (kind == 1) w = a0;
     else return T(0);
     return u * w;
+}
+
+// Store with the nontemporal hint, for results that no kernel reads soon
+// after (the next RK stage's tendencies, the diagnostics read by the next
+// stage kernel): a stream of such stores runs at close to the HBM write rate,
+// where write-allocating them in the caches costs far more.
+template <typename T>
+__device__ __forceinline__ void store_stream(T *p, T v) {
+    __builtin_nontemporal_store(v, p);
 }
 
 // ---------------------------------------------------------------------------

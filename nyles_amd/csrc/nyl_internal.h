@@ -1,11 +1,34 @@
 // nyl_internal.h -- host-side launcher declarations shared by the
 // translation units of libnyles_hip.so (not part of the public ABI).
 #pragma once
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include "nyl_device.h"
 
 namespace nyl {
+
+// Events of the kernel-class profiler: when set, the next kernel launched
+// through NYL_LAUNCH records its own start and end on them
+// (hipExtLaunchKernelGGL places the timestamps in the dispatch itself, so the
+// time is the kernel's alone, without the gaps an event record on each side
+// of the launch adds) and clears them.
+struct LaunchEvents {
+    hipEvent_t start, stop;
+};
+inline thread_local LaunchEvents g_launch_events = {nullptr, nullptr};
+
+#define NYL_LAUNCH(kernel, grid, block, shmem, stream, ...)                                 \
+    do {                                                                                    \
+        if (nyl::g_launch_events.start) {                                                   \
+            hipExtLaunchKernelGGL(kernel, grid, block, shmem, stream,                       \
+                                  nyl::g_launch_events.start, nyl::g_launch_events.stop, 0, \
+                                  __VA_ARGS__);                                             \
+            nyl::g_launch_events.start = nullptr;                                           \
+        } else {                                                                            \
+            hipLaunchKernelGGL(kernel, grid, block, shmem, stream, __VA_ARGS__);            \
+        }                                                                                   \
+    } while (0)
 
 // Arguments of the fused RK stage kernel (rhs + update).
 template <typename T>
@@ -27,7 +50,13 @@ struct StageArgs {
     // vc_d = viscosity * ids2[d] (core/viscosity.py:3-9)
     double vcx, vcy, vcz;
     int visc;
+    // model variant, mode 0 only (the variants run tendencies + update):
+    // bit 0 no vortex force (linear model), bit 1 no advection of b and no
+    // buoyancy (Euler3d)
+    int flags;
 };
+#define NYL_SF_NO_VF 1
+#define NYL_SF_NO_B 2
 
 // mode 0: tendencies only; 1..3: RK3_SSP stage update;
 // 4: LFAM3 predictor; 5: LFAM3 corrector (next = next + c*ds);
@@ -249,9 +278,20 @@ struct FinalItem { const double *partials; int nparts; int member; double ncell;
 #define NYL_BATCH_TPB 256
 int batch_blocks(int64_t ncell);   // blocks of one member at NYL_BATCH_TPB threads
 
+// Tile of the batched stage kernel: tpb cells, either rows of tpb
+// consecutive cells of a member (lx < 0) or a brick of 2^lx x 2^ly x
+// (tpb >> (lx + ly)) cells.
+struct StageTile {
+    int tpb, lx, ly;
+};
+#define NYL_N_TILES 15
+extern const StageTile kStageTiles[NYL_N_TILES];
+// tiles of one member for a tile shape
+int64_t stage_tiles(const Geo &g, const StageTile &t);
+
 template <typename T>
 hipError_t launch_b_stage(const StageItem<T> *items, int n, int64_t nblk, int mode, int perio,
-                          int tpb, const double *dt, hipStream_t s);
+                          const StageTile &tile, const double *dt, hipStream_t s);
 template <typename T>
 hipError_t launch_b_div(const DivItem<T> *items, int n, int64_t nblk, hipStream_t s);
 template <typename T>
@@ -301,5 +341,8 @@ hipError_t launch_nonfinite(const Geo &g, const T *a, const T *b, const T *c, co
                             double *partials, double *dev_out, hipStream_t s);
 
 hipError_t launch_fill(double *p, int64_t n, double v, hipStream_t s);
+// p[0..n) = 0 (a kernel, so it may be captured in a graph)
+template <typename T>
+hipError_t launch_zero(T *p, int64_t n, hipStream_t s);
 
 }  // namespace nyl

@@ -78,7 +78,7 @@ __global__ __launch_bounds__(256) void k_mg_sweep(Geo g, const T *__restrict__ s
 template <typename T>
 hipError_t launch_mg_sweep(const Geo &g, const T *src, T *dst, const T *b, double omega,
                            const MGTabs &tab, hipStream_t s) {
-    hipLaunchKernelGGL((k_mg_sweep<T>), mg_grid(g), dim3(MG_BX, MG_BY, 1), 0, s, g, src, dst, b,
+    NYL_LAUNCH((k_mg_sweep<T>), mg_grid(g), dim3(MG_BX, MG_BY, 1), 0, s, g, src, dst, b,
                        omega, tab);
     return hipGetLastError();
 }
@@ -159,7 +159,7 @@ template <typename Tf, typename Tc>
 hipError_t launch_mg_resid_restrict(const Geo &gf, const Geo &gc, const Tf *x, const Tf *b,
                                     Tc *bc, Tc *xc, const MGTabs &tab, hipStream_t s, Tc *yc,
                                     double omega) {
-    hipLaunchKernelGGL((k_mg_resid_restrict<Tf, Tc>), mg_grid(gc), dim3(MG_BX, MG_BY, 1), 0, s,
+    NYL_LAUNCH((k_mg_resid_restrict<Tf, Tc>), mg_grid(gc), dim3(MG_BX, MG_BY, 1), 0, s,
                        gf, gc, x, b, bc, xc, tab, yc, omega);
     return hipGetLastError();
 }
@@ -207,7 +207,7 @@ __global__ __launch_bounds__(256) void k_mg_prolong(Geo gf, Geo gc, Tf *__restri
 template <typename Tf, typename Tc>
 hipError_t launch_mg_prolong(const Geo &gf, const Geo &gc, Tf *xf, const Tc *xc,
                              const MGTabs &tab, hipStream_t s) {
-    hipLaunchKernelGGL((k_mg_prolong<Tf, Tc>), mg_grid(gf), dim3(MG_BX, MG_BY, 1), 0, s, gf, gc,
+    NYL_LAUNCH((k_mg_prolong<Tf, Tc>), mg_grid(gf), dim3(MG_BX, MG_BY, 1), 0, s, gf, gc,
                        xf, xc, tab);
     return hipGetLastError();
 }
@@ -247,9 +247,11 @@ __device__ __forceinline__ void block_max_store(double v, double *out, int tid) 
 // cells (two rows per thread) and marches up KZ planes.  The current plane,
 // with a one-cell ring, sits in LDS (three buffers, one barrier per plane);
 // the z-neighbours of a thread's own cells stay in registers.  Each global
-// value is loaded about once.  The arithmetic is that of k_mg_sweep (same
-// operand order), so the result is bit-identical; out-of-domain neighbours
-// along a wall axis are zero.
+// value is loaded about once.  The loads run one plane ahead of the
+// arithmetic: plane k+2 of x and plane k+1 of b are in flight while plane k
+// is computed, so a plane costs one memory latency, not two in a row.  The
+// arithmetic is that of k_mg_sweep (same operand order), so the result is
+// bit-identical; out-of-domain neighbours along a wall axis are zero.
 //
 // RESNORM: the block also sums the squared residual b + diag*src - sum6(src)
 //          of its INPUT over its cells into partials[block] (the convergence
@@ -257,12 +259,91 @@ __device__ __forceinline__ void block_max_store(double v, double *out, int tid) 
 #define SW_TX 64
 #define SW_TY 8
 #define SW_R (SW_TY / 4)
+#define SW_LX (SW_TX + 2)
+#define SW_LY (SW_TY + 2)
+
+// Per-thread plane access of the 2.5-D kernels.  Thread (tx, ty) of a
+// SW_TX x 4 block owns R cells of column i0 + tx (LDS rows row[r]) and, for
+// the first NRING threads, one cell of the one-cell ring around the tile.
+// Every load is issued from a valid address (clamped when the cell is outside
+// a wall axis) and its value selected when it is stored to LDS, so the load
+// of a plane can be in flight across the barrier and the previous plane's
+// arithmetic.
+template <int R>
+struct Plane25 {
+    static constexpr int NRING = 2 * SW_LX + 2 * SW_TY;
+    int off[R], row[R];   // in-plane offsets and LDS rows of the own cells
+    int roff, rlx, rly;   // ring cell: in-plane offset, LDS column and row
+    int rgi, rgj;         // ring cell: (wrapped) grid position
+    bool ring, rin;       // thread holds a ring cell; the cell is in the domain
+
+    // row[] is set by the caller first
+    __device__ __forceinline__ void init(const Geo &g, int i0, int j0, int tx, int tid) {
+#pragma unroll
+        for (int r = 0; r < R; r++) off[r] = (j0 + row[r] - 1) * (int)g.sy + i0 + tx;
+        ring = tid < NRING;
+        rin = false;
+        roff = off[0];
+        rlx = rly = rgi = rgj = 0;
+        if (ring) {
+            const int q = tid;
+            if (q < SW_LX) { rlx = q; rly = 0; }
+            else if (q < 2 * SW_LX) { rlx = q - SW_LX; rly = SW_LY - 1; }
+            else { const int t = q - 2 * SW_LX; rly = 1 + (t >> 1); rlx = (t & 1) ? SW_LX - 1 : 0; }
+            int gi = i0 + rlx - 1, gj = j0 + rly - 1;
+            bool in = true;
+            if (g.px) gi = wrapi(gi, g.nx); else in = in && gi >= 0 && gi < g.nx;
+            if (g.py) gj = wrapi(gj, g.ny); else in = in && gj >= 0 && gj < g.ny;
+            if (in) { roff = gj * (int)g.sy + gi; rgi = gi; rgj = gj; rin = true; }
+        }
+    }
+};
+
+// raw values of one plane as loaded (selected in put25)
+template <typename T, int R>
+struct Raw25 {
+    T own[R], ring;
+    int mw;     // plane index loaded (wrapped; 0 when outside a wall axis)
+    bool zin;   // the plane is inside the domain
+};
+
+template <typename T, int R>
+__device__ __forceinline__ void fetch25(const Plane25<R> &p, const Geo &g, const T *__restrict__ src,
+                                        int m, Raw25<T, R> &v) {
+    v.zin = g.pz || (m >= 0 && m < g.nz);
+    v.mw = g.pz ? wrapi(m, g.nz) : (v.zin ? m : 0);
+    const T *s = src + (int64_t)v.mw * g.sz;
+#pragma unroll
+    for (int r = 0; r < R; r++) v.own[r] = s[p.off[r]];
+    v.ring = s[p.roff];
+}
+
+// Store a fetched plane into an LDS buffer and the own values into own[]:
+// zero outside the domain, plus the prolongation of the coarse xc when
+// PROLONG (the value k_mg_prolong would store, same expression).
+template <typename T, int R, bool PROLONG>
+__device__ __forceinline__ void put25(const Plane25<R> &p, const Raw25<T, R> &v,
+                                      T (*buf)[SW_LX], int tx, T *own, const Geo &gc,
+                                      const T *__restrict__ xc, const MGTabs &tab, int i,
+                                      int j0) {
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        T t = v.zin ? v.own[r] : T(0);
+        if (PROLONG && v.zin) t = t + prolong_term<T, T>(gc, xc, tab, i, j0 + p.row[r] - 1, v.mw);
+        own[r] = t;
+        buf[p.row[r]][1 + tx] = t;
+    }
+    if (p.ring) {
+        T t = (v.zin && p.rin) ? v.ring : T(0);
+        if (PROLONG && v.zin && p.rin) t = t + prolong_term<T, T>(gc, xc, tab, p.rgi, p.rgj, v.mw);
+        buf[p.rly][p.rlx] = t;
+    }
+}
 
 //
-// PROLONG: src is first corrected by the prolongation of the coarse xc (the
-//          value k_mg_prolong would store, same expression), on the own cells
-//          and the ring; the corrected x is only read by this sweep (the next
-//          sweep reads dst), so it is not stored.
+// PROLONG: src is first corrected by the prolongation of the coarse xc, on
+//          the own cells and the ring; the corrected x is only read by this
+//          sweep (the next sweep reads dst), so it is not stored.
 //
 // ADDX: the result is not stored; it is added to the FP64 xo instead (the
 //       last sweep of an inner V-cycle of the mixed solve, whose output e is
@@ -275,89 +356,81 @@ __global__ __launch_bounds__(256) void k_mg_sweep25(Geo g, const T *__restrict__
                                                     double *__restrict__ partials, Geo gc,
                                                     const T *__restrict__ xc,
                                                     double *__restrict__ xo) {
-    constexpr int LX = SW_TX + 2, LY = SW_TY + 2, NRING = 2 * LX + 2 * SW_TY;
-    __shared__ T sp[3][LY][LX];
+    __shared__ T sp[3][SW_LY][SW_LX];
     const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * SW_TX + tx;
     const Blk3 bk = xcd_block();
     const int i0 = bk.x * SW_TX, j0 = bk.y * SW_TY, k0 = bk.z * kz;
     const T om = T(omega), cff1 = T(1.0 - omega);
     const int i = i0 + tx;
 
-    // ring position of this thread (one value per plane), -1 when none/outside
-    int64_t roff = -1;
-    int rlx = 0, rly = 0, rgi = 0, rgj = 0;
-    if (tid < NRING) {
-        const int q = tid;
-        if (q < LX) { rlx = q; rly = 0; }
-        else if (q < 2 * LX) { rlx = q - LX; rly = LY - 1; }
-        else { const int t = q - 2 * LX; rly = 1 + (t >> 1); rlx = (t & 1) ? LX - 1 : 0; }
-        int gi = i0 + rlx - 1, gj = j0 + rly - 1;
-        bool in = true;
-        if (g.px) gi = wrapi(gi, g.nx); else in = in && gi >= 0 && gi < g.nx;
-        if (g.py) gj = wrapi(gj, g.ny); else in = in && gj >= 0 && gj < g.ny;
-        if (in) { roff = (int64_t)gj * g.sy + gi; rgi = gi; rgj = gj; }
-    }
-    auto zok = [&](int m) { return g.pz || (m >= 0 && m < g.nz); };
-    auto zw = [&](int m) { return g.pz ? wrapi(m, g.nz) : m; };
-    // load plane m: own cells into registers `own` and LDS, ring into LDS
-    auto load_plane = [&](int m, int buf, T *own) {
-        const bool zin = zok(m);
-        const int mw = zw(m);
-        const int64_t base = (int64_t)mw * g.sz;
+    Plane25<SW_R> P;
 #pragma unroll
-        for (int r = 0; r < SW_R; r++) {
-            const int j = j0 + ty + 4 * r;
-            T v = T(0);
-            if (zin) {
-                v = src[base + (int64_t)j * g.sy + i];
-                if (PROLONG) v = v + prolong_term<T, T>(gc, xc, tab, i, j, mw);
-            }
-            own[r] = v;
-            sp[buf][1 + ty + 4 * r][1 + tx] = v;
-        }
-        if (tid < NRING) {
-            T v = T(0);
-            if (zin && roff >= 0) {
-                v = src[base + roff];
-                if (PROLONG) v = v + prolong_term<T, T>(gc, xc, tab, rgi, rgj, mw);
-            }
-            sp[buf][rly][rlx] = v;
-        }
-    };
-    // in-domain neighbour count per own row (x and y parts fixed, z varies)
-    int nxy[SW_R];
+    for (int r = 0; r < SW_R; r++) P.row[r] = 1 + ty + 4 * r;
+    P.init(g, i0, j0, tx, tid);
+    // diagonal and its inverse per own cell for 0, 1, 2 in-domain z-neighbours
+    // (the x and y parts of the neighbour count are fixed along the march)
+    T idg[3][SW_R], dg[3][SW_R];
 #pragma unroll
     for (int r = 0; r < SW_R; r++) {
-        const int j = j0 + ty + 4 * r;
-        nxy[r] = (g.px || i > 0) + (g.px || i < g.nx - 1) + (g.py || j > 0) + (g.py || j < g.ny - 1);
+        const int j = j0 + P.row[r] - 1;
+        const int nxy = (g.px || i > 0) + (g.px || i < g.nx - 1) + (g.py || j > 0)
+                      + (g.py || j < g.ny - 1);
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            idg[q][r] = T(tab.idiag[nxy + q]);
+            dg[q][r] = RESNORM ? T(tab.diag[nxy + q]) : T(0);
+        }
     }
 
-    T zm[SW_R], z0[SW_R], zp[SW_R];
+    T zm[SW_R], z0[SW_R], zp[SW_R], bc[SW_R], bn[SW_R];
+    double xoc[SW_R], xon[SW_R];
+    Raw25<T, SW_R> va, vb;
     double acc = 0.0;
-    // plane k lives in buffer (k - k0 + 1) % 3; loading plane k+1 reuses the
+    // plane k lives in buffer (k - k0 + 1) % 3; storing plane k+1 reuses the
     // buffer of plane k-2, whose readers all passed the previous barrier
-    load_plane(k0 - 1, 0, zm);
-    load_plane(k0, 1, z0);
+    fetch25(P, g, src, k0 - 1, va);
+    fetch25(P, g, src, k0, vb);
+    put25<T, SW_R, PROLONG>(P, va, sp[0], tx, zm, gc, xc, tab, i, j0);
+    put25<T, SW_R, PROLONG>(P, vb, sp[1], tx, z0, gc, xc, tab, i, j0);
+    fetch25(P, g, src, k0 + 1, va);
+#pragma unroll
+    for (int r = 0; r < SW_R; r++) {
+        bn[r] = b[(int64_t)k0 * g.sz + P.off[r]];
+        if (ADDX) xon[r] = xo[(int64_t)k0 * g.sz + P.off[r]];
+    }
     for (int k = k0; k < k0 + kz; k++) {
         const int cur = (k - k0 + 1) % 3;
-        load_plane(k + 1, (k - k0 + 2) % 3, zp);
+        put25<T, SW_R, PROLONG>(P, va, sp[(k - k0 + 2) % 3], tx, zp, gc, xc, tab, i, j0);
+#pragma unroll
+        for (int r = 0; r < SW_R; r++) {
+            bc[r] = bn[r];
+            if (ADDX) xoc[r] = xon[r];
+        }
+        if (k + 1 < k0 + kz) {
+            fetch25(P, g, src, k + 2, va);
+#pragma unroll
+            for (int r = 0; r < SW_R; r++) {
+                bn[r] = b[(int64_t)(k + 1) * g.sz + P.off[r]];
+                if (ADDX) xon[r] = xo[(int64_t)(k + 1) * g.sz + P.off[r]];
+            }
+        }
         __syncthreads();
         const int nz_ = (g.pz || k > 0) + (g.pz || k < g.nz - 1);
 #pragma unroll
         for (int r = 0; r < SW_R; r++) {
-            const int ly = 1 + ty + 4 * r, lx = 1 + tx;
-            const int j = j0 + ty + 4 * r;
+            const int ly = P.row[r], lx = 1 + tx;
             const T S = sp[cur][ly][lx - 1] + sp[cur][ly][lx + 1] + sp[cur][ly - 1][lx]
                       + sp[cur][ly + 1][lx] + zm[r] + zp[r];
-            const int64_t c = (int64_t)k * g.sz + (int64_t)j * g.sy + i;
-            const T bv = b[c];
-            const int nw = nxy[r] + nz_;
+            const int64_t c = (int64_t)k * g.sz + P.off[r];
+            const T bv = bc[r];
+            const T id = nz_ == 2 ? idg[2][r] : (nz_ == 1 ? idg[1][r] : idg[0][r]);
             if (RESNORM) {
-                const double rr = (double)(bv + T(tab.diag[nw]) * z0[r] - S);
+                const T d = nz_ == 2 ? dg[2][r] : (nz_ == 1 ? dg[1][r] : dg[0][r]);
+                const double rr = (double)(bv + d * z0[r] - S);
                 acc = acc + rr * rr;
             }
-            const T v = cff1 * z0[r] + om * (S - bv) * T(tab.idiag[nw]);
-            if (ADDX) xo[c] = xo[c] + (double)v;
+            const T v = cff1 * z0[r] + om * (S - bv) * id;
+            if (ADDX) xo[c] = xoc[r] + (double)v;
             else dst[c] = v;
             zm[r] = z0[r];
             z0[r] = zp[r];
@@ -374,72 +447,65 @@ __global__ __launch_bounds__(256) void k_mg_sweep25(Geo g, const T *__restrict__
 // (coarse 32 x 4) and marches up kz fine planes (kz even); thread (tx, ty)
 // holds the fine rows j0 + 2 ty and j0 + 2 ty + 1 of column i0 + tx, so the
 // y pair of a coarse cell is in one thread, the x pair in two adjacent lanes
-// and the z pair in consecutive planes.  The residuals are those of
-// residual_at and the eight are summed in the order of k_mg_resid_restrict,
-// so the coarse b is bitwise the same.
+// and the z pair in consecutive planes.  Loads run one plane ahead, as in
+// k_mg_sweep25.  The residuals are those of residual_at and the eight are
+// summed in the order of k_mg_resid_restrict, so the coarse b is bitwise the
+// same.
 template <typename Tf, typename Tc>
 __global__ __launch_bounds__(256) void k_mg_restrict25(Geo g, Geo gc, const Tf *__restrict__ x,
                                                        const Tf *__restrict__ b,
                                                        Tc *__restrict__ bc, Tc *__restrict__ xc,
                                                        MGTabs tab, int kz, Tc *__restrict__ yc,
                                                        double omega) {
-    constexpr int LX = SW_TX + 2, LY = SW_TY + 2, NRING = 2 * LX + 2 * SW_TY;
-    __shared__ Tf sp[3][LY][LX];
+    __shared__ Tf sp[3][SW_LY][SW_LX];
     const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * SW_TX + tx;
     const Blk3 bk = xcd_block();
     const int i0 = bk.x * SW_TX, j0 = bk.y * SW_TY, k0 = bk.z * kz;
     const int i = i0 + tx;
 
-    int64_t roff = -1;
-    int rlx = 0, rly = 0;
-    if (tid < NRING) {
-        const int q = tid;
-        if (q < LX) { rlx = q; rly = 0; }
-        else if (q < 2 * LX) { rlx = q - LX; rly = LY - 1; }
-        else { const int t = q - 2 * LX; rly = 1 + (t >> 1); rlx = (t & 1) ? LX - 1 : 0; }
-        int gi = i0 + rlx - 1, gj = j0 + rly - 1;
-        bool in = true;
-        if (g.px) gi = wrapi(gi, g.nx); else in = in && gi >= 0 && gi < g.nx;
-        if (g.py) gj = wrapi(gj, g.ny); else in = in && gj >= 0 && gj < g.ny;
-        if (in) roff = (int64_t)gj * g.sy + gi;
-    }
-    auto zok = [&](int m) { return g.pz || (m >= 0 && m < g.nz); };
-    auto zw = [&](int m) { return g.pz ? wrapi(m, g.nz) : m; };
-    auto load_plane = [&](int m, int buf, Tf *own) {
-        const bool zin = zok(m);
-        const int64_t base = (int64_t)zw(m) * g.sz;
-#pragma unroll
-        for (int r = 0; r < 2; r++) {
-            const int j = j0 + 2 * ty + r;
-            own[r] = zin ? x[base + (int64_t)j * g.sy + i] : Tf(0);
-            sp[buf][1 + 2 * ty + r][1 + tx] = own[r];
-        }
-        if (tid < NRING) sp[buf][rly][rlx] = (zin && roff >= 0) ? x[base + roff] : Tf(0);
-    };
-    int nxy[2];
+    Plane25<2> P;
+    P.row[0] = 1 + 2 * ty;
+    P.row[1] = 2 + 2 * ty;
+    P.init(g, i0, j0, tx, tid);
+    Tf dg[3][2];
 #pragma unroll
     for (int r = 0; r < 2; r++) {
         const int j = j0 + 2 * ty + r;
-        nxy[r] = (g.px || i > 0) + (g.px || i < g.nx - 1) + (g.py || j > 0) + (g.py || j < g.ny - 1);
+        const int nxy = (g.px || i > 0) + (g.px || i < g.nx - 1) + (g.py || j > 0)
+                      + (g.py || j < g.ny - 1);
+#pragma unroll
+        for (int q = 0; q < 3; q++) dg[q][r] = Tf(tab.diag[nxy + q]);
     }
-    Tf zm[2], z0[2], zp[2];
+    Tf zm[2], z0[2], zp[2], bcur[2], bn[2];
+    Raw25<Tf, 2> va, vb;
     Tf acc = Tf(0);
-    load_plane(k0 - 1, 0, zm);
-    load_plane(k0, 1, z0);
+    fetch25(P, g, x, k0 - 1, va);
+    fetch25(P, g, x, k0, vb);
+    put25<Tf, 2, false>(P, va, sp[0], tx, zm, g, x, tab, i, j0);
+    put25<Tf, 2, false>(P, vb, sp[1], tx, z0, g, x, tab, i, j0);
+    fetch25(P, g, x, k0 + 1, va);
+#pragma unroll
+    for (int r = 0; r < 2; r++) bn[r] = b[(int64_t)k0 * g.sz + P.off[r]];
     for (int k = k0; k < k0 + kz; k++) {
         const int cur = (k - k0 + 1) % 3;
-        load_plane(k + 1, (k - k0 + 2) % 3, zp);
+        put25<Tf, 2, false>(P, va, sp[(k - k0 + 2) % 3], tx, zp, g, x, tab, i, j0);
+#pragma unroll
+        for (int r = 0; r < 2; r++) bcur[r] = bn[r];
+        if (k + 1 < k0 + kz) {
+            fetch25(P, g, x, k + 2, va);
+#pragma unroll
+            for (int r = 0; r < 2; r++) bn[r] = b[(int64_t)(k + 1) * g.sz + P.off[r]];
+        }
         __syncthreads();
         const int nz_ = (g.pz || k > 0) + (g.pz || k < g.nz - 1);
         Tf r[2];
 #pragma unroll
         for (int rr = 0; rr < 2; rr++) {
-            const int ly = 1 + 2 * ty + rr, lx = 1 + tx;
-            const int j = j0 + 2 * ty + rr;
+            const int ly = P.row[rr], lx = 1 + tx;
             const Tf S = sp[cur][ly][lx - 1] + sp[cur][ly][lx + 1] + sp[cur][ly - 1][lx]
                        + sp[cur][ly + 1][lx] + zm[rr] + zp[rr];
-            const int64_t c = (int64_t)k * g.sz + (int64_t)j * g.sy + i;
-            r[rr] = b[c] + Tf(tab.diag[nxy[rr] + nz_]) * z0[rr] - S;
+            const Tf d = nz_ == 2 ? dg[2][rr] : (nz_ == 1 ? dg[1][rr] : dg[0][rr]);
+            r[rr] = bcur[rr] + d * z0[rr] - S;
             zm[rr] = z0[rr];
             z0[rr] = zp[rr];
         }
@@ -496,8 +562,8 @@ hipError_t launch_norm2(const Geo &g, const T *v, double *partials, double *dev_
                         hipStream_t s) {
     const int64_t n = (int64_t)g.nx * g.ny * g.nz;
     const int nb = red_blocks(n);
-    hipLaunchKernelGGL((k_norm2<T>), dim3(nb), dim3(RED_T), 0, s, n, v, partials);
-    hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(RED_T), 0, s, partials, nb, dev_out);
+    NYL_LAUNCH((k_norm2<T>), dim3(nb), dim3(RED_T), 0, s, n, v, partials);
+    NYL_LAUNCH(k_final_sum, dim3(1), dim3(RED_T), 0, s, partials, nb, dev_out);
     return hipGetLastError();
 }
 
@@ -524,8 +590,8 @@ hipError_t launch_resnorm2(const Geo &g, const T *x, const T *b, const MGTabs &t
                            double *partials, double *dev_out, hipStream_t s) {
     const int64_t n = (int64_t)g.nx * g.ny * g.nz;
     const int nb = red_blocks(n);
-    hipLaunchKernelGGL((k_resnorm2<T>), dim3(nb), dim3(RED_T), 0, s, g, x, b, tab, partials);
-    hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(RED_T), 0, s, partials, nb, dev_out);
+    NYL_LAUNCH((k_resnorm2<T>), dim3(nb), dim3(RED_T), 0, s, g, x, b, tab, partials);
+    NYL_LAUNCH(k_final_sum, dim3(1), dim3(RED_T), 0, s, partials, nb, dev_out);
     return hipGetLastError();
 }
 
@@ -553,9 +619,9 @@ hipError_t launch_maxu2(const Geo &g, const T *u, const T *v, const T *w, double
                         hipStream_t s) {
     const int64_t n = (int64_t)g.nx * g.ny * g.nz;
     const int nb = red_blocks(n);
-    hipLaunchKernelGGL((k_maxu2<T>), dim3(nb), dim3(RED_T), 0, s, n, u, v, w, idx2, idy2, idz2,
+    NYL_LAUNCH((k_maxu2<T>), dim3(nb), dim3(RED_T), 0, s, n, u, v, w, idx2, idy2, idz2,
                        partials);
-    hipLaunchKernelGGL(k_final_max, dim3(1), dim3(RED_T), 0, s, partials, nb, dev_out);
+    NYL_LAUNCH(k_final_max, dim3(1), dim3(RED_T), 0, s, partials, nb, dev_out);
     return hipGetLastError();
 }
 
@@ -574,8 +640,8 @@ hipError_t launch_sum(const Geo &g, const T *v, double *partials, double *dev_ou
                       hipStream_t s) {
     const int64_t n = (int64_t)g.nx * g.ny * g.nz;
     const int nb = red_blocks(n);
-    hipLaunchKernelGGL((k_sum<T>), dim3(nb), dim3(RED_T), 0, s, n, v, partials);
-    hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(RED_T), 0, s, partials, nb, dev_out);
+    NYL_LAUNCH((k_sum<T>), dim3(nb), dim3(RED_T), 0, s, n, v, partials);
+    NYL_LAUNCH(k_final_sum, dim3(1), dim3(RED_T), 0, s, partials, nb, dev_out);
     return hipGetLastError();
 }
 
@@ -601,8 +667,8 @@ hipError_t launch_nonfinite(const Geo &g, const T *a, const T *b, const T *c, co
                             double *partials, double *dev_out, hipStream_t s) {
     const int64_t n = (int64_t)g.nx * g.ny * g.nz;
     const int nb = red_blocks(n);
-    hipLaunchKernelGGL((k_nonfinite<T>), dim3(nb), dim3(RED_T), 0, s, n, a, b, c, d, partials);
-    hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(RED_T), 0, s, partials, nb, dev_out);
+    NYL_LAUNCH((k_nonfinite<T>), dim3(nb), dim3(RED_T), 0, s, n, a, b, c, d, partials);
+    NYL_LAUNCH(k_final_sum, dim3(1), dim3(RED_T), 0, s, partials, nb, dev_out);
     return hipGetLastError();
 }
 
@@ -674,7 +740,7 @@ __global__ __launch_bounds__(256) void k_add_correction(int64_t n, double *__res
 template <typename TI>
 hipError_t launch_add_correction(const Geo &g, double *x, const TI *e, hipStream_t s) {
     const int64_t n = (int64_t)g.nx * g.ny * g.nz;
-    hipLaunchKernelGGL((k_add_correction<TI>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
+    NYL_LAUNCH((k_add_correction<TI>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
                        n, x, e);
     return hipGetLastError();
 }
@@ -699,9 +765,9 @@ hipError_t launch_sum_sumsq(const Geo &g, const double *v, double *partials, dou
     const int64_t n = (int64_t)g.nx * g.ny * g.nz;
     int nb = red_blocks(n);
     if (nb > NYL_RED_BLOCKS / 2) nb = NYL_RED_BLOCKS / 2;
-    hipLaunchKernelGGL(k_sum_sumsq, dim3(nb), dim3(RED_T), 0, s, n, v, partials, nb);
-    hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(RED_T), 0, s, partials, nb, dev_out);
-    hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(RED_T), 0, s, partials + nb, nb, dev_out + 1);
+    NYL_LAUNCH(k_sum_sumsq, dim3(nb), dim3(RED_T), 0, s, n, v, partials, nb);
+    NYL_LAUNCH(k_final_sum, dim3(1), dim3(RED_T), 0, s, partials, nb, dev_out);
+    NYL_LAUNCH(k_final_sum, dim3(1), dim3(RED_T), 0, s, partials + nb, nb, dev_out + 1);
     return hipGetLastError();
 }
 
@@ -717,7 +783,7 @@ __global__ __launch_bounds__(64) void k_publish(const double *__restrict__ src, 
 
 hipError_t launch_publish(const double *src, int n, double *dst, uint32_t *flag, uint32_t seq,
                           hipStream_t s) {
-    hipLaunchKernelGGL(k_publish, dim3(1), dim3(64), 0, s, src, n, dst, flag, seq);
+    NYL_LAUNCH(k_publish, dim3(1), dim3(64), 0, s, src, n, dst, flag, seq);
     return hipGetLastError();
 }
 
@@ -763,9 +829,9 @@ hipError_t launch_kesum(const Geo &g, const T *u, const T *v, const T *w, double
                         hipStream_t s) {
     const int64_t n = (int64_t)g.nx * g.ny * g.nz;
     const int nb = red_blocks(n);
-    hipLaunchKernelGGL((k_kesum<T>), dim3(nb), dim3(RED_T), 0, s, g, u, v, w, idx2, idy2, idz2,
+    NYL_LAUNCH((k_kesum<T>), dim3(nb), dim3(RED_T), 0, s, g, u, v, w, idx2, idy2, idz2,
                        partials);
-    hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(RED_T), 0, s, partials, nb, dev_out);
+    NYL_LAUNCH(k_final_sum, dim3(1), dim3(RED_T), 0, s, partials, nb, dev_out);
     return hipGetLastError();
 }
 
@@ -790,7 +856,7 @@ __global__ __launch_bounds__(BT) void k_b_sweep(const SweepItem<T> *__restrict__
 template <typename T>
 hipError_t launch_b_sweep(const SweepItem<T> *items, int n, int64_t nblk, const MGTabs &tab,
                           const int *active, hipStream_t s) {
-    hipLaunchKernelGGL((k_b_sweep<T>), dim3((unsigned)nblk), BT, 0, s, items, n, tab, active);
+    NYL_LAUNCH((k_b_sweep<T>), dim3((unsigned)nblk), BT, 0, s, items, n, tab, active);
     return hipGetLastError();
 }
 
@@ -809,7 +875,7 @@ __global__ __launch_bounds__(BT) void k_b_restrict(const RestrItem<Tf, Tc> *__re
 template <typename Tf, typename Tc>
 hipError_t launch_b_restrict(const RestrItem<Tf, Tc> *items, int n, int64_t nblk,
                              const MGTabs &tab, const int *active, hipStream_t s) {
-    hipLaunchKernelGGL((k_b_restrict<Tf, Tc>), dim3((unsigned)nblk), BT, 0, s, items, n, tab,
+    NYL_LAUNCH((k_b_restrict<Tf, Tc>), dim3((unsigned)nblk), BT, 0, s, items, n, tab,
                        active);
     return hipGetLastError();
 }
@@ -829,7 +895,7 @@ __global__ __launch_bounds__(BT) void k_b_prolong(const ProlItem<Tf, Tc> *__rest
 template <typename Tf, typename Tc>
 hipError_t launch_b_prolong(const ProlItem<Tf, Tc> *items, int n, int64_t nblk,
                             const MGTabs &tab, const int *active, hipStream_t s) {
-    hipLaunchKernelGGL((k_b_prolong<Tf, Tc>), dim3((unsigned)nblk), BT, 0, s, items, n, tab,
+    NYL_LAUNCH((k_b_prolong<Tf, Tc>), dim3((unsigned)nblk), BT, 0, s, items, n, tab,
                        active);
     return hipGetLastError();
 }
@@ -873,9 +939,9 @@ template <typename T>
 hipError_t launch_b_reduce(const RedItem<T> *items, int n, int64_t nblk, int kind,
                            const MGTabs &tab, const int *active, hipStream_t s) {
     const dim3 grd((unsigned)nblk);
-    if (kind == 0) hipLaunchKernelGGL((k_b_reduce<T, 0>), grd, BT, 0, s, items, n, tab, active);
-    else if (kind == 1) hipLaunchKernelGGL((k_b_reduce<T, 1>), grd, BT, 0, s, items, n, tab, active);
-    else hipLaunchKernelGGL((k_b_reduce<T, 2>), grd, BT, 0, s, items, n, tab, active);
+    if (kind == 0) NYL_LAUNCH((k_b_reduce<T, 0>), grd, BT, 0, s, items, n, tab, active);
+    else if (kind == 1) NYL_LAUNCH((k_b_reduce<T, 1>), grd, BT, 0, s, items, n, tab, active);
+    else NYL_LAUNCH((k_b_reduce<T, 2>), grd, BT, 0, s, items, n, tab, active);
     return hipGetLastError();
 }
 
@@ -908,7 +974,7 @@ __global__ __launch_bounds__(BT) void k_b_outer_resid(const OuterItem *__restric
 
 hipError_t launch_b_outer_resid(const OuterItem *items, int n, int64_t nblk, const BMG &mg,
                                 const MGTabs &tab, const int *active, hipStream_t s) {
-    hipLaunchKernelGGL(k_b_outer_resid, dim3((unsigned)nblk), BT, 0, s, items, n, mg, tab,
+    NYL_LAUNCH(k_b_outer_resid, dim3((unsigned)nblk), BT, 0, s, items, n, mg, tab,
                        active);
     return hipGetLastError();
 }
@@ -963,25 +1029,34 @@ __global__ __launch_bounds__(RED_T) void k_b_final(const FinalItem *__restrict__
     }
 }
 
-// mg.any[0] = number of members still iterating
+// mg.any[0] = number of members still iterating, mg.any[1] = the most
+// V-cycles any member has run in the current solve
 __global__ __launch_bounds__(RED_T) void k_b_count(int n, BMG mg) {
-    __shared__ double tot;
-    double acc = 0.0;
-    for (int q = threadIdx.x; q < n; q += RED_T) acc = acc + (double)mg.active[q];
+    __shared__ double tot, most;
+    double acc = 0.0, mx = 0.0;
+    for (int q = threadIdx.x; q < n; q += RED_T) {
+        acc = acc + (double)mg.active[q];
+        mx = fmax(mx, (double)mg.nite[q]);
+    }
     block_sum_store(acc, &tot, threadIdx.x);
     __syncthreads();
-    if (threadIdx.x == 0) mg.any[0] = (int)tot;
+    block_max_store(mx, &most, threadIdx.x);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        mg.any[0] = (int)tot;
+        mg.any[1] = (int)most;
+    }
 }
 
 hipError_t launch_b_final(const FinalItem *items, int n, int kind, const BMG &mg, hipStream_t s) {
     switch (kind) {
-    case NYL_FIN_NORMB: hipLaunchKernelGGL((k_b_final<NYL_FIN_NORMB>), dim3(n), RED_T, 0, s, items, mg); break;
-    case NYL_FIN_INIT: hipLaunchKernelGGL((k_b_final<NYL_FIN_INIT>), dim3(n), RED_T, 0, s, items, mg); break;
-    case NYL_FIN_STEP: hipLaunchKernelGGL((k_b_final<NYL_FIN_STEP>), dim3(n), RED_T, 0, s, items, mg); break;
-    default: hipLaunchKernelGGL((k_b_final<NYL_FIN_SHIFT>), dim3(n), RED_T, 0, s, items, mg); break;
+    case NYL_FIN_NORMB: NYL_LAUNCH((k_b_final<NYL_FIN_NORMB>), dim3(n), RED_T, 0, s, items, mg); break;
+    case NYL_FIN_INIT: NYL_LAUNCH((k_b_final<NYL_FIN_INIT>), dim3(n), RED_T, 0, s, items, mg); break;
+    case NYL_FIN_STEP: NYL_LAUNCH((k_b_final<NYL_FIN_STEP>), dim3(n), RED_T, 0, s, items, mg); break;
+    default: NYL_LAUNCH((k_b_final<NYL_FIN_SHIFT>), dim3(n), RED_T, 0, s, items, mg); break;
     }
     if (kind == NYL_FIN_INIT || kind == NYL_FIN_STEP)
-        hipLaunchKernelGGL(k_b_count, dim3(1), RED_T, 0, s, n, mg);
+        NYL_LAUNCH(k_b_count, dim3(1), RED_T, 0, s, n, mg);
     return hipGetLastError();
 }
 
@@ -990,7 +1065,7 @@ __global__ __launch_bounds__(RED_T) void k_b_acc(int n, BMG mg, int mode) {
 }
 
 hipError_t launch_b_acc(int n, const BMG &mg, int mode, hipStream_t s) {
-    hipLaunchKernelGGL(k_b_acc, dim3(1), RED_T, 0, s, n, mg, mode);
+    NYL_LAUNCH(k_b_acc, dim3(1), RED_T, 0, s, n, mg, mode);
     return hipGetLastError();
 }
 
@@ -1236,7 +1311,7 @@ bool fused_fits(const Geo &gf, const Geo &gc) {
 template <typename Tf, typename Tc>
 hipError_t launch_mg_down(const Geo &gf, const Geo &gc, const Tf *y, const Tf *b, Tf *x, Tc *bc,
                           Tc *yc, double omega, const MGTabs &tab, hipStream_t s) {
-    hipLaunchKernelGGL((k_mg_down<Tf, Tc>), fused_grid(gf), dim3(FUSED_T), 0, s, gf, gc, y, b, x,
+    NYL_LAUNCH((k_mg_down<Tf, Tc>), fused_grid(gf), dim3(FUSED_T), 0, s, gf, gc, y, b, x,
                        bc, yc, omega, tab);
     return hipGetLastError();
 }
@@ -1244,7 +1319,7 @@ hipError_t launch_mg_down(const Geo &gf, const Geo &gc, const Tf *y, const Tf *b
 template <typename Tf, typename Tc>
 hipError_t launch_mg_up(const Geo &gf, const Geo &gc, const Tf *x, const Tc *xc, const Tf *b,
                         Tf *y, double omega, const MGTabs &tab, hipStream_t s) {
-    hipLaunchKernelGGL((k_mg_up<Tf, Tc>), fused_grid(gf), dim3(FUSED_T), 0, s, gf, gc, x, xc, b,
+    NYL_LAUNCH((k_mg_up<Tf, Tc>), fused_grid(gf), dim3(FUSED_T), 0, s, gf, gc, x, xc, b,
                        y, omega, tab);
     return hipGetLastError();
 }
@@ -1285,26 +1360,27 @@ hipError_t launch_mg_sweep25(const Geo &g, const T *src, T *dst, const T *b, dou
     dim3 grd(g.nx / SW_TX, g.ny / SW_TY, g.nz / kz), blk(SW_TX, 4);
     const Geo gcv = gc ? *gc : g;
     if (partials) {
-        hipLaunchKernelGGL((k_mg_sweep25<T, true, false, false>), grd, blk, 0, s, g, src, dst, b,
+        NYL_LAUNCH((k_mg_sweep25<T, true, false, false>), grd, blk, 0, s, g, src, dst, b,
                            omega, tab, kz, partials, gcv, xc, addx);
         const int nb = grd.x * grd.y * grd.z;
-        hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(RED_T), 0, s, partials, nb, dev_out);
+        NYL_LAUNCH(k_final_sum, dim3(1), dim3(RED_T), 0, s, partials, nb, dev_out);
     } else if (addx) {
-        hipLaunchKernelGGL((k_mg_sweep25<T, false, false, true>), grd, blk, 0, s, g, src, dst, b,
+        NYL_LAUNCH((k_mg_sweep25<T, false, false, true>), grd, blk, 0, s, g, src, dst, b,
                            omega, tab, kz, partials, gcv, xc, addx);
     } else if (xc) {
-        hipLaunchKernelGGL((k_mg_sweep25<T, false, true, false>), grd, blk, 0, s, g, src, dst, b,
+        NYL_LAUNCH((k_mg_sweep25<T, false, true, false>), grd, blk, 0, s, g, src, dst, b,
                            omega, tab, kz, partials, gcv, xc, addx);
     } else {
-        hipLaunchKernelGGL((k_mg_sweep25<T, false, false, false>), grd, blk, 0, s, g, src, dst,
+        NYL_LAUNCH((k_mg_sweep25<T, false, false, false>), grd, blk, 0, s, g, src, dst,
                            b, omega, tab, kz, partials, gcv, xc, addx);
     }
     return hipGetLastError();
 }
 
 // Outer residual of the mixed solve with the 2.5-D tiling of k_mg_sweep25
-// (FP64 planes of x in LDS): per cell the values of k_mg_outer_resid (same
-// operand order), the block sums of r**2 into partials[block].
+// (FP64 planes of x in LDS, loads one plane ahead): per cell the values of
+// k_mg_outer_resid (same operand order), the block sums of r**2 into
+// partials[block] (and of b**2 into partials_b[block] when given).
 template <typename TI>
 __global__ __launch_bounds__(256) void k_mg_outer25(Geo g, const double *__restrict__ x,
                                                     const double *__restrict__ b, double shift,
@@ -1313,70 +1389,67 @@ __global__ __launch_bounds__(256) void k_mg_outer25(Geo g, const double *__restr
                                                     MGTabs tab, int kz,
                                                     double *__restrict__ partials,
                                                     double *__restrict__ partials_b) {
-    constexpr int LX = SW_TX + 2, LY = SW_TY + 2, NRING = 2 * LX + 2 * SW_TY;
-    __shared__ double sp[3][LY][LX];
+    __shared__ double sp[3][SW_LY][SW_LX];
     const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * SW_TX + tx;
     const Blk3 bk = xcd_block();
     const int i0 = bk.x * SW_TX, j0 = bk.y * SW_TY, k0 = bk.z * kz;
     const int i = i0 + tx;
     const TI om = TI(omega), cff1 = TI(1.0 - omega);
 
-    int64_t roff = -1;
-    int rlx = 0, rly = 0;
-    if (tid < NRING) {
-        const int q = tid;
-        if (q < LX) { rlx = q; rly = 0; }
-        else if (q < 2 * LX) { rlx = q - LX; rly = LY - 1; }
-        else { const int t = q - 2 * LX; rly = 1 + (t >> 1); rlx = (t & 1) ? LX - 1 : 0; }
-        int gi = i0 + rlx - 1, gj = j0 + rly - 1;
-        bool in = true;
-        if (g.px) gi = wrapi(gi, g.nx); else in = in && gi >= 0 && gi < g.nx;
-        if (g.py) gj = wrapi(gj, g.ny); else in = in && gj >= 0 && gj < g.ny;
-        if (in) roff = (int64_t)gj * g.sy + gi;
-    }
-    auto zok = [&](int m) { return g.pz || (m >= 0 && m < g.nz); };
-    auto zw = [&](int m) { return g.pz ? wrapi(m, g.nz) : m; };
-    auto load_plane = [&](int m, int buf, double *own) {
-        const bool zin = zok(m);
-        const int64_t base = (int64_t)zw(m) * g.sz;
+    Plane25<SW_R> P;
 #pragma unroll
-        for (int r = 0; r < SW_R; r++) {
-            const int j = j0 + ty + 4 * r;
-            own[r] = zin ? x[base + (int64_t)j * g.sy + i] : 0.0;
-            sp[buf][1 + ty + 4 * r][1 + tx] = own[r];
-        }
-        if (tid < NRING) sp[buf][rly][rlx] = (zin && roff >= 0) ? x[base + roff] : 0.0;
-    };
-    int nxy[SW_R];
+    for (int r = 0; r < SW_R; r++) P.row[r] = 1 + ty + 4 * r;
+    P.init(g, i0, j0, tx, tid);
+    double dg[3][SW_R];
+    TI idg[3][SW_R];
 #pragma unroll
     for (int r = 0; r < SW_R; r++) {
-        const int j = j0 + ty + 4 * r;
-        nxy[r] = (g.px || i > 0) + (g.px || i < g.nx - 1) + (g.py || j > 0) + (g.py || j < g.ny - 1);
+        const int j = j0 + P.row[r] - 1;
+        const int nxy = (g.px || i > 0) + (g.px || i < g.nx - 1) + (g.py || j > 0)
+                      + (g.py || j < g.ny - 1);
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            dg[q][r] = tab.diag[nxy + q];
+            idg[q][r] = TI(tab.idiag[nxy + q]);
+        }
     }
-    double zm[SW_R], z0[SW_R], zp[SW_R];
+    double zm[SW_R], z0[SW_R], zp[SW_R], bcur[SW_R], bn[SW_R];
+    Raw25<double, SW_R> va, vb;
     double acc = 0.0, accb = 0.0;
-    load_plane(k0 - 1, 0, zm);
-    load_plane(k0, 1, z0);
+    fetch25(P, g, x, k0 - 1, va);
+    fetch25(P, g, x, k0, vb);
+    put25<double, SW_R, false>(P, va, sp[0], tx, zm, g, x, tab, i, j0);
+    put25<double, SW_R, false>(P, vb, sp[1], tx, z0, g, x, tab, i, j0);
+    fetch25(P, g, x, k0 + 1, va);
+#pragma unroll
+    for (int r = 0; r < SW_R; r++) bn[r] = b[(int64_t)k0 * g.sz + P.off[r]];
     for (int k = k0; k < k0 + kz; k++) {
         const int cur = (k - k0 + 1) % 3;
-        load_plane(k + 1, (k - k0 + 2) % 3, zp);
+        put25<double, SW_R, false>(P, va, sp[(k - k0 + 2) % 3], tx, zp, g, x, tab, i, j0);
+#pragma unroll
+        for (int r = 0; r < SW_R; r++) bcur[r] = bn[r];
+        if (k + 1 < k0 + kz) {
+            fetch25(P, g, x, k + 2, va);
+#pragma unroll
+            for (int r = 0; r < SW_R; r++) bn[r] = b[(int64_t)(k + 1) * g.sz + P.off[r]];
+        }
         __syncthreads();
         const int nz_ = (g.pz || k > 0) + (g.pz || k < g.nz - 1);
 #pragma unroll
         for (int r = 0; r < SW_R; r++) {
-            const int ly = 1 + ty + 4 * r, lx = 1 + tx;
-            const int j = j0 + ty + 4 * r;
+            const int ly = P.row[r], lx = 1 + tx;
             const double S = sp[cur][ly][lx - 1] + sp[cur][ly][lx + 1] + sp[cur][ly - 1][lx]
                            + sp[cur][ly + 1][lx] + zm[r] + zp[r];
-            const int64_t c = (int64_t)k * g.sz + (int64_t)j * g.sy + i;
-            const int nw = nxy[r] + nz_;
-            const double bc = b[c];
-            const double rr = (bc - shift) + tab.diag[nw] * z0[r] - S;
+            const int64_t c = (int64_t)k * g.sz + P.off[r];
+            const double d = nz_ == 2 ? dg[2][r] : (nz_ == 1 ? dg[1][r] : dg[0][r]);
+            const TI id = nz_ == 2 ? idg[2][r] : (nz_ == 1 ? idg[1][r] : idg[0][r]);
+            const double bv = bcur[r];
+            const double rr = (bv - shift) + d * z0[r] - S;
             acc = acc + rr * rr;
-            accb = accb + bc * bc;
+            accb = accb + bv * bv;
             const TI rf = TI(rr);
             r_out[c] = rf;
-            y_out[c] = cff1 * TI(0) + om * (TI(0) - rf) * TI(tab.idiag[nw]);
+            y_out[c] = cff1 * TI(0) + om * (TI(0) - rf) * id;
             zm[r] = z0[r];
             z0[r] = zp[r];
         }
@@ -1419,19 +1492,19 @@ hipError_t launch_mg_outer_resid(const Geo &g, const double *x, const double *b,
         const int nb = grd.x * grd.y * grd.z;
         if (with_bsq && 2 * nb > NYL_RED_BLOCKS) return hipErrorInvalidValue;
         double *pb = with_bsq ? partials + nb : nullptr;
-        hipLaunchKernelGGL((k_mg_outer25<TI>), grd, blk, 0, s, g, x, b, shift, r, y, omega, tab,
+        NYL_LAUNCH((k_mg_outer25<TI>), grd, blk, 0, s, g, x, b, shift, r, y, omega, tab,
                            kz, partials, pb);
-        hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(RED_T), 0, s, partials, nb, dev_out);
+        NYL_LAUNCH(k_final_sum, dim3(1), dim3(RED_T), 0, s, partials, nb, dev_out);
         if (with_bsq)
-            hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(RED_T), 0, s, pb, nb, dev_out + 1);
+            NYL_LAUNCH(k_final_sum, dim3(1), dim3(RED_T), 0, s, pb, nb, dev_out + 1);
         return hipGetLastError();
     }
     if (with_bsq) return hipErrorInvalidValue;
     const dim3 grd = outer_grid(g);
-    hipLaunchKernelGGL((k_mg_outer_resid<TI>), grd, dim3(MG_BX, MG_BY, 1), 0, s, g, x, b, shift,
+    NYL_LAUNCH((k_mg_outer_resid<TI>), grd, dim3(MG_BX, MG_BY, 1), 0, s, g, x, b, shift,
                        r, y, omega, tab, partials);
     const int nb = grd.x * grd.y * grd.z;
-    hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(RED_T), 0, s, partials, nb, dev_out);
+    NYL_LAUNCH(k_final_sum, dim3(1), dim3(RED_T), 0, s, partials, nb, dev_out);
     return hipGetLastError();
 }
 
@@ -1531,7 +1604,7 @@ hipError_t launch_mg_prolong25(const Geo &gf, const Geo &gc, Tf *xf, const Tc *x
                                const MGTabs &tab, hipStream_t s, Tf *yf) {
     const int kz = sweep25_kz(gf, 2);
     dim3 grd(gf.nx / SW_TX, gf.ny / SW_TY, gf.nz / kz), blk(SW_TX, 4);
-    hipLaunchKernelGGL((k_mg_prolong25<Tf, Tc>), grd, blk, 0, s, gf, gc, xf, xc, tab, kz, yf);
+    NYL_LAUNCH((k_mg_prolong25<Tf, Tc>), grd, blk, 0, s, gf, gc, xf, xc, tab, kz, yf);
     return hipGetLastError();
 }
 
@@ -1679,7 +1752,7 @@ hipError_t launch_mg_sweep2x25(const Geo &g, const T *src, T *dst, const T *b, d
     if (src == dst) return hipErrorInvalidValue;
     const int kz = sweep2x_kz(g);
     dim3 grd(g.nx / SW_TX, g.ny / SW_TY, g.nz / kz), blk(SW_TX, 4);
-    hipLaunchKernelGGL((k_mg_sweep2x25<T>), grd, blk, 0, s, g, src, dst, b, omega, tab, kz);
+    NYL_LAUNCH((k_mg_sweep2x25<T>), grd, blk, 0, s, g, src, dst, b, omega, tab, kz);
     return hipGetLastError();
 }
 template hipError_t launch_mg_sweep2x25<double>(const Geo &, const double *, double *,
@@ -1698,7 +1771,7 @@ hipError_t launch_mg_restrict25(const Geo &gf, const Geo &gc, const Tf *x, const
                                 Tc *xc, const MGTabs &tab, hipStream_t s, Tc *yc, double omega) {
     const int kz = sweep25_kz(gf, 2);
     dim3 grd(gf.nx / SW_TX, gf.ny / SW_TY, gf.nz / kz), blk(SW_TX, 4);
-    hipLaunchKernelGGL((k_mg_restrict25<Tf, Tc>), grd, blk, 0, s, gf, gc, x, b, bc, xc, tab, kz,
+    NYL_LAUNCH((k_mg_restrict25<Tf, Tc>), grd, blk, 0, s, gf, gc, x, b, bc, xc, tab, kz,
                        yc, omega);
     return hipGetLastError();
 }

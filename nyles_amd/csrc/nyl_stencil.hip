@@ -107,6 +107,10 @@ template <typename T>
 __device__ __forceinline__ void stb(T *__restrict__ p, uint32_t boff, T v) {
     *reinterpret_cast<T *>(reinterpret_cast<char *>(p) + boff) = v;
 }
+template <typename T>
+__device__ __forceinline__ void stb_stream(T *__restrict__ p, uint32_t boff, T v) {
+    store_stream(reinterpret_cast<T *>(reinterpret_cast<char *>(p) + boff), v);
+}
 
 // Vortex-force flux of one term at a cell (core/fortran_vortex_force.f90
 // WENO branch, :66-81 and :145-159): the vorticity component `om` is
@@ -280,23 +284,31 @@ __device__ __forceinline__ void stage_cell(const Geo &g, const StageArgs<T> &a, 
     if constexpr (PY != 0) Ly = per_line(j, g.ny, (int)g.sy * ES);
     if constexpr (PZ != 0) Lz = per_line(k, g.nz, (int)g.sz * ES);
 
+    // model variants (tendencies-only mode): the linear model has no vortex
+    // force (core/model_les.py:135-136), Euler3d neither advects b
+    // (core/model_les_euler.py:131) nor adds buoyancy (core/bernoulli.py:26-32)
+    const bool vf = MODE != 0 || !(a.flags & NYL_SF_NO_VF);
+    const bool with_b = MODE != 0 || !(a.flags & NYL_SF_NO_B);
+
     // tracer b: rhstrac, directions i, j, k (core/tracer.py:63-72)
     const T cx = T(a.idx2), cy = T(a.idy2), cz = T(a.idz2);
     T db = T(0);
     int64_t bx = c - i, by = c - (int64_t)j * g.sy, bz = c - (int64_t)k * g.sz;
     uint32_t cs = cb;
-    if constexpr (PX != 0) db = tracer_axis_per<T>(db, a.b, a.u, cx, cs, Lx);
-    else db = tracer_axis<T, 0>(db, a.b, a.u, cx, bx, 1, i, g.nx);
-    STAGE_SEQ_A(by, db);
-    STAGE_SEQ_A(cs, db);
-    if constexpr (PY != 0) db = tracer_axis_per<T>(db, a.b, a.v, cy, cs, Ly);
-    else db = tracer_axis<T, 0>(db, a.b, a.v, cy, by, g.sy, j, g.ny);
-    STAGE_SEQ_A(bz, db);
-    STAGE_SEQ_A(cs, db);
-    if constexpr (PZ != 0) db = tracer_axis_per<T>(db, a.b, a.w, cz, cs, Lz);
-    else db = tracer_axis<T, 0>(db, a.b, a.w, cz, bz, g.sz, k, g.nz);
-    STAGE_SEQ(bx, db);
-    STAGE_SEQ(cs, db);
+    if (with_b) {
+        if constexpr (PX != 0) db = tracer_axis_per<T>(db, a.b, a.u, cx, cs, Lx);
+        else db = tracer_axis<T, 0>(db, a.b, a.u, cx, bx, 1, i, g.nx);
+        STAGE_SEQ_A(by, db);
+        STAGE_SEQ_A(cs, db);
+        if constexpr (PY != 0) db = tracer_axis_per<T>(db, a.b, a.v, cy, cs, Ly);
+        else db = tracer_axis<T, 0>(db, a.b, a.v, cy, by, g.sy, j, g.ny);
+        STAGE_SEQ_A(bz, db);
+        STAGE_SEQ_A(cs, db);
+        if constexpr (PZ != 0) db = tracer_axis_per<T>(db, a.b, a.w, cz, cs, Lz);
+        else db = tracer_axis<T, 0>(db, a.b, a.w, cz, bz, g.sz, k, g.nz);
+        STAGE_SEQ(bx, db);
+        STAGE_SEQ(cs, db);
+    }
 
     // vortex force, triplets 'ikj', 'jik', 'kji' (core/vortex_force.py:69-81)
     const int64_t oex = PX ? (int64_t)(wrapi(i + 1, g.nx) - i) : 1;
@@ -308,6 +320,7 @@ __device__ __forceinline__ void stage_cell(const Geo &g, const StageArgs<T> &a, 
     const bool ey = PY || j < g.ny - 1;
     const bool ez = PZ || k < g.nz - 1;
     T du = T(0), dv = T(0), dw = T(0);
+    if (vf) {
     // direc(U_i, vor_k, du_j)
     if (ey) {
         if constexpr (PX != 0) dv = dv - vf_flux_per<T>(a.u, cx, a.oz, cs, Lx, bey);
@@ -348,6 +361,7 @@ __device__ __forceinline__ void stage_cell(const Geo &g, const StageArgs<T> &a, 
         if constexpr (PX != 0) dw = dw + vf_flux_per<T>(a.u, cx, a.oy, cs, Lx, bez);
         else dw = dw + vf_flux<T, 0>(a.u, cx, a.oy, bx, 1, i, g.nx, oez);
     }
+    }
 
     // bernoulli (core/bernoulli.py:14-23, core/fortran_bernoulli.f90:2-58)
     const T kec = ldb(a.ke, cb);
@@ -355,13 +369,15 @@ __device__ __forceinline__ void stage_cell(const Geo &g, const StageArgs<T> &a, 
     if (ey) dv = dv - (ldb(a.ke, cb + bey) - kec);
     if (ez) {
         const T cff = T(0.5 * a.dz);
-        dw = dw - (ldb(a.ke, cb + bez) - kec) + cff * (ldb(a.b, cb + bez) + ldb(a.b, cb));
+        if (with_b) dw = dw - (ldb(a.ke, cb + bez) - kec) + cff * (ldb(a.b, cb + bez) + ldb(a.b, cb));
+        else dw = dw - (ldb(a.ke, cb + bez) - kec);
     }
     // viscosity at the last stage (core/model_les.py:140-141)
     if (a.visc) viscosity_cell<T, PX, PY, PZ>(g, a, cb, i, j, k, du, dv, dw);
 
-    if (MODE == 0 || MODE == 1 || MODE == 2) {
-        stb(a.tb, cb, db); stb(a.tu, cb, du); stb(a.tv, cb, dv); stb(a.tw, cb, dw);
+    if (MODE == 0 || MODE == 1 || MODE == 2) {   // read at a later RK stage
+        stb_stream(a.tb, cb, db); stb_stream(a.tu, cb, du);
+        stb_stream(a.tv, cb, dv); stb_stream(a.tw, cb, dw);
     }
     const T cc = T(a.c);
     if (MODE == 1) {            // s += dt*ds0
@@ -420,13 +436,13 @@ template <typename T, int PX, int PY, int PZ>
 static void launch_stage_p(const Geo &g, const StageArgs<T> &a, int mode, hipStream_t s) {
     dim3 blk(NYL_BX, NYL_BY, 1), grd = cell_grid(g);
     switch (mode) {
-    case 0: hipLaunchKernelGGL((k_stage<T, 0, PX, PY, PZ>), grd, blk, 0, s, g, a); break;
-    case 1: hipLaunchKernelGGL((k_stage<T, 1, PX, PY, PZ>), grd, blk, 0, s, g, a); break;
-    case 2: hipLaunchKernelGGL((k_stage<T, 2, PX, PY, PZ>), grd, blk, 0, s, g, a); break;
-    case 3: hipLaunchKernelGGL((k_stage<T, 3, PX, PY, PZ>), grd, blk, 0, s, g, a); break;
-    case 4: hipLaunchKernelGGL((k_stage<T, 4, PX, PY, PZ>), grd, blk, 0, s, g, a); break;
-    case 5: hipLaunchKernelGGL((k_stage<T, 5, PX, PY, PZ>), grd, blk, 0, s, g, a); break;
-    case 6: hipLaunchKernelGGL((k_stage<T, 6, PX, PY, PZ>), grd, blk, 0, s, g, a); break;
+    case 0: NYL_LAUNCH((k_stage<T, 0, PX, PY, PZ>), grd, blk, 0, s, g, a); break;
+    case 1: NYL_LAUNCH((k_stage<T, 1, PX, PY, PZ>), grd, blk, 0, s, g, a); break;
+    case 2: NYL_LAUNCH((k_stage<T, 2, PX, PY, PZ>), grd, blk, 0, s, g, a); break;
+    case 3: NYL_LAUNCH((k_stage<T, 3, PX, PY, PZ>), grd, blk, 0, s, g, a); break;
+    case 4: NYL_LAUNCH((k_stage<T, 4, PX, PY, PZ>), grd, blk, 0, s, g, a); break;
+    case 5: NYL_LAUNCH((k_stage<T, 5, PX, PY, PZ>), grd, blk, 0, s, g, a); break;
+    case 6: NYL_LAUNCH((k_stage<T, 6, PX, PY, PZ>), grd, blk, 0, s, g, a); break;
     default: break;
     }
 }
@@ -766,7 +782,7 @@ __global__ __launch_bounds__(S25_NT) void k_stage25(Geo g, StageArgs<T> a, int k
             const T b0 = bw.v3, u0 = L.u0[ty * S25_W1 + tx + 1];
             const T v0 = L.v0[(ty + 1) * S25_W1 + tx], w0 = wc;
             if (MODE == 0 || MODE == 1 || MODE == 2) {
-                stb(a.tb, cb, db); stb(a.tu, cb, du); stb(a.tv, cb, dv); stb(a.tw, cb, dw);
+                stb_stream(a.tb, cb, db); stb_stream(a.tu, cb, du); stb_stream(a.tv, cb, dv); stb_stream(a.tw, cb, dw);
             }
             const T cc = T(a.c);
             if (MODE == 1) {            // s += dt*ds0
@@ -830,13 +846,13 @@ static void launch_stage25_p(const Geo &g, const StageArgs<T> &a, int mode, hipS
     dim3 blk(S25_NT), grd((g.nx + S25_TX - 1) / S25_TX, (g.ny + S25_TY - 1) / S25_TY,
                           (g.nz + kz - 1) / kz);
     switch (mode) {
-    case 0: hipLaunchKernelGGL((k_stage25<T, 0, PX, PY, PZ>), grd, blk, 0, s, g, a, kz); break;
-    case 1: hipLaunchKernelGGL((k_stage25<T, 1, PX, PY, PZ>), grd, blk, 0, s, g, a, kz); break;
-    case 2: hipLaunchKernelGGL((k_stage25<T, 2, PX, PY, PZ>), grd, blk, 0, s, g, a, kz); break;
-    case 3: hipLaunchKernelGGL((k_stage25<T, 3, PX, PY, PZ>), grd, blk, 0, s, g, a, kz); break;
-    case 4: hipLaunchKernelGGL((k_stage25<T, 4, PX, PY, PZ>), grd, blk, 0, s, g, a, kz); break;
-    case 5: hipLaunchKernelGGL((k_stage25<T, 5, PX, PY, PZ>), grd, blk, 0, s, g, a, kz); break;
-    case 6: hipLaunchKernelGGL((k_stage25<T, 6, PX, PY, PZ>), grd, blk, 0, s, g, a, kz); break;
+    case 0: NYL_LAUNCH((k_stage25<T, 0, PX, PY, PZ>), grd, blk, 0, s, g, a, kz); break;
+    case 1: NYL_LAUNCH((k_stage25<T, 1, PX, PY, PZ>), grd, blk, 0, s, g, a, kz); break;
+    case 2: NYL_LAUNCH((k_stage25<T, 2, PX, PY, PZ>), grd, blk, 0, s, g, a, kz); break;
+    case 3: NYL_LAUNCH((k_stage25<T, 3, PX, PY, PZ>), grd, blk, 0, s, g, a, kz); break;
+    case 4: NYL_LAUNCH((k_stage25<T, 4, PX, PY, PZ>), grd, blk, 0, s, g, a, kz); break;
+    case 5: NYL_LAUNCH((k_stage25<T, 5, PX, PY, PZ>), grd, blk, 0, s, g, a, kz); break;
+    case 6: NYL_LAUNCH((k_stage25<T, 6, PX, PY, PZ>), grd, blk, 0, s, g, a, kz); break;
     default: break;
     }
 }
@@ -844,7 +860,7 @@ static void launch_stage25_p(const Geo &g, const StageArgs<T> &a, int mode, hipS
 template <typename T>
 hipError_t launch_stage(const Geo &g, const StageArgs<T> &a, int mode, hipStream_t s) {
     if (mode < 0 || mode > 6) return hipErrorInvalidValue;
-    const bool t25 = s25_fits(g);
+    const bool t25 = s25_fits(g) && a.flags == 0;   // the 2.5-D form runs the LES only
     switch (g.px + 2 * g.py + 4 * g.pz) {
 #define NYL_STAGE_CASE(c, X, Y, Z)                                                   \
     case c:                                                                          \
@@ -922,13 +938,13 @@ template <typename T, int PX, int PY, int PZ>
 static void launch_tracer_p(const Geo &g, const TracerArgs<T> &a, int mode, hipStream_t s) {
     dim3 blk(NYL_BX, NYL_BY, 1), grd = cell_grid(g);
     switch (mode) {
-    case 0: hipLaunchKernelGGL((k_tracer_stage<T, 0, PX, PY, PZ>), grd, blk, 0, s, g, a); break;
-    case 1: hipLaunchKernelGGL((k_tracer_stage<T, 1, PX, PY, PZ>), grd, blk, 0, s, g, a); break;
-    case 2: hipLaunchKernelGGL((k_tracer_stage<T, 2, PX, PY, PZ>), grd, blk, 0, s, g, a); break;
-    case 3: hipLaunchKernelGGL((k_tracer_stage<T, 3, PX, PY, PZ>), grd, blk, 0, s, g, a); break;
-    case 4: hipLaunchKernelGGL((k_tracer_stage<T, 4, PX, PY, PZ>), grd, blk, 0, s, g, a); break;
-    case 5: hipLaunchKernelGGL((k_tracer_stage<T, 5, PX, PY, PZ>), grd, blk, 0, s, g, a); break;
-    case 6: hipLaunchKernelGGL((k_tracer_stage<T, 6, PX, PY, PZ>), grd, blk, 0, s, g, a); break;
+    case 0: NYL_LAUNCH((k_tracer_stage<T, 0, PX, PY, PZ>), grd, blk, 0, s, g, a); break;
+    case 1: NYL_LAUNCH((k_tracer_stage<T, 1, PX, PY, PZ>), grd, blk, 0, s, g, a); break;
+    case 2: NYL_LAUNCH((k_tracer_stage<T, 2, PX, PY, PZ>), grd, blk, 0, s, g, a); break;
+    case 3: NYL_LAUNCH((k_tracer_stage<T, 3, PX, PY, PZ>), grd, blk, 0, s, g, a); break;
+    case 4: NYL_LAUNCH((k_tracer_stage<T, 4, PX, PY, PZ>), grd, blk, 0, s, g, a); break;
+    case 5: NYL_LAUNCH((k_tracer_stage<T, 5, PX, PY, PZ>), grd, blk, 0, s, g, a); break;
+    case 6: NYL_LAUNCH((k_tracer_stage<T, 6, PX, PY, PZ>), grd, blk, 0, s, g, a); break;
     default: break;
     }
 }
@@ -978,12 +994,12 @@ template <typename T>
 hipError_t launch_update(int64_t n, const UpdateArgs<T> &a, int mode, hipStream_t s) {
     const dim3 grd((unsigned)((n + 255) / 256));
     switch (mode) {
-    case 1: hipLaunchKernelGGL((k_update<T, 1>), grd, dim3(256), 0, s, n, a); break;
-    case 2: hipLaunchKernelGGL((k_update<T, 2>), grd, dim3(256), 0, s, n, a); break;
-    case 3: hipLaunchKernelGGL((k_update<T, 3>), grd, dim3(256), 0, s, n, a); break;
-    case 4: hipLaunchKernelGGL((k_update<T, 4>), grd, dim3(256), 0, s, n, a); break;
-    case 5: hipLaunchKernelGGL((k_update<T, 5>), grd, dim3(256), 0, s, n, a); break;
-    case 6: hipLaunchKernelGGL((k_update<T, 6>), grd, dim3(256), 0, s, n, a); break;
+    case 1: NYL_LAUNCH((k_update<T, 1>), grd, dim3(256), 0, s, n, a); break;
+    case 2: NYL_LAUNCH((k_update<T, 2>), grd, dim3(256), 0, s, n, a); break;
+    case 3: NYL_LAUNCH((k_update<T, 3>), grd, dim3(256), 0, s, n, a); break;
+    case 4: NYL_LAUNCH((k_update<T, 4>), grd, dim3(256), 0, s, n, a); break;
+    case 5: NYL_LAUNCH((k_update<T, 5>), grd, dim3(256), 0, s, n, a); break;
+    case 6: NYL_LAUNCH((k_update<T, 6>), grd, dim3(256), 0, s, n, a); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -1017,6 +1033,12 @@ __device__ __forceinline__ void st4(double *p, const Q4<double> &q) {
     *reinterpret_cast<double2 *>(p) = make_double2(q.v[0], q.v[1]);
     *reinterpret_cast<double2 *>(p + 2) = make_double2(q.v[2], q.v[3]);
 }
+// st4 with the nontemporal hint (the four stores merge into wide ones)
+template <typename T>
+__device__ __forceinline__ void st4_stream(T *p, const Q4<T> &q) {
+#pragma unroll
+    for (int m = 0; m < 4; m++) store_stream(p + m, q.v[m]);
+}
 
 static inline dim3 cell_grid4(const Geo &g) {
     return dim3((g.nx / 4 + NYL_BX - 1) / NYL_BX, (g.ny + NYL_BY - 1) / NYL_BY, g.nz / cell_kz(g));
@@ -1029,9 +1051,17 @@ static inline bool quad_ok(const Geo &g, std::initializer_list<const void *> ptr
     return true;
 }
 
-// diag_cell for cells i0 .. i0+3
-template <typename T>
-__global__ __launch_bounds__(256) void k_diag4(Geo g, DiagArgs<T> a, int what) {
+// diag_cell for cells i0 .. i0+3; WHAT is the diag_cell selection as a
+// compile-time constant.  FP32: every load the selection needs is issued at
+// the top of a plane, together (one memory latency per plane; a neighbour
+// outside a wall axis is loaded from the own cell instead and never used),
+// and the results are stored with the nontemporal hint (the next stage
+// kernel reads them): 115 -> 95 us at 256^3.  FP64: the neighbour loads of
+// each part just before it and plain stores (the register cost of hoisting
+// the FP64 loads lowers occupancy; 206 vs 232 us, and the hint gains nothing).
+template <typename T, int WHAT>
+__global__ __launch_bounds__(256) void k_diag4(Geo g, DiagArgs<T> a) {
+    constexpr bool HOIST = sizeof(T) == 4;
     const Blk3 bk = CELL_BLOCK();
     const int i0 = (bk.x * NYL_BX + threadIdx.x) * 4;
     const int j = bk.y * NYL_BY + threadIdx.y;
@@ -1045,10 +1075,32 @@ __global__ __launch_bounds__(256) void k_diag4(Geo g, DiagArgs<T> a, int what) {
     const bool ey = g.py || j < g.ny - 1, hym = g.py || j > 0;
     const int64_t oyp = g.py ? (int64_t)(wrapi(j + 1, g.ny) - j) * g.sy : g.sy;
     const int64_t oym = g.py ? (int64_t)(j - wrapi(j - 1, g.ny)) * g.sy : g.sy;
+    auto out4 = [&](T *p, const Q4<T> &q) {
+        if constexpr (HOIST) st4_stream(p, q);
+        else st4(p, q);
+    };
     for (int k = bk.z * kz; k < (bk.z + 1) * kz; k++) {
         const int64_t c = (int64_t)k * g.sz + (int64_t)j * g.sy + i0;
+        const bool ez = g.pz || k < g.nz - 1, hzm = g.pz || k > 0;
+        const int64_t ozp = g.pz ? (int64_t)(wrapi(k + 1, g.nz) - k) * g.sz : g.sz;
+        const int64_t ozm = g.pz ? (int64_t)(k - wrapi(k - 1, g.nz)) * g.sz : g.sz;
         const Q4<T> u = ld4(a.u + c), v = ld4(a.v + c), w = ld4(a.w + c);
-        if (what & 1) {
+        T v4 = T(0), w4 = T(0), um0 = T(0);
+        Q4<T> uyp{}, wyp{}, vzp{}, uzp{}, vm{}, wm{};
+        if (HOIST && (WHAT & 2)) {
+            const int64_t cx = c + 3 + (ex3 ? oxp3 : 0);
+            const int64_t cy = c + (ey ? oyp : 0), cz = c + (ez ? ozp : 0);
+            v4 = a.v[cx];
+            w4 = a.w[cx];
+            uyp = ld4(a.u + cy); wyp = ld4(a.w + cy);
+            vzp = ld4(a.v + cz); uzp = ld4(a.u + cz);
+        }
+        if (HOIST && (WHAT & 4)) {
+            um0 = a.u[c - (hxm ? oxm : 0)];
+            vm = ld4(a.v + c - (hym ? oym : 0));
+            wm = ld4(a.w + c - (hzm ? ozm : 0));
+        }
+        if (WHAT & 1) {
             Q4<T> X, Y, Z;
 #pragma unroll
             for (int m = 0; m < 4; m++) {
@@ -1056,17 +1108,15 @@ __global__ __launch_bounds__(256) void k_diag4(Geo g, DiagArgs<T> a, int what) {
                 Y.v[m] = v.v[m] * T(a.idy2);
                 Z.v[m] = w.v[m] * T(a.idz2);
             }
-            st4(a.Ux + c, X); st4(a.Uy + c, Y); st4(a.Uz + c, Z);
+            out4(a.Ux + c, X); out4(a.Uy + c, Y); out4(a.Uz + c, Z);
         }
-        const bool ez = g.pz || k < g.nz - 1, hzm = g.pz || k > 0;
-        const int64_t ozp = g.pz ? (int64_t)(wrapi(k + 1, g.nz) - k) * g.sz : g.sz;
-        const int64_t ozm = g.pz ? (int64_t)(k - wrapi(k - 1, g.nz)) * g.sz : g.sz;
-        if (what & 2) {
-            const T v4 = ex3 ? a.v[c + 3 + oxp3] : T(0);
-            const T w4 = ex3 ? a.w[c + 3 + oxp3] : T(0);
-            Q4<T> uyp{}, wyp{}, vzp{}, uzp{};
-            if (ey) { uyp = ld4(a.u + c + oyp); wyp = ld4(a.w + c + oyp); }
-            if (ez) { vzp = ld4(a.v + c + ozp); uzp = ld4(a.u + c + ozp); }
+        if (WHAT & 2) {
+            if (!HOIST) {
+                v4 = ex3 ? a.v[c + 3 + oxp3] : T(0);
+                w4 = ex3 ? a.w[c + 3 + oxp3] : T(0);
+                if (ey) { uyp = ld4(a.u + c + oyp); wyp = ld4(a.w + c + oyp); }
+                if (ez) { vzp = ld4(a.v + c + ozp); uzp = ld4(a.u + c + ozp); }
+            }
             Q4<T> OX, OY, OZ;
 #pragma unroll
             for (int m = 0; m < 4; m++) {
@@ -1084,13 +1134,14 @@ __global__ __launch_bounds__(256) void k_diag4(Geo g, DiagArgs<T> a, int what) {
                 if (ez && ex) oy = uzp.v[m] - uc - wxp + wc;
                 OX.v[m] = ox; OY.v[m] = oy; OZ.v[m] = oz;
             }
-            st4(a.ox + c, OX); st4(a.oy + c, OY); st4(a.oz + c, OZ);
+            out4(a.ox + c, OX); out4(a.oy + c, OY); out4(a.oz + c, OZ);
         }
-        if (what & 4) {
-            const T um0 = hxm ? a.u[c - oxm] : T(0);
-            Q4<T> vm{}, wm{};
-            if (hym) vm = ld4(a.v + c - oym);
-            if (hzm) wm = ld4(a.w + c - ozm);
+        if (WHAT & 4) {
+            if (!HOIST) {
+                um0 = hxm ? a.u[c - oxm] : T(0);
+                if (hym) vm = ld4(a.v + c - oym);
+                if (hzm) wm = ld4(a.w + c - ozm);
+            }
             Q4<T> KE;
 #pragma unroll
             for (int m = 0; m < 4; m++) {
@@ -1111,8 +1162,23 @@ __global__ __launch_bounds__(256) void k_diag4(Geo g, DiagArgs<T> a, int what) {
                 }
                 KE.v[m] = ke;
             }
-            st4(a.ke + c, KE);
+            out4(a.ke + c, KE);
         }
+    }
+}
+
+template <typename T>
+static void launch_diag4(const Geo &g, const DiagArgs<T> &a, int what, hipStream_t s) {
+    const dim3 grd = cell_grid4(g), blk(NYL_BX, NYL_BY, 1);
+    switch (what & 7) {
+    case 1: NYL_LAUNCH((k_diag4<T, 1>), grd, blk, 0, s, g, a); break;
+    case 2: NYL_LAUNCH((k_diag4<T, 2>), grd, blk, 0, s, g, a); break;
+    case 3: NYL_LAUNCH((k_diag4<T, 3>), grd, blk, 0, s, g, a); break;
+    case 4: NYL_LAUNCH((k_diag4<T, 4>), grd, blk, 0, s, g, a); break;
+    case 5: NYL_LAUNCH((k_diag4<T, 5>), grd, blk, 0, s, g, a); break;
+    case 6: NYL_LAUNCH((k_diag4<T, 6>), grd, blk, 0, s, g, a); break;
+    case 7: NYL_LAUNCH((k_diag4<T, 7>), grd, blk, 0, s, g, a); break;
+    default: break;
     }
 }
 
@@ -1132,11 +1198,11 @@ __global__ __launch_bounds__(256) void k_div4(Geo g, DivArgs<T, TO> a) {
         const int64_t c = (int64_t)k * g.sz + (int64_t)j * g.sy + i0;
         const bool hzm = g.pz || k > 0;
         const int64_t ozm = g.pz ? (int64_t)(k - wrapi(k - 1, g.nz)) * g.sz : g.sz;
+        // every load at once; a neighbour outside a wall axis is read from
+        // the own cell and never used
         const Q4<T> u = ld4(a.u + c), v = ld4(a.v + c), w = ld4(a.w + c);
-        const T um0 = hxm ? a.u[c - oxm] : T(0);
-        Q4<T> vm{}, wm{};
-        if (hym) vm = ld4(a.v + c - oym);
-        if (hzm) wm = ld4(a.w + c - ozm);
+        const T um0 = a.u[c - (hxm ? oxm : 0)];
+        const Q4<T> vm = ld4(a.v + c - (hym ? oym : 0)), wm = ld4(a.w + c - (hzm ? ozm : 0));
         Q4<TO> D;
 #pragma unroll
         for (int m = 0; m < 4; m++) {
@@ -1257,9 +1323,9 @@ __global__ __launch_bounds__(256) void k_diag(Geo g, DiagArgs<T> a, int what) {
 template <typename T>
 hipError_t launch_diag(const Geo &g, const DiagArgs<T> &a, int what, hipStream_t s) {
     if (quad_ok(g, {a.u, a.v, a.w, a.Ux, a.Uy, a.Uz, a.ox, a.oy, a.oz, a.ke}))
-        hipLaunchKernelGGL((k_diag4<T>), cell_grid4(g), dim3(NYL_BX, NYL_BY, 1), 0, s, g, a, what);
+        launch_diag4<T>(g, a, what, s);
     else
-        hipLaunchKernelGGL((k_diag<T>), cell_grid_kz(g), dim3(NYL_BX, NYL_BY, 1), 0, s, g, a, what);
+        NYL_LAUNCH((k_diag<T>), cell_grid_kz(g), dim3(NYL_BX, NYL_BY, 1), 0, s, g, a, what);
     return hipGetLastError();
 }
 
@@ -1300,9 +1366,9 @@ hipError_t launch_div(const Geo &g, const T *u, const T *v, const T *w, double i
     DivArgs<T, TO> a;
     a.u = u; a.v = v; a.w = w; a.idx2 = idx2; a.idy2 = idy2; a.idz2 = idz2; a.out = out;
     if (quad_ok(g, {u, v, w, out}))
-        hipLaunchKernelGGL((k_div4<T, TO>), cell_grid4(g), dim3(NYL_BX, NYL_BY, 1), 0, s, g, a);
+        NYL_LAUNCH((k_div4<T, TO>), cell_grid4(g), dim3(NYL_BX, NYL_BY, 1), 0, s, g, a);
     else
-        hipLaunchKernelGGL((k_div<T, TO>), cell_grid_kz(g), dim3(NYL_BX, NYL_BY, 1), 0, s, g, a);
+        NYL_LAUNCH((k_div<T, TO>), cell_grid_kz(g), dim3(NYL_BX, NYL_BY, 1), 0, s, g, a);
     return hipGetLastError();
 }
 
@@ -1344,9 +1410,9 @@ hipError_t launch_correct(const Geo &g, T *u, T *v, T *w, const TX *x, double dx
     CorrectArgs<T, TX> a;
     a.u = u; a.v = v; a.w = w; a.x = x; a.dx2 = dx2;
     if (quad_ok(g, {u, v, w, x}))
-        hipLaunchKernelGGL((k_correct4<T, TX>), cell_grid4(g), dim3(NYL_BX, NYL_BY, 1), 0, s, g, a);
+        NYL_LAUNCH((k_correct4<T, TX>), cell_grid4(g), dim3(NYL_BX, NYL_BY, 1), 0, s, g, a);
     else
-        hipLaunchKernelGGL((k_correct<T, TX>), cell_grid_kz(g), dim3(NYL_BX, NYL_BY, 1), 0, s, g, a);
+        NYL_LAUNCH((k_correct<T, TX>), cell_grid_kz(g), dim3(NYL_BX, NYL_BY, 1), 0, s, g, a);
     return hipGetLastError();
 }
 
@@ -1359,7 +1425,7 @@ __global__ void k_pressure(int64_t n, const TX *__restrict__ x, double dx2, T *_
 template <typename TX, typename T>
 hipError_t launch_pressure(const Geo &g, const TX *x, double dx2, T *p, hipStream_t s) {
     const int64_t n = (int64_t)g.nx * g.ny * g.nz;
-    hipLaunchKernelGGL((k_pressure<TX, T>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
+    NYL_LAUNCH((k_pressure<TX, T>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
                        n, x, dx2, p);
     return hipGetLastError();
 }
@@ -1369,25 +1435,70 @@ __global__ void k_fill(double *p, int64_t n, double v) {
     if (q < n) p[q] = v;
 }
 
+template <typename T>
+__global__ void k_zero(T *p, int64_t n) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) p[q] = T(0);
+}
+
+template <typename T>
+hipError_t launch_zero(T *p, int64_t n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    NYL_LAUNCH((k_zero<T>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n);
+    return hipGetLastError();
+}
+template hipError_t launch_zero<float>(float *, int64_t, hipStream_t);
+template hipError_t launch_zero<double>(double *, int64_t, hipStream_t);
+
 hipError_t launch_fill(double *p, int64_t n, double v, hipStream_t s) {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_fill, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n, v);
+    NYL_LAUNCH(k_fill, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n, v);
     return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------
 // Ensemble batches: one launch covers every member (BHead), each block a
-// tile of TPB consecutive cells of one member, running the single-member
-// cell functions.  The stage kernel is persistent: a fixed grid strides over
-// all tiles; its RK coefficient is the host's dt, dt/4., dt/12. of the
-// member's dt (same IEEE operations).
+// tile of TPB cells of one member (rows of consecutive cells, or a brick:
+// StageTile), running the single-member cell functions.  The stage kernel
+// is persistent: a fixed grid strides over all tiles; its RK coefficient is
+// the host's dt, dt/4., dt/12. of the member's dt (same IEEE operations).
+
+// the autotuner's candidates: rows, then bricks from flat to cubic
+const StageTile kStageTiles[NYL_N_TILES] = {
+    {128, -1, 0}, {128, 5, 2}, {128, 4, 2}, {128, 3, 2},
+    {256, -1, 0}, {256, 6, 2}, {256, 5, 2}, {256, 4, 2}, {256, 3, 3},
+    {512, -1, 0}, {512, 6, 3}, {512, 5, 3}, {512, 5, 2}, {512, 4, 3}, {512, 3, 3}};
+
+int64_t stage_tiles(const Geo &g, const StageTile &t) {
+    if (t.lx < 0) return ((int64_t)g.nx * g.ny * g.nz + t.tpb - 1) / t.tpb;
+    const int tz = t.tpb >> (t.lx + t.ly);
+    return (int64_t)((g.nx + (1 << t.lx) - 1) >> t.lx) * ((g.ny + (1 << t.ly) - 1) >> t.ly)
+         * ((g.nz + tz - 1) / tz);
+}
+
+// cell (i, j, k) of thread th in tile `tile` of a member; false for a thread
+// past the member's edge
+template <int TPB>
+__device__ __forceinline__ bool tile_cell(const Geo &g, int64_t tile, int th, int lx, int ly,
+                                          int &i, int &j, int &k) {
+    if (lx < 0) return cell_of(g, tile * TPB + th, i, j, k);
+    const uint32_t bx = (uint32_t)(g.nx + (1 << lx) - 1) >> lx;
+    const uint32_t by = (uint32_t)(g.ny + (1 << ly) - 1) >> ly;
+    const uint32_t t = (uint32_t)tile, r = t / bx, c = r / by;
+    i = (int)(((t - r * bx) << lx) + (th & ((1 << lx) - 1)));
+    j = (int)(((r - c * by) << ly) + ((th >> lx) & ((1 << ly) - 1)));
+    k = (int)(c * (uint32_t)(TPB >> (lx + ly)) + (th >> (lx + ly)));
+    return i < g.nx && j < g.ny && k < g.nz;
+}
+
 template <typename T, int MODE, int PX, int PY, int PZ, int TPB>
 __global__ __launch_bounds__(TPB) void k_b_stage(const StageItem<T> *__restrict__ items, int n,
-                                                 int64_t nblk, const double *__restrict__ dt) {
+                                                 int64_t nblk, const double *__restrict__ dt,
+                                                 int lx, int ly) {
     for (int64_t blk = blockIdx.x; blk < nblk; blk += gridDim.x) {
         const StageItem<T> &it = find_item(items, n, blk);
         int i, j, k;
-        if (!cell_of(it.h.g, (blk - it.h.start) * TPB + threadIdx.x, i, j, k)) continue;
+        if (!tile_cell<TPB>(it.h.g, blk - it.h.start, threadIdx.x, lx, ly, i, j, k)) continue;
         StageArgs<T> a = it.a;
         const double d = dt[it.h.member];
         a.c = MODE == 1 ? d : (MODE == 2 ? d / 4. : d / 12.);
@@ -1399,40 +1510,40 @@ int batch_blocks(int64_t ncell) { return (int)((ncell + NYL_BATCH_TPB - 1) / NYL
 
 template <typename T, int PX, int PY, int PZ, int TPB>
 static void b_stage_p(const StageItem<T> *items, int n, int64_t nblk, int mode, const double *dt,
-                      hipStream_t s) {
+                      int lx, int ly, hipStream_t s) {
     const unsigned grid = (unsigned)(nblk < 2048 ? nblk : 2048);
     switch (mode) {
-    case 1: hipLaunchKernelGGL((k_b_stage<T, 1, PX, PY, PZ, TPB>), grid, TPB, 0, s, items, n, nblk, dt); break;
-    case 2: hipLaunchKernelGGL((k_b_stage<T, 2, PX, PY, PZ, TPB>), grid, TPB, 0, s, items, n, nblk, dt); break;
-    default: hipLaunchKernelGGL((k_b_stage<T, 3, PX, PY, PZ, TPB>), grid, TPB, 0, s, items, n, nblk, dt); break;
+    case 1: NYL_LAUNCH((k_b_stage<T, 1, PX, PY, PZ, TPB>), grid, TPB, 0, s, items, n, nblk, dt, lx, ly); break;
+    case 2: NYL_LAUNCH((k_b_stage<T, 2, PX, PY, PZ, TPB>), grid, TPB, 0, s, items, n, nblk, dt, lx, ly); break;
+    default: NYL_LAUNCH((k_b_stage<T, 3, PX, PY, PZ, TPB>), grid, TPB, 0, s, items, n, nblk, dt, lx, ly); break;
     }
 }
 
 template <typename T, int TPB>
 static void b_stage_t(const StageItem<T> *items, int n, int64_t nblk, int mode, int perio,
-                      const double *dt, hipStream_t s) {
+                      const double *dt, int lx, int ly, hipStream_t s) {
     switch (perio) {
-    case 0: b_stage_p<T, 0, 0, 0, TPB>(items, n, nblk, mode, dt, s); break;
-    case 1: b_stage_p<T, 1, 0, 0, TPB>(items, n, nblk, mode, dt, s); break;
-    case 2: b_stage_p<T, 0, 1, 0, TPB>(items, n, nblk, mode, dt, s); break;
-    case 3: b_stage_p<T, 1, 1, 0, TPB>(items, n, nblk, mode, dt, s); break;
-    case 4: b_stage_p<T, 0, 0, 1, TPB>(items, n, nblk, mode, dt, s); break;
-    case 5: b_stage_p<T, 1, 0, 1, TPB>(items, n, nblk, mode, dt, s); break;
-    case 6: b_stage_p<T, 0, 1, 1, TPB>(items, n, nblk, mode, dt, s); break;
-    default: b_stage_p<T, 1, 1, 1, TPB>(items, n, nblk, mode, dt, s); break;
+    case 0: b_stage_p<T, 0, 0, 0, TPB>(items, n, nblk, mode, dt, lx, ly, s); break;
+    case 1: b_stage_p<T, 1, 0, 0, TPB>(items, n, nblk, mode, dt, lx, ly, s); break;
+    case 2: b_stage_p<T, 0, 1, 0, TPB>(items, n, nblk, mode, dt, lx, ly, s); break;
+    case 3: b_stage_p<T, 1, 1, 0, TPB>(items, n, nblk, mode, dt, lx, ly, s); break;
+    case 4: b_stage_p<T, 0, 0, 1, TPB>(items, n, nblk, mode, dt, lx, ly, s); break;
+    case 5: b_stage_p<T, 1, 0, 1, TPB>(items, n, nblk, mode, dt, lx, ly, s); break;
+    case 6: b_stage_p<T, 0, 1, 1, TPB>(items, n, nblk, mode, dt, lx, ly, s); break;
+    default: b_stage_p<T, 1, 1, 1, TPB>(items, n, nblk, mode, dt, lx, ly, s); break;
     }
 }
 
-// tpb: threads per tile (the autotuned shape: a tile is tpb consecutive cells
-// of a member, i.e. tpb/nx rows of its x lines)
 template <typename T>
 hipError_t launch_b_stage(const StageItem<T> *items, int n, int64_t nblk, int mode, int perio,
-                          int tpb, const double *dt, hipStream_t s) {
+                          const StageTile &tile, const double *dt, hipStream_t s) {
     if (mode < 1 || mode > 3 || nblk <= 0) return hipErrorInvalidValue;
-    switch (tpb) {
-    case 128: b_stage_t<T, 128>(items, n, nblk, mode, perio, dt, s); break;
-    case 256: b_stage_t<T, 256>(items, n, nblk, mode, perio, dt, s); break;
-    case 512: b_stage_t<T, 512>(items, n, nblk, mode, perio, dt, s); break;
+    if (tile.lx >= 0 && (tile.ly < 0 || tile.lx + tile.ly > 9 || (tile.tpb >> (tile.lx + tile.ly)) < 1))
+        return hipErrorInvalidValue;
+    switch (tile.tpb) {
+    case 128: b_stage_t<T, 128>(items, n, nblk, mode, perio, dt, tile.lx, tile.ly, s); break;
+    case 256: b_stage_t<T, 256>(items, n, nblk, mode, perio, dt, tile.lx, tile.ly, s); break;
+    case 512: b_stage_t<T, 512>(items, n, nblk, mode, perio, dt, tile.lx, tile.ly, s); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -1450,7 +1561,7 @@ __global__ __launch_bounds__(NYL_BATCH_TPB) void k_b_div(const DivItem<T> *__res
 
 template <typename T>
 hipError_t launch_b_div(const DivItem<T> *items, int n, int64_t nblk, hipStream_t s) {
-    hipLaunchKernelGGL((k_b_div<T>), dim3((unsigned)nblk), NYL_BATCH_TPB, 0, s, items, n);
+    NYL_LAUNCH((k_b_div<T>), dim3((unsigned)nblk), NYL_BATCH_TPB, 0, s, items, n);
     return hipGetLastError();
 }
 
@@ -1466,7 +1577,7 @@ __global__ __launch_bounds__(NYL_BATCH_TPB) void k_b_correct(const CorrItem<T> *
 
 template <typename T>
 hipError_t launch_b_correct(const CorrItem<T> *items, int n, int64_t nblk, hipStream_t s) {
-    hipLaunchKernelGGL((k_b_correct<T>), dim3((unsigned)nblk), NYL_BATCH_TPB, 0, s, items, n);
+    NYL_LAUNCH((k_b_correct<T>), dim3((unsigned)nblk), NYL_BATCH_TPB, 0, s, items, n);
     return hipGetLastError();
 }
 
@@ -1482,13 +1593,13 @@ __global__ __launch_bounds__(NYL_BATCH_TPB) void k_b_diag(const DiagItem<T> *__r
 
 template <typename T>
 hipError_t launch_b_diag(const DiagItem<T> *items, int n, int64_t nblk, int what, hipStream_t s) {
-    hipLaunchKernelGGL((k_b_diag<T>), dim3((unsigned)nblk), NYL_BATCH_TPB, 0, s, items, n, what);
+    NYL_LAUNCH((k_b_diag<T>), dim3((unsigned)nblk), NYL_BATCH_TPB, 0, s, items, n, what);
     return hipGetLastError();
 }
 
 #define NYL_B_STENCIL_INST(T)                                                                    \
-    template hipError_t launch_b_stage<T>(const StageItem<T> *, int, int64_t, int, int, int,    \
-                                          const double *, hipStream_t);                         \
+    template hipError_t launch_b_stage<T>(const StageItem<T> *, int, int64_t, int, int,         \
+                                          const StageTile &, const double *, hipStream_t);      \
     template hipError_t launch_b_div<T>(const DivItem<T> *, int, int64_t, hipStream_t);         \
     template hipError_t launch_b_correct<T>(const CorrItem<T> *, int, int64_t, hipStream_t);    \
     template hipError_t launch_b_diag<T>(const DiagItem<T> *, int, int64_t, int, hipStream_t);

@@ -30,7 +30,8 @@ class DeviceModel:
 
     def __init__(self, nx, ny, nz, periodic, dx, dy, dz, idx2=None, idy2=None, idz2=None,
                  dx2=None, fparameter=0.0, precision=L.NYL_PREC_FP64, mg_maxite=20,
-                 mg_tol=1e-6, mg_omega=0.9, device=None, viscosity=0.0, n_tracers=0):
+                 mg_tol=1e-6, mg_omega=0.9, device=None, viscosity=0.0, n_tracers=0,
+                 model=0):
         lib = L.load()
         if not torch.cuda.is_available():
             raise L.NylesHipError("no GPU visible: the Nyles time step runs only on the GPU")
@@ -51,7 +52,9 @@ class DeviceModel:
         d.mg_omega = mg_omega
         d.viscosity = viscosity
         d.n_tracers = n_tracers
+        d.model = model
         self.n_tracers = n_tracers
+        self.model = model
         self.desc = d
         ctx = ctypes.c_void_p()
         L.check(lib.nyl_ctx_create(ctypes.byref(d), ctypes.byref(ctx)))
@@ -202,6 +205,36 @@ class DeviceModel:
             pass
 
 
+def tile_candidates():
+    """the stage tiles the batch autotuner tries, as (cells, tx, ty)"""
+    lib = L.load()
+    out = []
+    for q in range(L.NYL_TILE_CANDIDATES):
+        t = (ctypes.c_int32 * 3)()
+        L.check(lib.nyl_batch_tile_candidate(q, t))
+        out.append((t[0], t[1], t[2]))
+    return out
+
+
+def tile_name(t):
+    """"256" for a row tile of 256 cells, "32x4x2" for a brick"""
+    cells, tx, ty = t
+    return str(cells) if tx == 0 else "%dx%dx%d" % (tx, ty, cells // (tx * ty))
+
+
+def parse_tile(t):
+    """(cells, tx, ty) from an int (row tile), a tile name or a triple"""
+    if isinstance(t, int):
+        return (t, 0, 0)
+    if isinstance(t, str):
+        parts = [int(v) for v in t.split("x")]
+        if len(parts) == 1:
+            return (parts[0], 0, 0)
+        tx, ty, tz = parts
+        return (tx * ty * tz, tx, ty)
+    return tuple(int(v) for v in t)
+
+
 class DeviceBatch:
     """An ensemble of DeviceModel members advanced together (nyl_batch_*).
 
@@ -235,7 +268,7 @@ class DeviceBatch:
         L.check(self.lib.nyl_batch_bind_workspace(b, ctypes.c_void_p(self.ws.data_ptr()),
                                                   nbytes.value))
         L.check(self.lib.nyl_batch_set_graphs(b, int(bool(graphs))))
-        self.tile = 256
+        self.tile = (256, 0, 0)
         self.tune_ms = None
         if tile is not None:
             self.set_tile(tile)
@@ -243,16 +276,19 @@ class DeviceBatch:
             self.autotune()
 
     def autotune(self):
-        tpb = ctypes.c_int32()
-        ms = (ctypes.c_double * 3)()
-        L.check(self.lib.nyl_batch_autotune(self.batch, ctypes.byref(tpb), ms))
-        self.tile = tpb.value
-        self.tune_ms = {128: ms[0], 256: ms[1], 512: ms[2]}
+        """time RK stage 1 for every candidate stage tile, keep the fastest"""
+        t = (ctypes.c_int32 * 3)()
+        ms = (ctypes.c_double * L.NYL_TILE_CANDIDATES)()
+        L.check(self.lib.nyl_batch_autotune(self.batch, t, ms))
+        self.tile = (t[0], t[1], t[2])
+        self.tune_ms = {tile_name(c): ms[q] for q, c in enumerate(tile_candidates())}
         return self.tile
 
-    def set_tile(self, tpb):
-        L.check(self.lib.nyl_batch_set_tile(self.batch, int(tpb)))
-        self.tile = int(tpb)
+    def set_tile(self, tile):
+        """tile: cells of a row tile (int), a name ("256", "32x4x2") or (cells, tx, ty)"""
+        t = parse_tile(tile)
+        L.check(self.lib.nyl_batch_set_tile(self.batch, *t))
+        self.tile = t
 
     def set_graphs(self, on):
         """replay the step plans from hipGraphs (captured on next use) or not"""

@@ -23,7 +23,7 @@ import torch
 
 from . import _lib as L
 from . import workloads
-from .device import DeviceBatch, DeviceModel
+from .device import DeviceBatch, DeviceModel, parse_tile, tile_name
 
 GEOMETRY_XYZ = {"closed": (0, 0, 0), "perio_x": (1, 0, 0), "perio_y": (0, 1, 0),
                 "perio_xy": (1, 1, 0), "perio_xyz": (1, 1, 1)}
@@ -312,7 +312,9 @@ class Server:
             "mean_s": float(lat.mean()),
             "Mcell_steps_per_s": cellsteps / elapsed / 1e6,
             "batched_steps": steps, "mean_slots": slot_steps / max(steps, 1),
-            "batch_rebuilds": rebuilds, "tile": tile, "peak_resident_bytes": peak,
+            "batch_rebuilds": rebuilds,
+            "tile": tile_name(parse_tile(tile)) if tile is not None else None,
+            "peak_resident_bytes": peak,
             "budget_bytes": self.budget,
         }
 

@@ -338,8 +338,15 @@ class OracleLES:
 
     def __init__(self, nx, ny, nz, Lx=1.0, Ly=1.0, Lz=1.0, geometry="closed", nh=3,
                  rotating=False, coriolis=1.0, mg_maxite=20, mg_tol=1e-6, mg_omega=0.9,
-                 viscosity=0.0, n_tracers=0):
+                 viscosity=0.0, n_tracers=0, model="LES"):
         self.nx, self.ny, self.nz, self.nh = nx, ny, nz, nh
+        # param.model["modelname"] (core/nyles.py:92-100): "LES", "linear"
+        # (LES(linear=True), core/model_les.py:33-34) or "Euler3d"
+        # (core/model_les_euler.py)
+        if model not in ("LES", "linear", "Euler3d"):
+            raise ValueError("model must be LES, linear or Euler3d")
+        self.model = model
+        self.nonlinear = model != "linear"
         # core/model_les.py:36-43: passive tracers t0, t1, ... after b
         self.traclist = ["b"] + ["t%d" % q for q in range(n_tracers)]
         self.PROGNOSTIC = tuple(self.traclist) + ("u_i", "u_j", "u_k")
@@ -399,8 +406,9 @@ class OracleLES:
         for d in "ijk":
             ds["u"][d][...] = 0.0
         # Tracer_numerics.rhstrac, core/tracer.py:59-72 (no diffusion: LES calls
-        # it without `last`, core/model_les.py:133)
-        for name in self.traclist:
+        # it without `last`, core/model_les.py:133); Euler3d advects nothing
+        # (the call is commented out, core/model_les_euler.py:131)
+        for name in (self.traclist if self.model != "Euler3d" else ()):
             for d in "ijk":
                 velocity = view(self.U[d], d)
                 field = view(self.get(name), d)
@@ -408,8 +416,9 @@ class OracleLES:
                 if d == "i":
                     dfield[...] = 0.0
                 upwind(field, velocity, dfield)
-        # vortex_force, core/vortex_force.py:69-81
-        for k, j, i in ["ikj", "jik", "kji"]:
+        # vortex_force, core/vortex_force.py:69-81 (not in the linear model,
+        # core/model_les.py:135-136)
+        for k, j, i in (["ikj", "jik", "kji"] if self.nonlinear else []):
             u_i = flipview(ds["u"][i], j)
             u_k = flipview(ds["u"][k], j)
             U_i = flipview(self.U[i], j)
@@ -417,11 +426,12 @@ class OracleLES:
             w_j = flipview(self.vor[j], j)
             vortex_force_direc(U_k, w_j, u_i)
             vortex_force_flip(U_i, w_j, u_k)
-        # bernoulli, core/bernoulli.py:14-23
+        # bernoulli, core/bernoulli.py:14-23; Euler3d: bernoulli_euler, the
+        # gradient of ke only (core/bernoulli.py:26-32)
         for d in "ijk":
             du = view(ds["u"][d], d)
             ke = view(self.ke, d)
-            if d in "ij":
+            if d in "ij" or self.model == "Euler3d":
                 gradke(ke, du)
             else:
                 gradkeandb(ke, view(self.b, d), du, self.dz)
@@ -490,11 +500,12 @@ class OracleLES:
         for d in "ijk":
             self.fill(self.u[d])
         self.U_from_u()
-        self.vorticity()
-        self.kinenergy()
-        for d in "ijk":
-            self.fill(self.vor[d])
-        self.fill(self.ke)
+        if self.nonlinear:                       # core/model_les.py:122-127
+            self.vorticity()
+            self.kinenergy()
+            for d in "ijk":
+                self.fill(self.vor[d])
+            self.fill(self.ke)
 
     # -- core/timescheme.py:180-221 ----------------------------------------
     def _pairs(self, *ds):

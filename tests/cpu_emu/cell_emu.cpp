@@ -47,7 +47,14 @@ int check(int nx, int ny, int nz, int px, int py, int pz) {
             set(o1);
             run_blocks(cell_grid_kz(g), dim3(NYL_BX, NYL_BY, 1), [&]() { k_diag<T>(g, a, what); }, false);
             set(o2);
-            run_blocks(cell_grid4(g), dim3(NYL_BX, NYL_BY, 1), [&]() { k_diag4<T>(g, a, what); }, false);
+            run_blocks(cell_grid4(g), dim3(NYL_BX, NYL_BY, 1), [&]() {
+                switch (what) {
+                case 7: k_diag4<T, 7>(g, a); break;
+                case 6: k_diag4<T, 6>(g, a); break;
+                case 4: k_diag4<T, 4>(g, a); break;
+                case 2: k_diag4<T, 2>(g, a); break;
+                default: k_diag4<T, 1>(g, a); break;
+                } }, false);
             for (int f = 0; f < 7; f++) bad += ndiff(o1[f], o2[f]);
         }
     printf("cell4<%s,%s> %dx%dx%d p=%d%d%d: %d mismatches\n", sizeof(T) == 8 ? "f64" : "f32",

@@ -39,11 +39,13 @@ inline T __shfl_down(T v, unsigned d, int width = 64) {
 }
 typedef int hipError_t;
 typedef void *hipStream_t;
+typedef void *hipEvent_t;
 enum { hipSuccess = 0, hipErrorInvalidValue = 1 };
 enum { hipFuncAttributeMaxDynamicSharedMemorySize = 8 };
 inline hipError_t hipGetLastError() { return 0; }
 inline hipError_t hipFuncSetAttribute(const void *, int, int) { return 0; }
 #define hipLaunchKernelGGL(...) ((void)0)
+#define hipExtLaunchKernelGGL(...) ((void)0)
 struct float4 { float x, y, z, w; };
 inline float4 make_float4(float a, float b, float c, float d) { return {a, b, c, d}; }
 struct double2 { double x, y; };
@@ -53,3 +55,5 @@ inline void __threadfence_system() {}
 template <typename T, typename U>
 inline void __hip_atomic_store(T *p, U v, int, int) { *p = v; }
 inline float __builtin_amdgcn_rcpf(float x) { return 1.0f / x; }
+// a cache hint only: a plain store on the CPU
+#define __builtin_nontemporal_store(v, p) (*(p) = (v))

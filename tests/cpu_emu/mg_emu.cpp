@@ -25,6 +25,38 @@ int check_tiled(int nx, int ny, int nz, int px, int py, int pz) {
         k_mg_sweep25<T, false, false, false>(gf, x.data(), y2.data(), b.data(), om, tab, kzs,
                                              nullptr, gf, nullptr, nullptr); }, true);
     bad += ndiff(y1, y2);
+    // the sweep variants: residual norm of the input (same output), the
+    // prolongation of xc folded into the input, the result added to an FP64 x
+    {
+        std::vector<T> y3(nf);
+        std::vector<double> part(nf, 0.0), rn(NYL_RED_BLOCKS, 0.0);
+        run_blocks(gs, dim3(SW_TX, 4, 1), [&]() {
+            k_mg_sweep25<T, true, false, false>(gf, x.data(), y3.data(), b.data(), om, tab, kzs,
+                                                part.data(), gf, nullptr, nullptr); }, true);
+        const int nbr = red_blocks((int64_t)nf);
+        run_blocks(dim3(nbr), dim3(RED_T), [&]() {
+            k_resnorm2<T>(gf, x.data(), b.data(), tab, rn.data()); }, true);
+        double s1 = 0, s2 = 0;
+        for (double v : part) s1 += v;
+        for (double v : rn) s2 += v;
+        bad += ndiff(y1, y3) + (std::fabs(s1 - s2) > 1e-9 * s2);
+        std::vector<T> xp = x, yp1(nf), yp2(nf);
+        run_blocks(mg_grid(gf), dim3(MG_BX, MG_BY, 1), [&]() {
+            k_mg_prolong<T, T>(gf, gc, xp.data(), xc.data(), tab); }, false);
+        run_blocks(mg_grid(gf), dim3(MG_BX, MG_BY, 1), [&]() {
+            k_mg_sweep<T>(gf, xp.data(), yp1.data(), b.data(), om, tab); }, false);
+        run_blocks(gs, dim3(SW_TX, 4, 1), [&]() {
+            k_mg_sweep25<T, false, true, false>(gf, x.data(), yp2.data(), b.data(), om, tab, kzs,
+                                                nullptr, gc, xc.data(), nullptr); }, true);
+        bad += ndiff(yp1, yp2);
+        std::vector<double> xo1(nf), xo2(nf);
+        for (size_t q = 0; q < nf; q++) xo1[q] = xo2[q] = 0.25 * (double)q;
+        for (size_t q = 0; q < nf; q++) xo1[q] = xo1[q] + (double)y1[q];
+        run_blocks(gs, dim3(SW_TX, 4, 1), [&]() {
+            k_mg_sweep25<T, false, false, true>(gf, x.data(), nullptr, b.data(), om, tab, kzs,
+                                                nullptr, gf, nullptr, xo2.data()); }, true);
+        bad += ndiff(xo1, xo2);
+    }
     // restriction with the coarse level's first sweep
     std::vector<T> bc1(nc), bc2(nc), yc1(nc), yc2(nc), xd(nc);
     run_blocks(mg_grid(gc), dim3(MG_BX, MG_BY, 1), [&]() {

@@ -157,3 +157,36 @@ def test_member_used_alone_between_batch_steps(cmp):
     for q in range(2):
         _same(cmp, _state(mem[q]), _state(ref[q]), "member %d" % q)
     batch.close()
+
+
+def test_every_stage_tile_gives_the_same_bits(cmp):
+    """Row tiles and bricks of every candidate shape (members of 16 to 64
+    cells per axis, so some bricks overhang a member's edge) step the batch
+    to the same bits as the single-member path; the V-cycle cap learnt from
+    the previous step (V-cycles launched without a host read) changes
+    nothing either."""
+    from nyles_amd.device import DeviceBatch, tile_candidates, tile_name
+    print("case batch tile shapes", flush=True)
+    cands = tile_candidates()
+    assert len(cands) == 15 and len(set(cands)) == 15
+    for cells, tx, ty in cands:
+        assert cells in (128, 256, 512)
+        if tx:
+            assert cells % (tx * ty) == 0
+    singles = [_member(cmp, s, 60 + q)[1] for q, s in enumerate(MEMBERS)]
+    batched = [_member(cmp, s, 60 + q)[1] for q, s in enumerate(MEMBERS)]
+    dts = [_dt(d) for d in singles]
+    batch = DeviceBatch(batched, autotune=False)
+    for n, t in enumerate(cands):
+        print("  tile", tile_name(t), flush=True)
+        batch.set_tile(t)
+        assert batch.tile == t
+        nites = [d.step_rk3(dt)["nite"] for d, dt in zip(singles, dts)]
+        st = batch.step_rk3(dts)
+        assert [s["nite"] for s in st] == nites, (tile_name(t), st, nites)
+        for q in range(len(MEMBERS)):
+            _same(cmp, _state(batched[q]), _state(singles[q]),
+                  "tile %s member %d" % (tile_name(t), q))
+    batch.close()
+    for d in singles + batched:
+        d.close()

@@ -67,8 +67,17 @@ def test_lock_exchange_script_matches_oracle(tmp_path, scheme):
         got = mine.view("i")
         np.testing.assert_array_equal(got, ref, err_msg=name)
     files = sorted(os.listdir(nyl.IO.output_directory))
-    assert any(f.endswith(".npz") for f in files)
     assert "param.pkl" in files
+    frames = [f for f in files if f.endswith(".npz")]
+    # history every 0.1 with dt 0.05: the initial state, then after steps 2 and 4
+    assert len(frames) == 3
+    with np.load(os.path.join(nyl.IO.output_directory, frames[0])) as z:
+        assert int(z["n"]) == 0
+        np.testing.assert_array_equal(z["b"], st.b.interior(b0))
+    with np.load(os.path.join(nyl.IO.output_directory, frames[-1])) as z:
+        assert int(z["n"]) == nyl.n == 4
+        for key, mine in (("b", st.b), ("u_i", st.u["i"]), ("vor_j", st.vor["j"])):
+            np.testing.assert_array_equal(z[key], mine.interior(mine.view("i")), err_msg=key)
 
 
 def test_auto_dt_closed_forms(tmp_path):

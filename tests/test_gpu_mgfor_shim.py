@@ -26,8 +26,10 @@ def _ptr(thing):
     if isinstance(thing, np.ndarray):
         return thing.ctypes.data_as(ctypes.POINTER(ctypes.c_int32 if thing.dtype == np.int32
                                                    else ctypes.c_double))
-    if isinstance(thing, int):        # bool too: isinstance(True, int)
+    if isinstance(thing, (int, np.int32)):   # bool too: isinstance(True, int)
         return ctypes.byref(ctypes.c_int32(thing))
+    if isinstance(thing, np.int64):
+        return ctypes.byref(ctypes.c_int64(thing))
     if isinstance(thing, float):
         return ctypes.byref(ctypes.c_double(thing))
     raise ValueError(thing)
@@ -64,9 +66,9 @@ def test_mgfor_driver_sequence_matches_device_solve(geometry):
     shape = np.zeros((3,), dtype="i")
     get_pyshape(mg, _ptr(1), _ptr(shape))
     assert tuple(shape) == (nz + 2 * NH, ny + 2 * NH, nx + 2 * NH)
-    n1, n2, n3 = shape
     x = np.zeros(tuple(shape))
     b = np.zeros(tuple(shape))
+    n1, n2, n3 = x.shape       # core/mgfordriver.py:87-90
     rng = np.random.default_rng(TOPOLOGY[geometry])
     I = (slice(NH, -NH),) * 3
     dev = DeviceModel(nx, ny, nz, PERIODIC[geometry], 1.0, 1.0, 1.0)

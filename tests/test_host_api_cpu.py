@@ -147,3 +147,56 @@ def test_grid_coordinates_and_metric():
     np.testing.assert_allclose(xu - x, gr.dx / 2)
     zw = gr.z_vel["k"].view("i")
     np.testing.assert_allclose(zw - gr.z_b.view("i"), gr.dz / 2)
+
+
+def test_history_frames_hold_the_state_on_schedule(tmp_path):
+    """nylesIO: a frame at init, then one each time t reaches the next
+    multiple of timestep_history (core/nylesIO.py's schedule), and one at
+    finalize unless the last step was just written; every frame holds t, n
+    and the chosen variables (vectors as name_i/_j/_k) equal to the state
+    when it was written, interior cells or full arrays with include_halo."""
+    import nylesIO
+    import variables as var
+    for halo in (False, True):
+        param = _param("perio_x", n=(6, 4, 8))
+        s = var.get_state(param)
+        io_param = {"disk_space_warning": 0, "simplified_grid": True,
+                    "variables_in_history": ["b", "u"], "timestep_history": 0.125,
+                    "include_halo": halo, "myrank": 0, "n_tracers": 0,
+                    "datadir": str(tmp_path), "expname": "h%d" % halo, "mode": "overwrite"}
+        io = nylesIO.NylesIO(io_param)
+        rng = np.random.default_rng(3)
+
+        def perturb():
+            for name in ("b", "u_i", "u_j", "u_k"):
+                obj = s.b if name == "b" else s.u[name[-1]]
+                obj.view("i")[...] = rng.standard_normal(obj.view("i").shape)
+
+        def snap():
+            out = {"b": s.b.view("i").copy()}
+            for d in "ijk":
+                out["u_" + d] = s.u[d].view("i").copy()
+            return out
+
+        perturb()
+        want = [(0.0, 0, snap())]
+        io.init(s, None, t=0.0, n=0)
+        # dt = 1/16, history every 1/8 (exact binary): frames after steps 2, 4, 6
+        for n in range(1, 8):
+            t = n / 16
+            perturb()
+            io.write(s, t, n)
+            if n in (2, 4, 6):
+                want.append((t, n, snap()))
+        perturb()
+        io.finalize(s, 0.5, 8)
+        want.append((0.5, 8, snap()))
+        files = sorted(f for f in os.listdir(io.output_directory) if f.endswith(".npz"))
+        assert len(files) == len(want) == 5
+        for f, (tw, nw, st) in zip(files, want):
+            with np.load(os.path.join(io.output_directory, f)) as z:
+                assert float(z["t"]) == tw and int(z["n"]) == nw
+                assert sorted(z.files) == ["b", "n", "t", "u_i", "u_j", "u_k"]
+                for key, a in st.items():
+                    obj = s.b if key == "b" else s.u[key[-1]]
+                    np.testing.assert_array_equal(z[key], a if halo else obj.interior(a))

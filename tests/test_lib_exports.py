@@ -176,3 +176,23 @@ def test_mgfor_shim_symbols_are_found_as_the_reference_finds_them():
         hits = [n for n in names if "MOD_" + f in n]
         assert len(hits) == 1, (f, hits)
 
+
+
+def test_stage_tile_candidates_and_names():
+    """The batch autotuner's tile list (a host-side table): 15 distinct
+    shapes of 128, 256 and 512 cells, rows or bricks that multiply out;
+    names round-trip; an index past the list is refused."""
+    from nyles_amd import _lib
+    from nyles_amd.device import parse_tile, tile_candidates, tile_name
+    c = tile_candidates()
+    assert len(c) == _lib.NYL_TILE_CANDIDATES == 15 and len(set(c)) == 15
+    for cells, tx, ty in c:
+        assert cells in (128, 256, 512)
+        assert (tx, ty) == (0, 0) or (cells % (tx * ty) == 0 and tx * ty <= cells)
+        assert parse_tile(tile_name((cells, tx, ty))) == (cells, tx, ty)
+    assert sum(1 for t in c if t[1] == 0) == 3
+    assert parse_tile(256) == (256, 0, 0) and tile_name((512, 32, 4)) == "32x4x4"
+    L = _lib.load()
+    t = (ctypes.c_int32 * 3)()
+    assert L.nyl_batch_tile_candidate(15, t) == _lib.NYL_E_ARG
+    assert L.nyl_batch_tile_candidate(-1, t) == _lib.NYL_E_ARG

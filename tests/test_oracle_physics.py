@@ -170,3 +170,83 @@ def test_forcing_hook_sees_each_stage_time(oracle_mod):
     m.lfam3(dt, t=3.0)
     m.lfam3(dt, t=3.1)
     assert seen == [2.0, 3.0, 3.1, 3.1 + dt * .5]
+
+
+def _random_model(O, model, geometry, seed):
+    m = O.OracleLES(16, 12, 10, 2.0, 1.5, 1.0, geometry, model=model, n_tracers=1)
+    _shear_state(m, amp=1e-2, seed=seed)
+    rng = np.random.default_rng(seed + 1)
+    m.interior(m.b)[...] = rng.standard_normal(m.interior(m.b).shape)
+    m.interior(m.tracers["t0"])[...] = rng.standard_normal(m.interior(m.b).shape)
+    return m
+
+
+@pytest.mark.parametrize("geometry", ["closed", "perio_xy", "perio_xyz"])
+def test_linear_model_has_no_vortex_force(oracle_mod, geometry):
+    """LES(linear=True) (core/model_les.py:33-34,122-127,135-136): the
+    diagnosis leaves vorticity and ke as they were (zero here), so the
+    momentum tendency is the buoyancy term alone: du_x = du_y = 0 and
+    du_z = 0.5*dz*(b(k) + b(k+1)) on the faces inside the domain; b and the
+    tracer get the LES advective tendency.  The vorticity never reaches the
+    tendencies: planting one changes nothing."""
+    O = oracle_mod
+    lin = _random_model(O, "linear", geometry, 11)
+    les = _random_model(O, "LES", geometry, 11)
+    lin.diagnose_var()
+    les.diagnose_var()
+    for d in "ijk":
+        assert np.all(lin.vor[d] == 0.0)
+        assert np.array_equal(lin.u[d], les.u[d])
+    assert np.all(lin.ke == 0.0) and np.any(les.ke != 0.0)
+    ds, dl = lin._new_tendency(), les._new_tendency()
+    lin.rhs(ds)
+    les.rhs(dl)
+    assert np.array_equal(ds["b"], dl["b"]) and np.array_equal(ds["t0"], dl["t0"])
+    I = lin.interior
+    assert np.all(I(ds["u"]["i"]) == 0.0) and np.all(I(ds["u"]["j"]) == 0.0)
+    b = I(lin.b)
+    want = np.zeros_like(b)
+    if lin.perio[0]:
+        want[...] = 0.5 * lin.dz * (b + np.roll(b, -1, 0))
+    else:
+        want[:-1] = 0.5 * lin.dz * (b[:-1] + b[1:])
+    np.testing.assert_allclose(I(ds["u"]["k"]), want, rtol=1e-15, atol=0)
+    # a planted vorticity does not change the linear tendencies
+    for d in "ijk":
+        lin.vor[d][...] = np.random.default_rng(5).standard_normal(lin.shape)
+    ds2 = lin._new_tendency()
+    lin.rhs(ds2)
+    for c in "ijk":
+        assert np.array_equal(ds2["u"][c], ds["u"][c])
+
+
+@pytest.mark.parametrize("geometry", ["closed", "perio_x", "perio_xyz"])
+def test_euler3d_advects_nothing_and_has_no_buoyancy(oracle_mod, geometry):
+    """core/model_les_euler.py: b and the tracers get zero tendencies (the
+    rhstrac call is commented out, :131) and the momentum tendency does not
+    depend on b (bernoulli_euler, core/bernoulli.py:26-32); with b = 0 it
+    is the LES tendency bit for bit.  Over an RK3 step b and the tracer
+    keep their values."""
+    O = oracle_mod
+    eu = _random_model(O, "Euler3d", geometry, 21)
+    eu.diagnose_var()
+    ds = eu._new_tendency()
+    eu.rhs(ds)
+    assert np.all(ds["b"] == 0.0) and np.all(ds["t0"] == 0.0)
+    # b does not reach the momentum
+    eu2 = _random_model(O, "Euler3d", geometry, 21)
+    eu2.interior(eu2.b)[...] = 0.0
+    eu2.diagnose_var()
+    ds2 = eu2._new_tendency()
+    eu2.rhs(ds2)
+    les = _random_model(O, "LES", geometry, 21)
+    les.interior(les.b)[...] = 0.0
+    les.diagnose_var()
+    dl = les._new_tendency()
+    les.rhs(dl)
+    for c in "ijk":
+        assert np.array_equal(ds2["u"][c], ds["u"][c])
+        assert np.array_equal(dl["u"][c], ds["u"][c])
+    b0, t0 = eu.b.copy(), eu.tracers["t0"].copy()
+    eu.rk3_ssp(0.01)
+    assert np.array_equal(eu.b, b0) and np.array_equal(eu.tracers["t0"], t0)
