@@ -39,7 +39,7 @@
 extern "C" {
 #endif
 
-#define NYL_ABI_VERSION 2
+#define NYL_ABI_VERSION 3
 
 /* status codes */
 #define NYL_OK 0
@@ -82,6 +82,11 @@ typedef struct nyl_desc {
     int32_t n_tracers;           /* passive tracers t0.. advected like b
                                     (core/model_les.py:36-43), 0..NYL_MAX_TRACERS */
     int32_t model;               /* NYL_MODEL_*: param.model["modelname"] (core/nyles.py:92-100) */
+    int32_t upwind_a;            /* 0: WENO5 advection of b and the tracers (the reference's
+                                    build); 1, 3, 5: the linear upwind branch of that order
+                                    (core/fortran_upwind.f90:34-67, linear=.false. there) */
+    int32_t upwind_vf;           /* the same for the vortex force
+                                    (core/fortran_vortex_force.f90:43-65,121-144) */
 } nyl_desc;
 
 #define NYL_MAX_TRACERS 8

@@ -11,6 +11,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libnyles_hip.so")
 
 # status codes, precision modes, field ids, reductions and ops (mirror the header)
+ABI_VERSION = 3        # NYL_ABI_VERSION of include/nyles_hip.h
 NYL_OK = 0
 NYL_E_ARG, NYL_E_STATE, NYL_E_HIP, NYL_E_SIZE = -1, -2, -3, -4
 NYL_PREC_FP64 = 0
@@ -87,7 +88,7 @@ class NylDesc(ctypes.Structure):
         ("precision", ctypes.c_int32), ("mg_maxite", ctypes.c_int32),
         ("mg_tol", ctypes.c_double), ("mg_omega", ctypes.c_double),
         ("viscosity", ctypes.c_double), ("n_tracers", ctypes.c_int32),
-        ("model", ctypes.c_int32),
+        ("model", ctypes.c_int32), ("upwind_a", ctypes.c_int32), ("upwind_vf", ctypes.c_int32),
     ]
 
 
@@ -162,6 +163,9 @@ def load():
         f = getattr(L, name)
         f.restype = res
         f.argtypes = args
+    if L.nyl_abi_version() != ABI_VERSION:
+        raise NylesHipError("libnyles_hip.so has ABI %d, this binding expects %d: rebuild it"
+                            % (L.nyl_abi_version(), ABI_VERSION))
     _lib = L
     return L
 

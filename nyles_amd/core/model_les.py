@@ -105,6 +105,14 @@ class LES(object):
         else:
             mg_tol, mg_maxite, mg_omega = param["mg_tol"], param["mg_maxite"], param["mg_omega"]
         self.blowup_check = param["blowup_check"]
+        # device.upwind = "linear": the reference's linear upwind branches
+        # (core/fortran_upwind.f90:34-67, core/fortran_vortex_force.f90:43-65,
+        # 121-144; compiled out there) of orders orderA and orderVF
+        upwind_a = upwind_vf = 0
+        if param["upwind"] == "linear":
+            upwind_a, upwind_vf = self.orderA, self.orderVF
+            if upwind_a not in (1, 3, 5) or upwind_vf not in (1, 3, 5):
+                raise ValueError("the linear upwind branches run odd orders 1, 3 or 5")
         self.device = DeviceModel(
             param["nx"], param["ny"], param["nz"], GEOMETRY_PERIODIC_XYZ[param["geometry"]],
             grid.dx, grid.dy, grid.dz, idx2=grid.idx2, idy2=grid.idy2, idz2=grid.idz2,
@@ -112,7 +120,8 @@ class LES(object):
             mg_maxite=mg_maxite, mg_tol=mg_tol, mg_omega=mg_omega,
             viscosity=float(self.viscosity), n_tracers=n_tracers,
             model=L.NYL_MODEL_EULER if euler else (L.NYL_MODEL_LINEAR if linear
-                                                   else L.NYL_MODEL_LES))
+                                                   else L.NYL_MODEL_LES),
+            upwind_a=upwind_a, upwind_vf=upwind_vf)
         self.timescheme.bind(self.device, self)
         self.mg = MGInfo()
         st = self.state

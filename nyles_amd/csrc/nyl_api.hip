@@ -273,6 +273,8 @@ StageArgs<T> stage_args(nyl_ctx *c) {
     a.vcx = a.vcy = a.vcz = 0.0;
     a.visc = 0;
     a.flags = 0;
+    a.upw_a = c->d.upwind_a;
+    a.upw_vf = c->d.upwind_vf;
     return a;
 }
 
@@ -868,6 +870,7 @@ int stage_t(nyl_ctx *c, int st, int slot, double coef, double coef2, bool last) 
         ta.idx2 = c->d.idx2; ta.idy2 = c->d.idy2; ta.idz2 = c->d.idz2;
         ta.c = coef;
         ta.c2 = coef2;
+        ta.upw = c->d.upwind_a;
         LAUNCH(NYL_K_STAGE, (words / 4.0) * cells(c->g) * sizeof(T),
                launch_tracer_stage<T>(c->g, ta, st, c->stream));
     }
@@ -910,8 +913,9 @@ int update_t(nyl_ctx *c, int st, double coef, double coef2) {
 // forward, the LFAM3 corrector), which adds the viscosity
 int stage(nyl_ctx *c, int st, double coef, double coef2, bool last) {
     int rc;
-    if (c->d.model != NYL_MODEL_LES) {
-        // model variants: the tendencies into the slot the stage combination
+    if (c->d.model != NYL_MODEL_LES || c->d.upwind_a || c->d.upwind_vf) {
+        // model variants and the linear upwind branches: the tendencies
+        // into the slot the stage combination
         // reads last (0 for modes 1 and 4..6, 1 for mode 2, 2 for mode 3),
         // then the combination
         const int slot = st == 2 ? 1 : (st == 3 ? 2 : 0);
@@ -994,6 +998,17 @@ int nyl_ctx_create(const nyl_desc *desc, nyl_ctx **out) {
     if (!(d.viscosity >= 0.0)) return fail(NYL_E_ARG, "viscosity must be >= 0");
     if (d.model < NYL_MODEL_LES || d.model > NYL_MODEL_EULER)
         return fail(NYL_E_ARG, "model must be NYL_MODEL_LES, _LINEAR or _EULER");
+    for (int o : {d.upwind_a, d.upwind_vf})
+        if (o != 0 && o != 1 && o != 3 && o != 5)
+            return fail(NYL_E_ARG, "linear upwind orders are 1, 3 or 5 (0: WENO)");
+    if (d.upwind_a || d.upwind_vf) {
+        // the reference's closures overlap (and read outside the line) on
+        // wall lines shorter than 6 cells
+        const int n[3] = {d.nx, d.ny, d.nz}, p[3] = {d.periodic_x, d.periodic_y, d.periodic_z};
+        for (int q = 0; q < 3; q++)
+            if (n[q] < 6 && !p[q])
+                return fail(NYL_E_SIZE, "linear upwind needs 6 or more cells along a wall axis");
+    }
     nyl_ctx *c = new nyl_ctx();
     memset(c, 0, sizeof(*c));
     c->d = d;

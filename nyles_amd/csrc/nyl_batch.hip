@@ -673,8 +673,10 @@ int nyl_batch_create(nyl_ctx *const *members, int32_t n, nyl_batch **out) {
             return set_error(NYL_E_ARG, "members must share the precision mode");
         if (members[q]->ntr > 0)
             return set_error(NYL_E_ARG, "member %d has passive tracers: step it on its own", q);
-        if (members[q]->d.model != NYL_MODEL_LES)
-            return set_error(NYL_E_ARG, "member %d is not an LES model: step it on its own", q);
+        if (members[q]->d.model != NYL_MODEL_LES || members[q]->d.upwind_a ||
+            members[q]->d.upwind_vf)
+            return set_error(NYL_E_ARG, "member %d is not a WENO LES model: step it on its own",
+                             q);
         for (int r = 0; r < q; r++)
             if (members[r] == members[q]) return set_error(NYL_E_ARG, "member %d appears twice", q);
     }

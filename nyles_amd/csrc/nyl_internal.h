@@ -54,6 +54,9 @@ struct StageArgs {
     // bit 0 no vortex force (linear model), bit 1 no advection of b and no
     // buoyancy (Euler3d)
     int flags;
+    // linear upwind orders (1, 3, 5) of the tracer advection and of the
+    // vortex force, mode 0 only; 0: WENO (the branch the reference runs)
+    int upw_a, upw_vf;
 };
 #define NYL_SF_NO_VF 1
 #define NYL_SF_NO_B 2
@@ -77,6 +80,7 @@ struct TracerArgs {
     T *sb;                           // LFAM3 'before' state (modes 4, 6)
     double idx2, idy2, idz2;
     double c, c2;
+    int upw;                         // linear upwind order (mode 0 only), 0: WENO
 };
 template <typename T>
 hipError_t launch_tracer_stage(const Geo &g, const TracerArgs<T> &a, int mode, hipStream_t s);

@@ -162,6 +162,81 @@ __device__ __forceinline__ T tracer_axis(T d, const T *__restrict__ q_arr,
     return (d + fl) - fr;
 }
 
+// ---------------------------------------------------------------------------
+// The reference's linear upwind branches (linear=.false. in its build:
+// core/fortran_upwind.f90:30, core/fortran_vortex_force.f90:28,104), odd
+// orders 1, 3, 5.  REAL(4) literals widened; sums left to right as written.
+
+// Reconstruction at line position j of the values q(.) (a functor over
+// positions, wrapped on a periodic axis): RIGHT = false the left-biased qp,
+// true the right-biased qm.  On a wall line the order drops near the ends:
+// tracer flavour (core/interpolate_tracer.f90:45-80) first order at 0 and
+// n-1, third at 1 and n-2; vortex-force flavour (core/interpolate.f90:35-79)
+// qp(-1) = 0, first order at 0 and n-2 (and qm at n-1), third at 1 and n-3.
+template <typename T, bool VF, bool RIGHT, typename QF>
+__device__ __forceinline__ T lin_rec(const QF &q, int j, int n, int order, bool per) {
+    const T c1 = T(-(1.0f / 6.0f)), c2 = T(5.0f / 6.0f), c3 = T(2.0f / 6.0f);
+    const T b1 = T(2.0f / 60.0f), b2 = T(-(13.0f / 60.0f)), b3 = T(47.0f / 60.0f);
+    const T b4 = T(27.0f / 60.0f), b5 = T(-(3.0f / 60.0f));
+    int ord = order;
+    if (!per) {
+        if (VF && !RIGHT && j < 0) return T(0);
+        const int last1 = VF ? n - 2 : n - 1;          // first order at 0 and here
+        if (j <= 0 || j >= last1) ord = 1;
+        else if (order == 5 && (j == 1 || j == last1 - 1)) ord = 3;
+    }
+    if (ord == 1) return q(j);
+    if (ord == 3) {
+        if (RIGHT) return c3 * q(j - 1) + c2 * q(j) + c1 * q(j + 1);
+        return c1 * q(j - 1) + c2 * q(j) + c3 * q(j + 1);
+    }
+    if (RIGHT) return b5 * q(j - 2) + b4 * q(j - 1) + b3 * q(j) + b2 * q(j + 1) + b1 * q(j + 2);
+    return b1 * q(j - 2) + b2 * q(j - 1) + b3 * q(j) + b4 * q(j + 1) + b5 * q(j + 2);
+}
+
+// core/fortran_upwind.f90:34-67: d = (d + fx(m-1)) - fx(m), the face flux
+// fx(i) = up(i)*qp(i) + um(i)*qm(i+1) with up/um = 0.5*(U -+ |U|), no flux
+// through the ends of a wall line.  U_d = ud*cu as in tracer_axis.
+template <typename T, int PD>
+__device__ __forceinline__ T lin_tracer_axis(T d, const T *__restrict__ q_arr,
+                                             const T *__restrict__ ud, T cu, int64_t based,
+                                             int64_t sd, int m, int n, int order) {
+    auto pos = [&](int p) -> int64_t { return based + (int64_t)(PD ? wrapi(p, n) : p) * sd; };
+    auto q = [&](int p) -> T { return q_arr[pos(p)]; };
+    auto face = [&](int i) -> T {
+        const T ui = ud[pos(i)] * cu, au = fabs(ui);
+        const T up = T(0.5) * (ui + au), um = T(0.5) * (ui - au);
+        return up * lin_rec<T, false, false>(q, i, n, order, PD != 0)
+             + um * lin_rec<T, false, true>(q, i + 1, n, order, PD != 0);
+    };
+    const T fxm = (PD || m > 0) ? face(m - 1) : T(0);
+    const T fx = (PD || m < n - 1) ? face(m) : T(0);
+    return (d + fxm) - fx;
+}
+
+// core/fortran_vortex_force.f90:43-65 (SIGN < 0, direc) and :121-144
+// (SIGN > 0, flip): res = (res -+ qp(md-1)*up) -+ qm(md)*um, where the
+// vorticity is not shifted (vU(k) = vort(k)) and up/um split U_interp, the
+// WENO branch's u1d (UU averaged along e, then along d from UU_0 = 0).
+template <typename T, int PD, int SIGN>
+__device__ __forceinline__ T vf_lin(T res, const T *__restrict__ ud, T cu,
+                                    const T *__restrict__ om, int64_t based, int64_t sd, int md,
+                                    int nd, int64_t oe, int order) {
+    auto pos = [&](int p) -> int64_t { return based + (int64_t)(PD ? wrapi(p, nd) : p) * sd; };
+    auto UU = [&](int m) -> T {
+        const int64_t p = pos(m);
+        return T(0.5) * (ud[p] * cu + ud[p + oe] * cu);
+    };
+    const T uu_prev = (!PD && md == 0) ? T(0) : UU(md - 1);
+    const T ui = T(0.5) * (uu_prev + UU(md)), au = fabs(ui);
+    const T up = T(0.5) * (ui + au), um = T(0.5) * (ui - au);
+    auto v = [&](int p) -> T { return om[pos(p)]; };
+    const T a = lin_rec<T, true, false>(v, md - 1, nd, order, PD != 0) * up;
+    const T b = lin_rec<T, true, true>(v, md, nd, order, PD != 0) * um;
+    if (SIGN < 0) return (res - a) - b;
+    return (res + a) + b;
+}
+
 // Periodic axes: the byte offsets of the cells -3..+3 away along the axis
 // (wrapped), computed once per cell; every stencil load of that axis is then
 // at cb + off[..].  The loads and the arithmetic are those of the general
@@ -289,13 +364,20 @@ __device__ __forceinline__ void stage_cell(const Geo &g, const StageArgs<T> &a, 
     // (core/model_les_euler.py:131) nor adds buoyancy (core/bernoulli.py:26-32)
     const bool vf = MODE != 0 || !(a.flags & NYL_SF_NO_VF);
     const bool with_b = MODE != 0 || !(a.flags & NYL_SF_NO_B);
+    // the reference's linear upwind branches (tendencies-only mode)
+    const bool lin_a = MODE == 0 && a.upw_a != 0;
+    const bool lin_vf = MODE == 0 && a.upw_vf != 0;
 
     // tracer b: rhstrac, directions i, j, k (core/tracer.py:63-72)
     const T cx = T(a.idx2), cy = T(a.idy2), cz = T(a.idz2);
     T db = T(0);
     int64_t bx = c - i, by = c - (int64_t)j * g.sy, bz = c - (int64_t)k * g.sz;
     uint32_t cs = cb;
-    if (with_b) {
+    if (with_b && lin_a) {
+        db = lin_tracer_axis<T, PX>(db, a.b, a.u, cx, bx, 1, i, g.nx, a.upw_a);
+        db = lin_tracer_axis<T, PY>(db, a.b, a.v, cy, by, g.sy, j, g.ny, a.upw_a);
+        db = lin_tracer_axis<T, PZ>(db, a.b, a.w, cz, bz, g.sz, k, g.nz, a.upw_a);
+    } else if (with_b) {
         if constexpr (PX != 0) db = tracer_axis_per<T>(db, a.b, a.u, cx, cs, Lx);
         else db = tracer_axis<T, 0>(db, a.b, a.u, cx, bx, 1, i, g.nx);
         STAGE_SEQ_A(by, db);
@@ -320,7 +402,16 @@ __device__ __forceinline__ void stage_cell(const Geo &g, const StageArgs<T> &a, 
     const bool ey = PY || j < g.ny - 1;
     const bool ez = PZ || k < g.nz - 1;
     T du = T(0), dv = T(0), dw = T(0);
-    if (vf) {
+    if (vf && lin_vf) {
+        // the six terms in the order below, with the linear branch
+        const int o = a.upw_vf;
+        if (ey) dv = vf_lin<T, PX, -1>(dv, a.u, cx, a.oz, bx, 1, i, g.nx, oey, o);
+        if (ex) du = vf_lin<T, PY, +1>(du, a.v, cy, a.oz, by, g.sy, j, g.ny, oex, o);
+        if (ez) dw = vf_lin<T, PY, -1>(dw, a.v, cy, a.ox, by, g.sy, j, g.ny, oez, o);
+        if (ey) dv = vf_lin<T, PZ, +1>(dv, a.w, cz, a.ox, bz, g.sz, k, g.nz, oey, o);
+        if (ex) du = vf_lin<T, PZ, -1>(du, a.w, cz, a.oy, bz, g.sz, k, g.nz, oex, o);
+        if (ez) dw = vf_lin<T, PX, +1>(dw, a.u, cx, a.oy, bx, 1, i, g.nx, oez, o);
+    } else if (vf) {
     // direc(U_i, vor_k, du_j)
     if (ey) {
         if constexpr (PX != 0) dv = dv - vf_flux_per<T>(a.u, cx, a.oz, cs, Lx, bey);
@@ -895,14 +986,21 @@ __device__ __forceinline__ void tracer_cell(const Geo &g, const TracerArgs<T> &a
     const T cx = T(a.idx2), cy = T(a.idy2), cz = T(a.idz2);
     T dq = T(0);
     const int64_t bx = c - i, by = c - (int64_t)j * g.sy, bz = c - (int64_t)k * g.sz;
-    if constexpr (PX != 0) dq = tracer_axis_per<T>(dq, a.q, a.u, cx, cb, per_line(i, g.nx, ES));
-    else dq = tracer_axis<T, 0>(dq, a.q, a.u, cx, bx, 1, i, g.nx);
-    if constexpr (PY != 0)
-        dq = tracer_axis_per<T>(dq, a.q, a.v, cy, cb, per_line(j, g.ny, (int)g.sy * ES));
-    else dq = tracer_axis<T, 0>(dq, a.q, a.v, cy, by, g.sy, j, g.ny);
-    if constexpr (PZ != 0)
-        dq = tracer_axis_per<T>(dq, a.q, a.w, cz, cb, per_line(k, g.nz, (int)g.sz * ES));
-    else dq = tracer_axis<T, 0>(dq, a.q, a.w, cz, bz, g.sz, k, g.nz);
+    if (MODE == 0 && a.upw != 0) {   // the linear upwind branch (tendencies-only mode)
+        dq = lin_tracer_axis<T, PX>(dq, a.q, a.u, cx, bx, 1, i, g.nx, a.upw);
+        dq = lin_tracer_axis<T, PY>(dq, a.q, a.v, cy, by, g.sy, j, g.ny, a.upw);
+        dq = lin_tracer_axis<T, PZ>(dq, a.q, a.w, cz, bz, g.sz, k, g.nz, a.upw);
+    } else {
+        if constexpr (PX != 0)
+            dq = tracer_axis_per<T>(dq, a.q, a.u, cx, cb, per_line(i, g.nx, ES));
+        else dq = tracer_axis<T, 0>(dq, a.q, a.u, cx, bx, 1, i, g.nx);
+        if constexpr (PY != 0)
+            dq = tracer_axis_per<T>(dq, a.q, a.v, cy, cb, per_line(j, g.ny, (int)g.sy * ES));
+        else dq = tracer_axis<T, 0>(dq, a.q, a.v, cy, by, g.sy, j, g.ny);
+        if constexpr (PZ != 0)
+            dq = tracer_axis_per<T>(dq, a.q, a.w, cz, cb, per_line(k, g.nz, (int)g.sz * ES));
+        else dq = tracer_axis<T, 0>(dq, a.q, a.w, cz, bz, g.sz, k, g.nz);
+    }
 
     if (MODE == 0 || MODE == 1 || MODE == 2) stb(a.tout, cb, dq);
     const T cc = T(a.c);

@@ -31,7 +31,7 @@ class DeviceModel:
     def __init__(self, nx, ny, nz, periodic, dx, dy, dz, idx2=None, idy2=None, idz2=None,
                  dx2=None, fparameter=0.0, precision=L.NYL_PREC_FP64, mg_maxite=20,
                  mg_tol=1e-6, mg_omega=0.9, device=None, viscosity=0.0, n_tracers=0,
-                 model=0):
+                 model=0, upwind_a=0, upwind_vf=0):
         lib = L.load()
         if not torch.cuda.is_available():
             raise L.NylesHipError("no GPU visible: the Nyles time step runs only on the GPU")
@@ -53,6 +53,7 @@ class DeviceModel:
         d.viscosity = viscosity
         d.n_tracers = n_tracers
         d.model = model
+        d.upwind_a, d.upwind_vf = upwind_a, upwind_vf
         self.n_tracers = n_tracers
         self.model = model
         self.desc = d

@@ -12,19 +12,21 @@ GEOMETRY_XYZ = {g: tuple(reversed(p)) for g, p in O.GEOMETRY_PERIODIC.items()}
 
 def make_pair(nx, ny, nz, geometry, Lx=1.0, Ly=1.0, Lz=1.0, rotating=False, coriolis=1.0,
               mg_maxite=20, mg_tol=1e-6, mg_omega=0.9, precision=0, viscosity=0.0,
-              n_tracers=0, model="LES"):
+              n_tracers=0, model="LES", upwind_a=0, upwind_vf=0):
     from nyles_amd import _lib as L
     from nyles_amd.device import DeviceModel
     orc = O.OracleLES(nx, ny, nz, Lx, Ly, Lz, geometry, rotating=rotating, coriolis=coriolis,
                       mg_maxite=mg_maxite, mg_tol=mg_tol, mg_omega=mg_omega,
-                      viscosity=viscosity, n_tracers=n_tracers, model=model)
+                      viscosity=viscosity, n_tracers=n_tracers, model=model,
+                      upwind_a=upwind_a, upwind_vf=upwind_vf)
     model_id = {"LES": L.NYL_MODEL_LES, "linear": L.NYL_MODEL_LINEAR,
                 "Euler3d": L.NYL_MODEL_EULER}[model]
     dev = DeviceModel(nx, ny, nz, GEOMETRY_XYZ[geometry], orc.dx, orc.dy, orc.dz,
                       idx2=orc.idx2, idy2=orc.idy2, idz2=orc.idz2, dx2=orc.dx ** 2,
                       fparameter=orc.fparameter, mg_maxite=mg_maxite, mg_tol=mg_tol,
                       mg_omega=mg_omega, precision=precision, viscosity=viscosity,
-                      n_tracers=n_tracers, model=model_id)
+                      n_tracers=n_tracers, model=model_id, upwind_a=upwind_a,
+                      upwind_vf=upwind_vf)
     return orc, dev
 
 

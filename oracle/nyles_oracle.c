@@ -8,8 +8,11 @@
  * It restates, loop by loop and operation by operation, the Fortran
  * kernels of the reference (pvthinker/Nyles):
  *   core/weno.f90                 weno3, weno5, flux1d
- *   core/fortran_upwind.f90       upwind (WENO branch)
+ *   core/fortran_upwind.f90       upwind (WENO branch, and the linear
+ *                                 branch with core/interpolate_tracer.f90)
  *   core/fortran_vortex_force.f90 vortex_force_direc / vortex_force_flip
+ *                                 (WENO branch, and the linear branch with
+ *                                 core/interpolate.f90)
  *   core/fortran_vorticity.f90    vorticity
  *   core/fortran_bernoulli.f90    gradke, gradkeandb, div
  *   core/fortran_kinenergy.f90    kin
@@ -155,6 +158,185 @@ void orc_upwind(const double *trac, const double *u, double *dtrac,
                 AT(dtrac, k, j, 0) = AT(dtrac, k, j, 0) - flux[0];
                 for (int i = 1; i < n; i++)
                     AT(dtrac, k, j, i) = AT(dtrac, k, j, i) + flux[i - 1] - flux[i];
+            }
+        free(buf);
+    }
+}
+
+/* Linear upwind interpolation, odd orders 1, 3, 5, on a line q[0..n-1]
+ * (0-based).  Constants are the REAL(4) literals of the Fortran widened to
+ * double; the sums run left to right as written there.
+ * Tracer flavour, core/interpolate_tracer.f90:3-70: qp[i], qm[i] for
+ * i = 0..n-1, the first-order copy at both ends, third order next to them,
+ * fifth order in between. */
+static void interp_tracer(const double *q, double *qp, double *qm, int order, int n)
+{
+    const double c1 = -(double)(1.0f / 6.0f), c2 = (double)(5.0f / 6.0f), c3 = (double)(2.0f / 6.0f);
+    const double b1 = (double)(2.0f / 60.0f), b2 = -(double)(13.0f / 60.0f);
+    const double b3 = (double)(47.0f / 60.0f), b4 = (double)(27.0f / 60.0f);
+    const double b5 = -(double)(3.0f / 60.0f);
+    for (int i = 0; i < n; i++) { qp[i] = q[i]; qm[i] = q[i]; }
+    if (order == 1) return;
+    const int lo = order == 5 ? 2 : 1, hi = order == 5 ? n - 3 : n - 2;
+    if (order == 5) {
+        for (int i = lo; i <= hi; i++) {
+            qp[i] = b1 * q[i - 2] + b2 * q[i - 1] + b3 * q[i] + b4 * q[i + 1] + b5 * q[i + 2];
+            qm[i] = b5 * q[i - 2] + b4 * q[i - 1] + b3 * q[i] + b2 * q[i + 1] + b1 * q[i + 2];
+        }
+        const int e[2] = {1, n - 2};
+        for (int t = 0; t < 2; t++) {
+            const int i = e[t];
+            qp[i] = c1 * q[i - 1] + c2 * q[i] + c3 * q[i + 1];
+            qm[i] = c3 * q[i - 1] + c2 * q[i] + c1 * q[i + 1];
+        }
+    } else {
+        for (int i = lo; i <= hi; i++) {
+            qp[i] = c1 * q[i - 1] + c2 * q[i] + c3 * q[i + 1];
+            qm[i] = c3 * q[i - 1] + c2 * q[i] + c1 * q[i + 1];
+        }
+    }
+}
+
+/* Vortex-force flavour, core/interpolate.f90:3-79 (odd orders): qp is
+ * indexed 0..n-1 with qp[0] = 0 standing for Fortran's qp(0), so Fortran
+ * qp(i) is qp[i] here and Fortran qm(i) is qm[i-1]; that is qp[1..] and
+ * qm[0..] are the 1-based values shifted by one.  To keep one convention
+ * this takes 1-based arrays: qp[0..n-1] = Fortran qp(0..n-1) and
+ * qm[1..n] = Fortran qm(1..n) (qm[0] unused). */
+static void interp_vf(const double *v /* 1-based: v[1..n] */, double *qp, double *qm,
+                      int order, int n)
+{
+    const double c1 = -(double)(1.0f / 6.0f), c2 = (double)(5.0f / 6.0f), c3 = (double)(2.0f / 6.0f);
+    const double b1 = (double)(2.0f / 60.0f), b2 = -(double)(13.0f / 60.0f);
+    const double b3 = (double)(47.0f / 60.0f), b4 = (double)(27.0f / 60.0f);
+    const double b5 = -(double)(3.0f / 60.0f);
+    qp[0] = 0.0;
+    if (order == 5) {
+        qp[1] = v[1]; qm[1] = v[1];
+        qp[2] = c1 * v[1] + c2 * v[2] + c3 * v[3];
+        qm[2] = c3 * v[1] + c2 * v[2] + c1 * v[3];
+        for (int i = 3; i <= n - 3; i++) {
+            qp[i] = b1 * v[i - 2] + b2 * v[i - 1] + b3 * v[i] + b4 * v[i + 1] + b5 * v[i + 2];
+            qm[i] = b5 * v[i - 2] + b4 * v[i - 1] + b3 * v[i] + b2 * v[i + 1] + b1 * v[i + 2];
+        }
+        int i = n - 2;
+        qp[i] = c1 * v[i - 1] + c2 * v[i] + c3 * v[i + 1];
+        qm[i] = c3 * v[i - 1] + c2 * v[i] + c1 * v[i + 1];
+        i = n - 1;
+        qp[i] = v[i]; qm[i] = v[i];
+        qm[n] = v[n];
+    } else if (order == 3) {
+        qp[1] = v[1]; qm[1] = v[1];
+        for (int i = 2; i <= n - 2; i++) {
+            qp[i] = c1 * v[i - 1] + c2 * v[i] + c3 * v[i + 1];
+            qm[i] = c3 * v[i - 1] + c2 * v[i] + c1 * v[i + 1];
+        }
+        qp[n - 1] = v[n - 1]; qm[n - 1] = v[n - 1];
+        qm[n] = v[n];
+    } else {
+        for (int i = 1; i <= n - 1; i++) { qp[i] = v[i]; qm[i] = v[i]; }
+        qm[n] = v[n];
+    }
+}
+
+void orc_interp_tracer(const double *q, double *qp, double *qm, int order, int n)
+{
+    interp_tracer(q, qp, qm, order, n);
+}
+
+void orc_interp_vf(const double *v, double *qp, double *qm, int order, int n)
+{
+    interp_vf(v, qp, qm, order, n);
+}
+
+/* core/fortran_upwind.f90:34-67, the linear branch, odd orders: per line
+ * fx = up(i)*qp(i) + um(i)*qm(i+1) for the faces i = 1..n-1, with
+ * up/um = 0.5*(u -+ |u|); dtrac(i) = dtrac(i) + fxm - fx, and fx = 0 past
+ * the last face. */
+void orc_upwind_linear(const double *trac, const double *u, double *dtrac, int order,
+                       int l, int m, int n, i64 s0, i64 s1, i64 s2)
+{
+    #pragma omp parallel
+    {
+        double *buf = (double *)malloc(sizeof(double) * 5 * (size_t)n);
+        double *up = buf, *um = buf + n, *phi = buf + 2 * n, *qp = buf + 3 * n, *qm = buf + 4 * n;
+        #pragma omp for collapse(2) schedule(static)
+        for (int k = 0; k < l; k++)
+            for (int j = 0; j < m; j++) {
+                for (int i = 0; i < n; i++) {
+                    const double ui = AT(u, k, j, i), UU = fabs(ui);
+                    up[i] = 0.5 * (ui + UU);
+                    um[i] = 0.5 * (ui - UU);
+                    phi[i] = AT(trac, k, j, i);
+                }
+                interp_tracer(phi, qp, qm, order, n);
+                double fxm = 0.0;
+                for (int i = 0; i < n - 1; i++) {
+                    const double fx = up[i] * qp[i] + um[i] * qm[i + 1];
+                    AT(dtrac, k, j, i) = AT(dtrac, k, j, i) + fxm - fx;
+                    fxm = fx;
+                }
+                AT(dtrac, k, j, n - 1) = AT(dtrac, k, j, n - 1) + fxm - 0.0;
+            }
+        free(buf);
+    }
+}
+
+/* core/fortran_vortex_force.f90:43-65 (direc) and :121-144 (flip), the
+ * linear branch, odd orders: vU(k) = vort(k) (not shifted), U_interp as in
+ * the WENO branch, up/um its split; direc res(k) = res(k) - qp(k-1)*up(k)
+ * - qm(k)*um(k), flip res(i) = res(i) + qp(i-1)*up(i) + qm(i)*um(i). */
+void orc_vortex_force_direc_linear(const double *U, const double *vort, double *res, int order,
+                                   int m, int n, int l, i64 s0, i64 s1, i64 s2)
+{
+    #pragma omp parallel
+    {
+        double *buf = (double *)malloc(sizeof(double) * 5 * ((size_t)l + 1));
+        double *up = buf, *um = buf + (l + 1), *v = buf + 2 * (l + 1);
+        double *qp = buf + 3 * (l + 1), *qm = buf + 4 * (l + 1);
+        #pragma omp for collapse(2) schedule(static)
+        for (int j = 0; j < m; j++)
+            for (int i = 0; i < n - 1; i++) {
+                double UU0 = 0.0;
+                for (int k = 1; k <= l; k++) {
+                    const double UU1 = 0.5 * (AT(U, j, i, k - 1) + AT(U, j, i + 1, k - 1));
+                    v[k] = AT(vort, j, i, k - 1);
+                    const double Ui = 0.5 * (UU0 + UU1), UU = fabs(Ui);
+                    up[k] = 0.5 * (Ui + UU);
+                    um[k] = 0.5 * (Ui - UU);
+                    UU0 = UU1;
+                }
+                interp_vf(v, qp, qm, order, l);
+                for (int k = 1; k <= l; k++)
+                    AT(res, j, i, k - 1) = AT(res, j, i, k - 1) - qp[k - 1] * up[k] - qm[k] * um[k];
+            }
+        free(buf);
+    }
+}
+
+void orc_vortex_force_flip_linear(const double *U, const double *vort, double *res, int order,
+                                  int m, int n, int l, i64 s0, i64 s1, i64 s2)
+{
+    #pragma omp parallel
+    {
+        double *buf = (double *)malloc(sizeof(double) * 5 * ((size_t)n + 1));
+        double *up = buf, *um = buf + (n + 1), *v = buf + 2 * (n + 1);
+        double *qp = buf + 3 * (n + 1), *qm = buf + 4 * (n + 1);
+        #pragma omp for collapse(2) schedule(static)
+        for (int j = 0; j < m; j++)
+            for (int k = 0; k < l - 1; k++) {
+                double UU0 = 0.0;
+                for (int i = 1; i <= n; i++) {
+                    const double UU1 = 0.5 * (AT(U, j, i - 1, k) + AT(U, j, i - 1, k + 1));
+                    v[i] = AT(vort, j, i - 1, k);
+                    const double Ui = 0.5 * (UU0 + UU1), UU = fabs(Ui);
+                    up[i] = 0.5 * (Ui + UU);
+                    um[i] = 0.5 * (Ui - UU);
+                    UU0 = UU1;
+                }
+                interp_vf(v, qp, qm, order, n);
+                for (int i = 1; i <= n; i++)
+                    AT(res, j, i - 1, k) = AT(res, j, i - 1, k) + qp[i - 1] * up[i] + qm[i] * um[i];
             }
         free(buf);
     }

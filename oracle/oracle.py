@@ -62,6 +62,11 @@ def lib():
         for name in ("orc_upwind", "orc_vortex_force_direc", "orc_vortex_force_flip",
                      "orc_vorticity"):
             getattr(L, name).argtypes = [dp, dp, dp, ci, ci, ci, i64, i64, i64]
+        for name in ("orc_upwind_linear", "orc_vortex_force_direc_linear",
+                     "orc_vortex_force_flip_linear"):
+            getattr(L, name).argtypes = [dp, dp, dp, ci, ci, ci, ci, i64, i64, i64]
+        L.orc_interp_tracer.argtypes = [dp, dp, dp, ci, ci]
+        L.orc_interp_vf.argtypes = [dp, dp, dp, ci, ci]
         L.orc_gradke.argtypes = [dp, dp, ci, ci, ci, i64, i64, i64]
         L.orc_gradkeandb.argtypes = [dp, dp, dp, ctypes.c_double, ci, ci, ci, i64, i64, i64]
         L.orc_div.argtypes = [dp, dp, ci, ci, ci, ci, i64, i64, i64]
@@ -151,6 +156,45 @@ def vortex_force_direc(U, vort, res):
     """core/fortran_vortex_force.f90:10 vortex_force_direc(U, vort, res, order)."""
     _same_layout(U, vort, res)
     lib().orc_vortex_force_direc(_vptr(U), _vptr(vort), _vptr(res), *_view_args(res))
+
+
+def interp_tracer(q, order):
+    """core/interpolate_tracer.f90, odd orders: (qp, qm) on a line"""
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    qp, qm = np.empty_like(q), np.empty_like(q)
+    lib().orc_interp_tracer(_ptr(q), _ptr(qp), _ptr(qm), int(order), q.size)
+    return qp, qm
+
+
+def interp_vf(v, order):
+    """core/interpolate.f90, odd orders, on v(1..n): (qp(0..n-1), qm(1..n))"""
+    n = len(v)
+    v1 = np.zeros(n + 1)
+    v1[1:] = v
+    qp, qm = np.zeros(n + 1), np.zeros(n + 1)
+    lib().orc_interp_vf(_ptr(v1), _ptr(qp), _ptr(qm), int(order), n)
+    return qp[:n].copy(), qm[1:].copy()
+
+
+def upwind_linear(trac, u, dtrac, order):
+    """core/fortran_upwind.f90:34-67 (the linear branch), odd orders"""
+    _same_layout(trac, u, dtrac)
+    lib().orc_upwind_linear(_vptr(trac), _vptr(u), _vptr(dtrac), int(order),
+                            *_view_args(dtrac))
+
+
+def vortex_force_direc_linear(U, vort, res, order):
+    """core/fortran_vortex_force.f90:43-65 (the linear branch), odd orders"""
+    _same_layout(U, vort, res)
+    lib().orc_vortex_force_direc_linear(_vptr(U), _vptr(vort), _vptr(res), int(order),
+                                        *_view_args(res))
+
+
+def vortex_force_flip_linear(U, vort, res, order):
+    """core/fortran_vortex_force.f90:121-144 (the linear branch), odd orders"""
+    _same_layout(U, vort, res)
+    lib().orc_vortex_force_flip_linear(_vptr(U), _vptr(vort), _vptr(res), int(order),
+                                       *_view_args(res))
 
 
 def vortex_force_flip(U, vort, res):
@@ -338,7 +382,7 @@ class OracleLES:
 
     def __init__(self, nx, ny, nz, Lx=1.0, Ly=1.0, Lz=1.0, geometry="closed", nh=3,
                  rotating=False, coriolis=1.0, mg_maxite=20, mg_tol=1e-6, mg_omega=0.9,
-                 viscosity=0.0, n_tracers=0, model="LES"):
+                 viscosity=0.0, n_tracers=0, model="LES", upwind_a=0, upwind_vf=0):
         self.nx, self.ny, self.nz, self.nh = nx, ny, nz, nh
         # param.model["modelname"] (core/nyles.py:92-100): "LES", "linear"
         # (LES(linear=True), core/model_les.py:33-34) or "Euler3d"
@@ -347,6 +391,13 @@ class OracleLES:
             raise ValueError("model must be LES, linear or Euler3d")
         self.model = model
         self.nonlinear = model != "linear"
+        # the reference's linear upwind branches (linear=.false. there:
+        # core/fortran_upwind.f90:30, core/fortran_vortex_force.f90:28,104),
+        # of order orderA for the tracers and orderVF for the vortex force;
+        # 0 is the WENO branch the reference runs
+        if upwind_a not in (0, 1, 3, 5) or upwind_vf not in (0, 1, 3, 5):
+            raise ValueError("linear upwind orders are 1, 3 or 5 (0: WENO)")
+        self.upwind_a, self.upwind_vf = upwind_a, upwind_vf
         # core/model_les.py:36-43: passive tracers t0, t1, ... after b
         self.traclist = ["b"] + ["t%d" % q for q in range(n_tracers)]
         self.PROGNOSTIC = tuple(self.traclist) + ("u_i", "u_j", "u_k")
@@ -415,7 +466,10 @@ class OracleLES:
                 dfield = view(ds[name], d)
                 if d == "i":
                     dfield[...] = 0.0
-                upwind(field, velocity, dfield)
+                if self.upwind_a:
+                    upwind_linear(field, velocity, dfield, self.upwind_a)
+                else:
+                    upwind(field, velocity, dfield)
         # vortex_force, core/vortex_force.py:69-81 (not in the linear model,
         # core/model_les.py:135-136)
         for k, j, i in (["ikj", "jik", "kji"] if self.nonlinear else []):
@@ -424,8 +478,12 @@ class OracleLES:
             U_i = flipview(self.U[i], j)
             U_k = flipview(self.U[k], j)
             w_j = flipview(self.vor[j], j)
-            vortex_force_direc(U_k, w_j, u_i)
-            vortex_force_flip(U_i, w_j, u_k)
+            if self.upwind_vf:
+                vortex_force_direc_linear(U_k, w_j, u_i, self.upwind_vf)
+                vortex_force_flip_linear(U_i, w_j, u_k, self.upwind_vf)
+            else:
+                vortex_force_direc(U_k, w_j, u_i)
+                vortex_force_flip(U_i, w_j, u_k)
         # bernoulli, core/bernoulli.py:14-23; Euler3d: bernoulli_euler, the
         # gradient of ke only (core/bernoulli.py:26-32)
         for d in "ijk":

@@ -28,20 +28,22 @@ def test_library_exports_every_declared_symbol():
 def test_binding_sets_signatures_and_abi_version():
     from nyles_amd import _lib
     L = _lib.load()
-    assert L.nyl_abi_version() == 2
+    assert L.nyl_abi_version() == 3
     assert L.nyl_last_error() is not None
 
 
 def test_descriptor_layout_matches_header():
-    """nyl_desc: six int32, eight doubles, two int32, three doubles, two
-    int32 (128 bytes)."""
+    """nyl_desc: six int32, eight doubles, two int32, three doubles, four
+    int32 (136 bytes)."""
     from nyles_amd import _lib
-    assert ctypes.sizeof(_lib.NylDesc) == 128
+    assert ctypes.sizeof(_lib.NylDesc) == 136
     assert _lib.NylDesc.dx.offset == 24
     assert _lib.NylDesc.precision.offset == 88
     assert _lib.NylDesc.mg_tol.offset == 96
     assert _lib.NylDesc.viscosity.offset == 112
     assert _lib.NylDesc.n_tracers.offset == 120
+    assert _lib.NylDesc.model.offset == 124
+    assert _lib.NylDesc.upwind_vf.offset == 132
     assert ctypes.sizeof(_lib.NylMGStats) == 24
 
 
