@@ -291,11 +291,13 @@ void set_tendency_out(nyl_ctx *c, StageArgs<T> &a, int slot) {
 // residual of the mixed solve).
 // addx (level 0 of the mixed solve): the last sweep adds its result to the
 // FP64 solution instead of storing it (tiled levels), or an add follows.
-// Post-smoothing of the tiled levels as one two-sweep pass (off by default:
-// faster per step, 63 us against 2 x 36 us per 256^3 FP32 level, but the
-// pass runs at 0.40 of HBM peak by its own bytes)
+// Post-smoothing of the tiled FP64 levels as one two-sweep pass: src and b
+// are read once for two sweeps (256^3: 125 us against 2 x 77 us).  FP32
+// levels keep two sweeps: there the pass (92 VGPRs, two barriers per plane)
+// took 78 us against 2 x 32.5 us.  The pass can also add the second sweep's
+// result to an FP64 solution (the mixed solve's last sweep).
 #ifndef NYL_SWEEP2
-#define NYL_SWEEP2 0
+#define NYL_SWEEP2 1
 #endif
 
 // Levels 1 .. nlev-2 run the fused level kernels (down: second pre-sweep +
@@ -341,13 +343,16 @@ int smooth_level_t(nyl_ctx *c, int l, int xc_level, bool pre_done, double *addx)
             return NYL_OK;
         }
         const double bp = 2.125 * cells(f.g) * sizeof(Tf);
-        if (NYL_SWEEP2 && same && restrict25_fits(f.g) && tiled && !pre_done && !addx) {
-            // y = x + P(xc), then both post-smoothing sweeps y -> x in one pass
+        if (NYL_SWEEP2 && std::is_same<Tf, double>::value && same && restrict25_fits(f.g) &&
+            tiled && !pre_done) {
+            // y = x + P(xc), then both post-smoothing sweeps y -> x in one
+            // pass (with addx: the second sweep's result added to the FP64 x)
             Tf *yv = F<Tf>(c, f.y);
             LAUNCH(NYL_K_MG_PROLONG, bp,
                    launch_mg_prolong25<Tf, Tf>(f.g, q.g, x, F<Tf>(c, qx), c->tabs, c->stream, yv));
-            LAUNCH(NYL_K_MG_SWEEP2, by,
-                   launch_mg_sweep2x25<Tf>(f.g, yv, x, b, om, c->tabs, c->stream));
+            const double b2 = addx ? (2 * sizeof(Tf) + 2 * sizeof(double)) * cells(f.g) : by;
+            LAUNCH(NYL_K_MG_SWEEP2, b2,
+                   launch_mg_sweep2x25<Tf>(f.g, yv, x, b, om, c->tabs, c->stream, addx));
             return NYL_OK;
         }
         if (same && restrict25_fits(f.g)) {

@@ -218,7 +218,7 @@ hipError_t launch_mg_prolong25(const Geo &gf, const Geo &gc, Tf *xf, const Tc *x
 // two consecutive tiled sweeps src -> (tmp) -> dst in one pass (src != dst)
 template <typename T>
 hipError_t launch_mg_sweep2x25(const Geo &g, const T *src, T *dst, const T *b, double omega,
-                               const MGTabs &tab, hipStream_t s);
+                               const MGTabs &tab, hipStream_t s, double *addx = nullptr);
 template <typename Tf, typename Tc>
 hipError_t launch_mg_restrict25(const Geo &gf, const Geo &gc, const Tf *x, const Tf *b, Tc *bc,
                                 Tc *xc, const MGTabs &tab, hipStream_t s, Tc *yc = nullptr,

@@ -1622,7 +1622,8 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_mg_sweep2x25(Geo g, const T *__restrict__ src,
                                                       T *__restrict__ dst,
                                                       const T *__restrict__ b, double omega,
-                                                      MGTabs tab, int kz) {
+                                                      MGTabs tab, int kz,
+                                                      double *__restrict__ xo) {
     constexpr int NIN = S2_LY * S2_LX, NMID = S2_MY * S2_MX;
     constexpr int RIN = (NIN + 255) / 256, RMID = (NMID + 255) / 256;
     __shared__ T pin[3][NIN];
@@ -1665,32 +1666,65 @@ __global__ __launch_bounds__(256) void k_mg_sweep2x25(Geo g, const T *__restrict
         out_nxy[r] = (g.px || i > 0) + (g.px || i < g.nx - 1) + (g.py || j > 0) +
                      (g.py || j < g.ny - 1);
     }
+    // inverse diagonals for 0, 1, 2 in-domain z-neighbours (hoisted out of
+    // the march: the table is a kernel argument, read per lane otherwise)
+    T mid_id[3][RMID], out_id[3][SW_R];
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+#pragma unroll
+        for (int r = 0; r < RMID; r++) mid_id[q][r] = T(tab.idiag[mid_nxy[r] + q]);
+#pragma unroll
+        for (int r = 0; r < SW_R; r++) out_id[q][r] = T(tab.idiag[out_nxy[r] + q]);
+    }
     auto zin = [&](int m) { return g.pz || (m >= 0 && m < g.nz); };
-    auto zw = [&](int m) { return g.pz ? wrapg(m, g.nz) : m; };
+    auto zw = [&](int m) { return g.pz ? wrapg(m, g.nz) : (zin(m) ? m : 0); };
+    auto nzw_of = [&](int m) {   // in-domain z-neighbours of plane m (wrapped)
+        const int mw = g.pz ? wrapg(m, g.nz) : m;
+        return (g.pz || mw > 0) + (g.pz || mw < g.nz - 1);
+    };
+    auto pick = [](int nz_, const T (&t)[3]) { return nz_ == 2 ? t[2] : (nz_ == 1 ? t[1] : t[0]); };
     auto slot = [&](int m) { return (m - k0 + 3) % 3; };   // m >= k0 - 2
-    auto load_in = [&](int m) {
+    // Loads run ahead of their use: every load is issued from a valid
+    // address (in-plane offset 0, plane 0 outside a wall axis) and the value
+    // selected when it is used, so no branch forces a wait.
+    int in_ld[RIN], mid_ld[RMID];
+#pragma unroll
+    for (int r = 0; r < RIN; r++) in_ld[r] = in_off[r] >= 0 ? in_off[r] : 0;
+#pragma unroll
+    for (int r = 0; r < RMID; r++) mid_ld[r] = mid_off[r] >= 0 ? mid_off[r] : 0;
+    const int out_ld0 = (j0 + ty) * (int)g.sy + i0 + tx;
+    auto fetch_in = [&](int m, T (&v)[RIN]) {
+        const T *p = src + (int64_t)zw(m) * g.sz;
+#pragma unroll
+        for (int r = 0; r < RIN; r++) v[r] = p[in_ld[r]];
+    };
+    auto put_in = [&](int m, const T (&v)[RIN]) {
         T *P = pin[slot(m)];
         const bool zi = zin(m);
-        const int64_t base = (int64_t)zw(m) * g.sz;
-        T v[RIN];
-#pragma unroll
-        for (int r = 0; r < RIN; r++) v[r] = (zi && in_off[r] >= 0) ? src[base + in_off[r]] : T(0);
 #pragma unroll
         for (int r = 0; r < RIN; r++) {
             const int q = tid + 256 * r;
-            if (q < NIN) P[q] = v[r];
+            if (q < NIN) P[q] = (zi && in_off[r] >= 0) ? v[r] : T(0);
         }
     };
-    auto comp_mid = [&](int m) {
+    auto fetch_bmid = [&](int m, T (&v)[RMID]) {
+        const T *p = b + (int64_t)zw(m) * g.sz;
+#pragma unroll
+        for (int r = 0; r < RMID; r++) v[r] = p[mid_ld[r]];
+    };
+    auto fetch_bout = [&](int k, T (&v)[SW_R], double (&xv)[SW_R]) {
+        const int64_t o = (int64_t)k * g.sz + out_ld0;
+#pragma unroll
+        for (int r = 0; r < SW_R; r++) {
+            v[r] = b[o + (int64_t)4 * r * g.sy];
+            if (xo) xv[r] = xo[o + (int64_t)4 * r * g.sy];
+        }
+    };
+    auto comp_mid = [&](int m, const T (&bv)[RMID]) {
         T *M = mid[slot(m)];
         const T *P0 = pin[slot(m)], *Pm = pin[slot(m - 1)], *Pp = pin[slot(m + 1)];
         const bool zi = zin(m);
-        const int mw = zw(m);
-        const int nzw = (g.pz || mw > 0) + (g.pz || mw < g.nz - 1);
-        const int64_t base = (int64_t)mw * g.sz;
-        T bv[RMID];
-#pragma unroll
-        for (int r = 0; r < RMID; r++) bv[r] = (zi && mid_off[r] >= 0) ? b[base + mid_off[r]] : T(0);
+        const int nzw = nzw_of(m);
 #pragma unroll
         for (int r = 0; r < RMID; r++) {
             const int q = tid + 256 * r;
@@ -1700,12 +1734,13 @@ __global__ __launch_bounds__(256) void k_mg_sweep2x25(Geo g, const T *__restrict
                 const int lx = q % S2_MX, ly = q / S2_MX;
                 const int p = (ly + 1) * S2_LX + (lx + 1);
                 const T S = P0[p - 1] + P0[p + 1] + P0[p - S2_LX] + P0[p + S2_LX] + Pm[p] + Pp[p];
-                v = cff1 * P0[p] + om * (S - bv[r]) * T(tab.idiag[mid_nxy[r] + nzw]);
+                T idg[3] = {mid_id[0][r], mid_id[1][r], mid_id[2][r]};
+                v = cff1 * P0[p] + om * (S - bv[r]) * pick(nzw, idg);
             }
             M[q] = v;
         }
     };
-    auto comp_out = [&](int k) {
+    auto comp_out = [&](int k, const T (&bv)[SW_R], const double (&xv)[SW_R]) {
         const T *M0 = mid[slot(k)], *Mm = mid[slot(k - 1)], *Mp = mid[slot(k + 1)];
         const int nzw = (g.pz || k > 0) + (g.pz || k < g.nz - 1);
         const int i = i0 + tx;
@@ -1715,25 +1750,61 @@ __global__ __launch_bounds__(256) void k_mg_sweep2x25(Geo g, const T *__restrict
             const int p = (ty + 4 * r + 1) * S2_MX + (tx + 1);
             const int64_t c = (int64_t)k * g.sz + (int64_t)j * g.sy + i;
             const T S = M0[p - 1] + M0[p + 1] + M0[p - S2_MX] + M0[p + S2_MX] + Mm[p] + Mp[p];
-            dst[c] = cff1 * M0[p] + om * (S - b[c]) * T(tab.idiag[out_nxy[r] + nzw]);
+            T idg[3] = {out_id[0][r], out_id[1][r], out_id[2][r]};
+            const T v = cff1 * M0[p] + om * (S - bv[r]) * pick(nzw, idg);
+            if (xo) xo[c] = xv[r] + (double)v;   // ADDX of k_mg_sweep25
+            else dst[c] = v;
         }
     };
-    load_in(k0 - 2);
-    load_in(k0 - 1);
-    load_in(k0);
-    __syncthreads();
-    comp_mid(k0 - 1);
-    __syncthreads();
-    load_in(k0 + 1);
-    __syncthreads();
-    comp_mid(k0);
+    // prologue: input planes k0-2 .. k0+1 in LDS, first sweep of k0-1, k0
+    T vin[RIN], bm[RMID], bo[SW_R];
+    double xv[SW_R] = {};
+    {
+        T v0[RIN], v1[RIN], v2[RIN];
+        fetch_in(k0 - 2, v0);
+        fetch_in(k0 - 1, v1);
+        fetch_in(k0, v2);
+        fetch_in(k0 + 1, vin);
+        T b0[RMID];
+        fetch_bmid(k0 - 1, b0);
+        fetch_bmid(k0, bm);
+        put_in(k0 - 2, v0);
+        put_in(k0 - 1, v1);
+        put_in(k0, v2);
+        __syncthreads();
+        comp_mid(k0 - 1, b0);
+        __syncthreads();          // plane k0+1 replaces plane k0-2, just read
+        put_in(k0 + 1, vin);
+        __syncthreads();
+        comp_mid(k0, bm);
+    }
+    fetch_in(k0 + 2, vin);
+    fetch_bmid(k0 + 1, bm);
+    fetch_bout(k0, bo, xv);
+    __syncthreads();              // comp_mid(k0) read plane k0-1, replaced next
     for (int k = k0; k < k0 + kz; k++) {
-        __syncthreads();          // readers of the buffers reused below are done
-        load_in(k + 2);           // replaces plane k - 1 of the input
+        // plane k+2 of the input replaces plane k-1, whose last reader (the
+        // first sweep of plane k) passed the barrier before comp_out(k-1);
+        // two barriers per plane
+        put_in(k + 2, vin);
+        T bmc[RMID], boc[SW_R];
+        double xvc[SW_R];
+#pragma unroll
+        for (int r = 0; r < RMID; r++) bmc[r] = bm[r];
+#pragma unroll
+        for (int r = 0; r < SW_R; r++) {
+            boc[r] = bo[r];
+            xvc[r] = xv[r];
+        }
+        if (k + 1 < k0 + kz) {
+            fetch_in(k + 3, vin);
+            fetch_bmid(k + 2, bm);
+            fetch_bout(k + 1, bo, xv);
+        }
         __syncthreads();
-        comp_mid(k + 1);          // replaces plane k - 2 of the first sweep
+        comp_mid(k + 1, bmc);     // replaces plane k - 2 of the first sweep
         __syncthreads();
-        comp_out(k);
+        comp_out(k, boc, xvc);
     }
 }
 
@@ -1748,19 +1819,19 @@ static int sweep2x_kz(const Geo &g) {
 
 template <typename T>
 hipError_t launch_mg_sweep2x25(const Geo &g, const T *src, T *dst, const T *b, double omega,
-                               const MGTabs &tab, hipStream_t s) {
+                               const MGTabs &tab, hipStream_t s, double *addx) {
     if (src == dst) return hipErrorInvalidValue;
     const int kz = sweep2x_kz(g);
     dim3 grd(g.nx / SW_TX, g.ny / SW_TY, g.nz / kz), blk(SW_TX, 4);
-    NYL_LAUNCH((k_mg_sweep2x25<T>), grd, blk, 0, s, g, src, dst, b, omega, tab, kz);
+    NYL_LAUNCH((k_mg_sweep2x25<T>), grd, blk, 0, s, g, src, dst, b, omega, tab, kz, addx);
     return hipGetLastError();
 }
 template hipError_t launch_mg_sweep2x25<double>(const Geo &, const double *, double *,
                                                 const double *, double, const MGTabs &,
-                                                hipStream_t);
+                                                hipStream_t, double *);
 template hipError_t launch_mg_sweep2x25<float>(const Geo &, const float *, float *,
                                                const float *, double, const MGTabs &,
-                                               hipStream_t);
+                                               hipStream_t, double *);
 
 bool restrict25_fits(const Geo &g) {
     return sweep25_fits(g) && g.nz % 2 == 0;

@@ -83,8 +83,18 @@ int check_tiled(int nx, int ny, int nz, int px, int py, int pz) {
         k_mg_sweep<T>(gf, y1.data(), z1.data(), b.data(), om, tab); }, false);
     const int kz2 = nz % 4 == 0 ? 4 : 1;
     run_blocks(dim3(nx / SW_TX, ny / SW_TY, nz / kz2), dim3(SW_TX, 4, 1), [&]() {
-        k_mg_sweep2x25<T>(gf, x.data(), z2.data(), b.data(), om, tab, kz2); }, true);
+        k_mg_sweep2x25<T>(gf, x.data(), z2.data(), b.data(), om, tab, kz2, nullptr); }, true);
     bad += ndiff(z1, z2);
+    // ... with the result added to an FP64 x instead of stored
+    {
+        std::vector<double> xo1(nf), xo2(nf);
+        for (size_t q = 0; q < nf; q++) xo1[q] = xo2[q] = 0.5 - 0.125 * (double)q;
+        for (size_t q = 0; q < nf; q++) xo1[q] = xo1[q] + (double)z1[q];
+        run_blocks(dim3(nx / SW_TX, ny / SW_TY, nz / kz2), dim3(SW_TX, 4, 1), [&]() {
+            k_mg_sweep2x25<T>(gf, x.data(), nullptr, b.data(), om, tab, kz2, xo2.data()); },
+            true);
+        bad += ndiff(xo1, xo2);
+    }
     printf("tiled<%s> %dx%dx%d p=%d%d%d: %d mismatches\n", sizeof(T) == 8 ? "f64" : "f32",
            nx, ny, nz, px, py, pz, bad);
     return bad != 0;
