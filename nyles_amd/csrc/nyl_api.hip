@@ -522,13 +522,72 @@ int coarsest(nyl_ctx *c) {
     return NYL_OK;
 }
 
+// The smallest levels (fused, one type, at most NYL_TAIL_MAX_CELLS cells,
+// all of them with the coarsest level within the tail kernel's LDS) and the
+// coarsest level run in one single-workgroup launch (k_mg_tail): below a
+// few hundred cells a level's launches cost more than its work, while one
+// workgroup's issue rate makes a larger level cheaper as separate launches.
+#ifndef NYL_TAIL
+#define NYL_TAIL 1
+#endif
+#ifndef NYL_TAIL_MAX_CELLS
+#define NYL_TAIL_MAX_CELLS TAIL_T
+#endif
+// first level of the tail, -1 when there is none
+int tail_level(const nyl_ctx *c) {
+    const int L = c->nlev;
+    if (!NYL_TAIL || c->exact || L < 3 || L > NYL_TAIL_MAXLEV) return -1;
+    auto lds = [&](int l) {
+        const int64_t n = (int64_t)cells(c->lev[l].g) * (lev64(c, l) ? 8 : 4);
+        return 3.0 * (double)((n + 15) / 16 * 16);
+    };
+    double used = lds(L - 1);
+    int t0 = -1;
+    for (int l = L - 2; l >= 1; l--) {
+        used += lds(l);
+        if (!fused_level(c, l) || used > NYL_TAIL_LDS || cells(c->lev[l].g) > NYL_TAIL_MAX_CELLS ||
+            lev64(c, l) != lev64(c, L - 2))
+            break;
+        t0 = l;
+    }
+    return t0;
+}
+
+int tail(nyl_ctx *c, int t0) {
+    const int L = c->nlev;
+    TailArgs a;
+    memset(&a, 0, sizeof(a));
+    a.ws = c->ws;
+    double bytes = 0.0;
+    for (int l = 0; l < L; l++) {
+        const Level &v = c->lev[l];
+        a.lev[l].g = v.g;
+        a.lev[l].x = v.x;
+        a.lev[l].b = v.b;
+        a.lev[l].y = v.y;
+        if (l >= t0) bytes += 3.0 * cells(v.g) * (lev64(c, l) ? 8.0 : 4.0);
+    }
+    a.l0 = t0;
+    a.nlev = L;
+    a.omega = c->d.mg_omega;
+    a.tab = c->tabs;
+    const bool t64 = lev64(c, t0), c64 = lev64(c, L - 1);
+    if (t64 && c64) LAUNCH(NYL_K_MG_COARSE, bytes, (launch_mg_tail<double, double>(a, c->stream)));
+    else if (!t64 && c64) LAUNCH(NYL_K_MG_COARSE, bytes, (launch_mg_tail<float, double>(a, c->stream)));
+    else if (!t64 && !c64) LAUNCH(NYL_K_MG_COARSE, bytes, (launch_mg_tail<float, float>(a, c->stream)));
+    else return fail(NYL_E_STATE, "FP64 level above an FP32 level");
+    return NYL_OK;
+}
+
 // the part of a V-cycle below level 1: from level 2's smoothing (its first
 // sweep written by the restriction from level 1) down to the coarsest level
 // and back up to level 2's post-smoothing
 int coarse_cycle(nyl_ctx *c) {
     const int L = c->nlev;
+    const int t0 = tail_level(c);
+    const int down_end = t0 >= 0 ? t0 : L - 1;   // levels [1, down_end) run here
     int rc;
-    for (int l = 1; l < L - 1; l++) {
+    for (int l = 1; l < down_end; l++) {
         if (fused_level(c, l)) {
             if ((rc = fused_down(c, l))) return rc;
             continue;
@@ -536,8 +595,12 @@ int coarse_cycle(nyl_ctx *c) {
         if ((rc = smooth_level(c, l, -1, true))) return rc;
         if ((rc = restrict_level(c, l))) return rc;
     }
-    if ((rc = coarsest(c))) return rc;
-    for (int l = L - 2; l >= 1; l--) {
+    if (t0 >= 0) {
+        if ((rc = tail(c, t0))) return rc;
+    } else if ((rc = coarsest(c))) {
+        return rc;
+    }
+    for (int l = down_end - 1; l >= 1; l--) {
         if (fused_level(c, l)) {
             if ((rc = fused_up(c, l))) return rc;
             continue;

@@ -215,6 +215,44 @@ bool restrict25_fits(const Geo &g);
 template <typename Tf, typename Tc>
 hipError_t launch_mg_prolong25(const Geo &gf, const Geo &gc, Tf *xf, const Tc *xc,
                                const MGTabs &tab, hipStream_t s, Tf *yf = nullptr);
+// The smallest levels of a V-cycle in one workgroup (k_mg_tail): levels
+// l0 .. nlev-2 fused (exact halving), the coarsest smoothed (no direct
+// solve).  x, b, y: each level's fields in the workspace ws (of level l0,
+// b and y are read and y written); lx, lb, ly: their places in the kernel's
+// LDS, where all the tail's levels live (tail_layout).
+#define NYL_TAIL_MAXLEV 16
+#define TAIL_T 512
+#define NYL_TAIL_LDS 65536
+struct TailLevel {
+    Geo g;
+    uint64_t x, b, y;
+    uint32_t lx, lb, ly;
+};
+struct TailArgs {
+    char *ws;
+    TailLevel lev[NYL_TAIL_MAXLEV];
+    int l0, nlev;
+    double omega;
+    MGTabs tab;
+};
+// LDS places of levels l0 .. nlev-1 (es: bytes per value of the fused
+// levels, esc: of the coarsest); returns the bytes used
+inline uint64_t tail_layout(TailArgs &a, int es, int esc) {
+    uint64_t o = 0;
+    for (int l = a.l0; l < a.nlev; l++) {
+        const TailLevel &v = a.lev[l];
+        const uint64_t n = (uint64_t)v.g.nx * v.g.ny * v.g.nz * (l == a.nlev - 1 ? esc : es);
+        const uint64_t step = (n + 15) / 16 * 16;
+        a.lev[l].lx = (uint32_t)o;
+        a.lev[l].lb = (uint32_t)(o + step);
+        a.lev[l].ly = (uint32_t)(o + 2 * step);
+        o += 3 * step;
+    }
+    return o;
+}
+template <typename T, typename TC>
+hipError_t launch_mg_tail(const TailArgs &a, hipStream_t s);
+
 // two consecutive tiled sweeps src -> (tmp) -> dst in one pass (src != dst)
 template <typename T>
 hipError_t launch_mg_sweep2x25(const Geo &g, const T *src, T *dst, const T *b, double omega,

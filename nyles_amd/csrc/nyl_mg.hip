@@ -1336,6 +1336,227 @@ NYL_FUSED_INST(float, float)
 NYL_FUSED_INST(float, double)
 #undef NYL_FUSED_INST
 
+// ---------------------------------------------------------------------------
+// The tail's cell forms on a dense level in LDS: every neighbour index is a
+// valid cell (periodic axes wrapped, a wall neighbour clamped to the cell
+// itself) and the wall's zero is selected after the load, so a cell issues
+// all its loads at once instead of one branch per neighbour.  The values and
+// the order of the sums are those of neighbours(), sweep_value(),
+// residual_at() and prolong_term().
+template <typename T>
+struct Six {
+    T v[6];   // xm, xp, ym, yp, zm, zp
+    int nw;
+};
+template <typename T>
+__device__ __forceinline__ Six<T> six(const Geo &g, const T *a, int i, int j, int k, int c) {
+    const int sy = (int)g.sy, sz = (int)g.sz;
+    const bool bxm = g.px || i > 0, bxp = g.px || i < g.nx - 1;
+    const bool bym = g.py || j > 0, byp = g.py || j < g.ny - 1;
+    const bool bzm = g.pz || k > 0, bzp = g.pz || k < g.nz - 1;
+    const int im = g.px ? wrapi(i - 1, g.nx) : (bxm ? i - 1 : i);
+    const int ip = g.px ? wrapi(i + 1, g.nx) : (bxp ? i + 1 : i);
+    const int jm = g.py ? wrapi(j - 1, g.ny) : (bym ? j - 1 : j);
+    const int jp = g.py ? wrapi(j + 1, g.ny) : (byp ? j + 1 : j);
+    const int km = g.pz ? wrapi(k - 1, g.nz) : (bzm ? k - 1 : k);
+    const int kp = g.pz ? wrapi(k + 1, g.nz) : (bzp ? k + 1 : k);
+    const T xm = a[c + im - i], xp = a[c + ip - i];
+    const T ym = a[c + (jm - j) * sy], yp = a[c + (jp - j) * sy];
+    const T zm = a[c + (km - k) * sz], zp = a[c + (kp - k) * sz];
+    Six<T> r;
+    r.v[0] = bxm ? xm : T(0);
+    r.v[1] = bxp ? xp : T(0);
+    r.v[2] = bym ? ym : T(0);
+    r.v[3] = byp ? yp : T(0);
+    r.v[4] = bzm ? zm : T(0);
+    r.v[5] = bzp ? zp : T(0);
+    r.nw = (int)bxm + (int)bxp + (int)bym + (int)byp + (int)bzm + (int)bzp;
+    return r;
+}
+
+template <typename T>
+__device__ __forceinline__ T tail_sweep(const Geo &g, const T *src, const T *b, double omega,
+                                        const MGTabs &tab, int i, int j, int k, int c) {
+    const Six<T> n = six<T>(g, src, i, j, k, c);
+    const T S = n.v[0] + n.v[1] + n.v[2] + n.v[3] + n.v[4] + n.v[5];
+    const T om = T(omega), cff1 = T(1.0 - omega);
+    return cff1 * src[c] + om * (S - b[c]) * T(tab.idiag[n.nw]);
+}
+
+template <typename T>
+__device__ __forceinline__ T tail_resid(const Geo &g, const T *x, const T *b, const MGTabs &tab,
+                                        int i, int j, int k) {
+    const int c = k * (int)g.sz + j * (int)g.sy + i;
+    const Six<T> n = six<T>(g, x, i, j, k, c);
+    return b[c] + T(tab.diag[n.nw]) * x[c] - (n.v[0] + n.v[1] + n.v[2] + n.v[3] + n.v[4] + n.v[5]);
+}
+
+template <typename Tf, typename Tc>
+__device__ __forceinline__ void tail_restrict(const Geo &gf, const Geo &gc, const Tf *x,
+                                              const Tf *b, Tc *bc, Tc *yc, double omega,
+                                              const MGTabs &tab, int ic, int jc, int kc) {
+    const int i = 2 * ic, j = 2 * jc, k = 2 * kc;
+    const Tf r000 = tail_resid<Tf>(gf, x, b, tab, i, j, k);
+    const Tf r100 = tail_resid<Tf>(gf, x, b, tab, i + 1, j, k);
+    const Tf r010 = tail_resid<Tf>(gf, x, b, tab, i, j + 1, k);
+    const Tf r110 = tail_resid<Tf>(gf, x, b, tab, i + 1, j + 1, k);
+    const Tf r001 = tail_resid<Tf>(gf, x, b, tab, i, j, k + 1);
+    const Tf r101 = tail_resid<Tf>(gf, x, b, tab, i + 1, j, k + 1);
+    const Tf r011 = tail_resid<Tf>(gf, x, b, tab, i, j + 1, k + 1);
+    const Tf r111 = tail_resid<Tf>(gf, x, b, tab, i + 1, j + 1, k + 1);
+    const int cc = kc * (int)gc.sz + jc * (int)gc.sy + ic;
+    const Tc bv = Tc(Tf(0.5) * (r000 + r100 + r010 + r110 + r001 + r101 + r011 + r111));
+    bc[cc] = bv;
+    yc[cc] = sweep_from_zero<Tc>(gc, bv, omega, tab, ic, jc, kc);
+}
+
+template <typename Tf, typename Tc>
+__device__ __forceinline__ Tf tail_prolong(const Geo &gc, const Tc *xc, const MGTabs &tab, int i,
+                                           int j, int k) {
+    const int ic = i >> 1, jc = j >> 1, kc = k >> 1;
+    int icn = (i & 1) ? ic + 1 : ic - 1;
+    int jcn = (j & 1) ? jc + 1 : jc - 1;
+    int kcn = (k & 1) ? kc + 1 : kc - 1;
+    int mx = 1, my = 1, mz = 1;
+    if (gc.px) icn = wrapi(icn, gc.nx); else if (icn < 0 || icn >= gc.nx) mx = 0;
+    if (gc.py) jcn = wrapi(jcn, gc.ny); else if (jcn < 0 || jcn >= gc.ny) my = 0;
+    if (gc.pz) kcn = wrapi(kcn, gc.nz); else if (kcn < 0 || kcn >= gc.nz) mz = 0;
+    const int il = mx ? icn : ic, jl = my ? jcn : jc, kl = mz ? kcn : kc;
+    const int sy = (int)gc.sy, sz = (int)gc.sz;
+    const Tf a0 = Tf(xc[kc * sz + jc * sy + ic]), a1 = Tf(xc[kc * sz + jc * sy + il]);
+    const Tf a2 = Tf(xc[kc * sz + jl * sy + ic]), a3 = Tf(xc[kc * sz + jl * sy + il]);
+    const Tf c0 = Tf(xc[kl * sz + jc * sy + ic]), c1 = Tf(xc[kl * sz + jc * sy + il]);
+    const Tf c2 = Tf(xc[kl * sz + jl * sy + ic]), c3 = Tf(xc[kl * sz + jl * sy + il]);
+    const bool bx = mx != 0, by = my != 0, bz = mz != 0;
+    const Tf bmid = Tf(9) * a0 + Tf(3) * (bx ? a1 : Tf(0)) + Tf(3) * (by ? a2 : Tf(0)) +
+                    ((bx && by) ? a3 : Tf(0));
+    const Tf aside = Tf(9) * (bz ? c0 : Tf(0)) + Tf(3) * ((bz && bx) ? c1 : Tf(0)) +
+                     Tf(3) * ((bz && by) ? c2 : Tf(0)) + ((bz && bx && by) ? c3 : Tf(0));
+    return Tf(tab.pcoef[mx + 2 * my + 4 * mz]) * (Tf(3) * bmid + aside);
+}
+
+// The smallest levels of a V-cycle in one workgroup (k_mg_tail): the down
+// legs of the fused levels l0 .. L-2 (second pre-sweep y -> x, residual
+// restriction into level l+1's b and its first sweep from zero into y), the
+// coarsest level's second sweep (y -> x), and the up legs (x + P(xc) -> y,
+// then the two post-sweeps y -> x -> y, so the level's correction ends in y
+// as after k_mg_up).  All the levels live in LDS: level l0's b and y are
+// read once, its correction y written once, and nothing else of these
+// levels touches memory.  Each phase runs the per-cell kernels' expressions (sweep_cell,
+// restrict_cell, prolong_term, in the branch-free forms above) over the
+// level and a barrier separates the phases, so the result is bitwise that
+// of the separate launches: one launch instead of 2 per level + 1, which
+// dominate at these sizes.  One workgroup issues 64 lanes a cycle, so the
+// tail pays only for levels of at most a few hundred cells (a 16^3 level
+// costs more here than in its two launches across the chip).  T: the fused
+// levels' type, TC: the coarsest level's.
+template <typename T, typename TC>
+__global__ __launch_bounds__(TAIL_T) void k_mg_tail(TailArgs a) {
+    __shared__ double lds_d[NYL_TAIL_LDS / sizeof(double)];
+    __shared__ MGTabs tab;   // the coefficient tables, indexed per lane
+    char *lds = reinterpret_cast<char *>(lds_d);
+    const int tid = threadIdx.x;
+    if (tid < 7) {
+        tab.idiag[tid] = a.tab.idiag[tid];
+        tab.diag[tid] = a.tab.diag[tid];
+    }
+    if (tid < 8) tab.pcoef[tid] = a.tab.pcoef[tid];
+    auto S = [&](uint32_t off) { return reinterpret_cast<T *>(lds + off); };
+    auto SC = [&](uint32_t off) { return reinterpret_cast<TC *>(lds + off); };
+    auto each = [&](const Geo &g, auto fn) {
+        const int nx = g.nx, nxy = g.nx * g.ny, n = nxy * g.nz;
+#pragma nounroll
+        for (int q = tid; q < n; q += TAIL_T) {
+            const int k = q / nxy, r = q - k * nxy, j = r / nx, i = r - j * nx;
+            fn(i, j, k, k * (int)g.sz + j * (int)g.sy + i);
+        }
+    };
+    const int L = a.nlev;
+    {
+        const TailLevel &t = a.lev[a.l0];
+        const T *gy = reinterpret_cast<const T *>(a.ws + t.y);
+        const T *gb = reinterpret_cast<const T *>(a.ws + t.b);
+        each(t.g, [&](int, int, int, int c) {
+            const T yv = gy[c], bv = gb[c];
+            S(t.ly)[c] = yv;
+            S(t.lb)[c] = bv;
+        });
+        __syncthreads();
+    }
+    for (int l = a.l0; l <= L - 2; l++) {
+        const TailLevel &f = a.lev[l], &q = a.lev[l + 1];
+        T *x = S(f.lx);
+        const T *y = S(f.ly), *b = S(f.lb);
+        each(f.g, [&](int i, int j, int k, int c) {
+            x[c] = tail_sweep<T>(f.g, y, b, a.omega, tab, i, j, k, c);
+        });
+        __syncthreads();
+        if (l + 1 == L - 1) {
+            each(q.g, [&](int i, int j, int k, int) {
+                tail_restrict<T, TC>(f.g, q.g, x, b, SC(q.lb), SC(q.ly), a.omega, tab, i, j, k);
+            });
+        } else {
+            each(q.g, [&](int i, int j, int k, int) {
+                tail_restrict<T, T>(f.g, q.g, x, b, S(q.lb), S(q.ly), a.omega, tab, i, j, k);
+            });
+        }
+        __syncthreads();
+    }
+    {
+        const TailLevel &c = a.lev[L - 1];
+        TC *cx = SC(c.lx);
+        each(c.g, [&](int i, int j, int k, int q) {
+            cx[q] = tail_sweep<TC>(c.g, SC(c.ly), SC(c.lb), a.omega, tab, i, j, k, q);
+        });
+        __syncthreads();
+    }
+    for (int l = L - 2; l >= a.l0; l--) {
+        const TailLevel &f = a.lev[l], &q = a.lev[l + 1];
+        T *x = S(f.lx), *y = S(f.ly);
+        const T *b = S(f.lb);
+        if (l + 1 == L - 1) {
+            const TC *xc = SC(q.lx);
+            each(f.g, [&](int i, int j, int k, int c) {
+                y[c] = x[c] + tail_prolong<T, TC>(q.g, xc, tab, i, j, k);
+            });
+        } else {
+            const T *xc = S(q.ly);
+            each(f.g, [&](int i, int j, int k, int c) {
+                y[c] = x[c] + tail_prolong<T, T>(q.g, xc, tab, i, j, k);
+            });
+        }
+        __syncthreads();
+        each(f.g, [&](int i, int j, int k, int c) {
+            x[c] = tail_sweep<T>(f.g, y, b, a.omega, tab, i, j, k, c);
+        });
+        __syncthreads();
+        if (l > a.l0) {
+            each(f.g, [&](int i, int j, int k, int c) {
+                y[c] = tail_sweep<T>(f.g, x, b, a.omega, tab, i, j, k, c);
+            });
+            __syncthreads();
+        }
+    }
+    // level l0's last sweep goes straight to memory
+    const TailLevel &t = a.lev[a.l0];
+    T *gy = reinterpret_cast<T *>(a.ws + t.y);
+    each(t.g, [&](int i, int j, int k, int c) {
+        gy[c] = tail_sweep<T>(t.g, S(t.lx), S(t.lb), a.omega, tab, i, j, k, c);
+    });
+}
+
+template <typename T, typename TC>
+hipError_t launch_mg_tail(const TailArgs &a, hipStream_t s) {
+    if (a.nlev > NYL_TAIL_MAXLEV || a.l0 < 0 || a.l0 > a.nlev - 2) return hipErrorInvalidValue;
+    TailArgs t = a;
+    if (tail_layout(t, sizeof(T), sizeof(TC)) > NYL_TAIL_LDS) return hipErrorInvalidValue;
+    NYL_LAUNCH((k_mg_tail<T, TC>), dim3(1), dim3(TAIL_T), 0, s, t);
+    return hipGetLastError();
+}
+template hipError_t launch_mg_tail<double, double>(const TailArgs &, hipStream_t);
+template hipError_t launch_mg_tail<float, double>(const TailArgs &, hipStream_t);
+template hipError_t launch_mg_tail<float, float>(const TailArgs &, hipStream_t);
+
 bool sweep25_fits(const Geo &g) {
     return g.nx % SW_TX == 0 && g.ny % SW_TY == 0 && g.nx >= SW_TX;
 }

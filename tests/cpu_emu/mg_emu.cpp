@@ -137,6 +137,103 @@ int check_fused(int nx, int ny, int nz, int px, int py, int pz) {
     return bad != 0;
 }
 
+// The single-workgroup tail (down legs of levels l0 .. L-2, the coarsest
+// level's second sweep, the up legs) against the fused launches it replaces:
+// the same correction y of level l0, and nothing else written to memory (the
+// levels below l0 live in the kernel's LDS).
+template <typename T, typename TC>
+int check_tail(int nx, int ny, int nz, int px, int py, int pz, int L, int l0) {
+    MGTabs tab = mktab();
+    const double om = 0.9;
+    std::vector<Geo> g(L);
+    std::vector<uint64_t> off(3 * L);
+    uint64_t bytes = 0;
+    for (int l = 0; l < L; l++) {
+        g[l] = mk(nx >> l, ny >> l, nz >> l, px, py, pz);
+        const uint64_t es = l == L - 1 ? sizeof(TC) : sizeof(T);
+        for (int f = 0; f < 3; f++) {
+            off[3 * l + f] = bytes;
+            bytes += ((uint64_t)(nx >> l) * (ny >> l) * (nz >> l) * es + 255) / 256 * 256;
+        }
+    }
+    std::mt19937_64 rng(nx + 5 * L + 11 * l0 + px + 2 * py + 4 * pz);
+    std::vector<char> w1(bytes), w2;
+    for (int l = 0; l < L; l++) {
+        const size_t n = (size_t)(nx >> l) * (ny >> l) * (nz >> l);
+        for (int f = 0; f < 3; f++) {
+            if (l == L - 1) {
+                auto v = rnd<TC>(n, rng);
+                memcpy(w1.data() + off[3 * l + f], v.data(), n * sizeof(TC));
+            } else {
+                auto v = rnd<T>(n, rng);
+                memcpy(w1.data() + off[3 * l + f], v.data(), n * sizeof(T));
+            }
+        }
+    }
+    w2 = w1;
+    const std::vector<char> w0 = w1;
+    char *ws = w1.data();
+    auto P = [&](int l, int f) { return reinterpret_cast<T *>(ws + off[3 * l + f]); };
+    auto PC = [&](int l, int f) { return reinterpret_cast<TC *>(ws + off[3 * l + f]); };
+    // fields: 0 = x, 1 = b, 2 = y
+    for (int l = l0; l <= L - 2; l++) {
+        if (l + 1 == L - 1)
+            run_blocks(fused_grid(g[l]), dim3(FUSED_T, 1, 1), [&]() {
+                k_mg_down<T, TC>(g[l], g[l + 1], P(l, 2), P(l, 1), P(l, 0), PC(l + 1, 1),
+                                 PC(l + 1, 2), om, tab); }, true);
+        else
+            run_blocks(fused_grid(g[l]), dim3(FUSED_T, 1, 1), [&]() {
+                k_mg_down<T, T>(g[l], g[l + 1], P(l, 2), P(l, 1), P(l, 0), P(l + 1, 1),
+                                P(l + 1, 2), om, tab); }, true);
+    }
+    run_blocks(mg_grid(g[L - 1]), dim3(MG_BX, MG_BY, 1), [&]() {
+        k_mg_sweep<TC>(g[L - 1], PC(L - 1, 2), PC(L - 1, 0), PC(L - 1, 1), om, tab); }, false);
+    for (int l = L - 2; l >= l0; l--) {
+        if (l + 1 == L - 1)
+            run_blocks(fused_grid(g[l]), dim3(FUSED_T, 1, 1), [&]() {
+                k_mg_up<T, TC>(g[l], g[l + 1], P(l, 0), PC(l + 1, 0), P(l, 1), P(l, 2), om,
+                               tab); }, true);
+        else
+            run_blocks(fused_grid(g[l]), dim3(FUSED_T, 1, 1), [&]() {
+                k_mg_up<T, T>(g[l], g[l + 1], P(l, 0), P(l + 1, 2), P(l, 1), P(l, 2), om,
+                              tab); }, true);
+    }
+    TailArgs a;
+    memset(&a, 0, sizeof(a));
+    a.ws = w2.data();
+    for (int l = 0; l < L; l++) {
+        a.lev[l].g = g[l];
+        a.lev[l].x = off[3 * l];
+        a.lev[l].b = off[3 * l + 1];
+        a.lev[l].y = off[3 * l + 2];
+    }
+    a.l0 = l0;
+    a.nlev = L;
+    a.omega = om;
+    a.tab = tab;
+    if (tail_layout(a, sizeof(T), sizeof(TC)) > NYL_TAIL_LDS) return 1;
+    run_blocks(dim3(1), dim3(TAIL_T, 1, 1), [&]() { k_mg_tail<T, TC>(a); }, true);
+    int bad = 0;
+    auto cmp = [&](int l, int f) {
+        const size_t n = (size_t)(nx >> l) * (ny >> l) * (nz >> l);
+        const size_t es = l == L - 1 ? sizeof(TC) : sizeof(T);
+        bad += memcmp(w1.data() + off[3 * l + f], w2.data() + off[3 * l + f], n * es) != 0;
+    };
+    cmp(l0, 2);
+    for (int l = 0; l < L; l++) {   // nothing else in memory is written
+        for (int f = 0; f < 3; f++) {
+            if (l == l0 && f == 2) continue;
+            const size_t n = (size_t)(nx >> l) * (ny >> l) * (nz >> l);
+            const size_t es = l == L - 1 ? sizeof(TC) : sizeof(T);
+            bad += memcmp(w2.data() + off[3 * l + f], w0.data() + off[3 * l + f], n * es) != 0;
+        }
+    }
+    printf("tail<%s,%s> %dx%dx%d L=%d l0=%d p=%d%d%d: %d mismatches\n",
+           sizeof(T) == 8 ? "f64" : "f32", sizeof(TC) == 8 ? "f64" : "f32", nx, ny, nz, L, l0,
+           px, py, pz, bad);
+    return bad != 0;
+}
+
 // FP64 outer residual of the mixed solve: tiled (with and without the sum
 // of b**2) against per-cell; the partial sums add up to the same totals
 int check_outer(int nx, int ny, int nz, int px, int py, int pz) {
@@ -184,6 +281,12 @@ int main() {
             fails += check_fused<double, double>(s[0], s[1], s[2], px, py, pz);
             fails += check_fused<float, float>(s[0], s[1], s[2], px, py, pz);
             fails += check_fused<float, double>(s[0], s[1], s[2], px, py, pz);
+        }
+        if (p == 0 || p == 3 || p == 4 || p == 7) {
+            fails += check_tail<double, double>(32, 32, 32, px, py, pz, 5, 2);
+            fails += check_tail<double, double>(16, 8, 8, px, py, pz, 4, 1);
+            fails += check_tail<float, double>(32, 32, 16, px, py, pz, 5, 2);
+            fails += check_tail<float, float>(32, 32, 32, px, py, pz, 5, 1);
         }
     }
     printf(fails ? "FAIL %d\n" : "ALL OK\n", fails);
