@@ -154,6 +154,24 @@ hipError_t launch_mg_sweep(const Geo &g, const T *src, T *dst, const T *b, doubl
 // copy n doubles to coherent host memory, then store seq into flag
 hipError_t launch_publish(const double *src, int n, double *dst, uint32_t *flag, uint32_t seq,
                           hipStream_t s);
+// A request to publish n device results at src (copy to dst in coherent host
+// memory, then seq into flag at system scope) from the final-sum launch that
+// writes the last of them, src[n-1]: host code that polls for the results
+// right after the launches arms it, saving the separate k_publish launch.
+// The final-sum launch that matches the request by address takes it
+// (consumed); an unmatched request is published by k_publish as before.
+struct Publish {
+    const double *src;
+    int n;
+    double *dst;
+    uint32_t *flag;
+    uint32_t seq;
+    int consumed;
+};
+inline thread_local Publish g_publish = {};
+// the second stage of a deterministic sum (one block over np partials),
+// publishing the result when it completes the armed request
+hipError_t launch_final_sum(const double *partials, int np, double *out, hipStream_t s);
 // Fused level kernels (below the finest level).  down: x = sweep(y), then
 // the residual b - A x restricted into bc and the coarse first sweep from
 // zero into yc.  up: y = sweep(sweep(x + P xc)); x and xc are only read.

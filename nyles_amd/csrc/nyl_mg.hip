@@ -531,11 +531,30 @@ static inline int red_blocks(int64_t n) {
     return (int)(b < NYL_RED_BLOCKS ? (b < 1 ? 1 : b) : NYL_RED_BLOCKS);
 }
 
+// pub.dst non-null: thread 0, which stored *out = pub.src[pub.n - 1], copies
+// pub.src[0 .. n) to coherent host memory and then stores the sequence
+// number (the other results were written by earlier kernels)
 __global__ __launch_bounds__(RED_T) void k_final_sum(const double *__restrict__ partials, int np,
-                                                     double *__restrict__ out) {
+                                                     double *out, Publish pub) {
     double acc = 0.0;
     for (int q = threadIdx.x; q < np; q += RED_T) acc = acc + partials[q];
     block_sum_store(acc, out, threadIdx.x);
+    if (pub.dst && threadIdx.x == 0) {
+        for (int q = 0; q < pub.n; q++) pub.dst[q] = pub.src[q];
+        __threadfence_system();
+        __hip_atomic_store(pub.flag, pub.seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+}
+
+hipError_t launch_final_sum(const double *partials, int np, double *out, hipStream_t s) {
+    Publish pub = {};
+    Publish &r = g_publish;
+    if (r.dst && !r.consumed && r.n >= 1 && out == r.src + (r.n - 1)) {
+        pub = r;
+        r.consumed = 1;
+    }
+    NYL_LAUNCH(k_final_sum, dim3(1), dim3(RED_T), 0, s, partials, np, out, pub);
+    return hipGetLastError();
 }
 
 __global__ __launch_bounds__(RED_T) void k_final_max(const double *__restrict__ partials, int np,
@@ -563,7 +582,7 @@ hipError_t launch_norm2(const Geo &g, const T *v, double *partials, double *dev_
     const int64_t n = (int64_t)g.nx * g.ny * g.nz;
     const int nb = red_blocks(n);
     NYL_LAUNCH((k_norm2<T>), dim3(nb), dim3(RED_T), 0, s, n, v, partials);
-    NYL_LAUNCH(k_final_sum, dim3(1), dim3(RED_T), 0, s, partials, nb, dev_out);
+    (void)launch_final_sum(partials, nb, dev_out, s);
     return hipGetLastError();
 }
 
@@ -591,7 +610,7 @@ hipError_t launch_resnorm2(const Geo &g, const T *x, const T *b, const MGTabs &t
     const int64_t n = (int64_t)g.nx * g.ny * g.nz;
     const int nb = red_blocks(n);
     NYL_LAUNCH((k_resnorm2<T>), dim3(nb), dim3(RED_T), 0, s, g, x, b, tab, partials);
-    NYL_LAUNCH(k_final_sum, dim3(1), dim3(RED_T), 0, s, partials, nb, dev_out);
+    (void)launch_final_sum(partials, nb, dev_out, s);
     return hipGetLastError();
 }
 
@@ -641,7 +660,7 @@ hipError_t launch_sum(const Geo &g, const T *v, double *partials, double *dev_ou
     const int64_t n = (int64_t)g.nx * g.ny * g.nz;
     const int nb = red_blocks(n);
     NYL_LAUNCH((k_sum<T>), dim3(nb), dim3(RED_T), 0, s, n, v, partials);
-    NYL_LAUNCH(k_final_sum, dim3(1), dim3(RED_T), 0, s, partials, nb, dev_out);
+    (void)launch_final_sum(partials, nb, dev_out, s);
     return hipGetLastError();
 }
 
@@ -668,7 +687,7 @@ hipError_t launch_nonfinite(const Geo &g, const T *a, const T *b, const T *c, co
     const int64_t n = (int64_t)g.nx * g.ny * g.nz;
     const int nb = red_blocks(n);
     NYL_LAUNCH((k_nonfinite<T>), dim3(nb), dim3(RED_T), 0, s, n, a, b, c, d, partials);
-    NYL_LAUNCH(k_final_sum, dim3(1), dim3(RED_T), 0, s, partials, nb, dev_out);
+    (void)launch_final_sum(partials, nb, dev_out, s);
     return hipGetLastError();
 }
 
@@ -766,8 +785,8 @@ hipError_t launch_sum_sumsq(const Geo &g, const double *v, double *partials, dou
     int nb = red_blocks(n);
     if (nb > NYL_RED_BLOCKS / 2) nb = NYL_RED_BLOCKS / 2;
     NYL_LAUNCH(k_sum_sumsq, dim3(nb), dim3(RED_T), 0, s, n, v, partials, nb);
-    NYL_LAUNCH(k_final_sum, dim3(1), dim3(RED_T), 0, s, partials, nb, dev_out);
-    NYL_LAUNCH(k_final_sum, dim3(1), dim3(RED_T), 0, s, partials + nb, nb, dev_out + 1);
+    (void)launch_final_sum(partials, nb, dev_out, s);
+    (void)launch_final_sum(partials + nb, nb, dev_out + 1, s);
     return hipGetLastError();
 }
 
@@ -831,7 +850,7 @@ hipError_t launch_kesum(const Geo &g, const T *u, const T *v, const T *w, double
     const int nb = red_blocks(n);
     NYL_LAUNCH((k_kesum<T>), dim3(nb), dim3(RED_T), 0, s, g, u, v, w, idx2, idy2, idz2,
                        partials);
-    NYL_LAUNCH(k_final_sum, dim3(1), dim3(RED_T), 0, s, partials, nb, dev_out);
+    (void)launch_final_sum(partials, nb, dev_out, s);
     return hipGetLastError();
 }
 
@@ -1584,7 +1603,7 @@ hipError_t launch_mg_sweep25(const Geo &g, const T *src, T *dst, const T *b, dou
         NYL_LAUNCH((k_mg_sweep25<T, true, false, false>), grd, blk, 0, s, g, src, dst, b,
                            omega, tab, kz, partials, gcv, xc, addx);
         const int nb = grd.x * grd.y * grd.z;
-        NYL_LAUNCH(k_final_sum, dim3(1), dim3(RED_T), 0, s, partials, nb, dev_out);
+        (void)launch_final_sum(partials, nb, dev_out, s);
     } else if (addx) {
         NYL_LAUNCH((k_mg_sweep25<T, false, false, true>), grd, blk, 0, s, g, src, dst, b,
                            omega, tab, kz, partials, gcv, xc, addx);
@@ -1715,9 +1734,9 @@ hipError_t launch_mg_outer_resid(const Geo &g, const double *x, const double *b,
         double *pb = with_bsq ? partials + nb : nullptr;
         NYL_LAUNCH((k_mg_outer25<TI>), grd, blk, 0, s, g, x, b, shift, r, y, omega, tab,
                            kz, partials, pb);
-        NYL_LAUNCH(k_final_sum, dim3(1), dim3(RED_T), 0, s, partials, nb, dev_out);
+        (void)launch_final_sum(partials, nb, dev_out, s);
         if (with_bsq)
-            NYL_LAUNCH(k_final_sum, dim3(1), dim3(RED_T), 0, s, pb, nb, dev_out + 1);
+            (void)launch_final_sum(pb, nb, dev_out + 1, s);
         return hipGetLastError();
     }
     if (with_bsq) return hipErrorInvalidValue;
@@ -1725,7 +1744,7 @@ hipError_t launch_mg_outer_resid(const Geo &g, const double *x, const double *b,
     NYL_LAUNCH((k_mg_outer_resid<TI>), grd, dim3(MG_BX, MG_BY, 1), 0, s, g, x, b, shift,
                        r, y, omega, tab, partials);
     const int nb = grd.x * grd.y * grd.z;
-    NYL_LAUNCH(k_final_sum, dim3(1), dim3(RED_T), 0, s, partials, nb, dev_out);
+    (void)launch_final_sum(partials, nb, dev_out, s);
     return hipGetLastError();
 }
 
