@@ -309,11 +309,10 @@ void set_tendency_out(nyl_ctx *c, StageArgs<T> &a, int slot) {
 // residual of the mixed solve).
 // addx (level 0 of the mixed solve): the last sweep adds its result to the
 // FP64 solution instead of storing it (tiled levels), or an add follows.
-// Post-smoothing of the tiled FP64 levels as one two-sweep pass: src and b
-// are read once for two sweeps (256^3: 125 us against 2 x 77 us).  FP32
-// levels keep two sweeps: there the pass (92 VGPRs, two barriers per plane)
-// took 78 us against 2 x 32.5 us.  The pass can also add the second sweep's
-// result to an FP64 solution (the mixed solve's last sweep).
+// Post-smoothing of the tiled levels as one two-sweep pass after the
+// prolongation into y: src and b are read once for two sweeps (256^3:
+// 37.8 us FP32 and 93.4 us FP64 against 2 x 31.6 and 2 x 80 us).  The pass
+// can also add the second sweep's result to an FP64 solution.
 #ifndef NYL_SWEEP2
 #define NYL_SWEEP2 1
 #endif
@@ -361,8 +360,7 @@ int smooth_level_t(nyl_ctx *c, int l, int xc_level, bool pre_done, double *addx)
             return NYL_OK;
         }
         const double bp = 2.125 * cells(f.g) * sizeof(Tf);
-        if (NYL_SWEEP2 && std::is_same<Tf, double>::value && same && restrict25_fits(f.g) &&
-            tiled && !pre_done) {
+        if (NYL_SWEEP2 && same && restrict25_fits(f.g) && tiled && !pre_done) {
             // y = x + P(xc), then both post-smoothing sweeps y -> x in one
             // pass (with addx: the second sweep's result added to the FP64 x)
             Tf *yv = F<Tf>(c, f.y);

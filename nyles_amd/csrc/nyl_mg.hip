@@ -2048,23 +2048,285 @@ __global__ __launch_bounds__(256) void k_mg_sweep2x25(Geo g, const T *__restrict
     }
 }
 
-// planes per block of the two-sweep kernel: up to 32 dividing nz, keeping
-// at least 1024 blocks (each block re-reads four input planes, so the
-// longest march that still fills the chip is the fastest: 114 against
-// 119 us for 16 planes at 256^3 in FP64)
-static int sweep2x_kz(const Geo &g) {
+// Two consecutive sweeps (src -> tmp -> dst) in one 2.5-D march with the
+// structure of k_mg_sweep25 (k_mg_sweep2t).  A block owns a 64 x 8 tile and
+// thread (tx, ty) its rows ty and ty + 4; each thread keeps in registers the
+// z-windows of its input (planes k, k+1, k+2) and of the first sweep
+// (k-1, k, k+1), so only in-plane neighbours go through LDS.  The first sweep
+// is also needed on the one-cell ring around the tile: its 148 cells belong
+// to threads 0..147, which keep the input's z-window there too; the input's
+// second ring (156 cells, read in-plane only) belongs to threads 0..155.
+// Per plane: input plane k+1 (tile and both rings) into one of two LDS
+// slots, one barrier, the first sweep of plane k+1 on the tile and the ring
+// into one of two LDS slots, the second sweep of plane k on the tile (its
+// in-plane neighbours from the slot written one plane earlier).  Cells
+// outside the domain (wall axes) hold zero in both stages, as neighbours()
+// reads them, and every value is the per-cell sweep's expression, so dst is
+// bitwise that of two sweeps.  ADDX: the result is added to the FP64 xo
+// instead of stored.
+#define S2T_LX (SW_TX + 4)                       // input plane, two-cell ring
+#define S2T_LY (SW_TY + 4)
+#define S2T_MX (SW_TX + 2)                       // first-sweep plane, one-cell ring
+#define S2T_MY (SW_TY + 2)
+#define S2T_RING1 (2 * S2T_MX + 2 * SW_TY)       // 148 cells
+#define S2T_RING2 (2 * S2T_LX + 2 * (SW_TY + 2))  // 156 cells
+template <typename T, bool ADDX, int PZ>
+__global__ __launch_bounds__(256) void k_mg_sweep2t(Geo g, const T *__restrict__ src,
+                                                    T *__restrict__ dst,
+                                                    const T *__restrict__ b, double omega,
+                                                    MGTabs tab, int kz,
+                                                    double *__restrict__ xo) {
+    __shared__ T sin_[2][S2T_LY][S2T_LX];
+    __shared__ T smid[2][S2T_MY][S2T_MX];
+    const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * SW_TX + tx;
+    const Blk3 bk = xcd_block();
+    const int i0 = bk.x * SW_TX, j0 = bk.y * SW_TY, k0 = bk.z * kz;
+    const T om = T(omega), cff1 = T(1.0 - omega);
+    const int i = i0 + tx, sy = (int)g.sy;
+    // own cells: in-plane offsets, rows, inverse diagonals for 0, 1, 2
+    // in-domain z-neighbours
+    int off[SW_R], row[SW_R];
+    T id0[SW_R], id1[SW_R], id2[SW_R];
+#pragma unroll
+    for (int r = 0; r < SW_R; r++) {
+        row[r] = ty + 4 * r;
+        const int j = j0 + row[r];
+        off[r] = j * sy + i;
+        const int nxy = (g.px || i > 0) + (g.px || i < g.nx - 1) + (g.py || j > 0) +
+                        (g.py || j < g.ny - 1);
+        id0[r] = T(tab.idiag[nxy]);
+        id1[r] = T(tab.idiag[nxy + 1]);
+        id2[r] = T(tab.idiag[nxy + 2]);
+    }
+    // ring cell of the first sweep (threads 0..147): position (mx, my) in the
+    // first-sweep plane, (mx + 1, my + 1) in the input plane
+    const bool has1 = tid < S2T_RING1;
+    int mx = 0, my = 0;
+    if (tid < S2T_MX) { mx = tid; my = 0; }
+    else if (tid < 2 * S2T_MX) { mx = tid - S2T_MX; my = S2T_MY - 1; }
+    else { const int t = tid - 2 * S2T_MX; my = 1 + (t >> 1); mx = (t & 1) ? S2T_MX - 1 : 0; }
+    int off1 = 0;
+    bool in1 = false;
+    T rid0 = T(0), rid1 = T(0), rid2 = T(0);
+    if (has1) {
+        int gi = i0 - 1 + mx, gj = j0 - 1 + my;
+        bool in = true;
+        if (g.px) gi = wrapg(gi, g.nx); else in = in && gi >= 0 && gi < g.nx;
+        if (g.py) gj = wrapg(gj, g.ny); else in = in && gj >= 0 && gj < g.ny;
+        if (in) {
+            off1 = gj * sy + gi;
+            in1 = true;
+            const int nxy = (g.px || gi > 0) + (g.px || gi < g.nx - 1) + (g.py || gj > 0) +
+                            (g.py || gj < g.ny - 1);
+            rid0 = T(tab.idiag[nxy]);
+            rid1 = T(tab.idiag[nxy + 1]);
+            rid2 = T(tab.idiag[nxy + 2]);
+        }
+    }
+    // second ring of the input (threads 0..155): position (lx, ly)
+    const bool has2 = tid < S2T_RING2;
+    int lx2 = 0, ly2 = 0;
+    if (tid < S2T_LX) { lx2 = tid; ly2 = 0; }
+    else if (tid < 2 * S2T_LX) { lx2 = tid - S2T_LX; ly2 = S2T_LY - 1; }
+    else { const int t = tid - 2 * S2T_LX; ly2 = 1 + (t >> 1); lx2 = (t & 1) ? S2T_LX - 1 : 0; }
+    int off2 = 0;
+    bool in2 = false;
+    if (has2) {
+        int gi = i0 - 2 + lx2, gj = j0 - 2 + ly2;
+        bool in = true;
+        if (g.px) gi = wrapg(gi, g.nx); else in = in && gi >= 0 && gi < g.nx;
+        if (g.py) gj = wrapg(gj, g.ny); else in = in && gj >= 0 && gj < g.ny;
+        if (in) { off2 = gj * sy + gi; in2 = true; }
+    }
+
+    // z bookkeeping (PZ: periodic z, a compile-time flag so that the march
+    // carries no wrap arithmetic): in-domain test and in-domain z-neighbour
+    // count of plane m (the count only matters for in-domain planes)
+    auto zin = [&](int m) { return PZ || (m >= 0 && m < g.nz); };
+    auto nzw = [&](int m) { return PZ ? 2 : (m > 0) + (m < g.nz - 1); };
+    // a plane that can be loaded for plane m (the wrapped one; any plane
+    // outside a wall axis, whose values are not used), and the one for m + 1
+    auto pl = [&](int m) { return PZ ? wrapg(m, g.nz) : (zin(m) ? m : 0); };
+    auto nextp = [&](int q) { return PZ ? (q + 1 == g.nz ? 0 : q + 1) : (q + 1 < g.nz ? q + 1 : q); };
+    auto pick = [](int nz_, T a0, T a1, T a2) { return nz_ == 2 ? a2 : (nz_ == 1 ? a1 : a0); };
+    // loads from valid addresses; the values of out-of-domain cells are
+    // replaced by zero where they are used
+    // loads of plane q (a loadable plane index)
+    auto ld_in = [&](int q, T (&o)[SW_R], T &v1) {
+        const T *p = src + (int64_t)q * g.sz;
+#pragma unroll
+        for (int r = 0; r < SW_R; r++) o[r] = p[off[r]];
+        v1 = p[off1];
+    };
+    auto ld_b = [&](int q, T (&o)[SW_R], T &v1) {
+        const T *p = b + (int64_t)q * g.sz;
+#pragma unroll
+        for (int r = 0; r < SW_R; r++) o[r] = p[off[r]];
+        v1 = p[off1];
+    };
+    auto ld_in2 = [&](int q) { return src[(int64_t)q * g.sz + off2]; };
+    // input plane m into its LDS slot
+    auto put_in = [&](int m, const T (&o)[SW_R], T v1, T v2, bool ring2) {
+        const bool zi = zin(m);
+        T (*P)[S2T_LX] = sin_[m & 1];
+#pragma unroll
+        for (int r = 0; r < SW_R; r++) P[row[r] + 2][tx + 2] = zi ? o[r] : T(0);
+        if (has1) P[my + 1][mx + 1] = (zi && in1) ? v1 : T(0);
+        if (ring2 && has2) P[ly2][lx2] = (zi && in2) ? v2 : T(0);
+    };
+    // first sweep of plane m on the own cells (zm_, zp_: input planes m -+ 1)
+    auto mid_own = [&](int m, const T (&c_)[SW_R], const T (&zm_)[SW_R], const T (&zp_)[SW_R],
+                       const T (&bv)[SW_R], T (&out)[SW_R], bool store) {
+        const bool zi = zin(m), zl = zin(m - 1), zh = zin(m + 1);
+        const int nz_ = nzw(m);
+        const T (*P)[S2T_LX] = sin_[m & 1];
+#pragma unroll
+        for (int r = 0; r < SW_R; r++) {
+            const int ly = row[r] + 2, lx = tx + 2;
+            const T S = P[ly][lx - 1] + P[ly][lx + 1] + P[ly - 1][lx] + P[ly + 1][lx] +
+                        (zl ? zm_[r] : T(0)) + (zh ? zp_[r] : T(0));
+            const T v = cff1 * c_[r] + om * (S - bv[r]) * pick(nz_, id0[r], id1[r], id2[r]);
+            out[r] = zi ? v : T(0);
+            if (store) smid[m & 1][row[r] + 1][tx + 1] = out[r];
+        }
+    };
+    // first sweep of plane m on the ring cell
+    auto mid_ring = [&](int m, T c_, T zm_, T zp_, T bv) {
+        if (!has1) return;
+        const bool zi = zin(m), zl = zin(m - 1), zh = zin(m + 1);
+        const int nz_ = nzw(m);
+        const T (*P)[S2T_LX] = sin_[m & 1];
+        const int ly = my + 1, lx = mx + 1;
+        const T S = P[ly][lx - 1] + P[ly][lx + 1] + P[ly - 1][lx] + P[ly + 1][lx] +
+                    (zl ? zm_ : T(0)) + (zh ? zp_ : T(0));
+        const T v = cff1 * c_ + om * (S - bv) * pick(nz_, rid0, rid1, rid2);
+        smid[m & 1][my][mx] = (zi && in1) ? v : T(0);
+    };
+
+    // prologue: input planes k0-2 .. k0+2, first sweep of k0-1 (own) and k0
+    T A0[SW_R], A1[SW_R], A2[SW_R], A3[SW_R], A4[SW_R];
+    T a0, a1, a2, a3, a4;
+    ld_in(pl(k0 - 2), A0, a0);
+    ld_in(pl(k0 - 1), A1, a1);
+    ld_in(pl(k0), A2, a2);
+    ld_in(pl(k0 + 1), A3, a3);
+    ld_in(pl(k0 + 2), A4, a4);
+    const T o2a = ld_in2(pl(k0));
+    T o2 = ld_in2(pl(k0 + 1));
+    T Bp[SW_R], Bo[SW_R], Bm[SW_R], bp1, bo1, bm1;
+    ld_b(pl(k0 - 1), Bp, bp1);
+    ld_b(pl(k0), Bo, bo1);
+    ld_b(pl(k0 + 1), Bm, bm1);
+    int q3 = pl(k0 + 3), q2 = pl(k0 + 2);   // planes of the loads in the march
+    put_in(k0 - 1, A1, a1, T(0), false);
+    put_in(k0, A2, a2, o2a, true);
+    __syncthreads();
+    T Mm[SW_R], M0[SW_R], Mp[SW_R];
+    mid_own(k0 - 1, A1, A0, A2, Bp, Mm, false);
+    mid_own(k0, A2, A1, A3, Bo, M0, true);
+    mid_ring(k0, a2, a1, a3, bo1);
+    __syncthreads();              // plane k0+1 replaces plane k0-1, just read
+    // z-windows: input k, k+1, k+2 (own and ring), b of k (output) and k+1
+    T I0[SW_R], I1[SW_R], I2[SW_R];
+    T r0 = a2, r1 = a3, r2 = a4;
+#pragma unroll
+    for (int r = 0; r < SW_R; r++) { I0[r] = A2[r]; I1[r] = A3[r]; I2[r] = A4[r]; }
+    double xv[SW_R] = {};
+    if (ADDX) {
+#pragma unroll
+        for (int r = 0; r < SW_R; r++) xv[r] = xo[(int64_t)k0 * g.sz + off[r]];
+    }
+    for (int k = k0; k < k0 + kz; k++) {
+        put_in(k + 1, I1, r1, o2, true);
+        // loads for the next plane: input k+3, second ring k+2, b of k+2
+        T I3[SW_R] = {}, Bn[SW_R] = {}, r3 = T(0), bn1 = T(0), o2n = T(0);
+        double xvn[SW_R] = {};
+        const bool more = k + 1 < k0 + kz;
+        if (more) {
+            ld_in(q3, I3, r3);
+            o2n = ld_in2(q2);
+            ld_b(q2, Bn, bn1);
+            if (ADDX) {
+#pragma unroll
+                for (int r = 0; r < SW_R; r++) xvn[r] = xo[(int64_t)(k + 1) * g.sz + off[r]];
+            }
+        }
+        __syncthreads();
+        mid_own(k + 1, I1, I0, I2, Bm, Mp, true);
+        mid_ring(k + 1, r1, r0, r2, bm1);
+        // second sweep of plane k (in-plane neighbours from the slot of k)
+        {
+            const int nz_ = nzw(k);
+            const T (*Q)[S2T_MX] = smid[k & 1];
+#pragma unroll
+            for (int r = 0; r < SW_R; r++) {
+                const int ly = row[r] + 1, lx = tx + 1;
+                const T S = Q[ly][lx - 1] + Q[ly][lx + 1] + Q[ly - 1][lx] + Q[ly + 1][lx] +
+                            Mm[r] + Mp[r];
+                const T v = cff1 * M0[r] + om * (S - Bo[r]) * pick(nz_, id0[r], id1[r], id2[r]);
+                const int64_t c = (int64_t)k * g.sz + off[r];
+                if (ADDX) xo[c] = xv[r] + (double)v;
+                else dst[c] = v;
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < SW_R; r++) {
+            Mm[r] = M0[r];
+            M0[r] = Mp[r];
+            I0[r] = I1[r];
+            I1[r] = I2[r];
+            I2[r] = I3[r];
+            Bo[r] = Bm[r];
+            Bm[r] = Bn[r];
+            if (ADDX) xv[r] = xvn[r];
+        }
+        r0 = r1;
+        r1 = r2;
+        r2 = r3;
+        bm1 = bn1;
+        o2 = o2n;
+        q3 = nextp(q3);
+        q2 = nextp(q2);
+    }
+}
+
+// planes per block of the two-sweep pass: the longest march (at most 16
+// planes in FP32, 32 in FP64, dividing nz) that keeps at least 1024 blocks;
+// each block re-reads four input planes.  256^3 (k_mg_sweep2t): FP32 37.8 us
+// at 16 planes (41.0 at 32), FP64 93.4 us at 32 (95.6 at 16).
+static int sweep2x_kz(const Geo &g, int es) {
     const int64_t cols = (int64_t)(g.nx / SW_TX) * (g.ny / SW_TY);
-    int kz = 32;
+    int kz = es == 4 ? 16 : 32;
     while (kz > 1 && (g.nz % kz != 0 || cols * (g.nz / kz) < 1024)) kz >>= 1;
     return kz;
 }
 
+// the two-sweep pass with register z-windows (k_mg_sweep2t) in place of
+// k_mg_sweep2x25
+#ifndef NYL_SWEEP2T
+#define NYL_SWEEP2T 1
+#endif
 template <typename T>
 hipError_t launch_mg_sweep2x25(const Geo &g, const T *src, T *dst, const T *b, double omega,
                                const MGTabs &tab, hipStream_t s, double *addx) {
     if (src == dst) return hipErrorInvalidValue;
-    const int kz = sweep2x_kz(g);
+    const int kz = sweep2x_kz(g, (int)sizeof(T));
     dim3 grd(g.nx / SW_TX, g.ny / SW_TY, g.nz / kz), blk(SW_TX, 4);
+    if (NYL_SWEEP2T) {
+        if (addx && g.pz)
+            NYL_LAUNCH((k_mg_sweep2t<T, true, 1>), grd, blk, 0, s, g, src, dst, b, omega, tab,
+                       kz, addx);
+        else if (addx)
+            NYL_LAUNCH((k_mg_sweep2t<T, true, 0>), grd, blk, 0, s, g, src, dst, b, omega, tab,
+                       kz, addx);
+        else if (g.pz)
+            NYL_LAUNCH((k_mg_sweep2t<T, false, 1>), grd, blk, 0, s, g, src, dst, b, omega, tab,
+                       kz, addx);
+        else
+            NYL_LAUNCH((k_mg_sweep2t<T, false, 0>), grd, blk, 0, s, g, src, dst, b, omega, tab,
+                       kz, addx);
+        return hipGetLastError();
+    }
     NYL_LAUNCH((k_mg_sweep2x25<T>), grd, blk, 0, s, g, src, dst, b, omega, tab, kz, addx);
     return hipGetLastError();
 }

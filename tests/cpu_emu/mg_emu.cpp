@@ -95,6 +95,22 @@ int check_tiled(int nx, int ny, int nz, int px, int py, int pz) {
             true);
         bad += ndiff(xo1, xo2);
     }
+    // ... and the pass with register z-windows, for several march lengths
+    for (int kzt : {1, 2, 3, 4, 8}) {
+        if (nz % kzt) continue;
+        std::vector<T> z4(nf);
+        run_blocks(dim3(nx / SW_TX, ny / SW_TY, nz / kzt), dim3(SW_TX, 4, 1), [&]() {
+            (pz ? k_mg_sweep2t<T, false, 1> : k_mg_sweep2t<T, false, 0>)(gf, x.data(), z4.data(), b.data(), om, tab, kzt, nullptr); },
+            true);
+        bad += ndiff(z1, z4);
+        std::vector<double> xa(nf), xb(nf);
+        for (size_t q = 0; q < nf; q++) xa[q] = xb[q] = 0.5 - 0.125 * (double)q;
+        for (size_t q = 0; q < nf; q++) xa[q] = xa[q] + (double)z1[q];
+        run_blocks(dim3(nx / SW_TX, ny / SW_TY, nz / kzt), dim3(SW_TX, 4, 1), [&]() {
+            (pz ? k_mg_sweep2t<T, true, 1> : k_mg_sweep2t<T, true, 0>)(gf, x.data(), nullptr, b.data(), om, tab, kzt, xb.data()); },
+            true);
+        bad += ndiff(xa, xb);
+    }
     printf("tiled<%s> %dx%dx%d p=%d%d%d: %d mismatches\n", sizeof(T) == 8 ? "f64" : "f32",
            nx, ny, nz, px, py, pz, bad);
     return bad != 0;
@@ -274,6 +290,8 @@ int main() {
         if (p == 0 || p == 3 || p == 4 || p == 7) {   // 000, 110, 001, 111
             fails += check_tiled<double>(64, 8, 8, px, py, pz);
             fails += check_tiled<float>(128, 16, 4, px, py, pz);
+            fails += check_tiled<float>(64, 16, 6, px, py, pz);
+            fails += check_tiled<double>(64, 8, 2, px, py, pz);
             fails += check_outer(64, 8, 16, px, py, pz);
         }
         const int sizes[][3] = {{8, 8, 8}, {2, 2, 4}, {4, 2, 2}, {20, 12, 6}};
