@@ -23,11 +23,13 @@ reported as p50/p99 time-to-completion and throughput, and a Rayleigh-Taylor
 both precisions.
 
 Prints ONE JSON line.  ``roofline`` prices the dominant kernel of the
-headline path, the damped-Jacobi smoother sweep on the finest multigrid level
-(k_mg_sweep25, the kernel with the largest time per step): achieved =
-its algorithmic HBM bytes per launch (each input field read once, each output
-written once) divided by its mean launch time, both measured with HIP events
-on the library's stream over the timed steps.  ``cpu_baseline`` times the
+headline path, the two-sweep damped-Jacobi smoothing pass on the finest
+multigrid level (k_mg_sweep2t: both pre-smoothing sweeps with the residual
+norm of the convergence test, or both post-smoothing sweeps; the kernel with
+the largest time per step): achieved = its algorithmic HBM bytes per launch
+(each input field read once, each output written once: 12 bytes per cell in
+FP32 for two sweeps) divided by its mean launch time, both measured with HIP
+events on the library's stream over the timed steps.  ``cpu_baseline`` times the
 FP64 CPU oracle (C restatement of the reference kernels + numpy
 orchestration, OpenMP) on a bounded sample of the same workload.
 """
@@ -50,9 +52,10 @@ def parse():
     ap.add_argument("--n", type=int, default=256, help="grid cells per axis")
     ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed",
                     help="path of the headline value (the other one is timed beside it)")
-    ap.add_argument("--roofline-kernel", default="mg_sweep_fine",
-                    help="kernel class priced against the roofline (default: the smoother "
-                         "sweep on the finest level, the kernel with the most time per step)")
+    ap.add_argument("--roofline-kernel", default="mg_sweep2_fine",
+                    help="kernel class priced against the roofline (default: the two-sweep "
+                         "smoothing pass on the finest level, the kernel with the most time "
+                         "per step)")
     ap.add_argument("--err-steps", type=int, default=200,
                     help="steps of the mixed-vs-FP64 error check (0: skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -285,9 +288,9 @@ def measured_traffic(kclass, precision, n):
     measured on the current kernel sources (its recorded source digest
     matches); None otherwise."""
     from nyles_amd import profiling
-    if (kclass, precision, n) != ("mg_sweep_fine", "mixed", 256):
+    if (precision, n) != ("mixed", 256):
         return None
-    path = os.path.join(ROOT, "profiles", "traffic_mg_sweep_fine_taylor_green_256_mixed.md")
+    path = os.path.join(ROOT, "profiles", "traffic_%s_taylor_green_256_mixed.md" % kclass)
     found, digest = None, None
     try:
         with open(path) as f:

@@ -70,7 +70,8 @@ EXPORTED = [
 
 # kernel classes of nyl_prof_start
 KERNEL_CLASSES = {"stage": 1, "mg_sweep": 2, "mg_restrict": 3, "mg_prolong": 4, "diag": 5,
-                  "div": 6, "correct": 7, "resnorm": 8, "mg_update": 9, "mg_sweep_fine": 10, "mg_sweep2": 11}
+                  "div": 6, "correct": 7, "resnorm": 8, "mg_update": 9, "mg_sweep_fine": 10, "mg_sweep2": 11,
+                  "mg_coarse": 12, "mg_sweep2_fine": 13}
 
 
 class NylesHipError(RuntimeError):

@@ -184,9 +184,13 @@ constexpr int kFineLevel = 0x100;   // launch works on the finest multigrid leve
 
 template <typename Fn>
 int timed_launch(nyl_ctx *c, int cls, double bytes, Fn fn) {
-    // a level-1 sweep also belongs to the NYL_K_MG_SWEEP_FINE class
-    const bool on = c->prof_cls == (cls & 0xff) ||
-                    (c->prof_cls == NYL_K_MG_SWEEP_FINE && (cls & kFineLevel));
+    // a level-1 sweep (two-sweep pass) also belongs to the NYL_K_MG_SWEEP_FINE
+    // (NYL_K_MG_SWEEP2_FINE) class
+    const int base = cls & 0xff;
+    const bool fine = (cls & kFineLevel) != 0;
+    const bool on = c->prof_cls == base ||
+                    (c->prof_cls == NYL_K_MG_SWEEP_FINE && fine && base == NYL_K_MG_SWEEP) ||
+                    (c->prof_cls == NYL_K_MG_SWEEP2_FINE && fine && base == NYL_K_MG_SWEEP2);
     if (on) {
         if (c->prof_used * 2 + 2 > c->prof_nev) {
             const int cap = (int)(sizeof(c->prof_ev) / sizeof(c->prof_ev[0]));
@@ -338,7 +342,8 @@ uint64_t sol_off(const nyl_ctx *c, int l) {
 }
 
 template <typename Tf>
-int smooth_level_t(nyl_ctx *c, int l, int xc_level, bool pre_done, double *addx) {
+int smooth_level_t(nyl_ctx *c, int l, int xc_level, bool pre_done, double *addx,
+                   bool sol_in_y) {
     Level &f = c->lev[l];
     const double om = c->d.mg_omega;
     Tf *x = F<Tf>(c, f.x), *b = F<Tf>(c, f.b), *y = F<Tf>(c, f.y);
@@ -360,6 +365,19 @@ int smooth_level_t(nyl_ctx *c, int l, int xc_level, bool pre_done, double *addx)
             return NYL_OK;
         }
         const double bp = 2.125 * cells(f.g) * sizeof(Tf);
+        if (sol_in_y) {
+            // the level's solution is in y (both pre-sweeps in one pass):
+            // y += P(xc) in place, then both post-sweeps y -> x
+            if (!(NYL_SWEEP2 && same && restrict25_fits(f.g) && tiled))
+                return fail(NYL_E_STATE, "level solution in y without the two-sweep pass");
+            Tf *yv = F<Tf>(c, f.y);
+            LAUNCH(NYL_K_MG_PROLONG, bp,
+                   launch_mg_prolong25<Tf, Tf>(f.g, q.g, yv, F<Tf>(c, qx), c->tabs, c->stream));
+            const double b2 = addx ? (2 * sizeof(Tf) + 2 * sizeof(double)) * cells(f.g) : by;
+            LAUNCH(NYL_K_MG_SWEEP2 | (l == 0 ? kFineLevel : 0), b2,
+                   launch_mg_sweep2x25<Tf>(f.g, yv, x, b, om, c->tabs, c->stream, addx));
+            return NYL_OK;
+        }
         if (NYL_SWEEP2 && same && restrict25_fits(f.g) && tiled && !pre_done) {
             // y = x + P(xc), then both post-smoothing sweeps y -> x in one
             // pass (with addx: the second sweep's result added to the FP64 x)
@@ -367,7 +385,7 @@ int smooth_level_t(nyl_ctx *c, int l, int xc_level, bool pre_done, double *addx)
             LAUNCH(NYL_K_MG_PROLONG, bp,
                    launch_mg_prolong25<Tf, Tf>(f.g, q.g, x, F<Tf>(c, qx), c->tabs, c->stream, yv));
             const double b2 = addx ? (2 * sizeof(Tf) + 2 * sizeof(double)) * cells(f.g) : by;
-            LAUNCH(NYL_K_MG_SWEEP2, b2,
+            LAUNCH(NYL_K_MG_SWEEP2 | (l == 0 ? kFineLevel : 0), b2,
                    launch_mg_sweep2x25<Tf>(f.g, yv, x, b, om, c->tabs, c->stream, addx));
             return NYL_OK;
         }
@@ -416,46 +434,48 @@ int smooth_level_t(nyl_ctx *c, int l, int xc_level, bool pre_done, double *addx)
 }
 
 int smooth_level(nyl_ctx *c, int l, int xc_level, bool pre_done = false,
-                 double *addx = nullptr) {
-    return lev64(c, l) ? smooth_level_t<double>(c, l, xc_level, pre_done, addx)
-                       : smooth_level_t<float>(c, l, xc_level, pre_done, addx);
+                 double *addx = nullptr, bool sol_in_y = false) {
+    return lev64(c, l) ? smooth_level_t<double>(c, l, xc_level, pre_done, addx, sol_in_y)
+                       : smooth_level_t<float>(c, l, xc_level, pre_done, addx, sol_in_y);
 }
 
 // residual of level l restricted into level l + 1, with level l+1's first
-// smoothing sweep from x = 0 written to its y
-int restrict_level(nyl_ctx *c, int l) {
+// smoothing sweep from x = 0 written to its y (from_y: level l's solution is
+// in its y)
+int restrict_level(nyl_ctx *c, int l, bool from_y = false) {
     Level &f = c->lev[l], &q = c->lev[l + 1];
+    const uint64_t fx = from_y ? f.y : f.x;   // the level's solution
     const bool f64 = lev64(c, l), c64 = lev64(c, l + 1);
     const double om = c->d.mg_omega;
     const double bytes = 2.25 * cells(f.g) * (f64 ? sizeof(double) : sizeof(float));
     if (f64 == c64 && restrict25_fits(f.g)) {
         if (f64)
             LAUNCH(NYL_K_MG_RESTRICT, bytes,
-                   launch_mg_restrict25<double, double>(f.g, q.g, F<double>(c, f.x),
+                   launch_mg_restrict25<double, double>(f.g, q.g, F<double>(c, fx),
                                                         F<double>(c, f.b), F<double>(c, q.b),
                                                         F<double>(c, q.x), c->tabs, c->stream,
                                                         F<double>(c, q.y), om));
         else
             LAUNCH(NYL_K_MG_RESTRICT, bytes,
-                   launch_mg_restrict25<float, float>(f.g, q.g, F<float>(c, f.x),
+                   launch_mg_restrict25<float, float>(f.g, q.g, F<float>(c, fx),
                                                       F<float>(c, f.b), F<float>(c, q.b),
                                                       F<float>(c, q.x), c->tabs, c->stream,
                                                       F<float>(c, q.y), om));
     } else if (f64 && c64) {
         LAUNCH(NYL_K_MG_RESTRICT, bytes,
-               launch_mg_resid_restrict<double, double>(f.g, q.g, F<double>(c, f.x),
+               launch_mg_resid_restrict<double, double>(f.g, q.g, F<double>(c, fx),
                                                         F<double>(c, f.b), F<double>(c, q.b),
                                                         F<double>(c, q.x), c->tabs, c->stream,
                                                         F<double>(c, q.y), om));
     } else if (!f64 && c64) {
         LAUNCH(NYL_K_MG_RESTRICT, bytes,
-               launch_mg_resid_restrict<float, double>(f.g, q.g, F<float>(c, f.x),
+               launch_mg_resid_restrict<float, double>(f.g, q.g, F<float>(c, fx),
                                                        F<float>(c, f.b), F<double>(c, q.b),
                                                        F<double>(c, q.x), c->tabs, c->stream,
                                                        F<double>(c, q.y), om));
     } else if (!f64 && !c64) {
         LAUNCH(NYL_K_MG_RESTRICT, bytes,
-               launch_mg_resid_restrict<float, float>(f.g, q.g, F<float>(c, f.x),
+               launch_mg_resid_restrict<float, float>(f.g, q.g, F<float>(c, fx),
                                                       F<float>(c, f.b), F<float>(c, q.b),
                                                       F<float>(c, q.x), c->tabs, c->stream,
                                                       F<float>(c, q.y), om));
@@ -640,7 +660,8 @@ void drop_graphs(nyl_ctx *c) {
 int coarse_cycle_run(nyl_ctx *c) {
     const int pc = c->prof_cls;
     const bool timed = pc == NYL_K_MG_SWEEP || pc == NYL_K_MG_RESTRICT || pc == NYL_K_MG_PROLONG ||
-                       pc == NYL_K_MG_SWEEP2 || pc == NYL_K_MG_COARSE;
+                       pc == NYL_K_MG_SWEEP2 || pc == NYL_K_MG_COARSE ||
+                       pc == NYL_K_MG_SWEEP2_FINE;
     if (!c->use_graphs || !c->stream || timed) return coarse_cycle(c);
     if (!c->g_coarse) {
         hipGraph_t graph;
@@ -663,16 +684,31 @@ int coarse_cycle_run(nyl_ctx *c) {
 // ran into the spare buffer.  addx: the mixed solve's x, which receives the
 // level-1 result (x += e).  Below level 1 the first sweep of each level is
 // written by the restriction.
-int vcycle(nyl_ctx *c, bool pre_done, double *addx = nullptr) {
+// pre: the level-1 pre-smoothing done before the call, 0 none, 1 the first
+// sweep (in y), 2 both (in y, by the two-sweep pass of the convergence
+// test): then the restriction reads y, the prolongation adds to y in place
+// and the post-smoothing pass writes x.
+int vcycle(nyl_ctx *c, int pre, double *addx = nullptr) {
     int rc;
-    if ((rc = smooth_level(c, 0, -1, pre_done))) return rc;
-    if ((rc = restrict_level(c, 0))) return rc;
+    if (pre < 2 && (rc = smooth_level(c, 0, -1, pre == 1))) return rc;
+    if ((rc = restrict_level(c, 0, pre == 2))) return rc;
     if (c->nlev > 2) {
         if ((rc = coarse_cycle_run(c))) return rc;
     } else {
         if ((rc = coarsest(c))) return rc;
     }
-    return smooth_level(c, 0, 1, false, addx);
+    return smooth_level(c, 0, 1, false, addx, pre == 2);
+}
+
+// The convergence test's residual norm can come with both pre-smoothing
+// sweeps of the next V-cycle (k_mg_sweep2t, solution into y): level 1 tiled
+// for the two-sweep pass and level 2 of the same type (the post-smoothing
+// that follows uses the pass too)
+bool spec2_fits(const nyl_ctx *c) {
+    const Level &f = c->lev[0];
+    return NYL_SWEEP2 && c->nlev >= 2 && sweep25_fits(f.g) && restrict25_fits(f.g) &&
+           lev64(c, 0) == lev64(c, 1) &&
+           sweep2x_blocks(f.g, lev64(c, 0) ? 8 : 4) <= NYL_RED_BLOCKS;
 }
 
 void fill_stats(nyl_ctx *c, nyl_mg_stats *st, int nite, double res, double normb) {
@@ -693,22 +729,29 @@ int mg_solve_fp64(nyl_ctx *c, nyl_mg_stats *st) {
     Level &f = c->lev[0];
     double *part = F<double>(c, c->partials), *res_dev = F<double>(c, c->result);
     const bool spec = sweep25_fits(f.g) && sweep25_blocks(f.g) <= NYL_RED_BLOCKS;
+    const bool spec2 = spec2_fits(c);
     int rc;
     double normb, rn, res;
     arm_publish(c, 1);
     HIPCHK(launch_norm2<double>(f.g, F<double>(c, f.b), part, res_dev, c->stream));
     if ((rc = read_result(c, &normb))) return rc;
-    bool pending = false;
+    int pending = 0;   // pre-smoothing sweeps of the next V-cycle already done
     auto normresidual = [&](double *out) -> int {
-        pending = false;
+        pending = 0;
         if (normb > 0.0) {
             arm_publish(c, 1);
-            if (spec) {
+            if (spec2) {
+                LAUNCH(NYL_K_MG_SWEEP2 | kFineLevel, 3 * cells(f.g) * sizeof(double),
+                       launch_mg_sweep2x25<double>(f.g, F<double>(c, f.x), F<double>(c, f.y),
+                                                   F<double>(c, f.b), c->d.mg_omega, c->tabs,
+                                                   c->stream, nullptr, part, res_dev));
+                pending = 2;
+            } else if (spec) {
                 LAUNCH(NYL_K_MG_SWEEP | kFineLevel, 3 * cells(f.g) * sizeof(double),
                        launch_mg_sweep25<double>(f.g, F<double>(c, f.x), F<double>(c, f.y),
                                                  F<double>(c, f.b), c->d.mg_omega, c->tabs,
                                                  part, res_dev, c->stream));
-                pending = true;
+                pending = 1;
             } else {
                 LAUNCH(NYL_K_RESNORM, 2 * cells(f.g) * sizeof(double),
                        launch_resnorm2<double>(f.g, F<double>(c, f.x), F<double>(c, f.b), c->tabs,
@@ -763,6 +806,7 @@ int mg_solve_mixed(nyl_ctx *c, nyl_mg_stats *st) {
     double *xo = F<double>(c, c->mgx), *bo = F<double>(c, c->mgb);
     float *e = F<float>(c, f.x), *r = F<float>(c, f.b), *y = F<float>(c, f.y);
     const bool spec = sweep25_fits(f.g) && sweep25_blocks(f.g) <= NYL_RED_BLOCKS;
+    const bool spec2 = spec2_fits(c);
     int rc;
     double rn, res;
     // sum of b and of b**2 in FP64: the mean to remove and the norm of b - mean
@@ -789,10 +833,15 @@ int mg_solve_mixed(nyl_ctx *c, nyl_mg_stats *st) {
         }
         return NYL_OK;
     };
-    // FP32 residual r - A e (speculative first sweep of the next V-cycle)
+    // FP32 residual r - A e (speculative pre-smoothing of the next V-cycle:
+    // both sweeps into y, or the first)
     auto inner = [&](double *out) -> int {
         arm_publish(c, 1);
-        if (spec) {
+        if (spec2) {
+            LAUNCH(NYL_K_MG_SWEEP2 | kFineLevel, 3 * cells(f.g) * sizeof(float),
+                   launch_mg_sweep2x25<float>(f.g, e, y, r, c->d.mg_omega, c->tabs, c->stream,
+                                              nullptr, part, res_dev));
+        } else if (spec) {
             LAUNCH(NYL_K_MG_SWEEP | kFineLevel, 3 * cells(f.g) * sizeof(float),
                    launch_mg_sweep25<float>(f.g, e, y, r, c->d.mg_omega, c->tabs, part, res_dev,
                                             c->stream));
@@ -811,7 +860,7 @@ int mg_solve_mixed(nyl_ctx *c, nyl_mg_stats *st) {
     int nite = 0;
     bool at_max = false;
     while (!(res < c->d.mg_tol) && !at_max) {
-        bool pending = true;       // y holds the first sweep from e = 0
+        int pending = 1;           // y holds the first sweep from e = 0
         const bool rel = res * kInnerRed2 > c->d.mg_tol;
         const double target = rel ? res * kInnerRed2 : c->d.mg_tol;
         for (int k = 1;; k++) {
@@ -821,7 +870,7 @@ int mg_solve_mixed(nyl_ctx *c, nyl_mg_stats *st) {
             if (at_max || (rel && k >= kInnerMax)) break;
             double rin;
             if ((rc = inner(&rin))) return rc;
-            pending = spec;
+            pending = spec2 ? 2 : spec ? 1 : 0;
             if (rin < target) break;
         }
         LAUNCH(NYL_K_MG_UPDATE, (2 * sizeof(double) + sizeof(float)) * cells(f.g),

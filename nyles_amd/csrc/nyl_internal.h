@@ -271,10 +271,15 @@ inline uint64_t tail_layout(TailArgs &a, int es, int esc) {
 template <typename T, typename TC>
 hipError_t launch_mg_tail(const TailArgs &a, hipStream_t s);
 
-// two consecutive tiled sweeps src -> (tmp) -> dst in one pass (src != dst)
+// two consecutive tiled sweeps src -> (tmp) -> dst in one pass (src != dst);
+// addx: the result added to that FP64 field instead of stored; partials:
+// also the residual norm of src into dev_out (needs sweep2x_blocks(g, es)
+// <= NYL_RED_BLOCKS partials)
 template <typename T>
 hipError_t launch_mg_sweep2x25(const Geo &g, const T *src, T *dst, const T *b, double omega,
-                               const MGTabs &tab, hipStream_t s, double *addx = nullptr);
+                               const MGTabs &tab, hipStream_t s, double *addx = nullptr,
+                               double *partials = nullptr, double *dev_out = nullptr);
+int sweep2x_blocks(const Geo &g, int es);
 template <typename Tf, typename Tc>
 hipError_t launch_mg_restrict25(const Geo &gf, const Geo &gc, const Tf *x, const Tf *b, Tc *bc,
                                 Tc *xc, const MGTabs &tab, hipStream_t s, Tc *yc = nullptr,

@@ -2063,19 +2063,23 @@ __global__ __launch_bounds__(256) void k_mg_sweep2x25(Geo g, const T *__restrict
 // outside the domain (wall axes) hold zero in both stages, as neighbours()
 // reads them, and every value is the per-cell sweep's expression, so dst is
 // bitwise that of two sweeps.  ADDX: the result is added to the FP64 xo
-// instead of stored.
+// instead of stored.  RESNORM: the block's partial sum of the input's squared
+// residuals (b + diag x - sum6(x), the first sweep's terms) is stored in
+// partials, as k_mg_sweep25 does (the speculative pre-smoothing of the next
+// V-cycle, launched with the convergence test).
 #define S2T_LX (SW_TX + 4)                       // input plane, two-cell ring
 #define S2T_LY (SW_TY + 4)
 #define S2T_MX (SW_TX + 2)                       // first-sweep plane, one-cell ring
 #define S2T_MY (SW_TY + 2)
 #define S2T_RING1 (2 * S2T_MX + 2 * SW_TY)       // 148 cells
 #define S2T_RING2 (2 * S2T_LX + 2 * (SW_TY + 2))  // 156 cells
-template <typename T, bool ADDX, int PZ>
+template <typename T, bool ADDX, int PZ, bool RESNORM>
 __global__ __launch_bounds__(256) void k_mg_sweep2t(Geo g, const T *__restrict__ src,
                                                     T *__restrict__ dst,
                                                     const T *__restrict__ b, double omega,
                                                     MGTabs tab, int kz,
-                                                    double *__restrict__ xo) {
+                                                    double *__restrict__ xo,
+                                                    double *__restrict__ partials) {
     __shared__ T sin_[2][S2T_LY][S2T_LX];
     __shared__ T smid[2][S2T_MY][S2T_MX];
     const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * SW_TX + tx;
@@ -2086,7 +2090,7 @@ __global__ __launch_bounds__(256) void k_mg_sweep2t(Geo g, const T *__restrict__
     // own cells: in-plane offsets, rows, inverse diagonals for 0, 1, 2
     // in-domain z-neighbours
     int off[SW_R], row[SW_R];
-    T id0[SW_R], id1[SW_R], id2[SW_R];
+    T id0[SW_R], id1[SW_R], id2[SW_R], dg0[SW_R], dg1[SW_R], dg2[SW_R];
 #pragma unroll
     for (int r = 0; r < SW_R; r++) {
         row[r] = ty + 4 * r;
@@ -2097,7 +2101,11 @@ __global__ __launch_bounds__(256) void k_mg_sweep2t(Geo g, const T *__restrict__
         id0[r] = T(tab.idiag[nxy]);
         id1[r] = T(tab.idiag[nxy + 1]);
         id2[r] = T(tab.idiag[nxy + 2]);
+        dg0[r] = RESNORM ? T(tab.diag[nxy]) : T(0);
+        dg1[r] = RESNORM ? T(tab.diag[nxy + 1]) : T(0);
+        dg2[r] = RESNORM ? T(tab.diag[nxy + 2]) : T(0);
     }
+    double acc = 0.0;   // RESNORM: sum of the input's squared residuals
     // ring cell of the first sweep (threads 0..147): position (mx, my) in the
     // first-sweep plane, (mx + 1, my + 1) in the input plane
     const bool has1 = tid < S2T_RING1;
@@ -2174,9 +2182,11 @@ __global__ __launch_bounds__(256) void k_mg_sweep2t(Geo g, const T *__restrict__
         if (has1) P[my + 1][mx + 1] = (zi && in1) ? v1 : T(0);
         if (ring2 && has2) P[ly2][lx2] = (zi && in2) ? v2 : T(0);
     };
-    // first sweep of plane m on the own cells (zm_, zp_: input planes m -+ 1)
+    // first sweep of plane m on the own cells (zm_, zp_: input planes m -+ 1);
+    // own: m is one of the block's planes (RESNORM: its residuals are summed,
+    // as in k_mg_sweep25)
     auto mid_own = [&](int m, const T (&c_)[SW_R], const T (&zm_)[SW_R], const T (&zp_)[SW_R],
-                       const T (&bv)[SW_R], T (&out)[SW_R], bool store) {
+                       const T (&bv)[SW_R], T (&out)[SW_R], bool store, bool own) {
         const bool zi = zin(m), zl = zin(m - 1), zh = zin(m + 1);
         const int nz_ = nzw(m);
         const T (*P)[S2T_LX] = sin_[m & 1];
@@ -2185,6 +2195,11 @@ __global__ __launch_bounds__(256) void k_mg_sweep2t(Geo g, const T *__restrict__
             const int ly = row[r] + 2, lx = tx + 2;
             const T S = P[ly][lx - 1] + P[ly][lx + 1] + P[ly - 1][lx] + P[ly + 1][lx] +
                         (zl ? zm_[r] : T(0)) + (zh ? zp_[r] : T(0));
+            if (RESNORM && own) {
+                const T d = pick(nz_, dg0[r], dg1[r], dg2[r]);
+                const double rr = (double)(bv[r] + d * c_[r] - S);
+                acc = acc + rr * rr;
+            }
             const T v = cff1 * c_[r] + om * (S - bv[r]) * pick(nz_, id0[r], id1[r], id2[r]);
             out[r] = zi ? v : T(0);
             if (store) smid[m & 1][row[r] + 1][tx + 1] = out[r];
@@ -2222,8 +2237,8 @@ __global__ __launch_bounds__(256) void k_mg_sweep2t(Geo g, const T *__restrict__
     put_in(k0, A2, a2, o2a, true);
     __syncthreads();
     T Mm[SW_R], M0[SW_R], Mp[SW_R];
-    mid_own(k0 - 1, A1, A0, A2, Bp, Mm, false);
-    mid_own(k0, A2, A1, A3, Bo, M0, true);
+    mid_own(k0 - 1, A1, A0, A2, Bp, Mm, false, false);
+    mid_own(k0, A2, A1, A3, Bo, M0, true, true);
     mid_ring(k0, a2, a1, a3, bo1);
     __syncthreads();              // plane k0+1 replaces plane k0-1, just read
     // z-windows: input k, k+1, k+2 (own and ring), b of k (output) and k+1
@@ -2252,7 +2267,7 @@ __global__ __launch_bounds__(256) void k_mg_sweep2t(Geo g, const T *__restrict__
             }
         }
         __syncthreads();
-        mid_own(k + 1, I1, I0, I2, Bm, Mp, true);
+        mid_own(k + 1, I1, I0, I2, Bm, Mp, true, k + 1 < k0 + kz);
         mid_ring(k + 1, r1, r0, r2, bm1);
         // second sweep of plane k (in-plane neighbours from the slot of k)
         {
@@ -2288,6 +2303,11 @@ __global__ __launch_bounds__(256) void k_mg_sweep2t(Geo g, const T *__restrict__
         q3 = nextp(q3);
         q2 = nextp(q2);
     }
+    if (RESNORM) {
+        __syncthreads();
+        const int nb = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+        block_sum_store(acc, partials + nb, tid);
+    }
 }
 
 // planes per block of the two-sweep pass: the longest march (at most 16
@@ -2308,34 +2328,44 @@ static int sweep2x_kz(const Geo &g, int es) {
 #endif
 template <typename T>
 hipError_t launch_mg_sweep2x25(const Geo &g, const T *src, T *dst, const T *b, double omega,
-                               const MGTabs &tab, hipStream_t s, double *addx) {
-    if (src == dst) return hipErrorInvalidValue;
+                               const MGTabs &tab, hipStream_t s, double *addx,
+                               double *partials, double *dev_out) {
+    if (src == dst || (addx && partials)) return hipErrorInvalidValue;
     const int kz = sweep2x_kz(g, (int)sizeof(T));
     dim3 grd(g.nx / SW_TX, g.ny / SW_TY, g.nz / kz), blk(SW_TX, 4);
     if (NYL_SWEEP2T) {
-        if (addx && g.pz)
-            NYL_LAUNCH((k_mg_sweep2t<T, true, 1>), grd, blk, 0, s, g, src, dst, b, omega, tab,
-                       kz, addx);
-        else if (addx)
-            NYL_LAUNCH((k_mg_sweep2t<T, true, 0>), grd, blk, 0, s, g, src, dst, b, omega, tab,
-                       kz, addx);
-        else if (g.pz)
-            NYL_LAUNCH((k_mg_sweep2t<T, false, 1>), grd, blk, 0, s, g, src, dst, b, omega, tab,
-                       kz, addx);
-        else
-            NYL_LAUNCH((k_mg_sweep2t<T, false, 0>), grd, blk, 0, s, g, src, dst, b, omega, tab,
-                       kz, addx);
+#define S2T_GO(A, Z, R)                                                                         \
+    NYL_LAUNCH((k_mg_sweep2t<T, A, Z, R>), grd, blk, 0, s, g, src, dst, b, omega, tab, kz, addx, \
+               partials)
+        if (partials) {
+            if (g.pz) S2T_GO(false, 1, true);
+            else S2T_GO(false, 0, true);
+            (void)launch_final_sum(partials, (int)(grd.x * grd.y * grd.z), dev_out, s);
+        } else if (addx) {
+            if (g.pz) S2T_GO(true, 1, false);
+            else S2T_GO(true, 0, false);
+        } else {
+            if (g.pz) S2T_GO(false, 1, false);
+            else S2T_GO(false, 0, false);
+        }
+#undef S2T_GO
         return hipGetLastError();
     }
+    if (partials) return hipErrorInvalidValue;   // the residual norm needs k_mg_sweep2t
     NYL_LAUNCH((k_mg_sweep2x25<T>), grd, blk, 0, s, g, src, dst, b, omega, tab, kz, addx);
     return hipGetLastError();
 }
+
+// blocks of the two-sweep pass (its residual norm needs as many partials)
+int sweep2x_blocks(const Geo &g, int es) {
+    return (g.nx / SW_TX) * (g.ny / SW_TY) * (g.nz / sweep2x_kz(g, es));
+}
 template hipError_t launch_mg_sweep2x25<double>(const Geo &, const double *, double *,
                                                 const double *, double, const MGTabs &,
-                                                hipStream_t, double *);
+                                                hipStream_t, double *, double *, double *);
 template hipError_t launch_mg_sweep2x25<float>(const Geo &, const float *, float *,
                                                const float *, double, const MGTabs &,
-                                               hipStream_t, double *);
+                                               hipStream_t, double *, double *, double *);
 
 bool restrict25_fits(const Geo &g) {
     return sweep25_fits(g) && g.nz % 2 == 0;
