@@ -823,8 +823,9 @@ int mg_solve_mixed(nyl_ctx *c, nyl_mg_stats *st) {
         if (normb > 0.0) {
             arm_publish(c, 1);
             LAUNCH(NYL_K_RESNORM, (2 * sizeof(double) + 2 * sizeof(float)) * cells(f.g),
-                   launch_mg_outer_resid<float>(f.g, xo, bo, shift, r, y, c->d.mg_omega, c->tabs,
-                                                part, res_dev, c->stream, false));
+                   launch_mg_outer_resid<float>(f.g, xo, bo, shift, r, spec2 ? nullptr : y,
+                                                c->d.mg_omega, c->tabs, part, res_dev,
+                                                c->stream, false));
             int r2 = read_result(c, &rn);
             if (r2) return r2;
             *out = rn / normb;
@@ -860,7 +861,16 @@ int mg_solve_mixed(nyl_ctx *c, nyl_mg_stats *st) {
     int nite = 0;
     bool at_max = false;
     while (!(res < c->d.mg_tol) && !at_max) {
-        int pending = 1;           // y holds the first sweep from e = 0
+        // both pre-smoothing sweeps from e = 0 in one pass into y, or the first
+        // one (written by the outer residual)
+        int pending = 1;
+        if (spec2) {
+            // (not in the fine-level class of the roofline: it reads one field)
+            LAUNCH(NYL_K_MG_SWEEP2, 2 * cells(f.g) * sizeof(float),
+                   launch_mg_sweep2x25<float>(f.g, nullptr, y, r, c->d.mg_omega, c->tabs,
+                                              c->stream));
+            pending = 2;
+        }
         const bool rel = res * kInnerRed2 > c->d.mg_tol;
         const double target = rel ? res * kInnerRed2 : c->d.mg_tol;
         for (int k = 1;; k++) {

@@ -271,10 +271,10 @@ inline uint64_t tail_layout(TailArgs &a, int es, int esc) {
 template <typename T, typename TC>
 hipError_t launch_mg_tail(const TailArgs &a, hipStream_t s);
 
-// two consecutive tiled sweeps src -> (tmp) -> dst in one pass (src != dst);
-// addx: the result added to that FP64 field instead of stored; partials:
-// also the residual norm of src into dev_out (needs sweep2x_blocks(g, es)
-// <= NYL_RED_BLOCKS partials)
+// two consecutive tiled sweeps src -> (tmp) -> dst in one pass (src != dst;
+// src null: from src = 0, nothing read); addx: the result added to that
+// FP64 field instead of stored; partials: also the residual norm of src into
+// dev_out (needs sweep2x_blocks(g, es) <= NYL_RED_BLOCKS partials)
 template <typename T>
 hipError_t launch_mg_sweep2x25(const Geo &g, const T *src, T *dst, const T *b, double omega,
                                const MGTabs &tab, hipStream_t s, double *addx = nullptr,

@@ -726,7 +726,7 @@ __global__ __launch_bounds__(256) void k_mg_outer_resid(Geo g, const double *__r
             acc = acc + r * r;
             const TI rf = TI(r);
             r_out[c] = rf;
-            y_out[c] = cff1 * TI(0) + om * (TI(0) - rf) * TI(tab.idiag[n.nwet]);
+            if (y_out) y_out[c] = cff1 * TI(0) + om * (TI(0) - rf) * TI(tab.idiag[n.nwet]);
         }
     }
     const int nb = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
@@ -1689,7 +1689,7 @@ __global__ __launch_bounds__(256) void k_mg_outer25(Geo g, const double *__restr
             accb = accb + bv * bv;
             const TI rf = TI(rr);
             r_out[c] = rf;
-            y_out[c] = cff1 * TI(0) + om * (TI(0) - rf) * id;
+            if (y_out) y_out[c] = cff1 * TI(0) + om * (TI(0) - rf) * id;
             zm[r] = z0[r];
             z0[r] = zp[r];
         }
@@ -2062,8 +2062,10 @@ __global__ __launch_bounds__(256) void k_mg_sweep2x25(Geo g, const T *__restrict
 // in-plane neighbours from the slot written one plane earlier).  Cells
 // outside the domain (wall axes) hold zero in both stages, as neighbours()
 // reads them, and every value is the per-cell sweep's expression, so dst is
-// bitwise that of two sweeps.  ADDX: the result is added to the FP64 xo
-// instead of stored.  RESNORM: the block's partial sum of the input's squared
+// bitwise that of two sweeps.  ZSRC: the input is zero and src is not read
+// (the first sweep is then sweep_from_zero's expression: the pre-smoothing of
+// a solve from x = 0).  ADDX: the result is added to the FP64 xo instead of
+// stored.  RESNORM: the block's partial sum of the input's squared
 // residuals (b + diag x - sum6(x), the first sweep's terms) is stored in
 // partials, as k_mg_sweep25 does (the speculative pre-smoothing of the next
 // V-cycle, launched with the convergence test).
@@ -2073,7 +2075,7 @@ __global__ __launch_bounds__(256) void k_mg_sweep2x25(Geo g, const T *__restrict
 #define S2T_MY (SW_TY + 2)
 #define S2T_RING1 (2 * S2T_MX + 2 * SW_TY)       // 148 cells
 #define S2T_RING2 (2 * S2T_LX + 2 * (SW_TY + 2))  // 156 cells
-template <typename T, bool ADDX, int PZ, bool RESNORM>
+template <typename T, bool ADDX, int PZ, bool RESNORM, bool ZSRC>
 __global__ __launch_bounds__(256) void k_mg_sweep2t(Geo g, const T *__restrict__ src,
                                                     T *__restrict__ dst,
                                                     const T *__restrict__ b, double omega,
@@ -2161,6 +2163,12 @@ __global__ __launch_bounds__(256) void k_mg_sweep2t(Geo g, const T *__restrict__
     // replaced by zero where they are used
     // loads of plane q (a loadable plane index)
     auto ld_in = [&](int q, T (&o)[SW_R], T &v1) {
+        if (ZSRC) {   // the input is zero: nothing is read
+#pragma unroll
+            for (int r = 0; r < SW_R; r++) o[r] = T(0);
+            v1 = T(0);
+            return;
+        }
         const T *p = src + (int64_t)q * g.sz;
 #pragma unroll
         for (int r = 0; r < SW_R; r++) o[r] = p[off[r]];
@@ -2172,7 +2180,7 @@ __global__ __launch_bounds__(256) void k_mg_sweep2t(Geo g, const T *__restrict__
         for (int r = 0; r < SW_R; r++) o[r] = p[off[r]];
         v1 = p[off1];
     };
-    auto ld_in2 = [&](int q) { return src[(int64_t)q * g.sz + off2]; };
+    auto ld_in2 = [&](int q) { return ZSRC ? T(0) : src[(int64_t)q * g.sz + off2]; };
     // input plane m into its LDS slot
     auto put_in = [&](int m, const T (&o)[SW_R], T v1, T v2, bool ring2) {
         const bool zi = zin(m);
@@ -2330,14 +2338,22 @@ template <typename T>
 hipError_t launch_mg_sweep2x25(const Geo &g, const T *src, T *dst, const T *b, double omega,
                                const MGTabs &tab, hipStream_t s, double *addx,
                                double *partials, double *dev_out) {
-    if (src == dst || (addx && partials)) return hipErrorInvalidValue;
+    if (src == dst || (addx && partials) || (!src && (addx || partials)))
+        return hipErrorInvalidValue;
     const int kz = sweep2x_kz(g, (int)sizeof(T));
     dim3 grd(g.nx / SW_TX, g.ny / SW_TY, g.nz / kz), blk(SW_TX, 4);
     if (NYL_SWEEP2T) {
-#define S2T_GO(A, Z, R)                                                                         \
-    NYL_LAUNCH((k_mg_sweep2t<T, A, Z, R>), grd, blk, 0, s, g, src, dst, b, omega, tab, kz, addx, \
-               partials)
-        if (partials) {
+#define S2T_GO(A, Z, R)                                                                    \
+    NYL_LAUNCH((k_mg_sweep2t<T, A, Z, R, false>), grd, blk, 0, s, g, src, dst, b, omega, tab, \
+               kz, addx, partials)
+        if (!src) {   // from x = 0
+            if (g.pz)
+                NYL_LAUNCH((k_mg_sweep2t<T, false, 1, false, true>), grd, blk, 0, s, g, src, dst,
+                           b, omega, tab, kz, addx, partials);
+            else
+                NYL_LAUNCH((k_mg_sweep2t<T, false, 0, false, true>), grd, blk, 0, s, g, src, dst,
+                           b, omega, tab, kz, addx, partials);
+        } else if (partials) {
             if (g.pz) S2T_GO(false, 1, true);
             else S2T_GO(false, 0, true);
             (void)launch_final_sum(partials, (int)(grd.x * grd.y * grd.z), dev_out, s);
@@ -2351,7 +2367,7 @@ hipError_t launch_mg_sweep2x25(const Geo &g, const T *src, T *dst, const T *b, d
 #undef S2T_GO
         return hipGetLastError();
     }
-    if (partials) return hipErrorInvalidValue;   // the residual norm needs k_mg_sweep2t
+    if (partials || !src) return hipErrorInvalidValue;   // these need k_mg_sweep2t
     NYL_LAUNCH((k_mg_sweep2x25<T>), grd, blk, 0, s, g, src, dst, b, omega, tab, kz, addx);
     return hipGetLastError();
 }

@@ -100,14 +100,14 @@ int check_tiled(int nx, int ny, int nz, int px, int py, int pz) {
         if (nz % kzt) continue;
         std::vector<T> z4(nf);
         run_blocks(dim3(nx / SW_TX, ny / SW_TY, nz / kzt), dim3(SW_TX, 4, 1), [&]() {
-            (pz ? k_mg_sweep2t<T, false, 1, false> : k_mg_sweep2t<T, false, 0, false>)(
+            (pz ? k_mg_sweep2t<T, false, 1, false, false> : k_mg_sweep2t<T, false, 0, false, false>)(
                 gf, x.data(), z4.data(), b.data(), om, tab, kzt, nullptr, nullptr); }, true);
         bad += ndiff(z1, z4);
         // with the residual norm of the input: same output, same sum
         std::vector<T> z5(nf);
         std::vector<double> pr(NYL_RED_BLOCKS, 0.0), rn2(NYL_RED_BLOCKS, 0.0);
         run_blocks(dim3(nx / SW_TX, ny / SW_TY, nz / kzt), dim3(SW_TX, 4, 1), [&]() {
-            (pz ? k_mg_sweep2t<T, false, 1, true> : k_mg_sweep2t<T, false, 0, true>)(
+            (pz ? k_mg_sweep2t<T, false, 1, true, false> : k_mg_sweep2t<T, false, 0, true, false>)(
                 gf, x.data(), z5.data(), b.data(), om, tab, kzt, nullptr, pr.data()); }, true);
         run_blocks(dim3(red_blocks((int64_t)nf)), dim3(RED_T), [&]() {
             k_resnorm2<T>(gf, x.data(), b.data(), tab, rn2.data()); }, true);
@@ -115,11 +115,21 @@ int check_tiled(int nx, int ny, int nz, int px, int py, int pz) {
         for (double v : pr) t1 += v;
         for (double v : rn2) t2 += v;
         bad += ndiff(z1, z5) + (std::fabs(t1 - t2) > 1e-9 * t2);
+        // from a zero input (src not read) against two sweeps of zeros
+        std::vector<T> zero(nf, T(0)), s1(nf), s2(nf), z6(nf);
+        run_blocks(mg_grid(gf), dim3(MG_BX, MG_BY, 1), [&]() {
+            k_mg_sweep<T>(gf, zero.data(), s1.data(), b.data(), om, tab); }, false);
+        run_blocks(mg_grid(gf), dim3(MG_BX, MG_BY, 1), [&]() {
+            k_mg_sweep<T>(gf, s1.data(), s2.data(), b.data(), om, tab); }, false);
+        run_blocks(dim3(nx / SW_TX, ny / SW_TY, nz / kzt), dim3(SW_TX, 4, 1), [&]() {
+            (pz ? k_mg_sweep2t<T, false, 1, false, true> : k_mg_sweep2t<T, false, 0, false, true>)(
+                gf, nullptr, z6.data(), b.data(), om, tab, kzt, nullptr, nullptr); }, true);
+        bad += ndiff(s2, z6);
         std::vector<double> xa(nf), xb(nf);
         for (size_t q = 0; q < nf; q++) xa[q] = xb[q] = 0.5 - 0.125 * (double)q;
         for (size_t q = 0; q < nf; q++) xa[q] = xa[q] + (double)z1[q];
         run_blocks(dim3(nx / SW_TX, ny / SW_TY, nz / kzt), dim3(SW_TX, 4, 1), [&]() {
-            (pz ? k_mg_sweep2t<T, true, 1, false> : k_mg_sweep2t<T, true, 0, false>)(
+            (pz ? k_mg_sweep2t<T, true, 1, false, false> : k_mg_sweep2t<T, true, 0, false, false>)(
                 gf, x.data(), nullptr, b.data(), om, tab, kzt, xb.data(), nullptr); },
             true);
         bad += ndiff(xa, xb);
