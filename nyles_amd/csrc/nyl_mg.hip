@@ -1752,73 +1752,106 @@ hipError_t launch_mg_outer_resid(const Geo &g, const double *x, const double *b,
 // j0 + ty + 4r of column i0 + tx) and marches up kz fine planes (kz even).
 // The coarse values its cells use (a 34 x 6 footprint per coarse plane,
 // zero outside a wall axis, wrapped along a periodic one: what the
-// guarded reads of prolong_term return) sit in LDS for the three coarse
-// planes kc-1, kc, kc+1 of the current fine pair (2kc, 2kc+1).  Each fine
-// cell evaluates prolong_term's expression in its order: bitwise the same.
+// guarded reads of prolong_term return) sit in a five-plane LDS ring: the
+// fine pair (2kc, 2kc+1) reads coarse planes kc-1, kc, kc+1 while plane
+// kc+2 is stored, so one barrier per coarse plane separates a slot's last
+// reader from its next writer.  The next coarse value and the next fine
+// pair are loaded one coarse plane ahead.  Each fine cell evaluates
+// prolong_term's expression in its order: bitwise the same.
 #define PR_CX (SW_TX / 2 + 2)
 #define PR_CY (SW_TY / 2 + 2)
+#define PR_RING 5
 template <typename Tf, typename Tc>
 __global__ __launch_bounds__(256) void k_mg_prolong25(Geo gf, Geo gc, Tf *__restrict__ xf,
                                                       const Tc *__restrict__ xc, MGTabs tab,
                                                       int kz, Tf *__restrict__ yf) {
-    __shared__ Tf cp[3][PR_CY][PR_CX];
+    __shared__ Tf cp[PR_RING][PR_CY][PR_CX];
     const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * SW_TX + tx;
     const Blk3 bk = xcd_block();
     const int i0 = bk.x * SW_TX, j0 = bk.y * SW_TY, k0 = bk.z * kz;
     const int i = i0 + tx;
     const int ci0 = (i0 >> 1) - 1, cj0 = (j0 >> 1) - 1;
     // coarse footprint cell loaded by this thread (one per coarse plane)
-    int64_t coff = -1;
+    const bool has_c = tid < PR_CX * PR_CY;
+    int coff = 0;   // a valid offset; cin: the cell is in the domain
+    bool cin = false;
     int clx = 0, cly = 0;
-    if (tid < PR_CX * PR_CY) {
+    if (has_c) {
         clx = tid % PR_CX;
         cly = tid / PR_CX;
         int gi = ci0 + clx, gj = cj0 + cly;
         bool in = true;
         if (gc.px) gi = wrapi(gi, gc.nx); else in = in && gi >= 0 && gi < gc.nx;
         if (gc.py) gj = wrapi(gj, gc.ny); else in = in && gj >= 0 && gj < gc.ny;
-        if (in) coff = (int64_t)gj * gc.sy + gi;
+        if (in) { coff = gj * (int)gc.sy + gi; cin = true; }
     }
-    auto load_coarse = [&](int m, int buf) {
-        if (tid < PR_CX * PR_CY) {
-            const bool zin = gc.pz || (m >= 0 && m < gc.nz);
-            const int mw = gc.pz ? wrapi(m, gc.nz) : m;
-            cp[buf][cly][clx] = (zin && coff >= 0) ? Tf(xc[(int64_t)mw * gc.sz + coff]) : Tf(0);
-        }
+    auto czin = [&](int m) { return gc.pz || (m >= 0 && m < gc.nz); };
+    // the coarse value of plane m (from a valid address; zero outside)
+    auto ldc = [&](int m) -> Tf {
+        const bool zi = czin(m);
+        const int mw = gc.pz ? wrapg(m, gc.nz) : (zi ? m : 0);
+        const Tf v = Tf(xc[(int64_t)mw * gc.sz + coff]);
+        return (zi && cin) ? v : Tf(0);
     };
-    // x and y parts of the stencil, fixed per thread and row
+    const int kc0 = k0 >> 1;
+    auto slot = [&](int m) { return (m - kc0 + 1) % PR_RING; };   // m >= kc0 - 1
+    auto putc = [&](int m, Tf v) {
+        if (has_c) cp[slot(m)][cly][clx] = v;
+    };
+    // x and y parts of the stencil, fixed per thread and row; the two
+    // coefficients for the z side in or out of the domain
     const int ic = i >> 1;
     const int icn = (i & 1) ? ic + 1 : ic - 1;
     const int mx = (gc.px || (icn >= 0 && icn < gc.nx)) ? 1 : 0;
     const int lx = ic - ci0, lxn = icn - ci0;
-    int ly[SW_R], lyn[SW_R], my[SW_R];
+    int ly[SW_R], lyn[SW_R], off[SW_R];
+    Tf pc0[SW_R], pc1[SW_R];
 #pragma unroll
     for (int r = 0; r < SW_R; r++) {
         const int j = j0 + ty + 4 * r;
         const int jc = j >> 1;
         const int jcn = (j & 1) ? jc + 1 : jc - 1;
-        my[r] = (gc.py || (jcn >= 0 && jcn < gc.ny)) ? 1 : 0;
+        const int my = (gc.py || (jcn >= 0 && jcn < gc.ny)) ? 1 : 0;
         ly[r] = jc - cj0;
         lyn[r] = jcn - cj0;
+        off[r] = j * (int)gf.sy + i;
+        pc0[r] = Tf(tab.pcoef[mx + 2 * my]);
+        pc1[r] = Tf(tab.pcoef[mx + 2 * my + 4]);
     }
-    const int kc0 = k0 >> 1;
-    // ring slot of coarse plane m: (m - kc0 + 1) mod 3
-    load_coarse(kc0 - 1, 0);
-    load_coarse(kc0, 1);
-    load_coarse(kc0 + 1, 2);
-    __syncthreads();
-    for (int kc = kc0; kc < kc0 + kz / 2; kc++) {
-        if (kc > kc0) {
-            __syncthreads();                       // readers of plane kc-2 are done
-            load_coarse(kc + 1, (kc + 1 - kc0 + 1) % 3);
-            __syncthreads();
+    const int kce = kc0 + kz / 2;
+    putc(kc0 - 1, ldc(kc0 - 1));
+    putc(kc0, ldc(kc0));
+    putc(kc0 + 1, ldc(kc0 + 1));
+    Tf cnext = ldc(kc0 + 2);
+    Tf fv[2][SW_R];
+#pragma unroll
+    for (int h = 0; h < 2; h++)
+#pragma unroll
+        for (int r = 0; r < SW_R; r++) fv[h][r] = xf[(int64_t)(k0 + h) * gf.sz + off[r]];
+    for (int kc = kc0; kc < kce; kc++) {
+        const bool more = kc + 1 < kce;
+        // coarse plane kc+2 into the slot of kc-3, last read two planes ago
+        if (more) putc(kc + 2, cnext);
+        Tf fc[2][SW_R];
+#pragma unroll
+        for (int h = 0; h < 2; h++)
+#pragma unroll
+            for (int r = 0; r < SW_R; r++) fc[h][r] = fv[h][r];
+        if (more) {
+            if (kc + 2 < kce) cnext = ldc(kc + 3);
+#pragma unroll
+            for (int h = 0; h < 2; h++)
+#pragma unroll
+                for (int r = 0; r < SW_R; r++)
+                    fv[h][r] = xf[(int64_t)(2 * kc + 2 + h) * gf.sz + off[r]];
         }
-        const int sm = (kc - 1 - kc0 + 1) % 3, s0 = (kc - kc0 + 1) % 3, sp = (kc + 1 - kc0 + 1) % 3;
+        __syncthreads();
+        const int sm = slot(kc - 1), s0 = slot(kc), sp = slot(kc + 1);
 #pragma unroll
         for (int h = 0; h < 2; h++) {              // fine planes 2kc (side kc-1), 2kc+1 (side kc+1)
             const int k = 2 * kc + h;
             const int kcn = h ? kc + 1 : kc - 1;
-            const int mz = (gc.pz || (kcn >= 0 && kcn < gc.nz)) ? 1 : 0;
+            const bool mz = gc.pz || (kcn >= 0 && kcn < gc.nz);
             const int sn = h ? sp : sm;
 #pragma unroll
             for (int r = 0; r < SW_R; r++) {
@@ -1828,10 +1861,9 @@ __global__ __launch_bounds__(256) void k_mg_prolong25(Geo gf, Geo gc, Tf *__rest
                 };
                 const Tf bmid = A(s0);
                 const Tf aside = A(sn);
-                const Tf term = Tf(tab.pcoef[mx + 2 * my[r] + 4 * mz]) * (Tf(3) * bmid + aside);
-                const int j = j0 + ty + 4 * r;
-                const int64_t c = (int64_t)k * gf.sz + (int64_t)j * gf.sy + i;
-                const Tf v = xf[c] + term;
+                const Tf term = (mz ? pc1[r] : pc0[r]) * (Tf(3) * bmid + aside);
+                const int64_t c = (int64_t)k * gf.sz + off[r];
+                const Tf v = fc[h][r] + term;
                 if (yf) yf[c] = v;
                 else xf[c] = v;
             }

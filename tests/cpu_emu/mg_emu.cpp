@@ -77,6 +77,16 @@ int check_tiled(int nx, int ny, int nz, int px, int py, int pz) {
     run_blocks(dim3(nx / SW_TX, ny / SW_TY, nz / kzr), dim3(SW_TX, 4, 1), [&]() {
         k_mg_prolong25<T, T>(gf, gc, p4.data(), xc.data(), tab, kzr, p3.data()); }, true);
     bad += ndiff(p1, p2) + ndiff(p1, p3) + ndiff(p4, x0);
+    for (int kzp : {2, 4, 6, 8}) {   // every march length of the coarse-plane ring
+        if (nz % kzp) continue;
+        std::vector<T> p5 = x, p6(nf);
+        run_blocks(dim3(nx / SW_TX, ny / SW_TY, nz / kzp), dim3(SW_TX, 4, 1), [&]() {
+            k_mg_prolong25<T, T>(gf, gc, p5.data(), xc.data(), tab, kzp, nullptr); }, true);
+        std::vector<T> x1 = x;
+        run_blocks(dim3(nx / SW_TX, ny / SW_TY, nz / kzp), dim3(SW_TX, 4, 1), [&]() {
+            k_mg_prolong25<T, T>(gf, gc, x1.data(), xc.data(), tab, kzp, p6.data()); }, true);
+        bad += ndiff(p1, p5) + ndiff(p1, p6);
+    }
     // two sweeps in one pass
     std::vector<T> z1(nf), z2(nf);
     run_blocks(mg_grid(gf), dim3(MG_BX, MG_BY, 1), [&]() {
