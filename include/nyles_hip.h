@@ -267,7 +267,7 @@ int nyl_reduce_noalloc(const nyl_desc *desc, int32_t dtype, const void *b, const
 #define NYL_K_RESNORM 8      /* residual norm (two launches); mixed mode: outer FP64 residual */
 #define NYL_K_MG_UPDATE 9    /* mixed mode: x += e of the defect correction */
 #define NYL_K_MG_SWEEP_FINE 10  /* the NYL_K_MG_SWEEP launches on the finest level only */
-#define NYL_K_MG_SWEEP2 11   /* two post-smoothing sweeps fused in one pass */
+#define NYL_K_MG_SWEEP2 11   /* two sweeps fused in one pass (pre- or post-smoothing) */
 #define NYL_K_MG_COARSE 12   /* direct solve of a coarsest level with an odd size */
 #define NYL_K_MG_SWEEP2_FINE 13  /* the NYL_K_MG_SWEEP2 launches on the finest level only */
 int nyl_prof_start(nyl_ctx *ctx, int kernel_class);
