@@ -354,6 +354,19 @@ int smooth_level_t(nyl_ctx *c, int l, int xc_level, bool pre_done, double *addx,
         Level &q = c->lev[xc_level];
         const uint64_t qx = sol_off(c, xc_level);
         const bool same = lev64(c, xc_level) == std::is_same<Tf, double>::value;
+        if (sol_in_y) {
+            // the level's solution is in y (both pre-sweeps in one pass):
+            // y += P(xc) in place, then both post-sweeps y -> x
+            if (!(NYL_SWEEP2 && same && restrict25_fits(f.g) && tiled))
+                return fail(NYL_E_STATE, "level solution in y without the two-sweep pass");
+            Tf *yv = F<Tf>(c, f.y);
+            LAUNCH(NYL_K_MG_PROLONG, 2.125 * cells(f.g) * sizeof(Tf),
+                   launch_mg_prolong25<Tf, Tf>(f.g, q.g, yv, F<Tf>(c, qx), c->tabs, c->stream));
+            const double b2 = addx ? (2 * sizeof(Tf) + 2 * sizeof(double)) * cells(f.g) : by;
+            LAUNCH(NYL_K_MG_SWEEP2 | (l == 0 ? kFineLevel : 0), b2,
+                   launch_mg_sweep2x25<Tf>(f.g, yv, x, b, om, c->tabs, c->stream, addx));
+            return NYL_OK;
+        }
         if (NYL_FUSE_PROLONG && tiled && same) {
             // prolongation fused into the first sweep: x + P(xc) -> y, then y -> x
             const double bp = (3 + 0.125) * cells(f.g) * sizeof(Tf);
@@ -365,19 +378,6 @@ int smooth_level_t(nyl_ctx *c, int l, int xc_level, bool pre_done, double *addx,
             return NYL_OK;
         }
         const double bp = 2.125 * cells(f.g) * sizeof(Tf);
-        if (sol_in_y) {
-            // the level's solution is in y (both pre-sweeps in one pass):
-            // y += P(xc) in place, then both post-sweeps y -> x
-            if (!(NYL_SWEEP2 && same && restrict25_fits(f.g) && tiled))
-                return fail(NYL_E_STATE, "level solution in y without the two-sweep pass");
-            Tf *yv = F<Tf>(c, f.y);
-            LAUNCH(NYL_K_MG_PROLONG, bp,
-                   launch_mg_prolong25<Tf, Tf>(f.g, q.g, yv, F<Tf>(c, qx), c->tabs, c->stream));
-            const double b2 = addx ? (2 * sizeof(Tf) + 2 * sizeof(double)) * cells(f.g) : by;
-            LAUNCH(NYL_K_MG_SWEEP2 | (l == 0 ? kFineLevel : 0), b2,
-                   launch_mg_sweep2x25<Tf>(f.g, yv, x, b, om, c->tabs, c->stream, addx));
-            return NYL_OK;
-        }
         if (NYL_SWEEP2 && same && restrict25_fits(f.g) && tiled && !pre_done) {
             // y = x + P(xc), then both post-smoothing sweeps y -> x in one
             // pass (with addx: the second sweep's result added to the FP64 x)
@@ -722,9 +722,10 @@ void fill_stats(nyl_ctx *c, nyl_mg_stats *st, int nite, double res, double normb
 }
 
 // core/mgfor/solvers.f90:8-33 with normresidual (core/mgfor/operators.f90:81-96).
-// When level 1 uses the tiled sweep, the residual norm of x is computed by
-// the first pre-smoothing sweep of the next V-cycle, launched before the
-// test: if the test stops the solve, that sweep's output (in y) is unused.
+// When level 1 uses the tiled sweeps, the residual norm of x is computed by
+// the pre-smoothing of the next V-cycle, launched before the test (both
+// sweeps in the two-sweep pass, or the first one): if the test stops the
+// solve, that output (in y) is unused and x is untouched.
 int mg_solve_fp64(nyl_ctx *c, nyl_mg_stats *st) {
     Level &f = c->lev[0];
     double *part = F<double>(c, c->partials), *res_dev = F<double>(c, c->result);
@@ -788,7 +789,7 @@ int mg_solve_fp64(nyl_ctx *c, nyl_mg_stats *st) {
 // FP64 and solves A e = r from e = 0 with FP32 V-cycles under mgfor's loop:
 // before each further V-cycle the squared residual r - A e = b - A(x + e),
 // evaluated in FP32 and summed in FP64, is tested against tol*|b|^2 (fused
-// into that V-cycle's first sweep, as in the FP64 path).  Then x += e in FP64
+// into that V-cycle's pre-smoothing, as in the FP64 path).  Then x += e in FP64
 // and the FP64 residual confirms the stop (or starts another outer step).
 // The V-cycle count follows mgfor's rule (nite, maxite).
 // One inner solve resolves a limited reduction of r: A e of a smooth e

@@ -1588,6 +1588,16 @@ static int sweep25_kz(const Geo &g, int kmin) {
     return kz;
 }
 
+// planes per block of the restriction and the prolongation (even): up to 16
+// in FP32 (256^3: 28.4 and 26.8 us against 29.4 and 28.0 at 8 planes), 8 in
+// FP64, keeping at least 1024 blocks
+static int rp25_kz(const Geo &g, int es) {
+    const int64_t cols = (int64_t)(g.nx / SW_TX) * (g.ny / SW_TY);
+    int kz = es == 4 ? 16 : 8;
+    while (kz > 2 && (g.nz % kz != 0 || cols * (g.nz / kz) < 1024)) kz >>= 1;
+    return kz;
+}
+
 int sweep25_blocks(const Geo &g) {
     return (g.nx / SW_TX) * (g.ny / SW_TY) * (g.nz / sweep25_kz(g, 1));
 }
@@ -1874,7 +1884,7 @@ __global__ __launch_bounds__(256) void k_mg_prolong25(Geo gf, Geo gc, Tf *__rest
 template <typename Tf, typename Tc>
 hipError_t launch_mg_prolong25(const Geo &gf, const Geo &gc, Tf *xf, const Tc *xc,
                                const MGTabs &tab, hipStream_t s, Tf *yf) {
-    const int kz = sweep25_kz(gf, 2);
+    const int kz = rp25_kz(gf, (int)sizeof(Tf));
     dim3 grd(gf.nx / SW_TX, gf.ny / SW_TY, gf.nz / kz), blk(SW_TX, 4);
     NYL_LAUNCH((k_mg_prolong25<Tf, Tc>), grd, blk, 0, s, gf, gc, xf, xc, tab, kz, yf);
     return hipGetLastError();
@@ -2422,7 +2432,7 @@ bool restrict25_fits(const Geo &g) {
 template <typename Tf, typename Tc>
 hipError_t launch_mg_restrict25(const Geo &gf, const Geo &gc, const Tf *x, const Tf *b, Tc *bc,
                                 Tc *xc, const MGTabs &tab, hipStream_t s, Tc *yc, double omega) {
-    const int kz = sweep25_kz(gf, 2);
+    const int kz = rp25_kz(gf, (int)sizeof(Tf));
     dim3 grd(gf.nx / SW_TX, gf.ny / SW_TY, gf.nz / kz), blk(SW_TX, 4);
     NYL_LAUNCH((k_mg_restrict25<Tf, Tc>), grd, blk, 0, s, gf, gc, x, b, bc, xc, tab, kz,
                        yc, omega);

@@ -67,6 +67,14 @@ int check_tiled(int nx, int ny, int nz, int px, int py, int pz) {
         k_mg_restrict25<T, T>(gf, gc, x.data(), b.data(), bc2.data(), xd.data(), tab, kzr,
                               yc2.data(), om); }, true);
     bad += ndiff(bc1, bc2) + ndiff(yc1, yc2);
+    for (int kzq : {2, 4, 6, 8}) {   // every even march length
+        if (nz % kzq) continue;
+        std::vector<T> bc3(nc), yc3(nc);
+        run_blocks(dim3(nx / SW_TX, ny / SW_TY, nz / kzq), dim3(SW_TX, 4, 1), [&]() {
+            k_mg_restrict25<T, T>(gf, gc, x.data(), b.data(), bc3.data(), xd.data(), tab, kzq,
+                                  yc3.data(), om); }, true);
+        bad += ndiff(bc1, bc3) + ndiff(yc1, yc3);
+    }
     // prolongation, in place and into another array
     std::vector<T> p1 = x, p2 = x, p3(nf), x0 = x;
     run_blocks(mg_grid(gf), dim3(MG_BX, MG_BY, 1), [&]() {
