@@ -277,8 +277,11 @@ int nyl_prof_stop(nyl_ctx *ctx, double *total_ms, int64_t *launches, double *alg
  * 0: one fsmoother3d sweep NYL_F_MGX -> NYL_F_MGY with the per-cell kernel;
  * 1: the same sweep with the tiled kernel; 2: tiled sweep that also reduces
  * the squared residual of NYL_F_MGX (read it with nyl_reduce_last); 3: that
- * residual norm alone.  Used by the parity tests; no reference counterpart
- * beyond the sweep itself (core/mgfor/basicoperators.f90:363-400). */
+ * residual norm alone; 4: two sweeps NYL_F_MGX -> NYL_F_MGY in one tiled
+ * pass; 5: the same with the residual norm of NYL_F_MGX; 6: two sweeps from
+ * zero into NYL_F_MGY (NYL_F_MGX not read).  Used by the parity tests; no
+ * reference counterpart beyond the sweep itself
+ * (core/mgfor/basicoperators.f90:363-400). */
 int nyl_debug_mg(nyl_ctx *ctx, int what);
 int nyl_reduce_last(nyl_ctx *ctx, double *out);
 

@@ -1442,6 +1442,17 @@ int nyl_debug_mg(nyl_ctx *c, int what) {
     case 3:
         HIPCHK(launch_resnorm2<double>(f.g, x, b, c->tabs, part, res_dev, c->stream));
         break;
+    case 4:
+    case 5:
+    case 6:
+        if (!sweep25_fits(f.g)) return fail(NYL_E_SIZE, "level 1 does not fit the tiled sweep");
+        if (what == 5 && sweep2x_blocks(f.g, 8) > NYL_RED_BLOCKS)
+            return fail(NYL_E_SIZE, "too many blocks for the residual norm");
+        HIPCHK(launch_mg_sweep2x25<double>(f.g, what == 6 ? nullptr : x, y, b, c->d.mg_omega,
+                                           c->tabs, c->stream, nullptr,
+                                           what == 5 ? part : nullptr,
+                                           what == 5 ? res_dev : nullptr));
+        break;
     default:
         return fail(NYL_E_ARG, "unknown debug operation %d", what);
     }

@@ -167,9 +167,9 @@ class DeviceModel:
 
     def debug_mg(self, what):
         """Level-1 multigrid kernel check (see nyl_debug_mg); returns the
-        reduced residual norm for what = 2, 3."""
+        reduced residual norm for what = 2, 3, 5."""
         L.check(self.lib.nyl_debug_mg(self.ctx, what))
-        if what in (2, 3):
+        if what in (2, 3, 5):
             out = ctypes.c_double()
             L.check(self.lib.nyl_reduce_last(self.ctx, ctypes.byref(out)))
             return out.value

@@ -214,3 +214,42 @@ def test_tiled_sweep_matches_per_cell_sweep(cmp, geometry, size):
     r3 = dev.debug_mg(3)
     assert abs(r2 - r3) <= 1e-12 * r3
     dev.close()
+
+
+@pytest.mark.parametrize("size", SWEEP_SIZES, ids=["%dx%dx%d" % s for s in SWEEP_SIZES])
+@pytest.mark.parametrize("geometry", GEOMETRIES)
+def test_two_sweep_pass_matches_two_sweeps(cmp, geometry, size):
+    """The two-sweep pass (k_mg_sweep2t: plain, with the residual norm of its
+    input, and from a zero input) equals two per-cell sweeps bitwise; its
+    residual norm equals the standalone one to rounding."""
+    from nyles_amd import _lib as L
+    print("case sweep2", geometry, size, flush=True)
+    nx, ny, nz = size
+    orc, dev = cmp.make_pair(nx, ny, nz, geometry)
+    rng = np.random.default_rng(13)
+    x0 = rng.standard_normal((nz, ny, nx))
+    dev.upload(L.NYL_F_MGB, rng.standard_normal((nz, ny, nx)))
+
+    def two_sweeps(x):
+        dev.upload(L.NYL_F_MGX, x)
+        dev.debug_mg(0)
+        dev.upload(L.NYL_F_MGX, dev.download(L.NYL_F_MGY))
+        dev.debug_mg(0)
+        return dev.download(L.NYL_F_MGY)
+
+    want = two_sweeps(x0)
+    want0 = two_sweeps(np.zeros_like(x0))
+    dev.upload(L.NYL_F_MGX, x0)
+    dev.upload(L.NYL_F_MGY, np.zeros_like(x0))
+    dev.debug_mg(4)
+    _assert_same("two-sweep pass", dev.download(L.NYL_F_MGY), want)
+    dev.upload(L.NYL_F_MGY, np.zeros_like(x0))
+    r5 = dev.debug_mg(5)
+    _assert_same("two-sweep pass + norm", dev.download(L.NYL_F_MGY), want)
+    r3 = dev.debug_mg(3)
+    assert abs(r5 - r3) <= 1e-12 * r3
+    dev.upload(L.NYL_F_MGX, np.full_like(x0, np.nan))   # not read
+    dev.upload(L.NYL_F_MGY, np.zeros_like(x0))
+    dev.debug_mg(6)
+    _assert_same("two-sweep pass from zero", dev.download(L.NYL_F_MGY), want0)
+    dev.close()
