@@ -16,17 +16,21 @@ REF = os.environ.get("NYL_REFERENCE_TREE", "/root/reference")
 CITE = re.compile(r"((?:core|experiments|tools|doc|docs)/[A-Za-z0-9_./-]+\.(?:py|f90|json|txt|md|png|cfg))"
                   r":(\d+)(?:-(\d+))?")
 SCAN_EXT = (".py", ".hip", ".h", ".c", ".cpp", ".md", ".json")
-SKIP_DIRS = {".git", "_work", "gpurun_out", "__pycache__", "_build", ".pytest_cache"}
-# driver/judge-written records and the survey quote the reference themselves
-SKIP_FILES = {"SURVEY.md", "VERDICT.md", "ADVICE.md", "BASELINE.md", "BASELINE.json"}
+SCAN_DIRS = ("nyles_amd", "include", "oracle", "tests", "profiles")
+SKIP_DIRS = {"__pycache__", "_build", ".pytest_cache"}
+DOCS = ("DESIGN.md", "INTEGRATION.md")   # the repository's own documents
 
 
 def _files():
-    for dp, dns, fns in os.walk(ROOT):
-        dns[:] = [d for d in dns if d not in SKIP_DIRS]
-        for fn in fns:
-            if fn.endswith(SCAN_EXT) and fn not in SKIP_FILES and not re.match(r"[A-Z]+_r\d+\.json", fn):
-                yield os.path.join(dp, fn)
+    for d in SCAN_DIRS:
+        for dp, dns, fns in os.walk(os.path.join(ROOT, d)):
+            dns[:] = [x for x in dns if x not in SKIP_DIRS]
+            for fn in fns:
+                if fn.endswith(SCAN_EXT):
+                    yield os.path.join(dp, fn)
+    for fn in sorted(os.listdir(ROOT)):
+        if fn.endswith(".py") or fn in DOCS:
+            yield os.path.join(ROOT, fn)
 
 
 def _nlines(path, cache={}):
